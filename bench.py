@@ -1,0 +1,223 @@
+#!/usr/bin/env python3
+"""Benchmark of the mesh-propagation hot path on one MI355X.
+
+Workload: Clements MZI mesh, N=256 modes, L=256 layers, ideal 50:50 splitters, phases
+U[0, 2pi) (float32), B=4096 complex64 unit-norm CN(0,1) input vectors.  One step is one
+fwd+grad pass of the loss J = sum |U x - y|^2 with y = U_target x (U_target a second
+random-phase mesh, precomputed): fwdprop, then backdiff with dJ/du_out = 2 (u_out - y)
+fused into its load, accumulating dJ/dp.  Inputs are resident in HBM before timing.
+
+Prints one JSON line.  value = MZI-vector updates/s = B * sum(lens) / step time (an update
+is one active crossing applied to one vector through the forward and the adjoint pass,
+128 FLOP).  The roofline object is for the dominant kernel (backdiff propagation, 96 FLOP
+per update), timed with HIP events on the stream it runs on.  cpu_baseline times the
+compiled reference CPU engine (c/dot.cpp, built into oracle/_ref by oracle/Makefile) on a
+bounded slice of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "MZI-vector updates/s fwd+grad, N=256 L=256 B=4096; p50/p99 request latency"
+N, L, B = 256, 256, 4096
+FP32_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(p, u, y, seconds_target=15.0):
+    """Reference c/dot.cpp (complex128, as mesh.py:98-113 calls it) on a slice of the batch:
+    mode 0 (forward) + mode 2 (VJP).  Falls back to the NumPy oracle ("port") if the compiled
+    reference is absent."""
+    from oracle import meshprop_oracle as O
+    g = O.clements(N)
+    T = O.slot_T(g, "mzi", p.astype(np.float64))
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    threads = max(1, min(16, cores))
+    nact = int(np.sum(g.lens))
+    try:
+        from oracle import ref_engine
+        if not ref_engine.available():
+            raise FileNotFoundError(ref_engine.LIB_PATH)
+        kind = "reference"
+
+        def one(bs):
+            x = u[:bs].astype(np.complex128)
+            t0 = time.perf_counter()
+            out = ref_engine.run(g, T, x, 0, threads=threads)
+            ref_engine.run(g, T, out, 2, dv=2 * (out - y[:bs]), threads=threads)
+            return time.perf_counter() - t0
+    except Exception:
+        kind = "port"
+        threads = 1
+
+        def one(bs):
+            x = u[:bs].astype(np.complex128)
+            t0 = time.perf_counter()
+            out = O.fwd(g, T, x)
+            O.vjp_inverse(g, T, out, 2 * (out - y[:bs]))
+            return time.perf_counter() - t0
+    bs = 64
+    t = one(bs)
+    # grow the slice toward ~seconds_target of CPU work, within the batch, then repeat it
+    while t < seconds_target / 4 and bs < B:
+        bs = min(B, bs * 4)
+        t = one(bs)
+    times = [t]
+    while sum(times) < seconds_target and len(times) < 7:
+        times.append(one(bs))
+    t = float(np.median(times))
+    value = bs * nact / t
+    return {"value": value, "unit": "MZI-vector updates/s", "cores": threads, "kind": kind,
+            "sample": f"c/dot.cpp complex128 fwd (mode 0) + VJP (mode 2), same N=256 L=256 Clements mesh, "
+                      f"{bs} of the {B} vectors, {threads} OpenMP threads, median of {len(times)} runs, {t:.2f} s each"
+            if kind == "reference" else f"NumPy FP64 oracle fwd + inverse-propagation VJP, {bs} vectors, {t:.2f} s"}
+
+
+def load_traffic():
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 --pmc summary."""
+    path = os.path.join(ROOT, "profiles", "k_bwd_c2_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--tile-fwd", type=str, default="")
+    ap.add_argument("--tile-bwd", type=str, default="")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the workload runs on a single MI355X (batch sharding and the dp "
+              "all-reduce across GPUs are described in DESIGN.md, not run).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import meshes_amd._lib as Lb
+    from meshes_amd import MeshNetworkGPU, geometry
+
+    dev = torch.device("cuda", 0)
+    lens, shifts = geometry.clements(N)
+    p = np.random.RandomState(2).uniform(0, 2 * np.pi, (L, N // 2, 2)).astype(np.float32)
+    pt = np.random.RandomState(3).uniform(0, 2 * np.pi, (L, N // 2, 2)).astype(np.float32)
+    x = geometry.unit_cn(np.random.RandomState(4), B, N).astype(np.complex64)
+    tile_f = tuple(int(t) for t in args.tile_fwd.split(",")) if args.tile_fwd else None
+    tile_b = tuple(int(t) for t in args.tile_bwd.split(",")) if args.tile_bwd else None
+    mesh = MeshNetworkGPU(N, L, lens, shifts, p, np.zeros(0), "mzi", is_phase=False)
+    target = MeshNetworkGPU(N, L, lens, shifts, pt, np.zeros(0), "mzi", is_phase=False)
+    xb = torch.as_tensor(x).to(dev).contiguous()          # (B, N): Fortran (N, B) buffer
+    yb = target.dot(xb.T).T.contiguous()
+    uout = torch.empty_like(xb)
+    uin = torch.empty_like(xb)
+    dp = torch.zeros(mesh.n_phase, dtype=torch.float32, device=dev)
+    ws_f_n = Lb.workspace_size(Lb.OP_FWDPROP, "mzi", N, L, B, False, tile_f)
+    ws_b_n = Lb.workspace_size(Lb.OP_BACKDIFF, "mzi", N, L, B, False, tile_b)
+    ws_f = torch.empty(ws_f_n, dtype=torch.uint8, device=dev)
+    ws_b = torch.empty(ws_b_n, dtype=torch.uint8, device=dev)
+    geo = (mesh.lens, mesh.shifts)
+    ldp = N
+
+    def fwd():
+        Lb.run(Lb.OP_FWDPROP, "mzi", N, L, B, False, *geo, mesh.p_phase, None, ldp, None, 0, xb, None, uout, None,
+               N, tile=tile_f, workspace=ws_f, ws_bytes=ws_f_n)
+
+    def bwd(flags):
+        Lb.run(Lb.OP_BACKDIFF, "mzi", N, L, B, False, *geo, mesh.p_phase, dp, ldp, None, 0, uout, None, uin, None,
+               N, target=yb, tile=tile_b, workspace=ws_b, ws_bytes=ws_b_n, flags=flags)
+
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(6)] for _ in range(args.steps)]
+
+    def step(e=None):
+        if e:
+            e[0].record()
+        dp.zero_()
+        fwd()
+        if e:
+            e[1].record()
+        bwd(Lb.ONLY_PROLOGUE)
+        if e:
+            e[2].record()
+        bwd(Lb.SKIP_PROLOGUE | Lb.SKIP_REDUCE)
+        if e:
+            e[3].record()
+        bwd(Lb.ONLY_REDUCE)
+        if e:
+            e[4].record()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(ev[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+
+    nact = int(np.sum(lens))
+    updates = B * nact
+    ms_step = wall * 1e3 / args.steps
+    step_ms = np.array([ev[k][0].elapsed_time(ev[k][4]) for k in range(args.steps)])
+    kb_ms = np.array([ev[k][2].elapsed_time(ev[k][3]) for k in range(args.steps)])
+    fwd_ms = np.array([ev[k][0].elapsed_time(ev[k][1]) for k in range(args.steps)])
+    kb_avg = float(np.mean(kb_ms))
+    flop_bwd = 96.0 * updates
+    achieved = flop_bwd / (kb_avg * 1e-3) / 1e12
+    # correctness of the measured configuration: reconstructed input vs x, gradient finite
+    rec_err = float(torch.linalg.norm(uin - xb) / torch.linalg.norm(xb))
+    out = {
+        "metric": METRIC,
+        "value": updates / (ms_step * 1e-3),
+        "unit": "MZI-vector updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (complex64 fields, float32 phases)",
+        "data": "synthetic: seeded U[0,2pi) phases, CN(0,1) unit-norm inputs, target y = U_target x",
+        "config": {"workload": "Clements MZI mesh fwd+grad (fwdprop + backdiff with fused L2 loss gradient), "
+                               "N=256 L=256 B=4096, ideal splitters, complex64",
+                   "N": N, "L": L, "B": B, "active_crossings": nact, "updates_per_step": updates,
+                   "flop_per_update": 128, "tile_fwd": tile_f, "tile_bwd": tile_b},
+        "latency_ms": {"p50": float(np.percentile(step_ms, 50)), "p99": float(np.percentile(step_ms, 99)),
+                       "what": "per-step fwd+grad latency (HIP events)"},
+        "kernels_ms": {"fwdprop_total": float(np.mean(fwd_ms)), "backdiff_propagation": kb_avg},
+        "gflops_per_s_fwd_plus_grad": 128.0 * updates / (ms_step * 1e-3) / 1e9,
+        "reconstruction_rel_err": rec_err,
+        "roofline": {"bound": "mfma", "compute": "fp32 VALU (2x2 complex rotations: no MFMA shape)",
+                     "kernel": "k_bwd<complex64> (backdiff propagation)",
+                     "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": load_traffic(),
+                     "flop_per_launch": flop_bwd, "avg_launch_ms": kb_avg},
+    }
+    if not args.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline(p, x, yb.cpu().numpy(), args.cpu_seconds)
+        except Exception as e:  # the GPU figure stands without it
+            out["cpu_baseline"] = {"value": None, "error": repr(e)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

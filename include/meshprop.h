@@ -1,0 +1,162 @@
+/*
+ * meshprop.h -- C ABI of the MI355X (gfx950) mesh-propagation library libmeshprop.so.
+ *
+ * Batched propagation of complex (or, for orthogonal crossings, real) field vectors
+ * through L layers of 2x2 photonic crossings, in three directions: forward (fwdprop),
+ * forward-mode differentiation (fwddiff) and reverse-mode differentiation (backdiff).
+ *
+ * Conventions shared with the CUDA kernels these entry points replace:
+ *   - N modes (even), L layers, B vectors.  Fields are column-major (N, B) arrays with
+ *     leading dimension ldu >= N: vector b starts at u + b*ldu (gpu/mesh.py:150-155).
+ *     Element type is complex64 (interleaved float re/im) for MESHPROP_MZI, MESHPROP_SYM
+ *     and MESHPROP_MZI_GENERIC, and float32 for MESHPROP_ORTH (gpu/mesh.py:73).
+ *   - Parameters use the slot layout p[l*ldp + m*n_p + k], s[l*lds + m*n_s + k] with
+ *     ldp = N/2*n_p, lds = N/2*n_s (gpu/mesh.py:154-155, gpu/gmem.cu:236-240).
+ *     n_p = 2,2,1,2 and n_s = 2,1,0,14 for MZI, SYM, ORTH, MZI_GENERIC.
+ *   - NN geometry: lens[l], shifts[l]; slot m of layer l couples waveguides
+ *     (2m + shifts[l]%2, 2m + shifts[l]%2 + 1) and is active iff
+ *     shifts[l]/2 <= m < shifts[l]/2 + lens[l]; inactive slots are the identity.
+ *     FFT geometry (the *_ft entry points): strides[l]; slot c = blk*S + j, S = 2^strides[l],
+ *     couples (2*S*blk + j, 2*S*blk + j + S) (gpu/fwdprop.cu:48-93, gpu/test.py:73-77).
+ *   - Gradient convention: dJdu = dJ/dRe(u) + i dJ/dIm(u); dp += Re sum_ij dJdT_ij dT_ij/dp
+ *     with dJdT_ij = sum_b conj(dJdu_i) u_j (gpu/crossing.cu:150-203, 285-302).
+ *
+ * Every pointer argument except `stream` is a device pointer.  Geometry arrays
+ * (lens, shifts, strides) are int32 device arrays of length L, as the reference kernels
+ * take them.  Calls are asynchronous on `stream` (a hipStream_t; NULL = default stream),
+ * never synchronise and never allocate: the caller provides a device workspace of at
+ * least meshprop_workspace_size(...) bytes.  Nullable pointers: s (ideal splitters),
+ * du_in (zero tangent), du_out, dJdu_in.  backdiff supports u_in == u_out and
+ * dJdu_in == dJdu_out (in place), as gpu/mesh.py:228-231 calls it.  `dp` of backdiff is
+ * accumulated into (gpu/mesh.py:214 zero-fills it first).  Each entry point returns a
+ * meshprop_status code.
+ */
+#ifndef MESHPROP_H
+#define MESHPROP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum meshprop_crossing {
+    MESHPROP_MZI = 0,          /* T(theta, phi; alpha, beta): crossing.py:216-221, gpu/crossing.cu:36-84 */
+    MESHPROP_SYM = 1,          /* T(theta, phi; alpha): crossing.py:377-381, gpu/crossing.cu:86-126 */
+    MESHPROP_ORTH = 2,         /* real T(theta): gpu/crossing.cu:128-148 */
+    MESHPROP_MZI_GENERIC = 3   /* lossy MZI with 14 imperfection values: crossing.py:444-491 */
+};
+
+enum meshprop_status {
+    MESHPROP_OK = 0,
+    MESHPROP_ERR_ARG = 1,          /* bad size, stride or null pointer */
+    MESHPROP_ERR_WORKSPACE = 2,    /* workspace missing or smaller than meshprop_workspace_size */
+    MESHPROP_ERR_UNSUPPORTED = 3,  /* mode != 0, N beyond 1024, FFT stride that does not tile N */
+    MESHPROP_ERR_HIP = 4           /* a HIP launch failed; see meshprop_last_hip_error() */
+};
+
+enum meshprop_op {
+    MESHPROP_OP_FWDPROP = 0,
+    MESHPROP_OP_FWDDIFF = 1,
+    MESHPROP_OP_BACKDIFF = 2
+};
+
+/* Optional tile choice.  Zero fields mean "library default".  w: waveguides per lane
+ * (2, 4, 8, 16); v: vectors per lane (1, 2, 4, 8); waves: wave64s per workgroup (1..4);
+ * chunk: layers per LDS reduction stage of backdiff. */
+typedef struct meshprop_tile {
+    int w, v, waves, chunk;
+} meshprop_tile;
+
+/* Bytes of device workspace the given call needs (0 is possible). */
+size_t meshprop_workspace_size(int op, int X, int N, int L, int B, int fft, const meshprop_tile *tile);
+
+/* Replaces fwdprop_N{64K}_{mzi,sym,orth} (gpu/fwdprop.cu:25-27; launched at gpu/mesh.py:158-163). */
+int meshprop_fwdprop(int X, int N, int L, int B, const int *lens, const int *shifts,
+                     const float *p, int ldp, const float *s, int lds,
+                     const void *u_in, void *u_out, int ldu, int mode,
+                     void *workspace, size_t workspace_bytes, void *stream);
+
+/* Replaces fwdprop_ft_N{64K}_{X} (gpu/consts.cuh:31-33 argument list with strides; gpu/test.py:48-56). */
+int meshprop_fwdprop_ft(int X, int N, int L, int B, const int *strides,
+                        const float *p, int ldp, const float *s, int lds,
+                        const void *u_in, void *u_out, int ldu, int mode,
+                        void *workspace, size_t workspace_bytes, void *stream);
+
+/* Replaces fwddiff_N{64K}_{X} (gpu/fwddiff.cu:24-27; launched at gpu/mesh.py:165-174).
+ * dp is the tangent direction in parameter space (slot layout, same ldp as p). */
+int meshprop_fwddiff(int X, int N, int L, int B, const int *lens, const int *shifts,
+                     const float *p, const float *dp, int ldp, const float *s, int lds,
+                     const void *u_in, const void *du_in, void *u_out, void *du_out, int ldu, int mode,
+                     void *workspace, size_t workspace_bytes, void *stream);
+
+/* Replaces fwddiff_ft_N{64K}_{X}. */
+int meshprop_fwddiff_ft(int X, int N, int L, int B, const int *strides,
+                        const float *p, const float *dp, int ldp, const float *s, int lds,
+                        const void *u_in, const void *du_in, void *u_out, void *du_out, int ldu, int mode,
+                        void *workspace, size_t workspace_bytes, void *stream);
+
+/* Replaces backdiff_N{64K}_{X} (gpu/backdiff.cu:24-27; launched at gpu/mesh.py:226-231).
+ * Reconstructs u_in from u_out by inverse propagation and accumulates dp.  For
+ * MESHPROP_MZI_GENERIC (non-unitary) the reconstruction uses T^-1 and the adjoint T^H. */
+int meshprop_backdiff(int X, int N, int L, int B, const int *lens, const int *shifts,
+                      const float *p, float *dp, int ldp, const float *s, int lds,
+                      const void *u_out, const void *dJdu_out, void *u_in, void *dJdu_in, int ldu, int mode,
+                      void *workspace, size_t workspace_bytes, void *stream);
+
+/* Replaces backdiff_ft_N{64K}_{X}. */
+int meshprop_backdiff_ft(int X, int N, int L, int B, const int *strides,
+                         const float *p, float *dp, int ldp, const float *s, int lds,
+                         const void *u_out, const void *dJdu_out, void *u_in, void *dJdu_in, int ldu, int mode,
+                         void *workspace, size_t workspace_bytes, void *stream);
+
+/*
+ * Extended entry point behind all of the above (no reference counterpart: the reference
+ * has one kernel per size and geometry).  Adds a tile override, stage flags and two forms
+ * the reference kernels do not have:
+ *   - target (backdiff only, nullable): if set, dJdu_out is ignored and the loss gradient
+ *     of J = sum |u_out - target|^2, dJdu_out = 2 (u_out - target), is formed in registers
+ *     at load time (the loss of gpu/test.py:131, fused into the adjoint's load);
+ *   - ckpt/ckpt_every (nullable / 0): fwdprop writes the field entering every layer
+ *     l % ckpt_every == 0 into ckpt[(l/ckpt_every)*B*ldu ...]; backdiff reloads it there
+ *     instead of trusting the reconstruction (checkpointed adjoint for lossy meshes).
+ * fft selects the strides geometry (geom0 = strides, geom1 unused) or NN (geom0 = lens,
+ * geom1 = shifts).  dp/du_in/du_out are used by fwddiff; dp/dJdu_out/dJdu_in by backdiff.
+ */
+int meshprop_run(int op, int X, int N, int L, int B, int fft, const int *geom0, const int *geom1,
+                 const float *p, float *dp, int ldp, const float *s, int lds,
+                 const void *u_a, const void *du_a, void *u_b, void *du_b, int ldu, int mode,
+                 const void *target, void *ckpt, int ckpt_every, const meshprop_tile *tile, int flags,
+                 void *workspace, size_t workspace_bytes, void *stream);
+
+/* flags of meshprop_run.  A call is three stages: (1) the prologue that builds the row
+ * tables from p, s (and the tangent dp for fwddiff) into the workspace, (2) the
+ * propagation kernel, (3) for backdiff, the reduction of dp partials into dp.
+ * MESHPROP_ONLY_PROLOGUE runs (1) only; MESHPROP_SKIP_PROLOGUE reuses tables a previous
+ * call with the same op, crossing, sizes, geometry, parameters and tile left in this
+ * workspace (parameters held resident between calls); MESHPROP_SKIP_REDUCE leaves the
+ * partials in the workspace and MESHPROP_ONLY_REDUCE runs (3) alone. */
+enum meshprop_flags {
+    MESHPROP_ONLY_PROLOGUE = 1,
+    MESHPROP_SKIP_PROLOGUE = 2,
+    MESHPROP_SKIP_REDUCE = 4,
+    MESHPROP_ONLY_REDUCE = 8
+};
+
+/* The tile the library would use for a call (after defaults and clamping). */
+int meshprop_resolve_tile(int op, int X, int N, int B, meshprop_tile *out);
+
+/* Last HIP error code seen by a failing call on this thread (hipError_t as int). */
+int meshprop_last_hip_error(void);
+
+/* Human-readable status. */
+const char *meshprop_status_string(int status);
+
+/* Library version string. */
+const char *meshprop_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MESHPROP_H */

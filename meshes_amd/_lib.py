@@ -1,0 +1,130 @@
+"""ctypes binding of libmeshprop.so (the C ABI declared in include/meshprop.h).
+
+The library is the only compute path: there is no CPU fallback.  If it is missing, every
+entry point raises ``MeshpropUnavailable``.  torch is imported before the library is
+loaded so that both share the HIP runtime torch already mapped.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import torch  # noqa: F401  (must precede loading libmeshprop.so: one HIP runtime per process)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("MESHPROP_LIB", os.path.join(_HERE, "libmeshprop.so"))
+
+MZI, SYM, ORTH, MZI_GENERIC = 0, 1, 2, 3
+OP_FWDPROP, OP_FWDDIFF, OP_BACKDIFF = 0, 1, 2
+X_CODE = {"mzi": MZI, "sym": SYM, "orth": ORTH, "mzi_generic": MZI_GENERIC}
+N_PHASE = {"mzi": 2, "sym": 2, "orth": 1, "mzi_generic": 2}
+N_SPLIT = {"mzi": 2, "sym": 1, "orth": 0, "mzi_generic": 14}
+
+STATUS = {0: "ok", 1: "invalid argument", 2: "workspace missing or too small", 3: "unsupported configuration",
+          4: "HIP launch error"}
+
+# Every symbol include/meshprop.h declares.
+EXPORTS = (
+    "meshprop_workspace_size", "meshprop_fwdprop", "meshprop_fwdprop_ft", "meshprop_fwddiff",
+    "meshprop_fwddiff_ft", "meshprop_backdiff", "meshprop_backdiff_ft", "meshprop_run",
+    "meshprop_resolve_tile", "meshprop_last_hip_error", "meshprop_status_string", "meshprop_version",
+)
+
+
+class MeshpropUnavailable(RuntimeError):
+    pass
+
+
+class MeshpropError(RuntimeError):
+    pass
+
+
+class Tile(ctypes.Structure):
+    _fields_ = [("w", ctypes.c_int), ("v", ctypes.c_int), ("waves", ctypes.c_int), ("chunk", ctypes.c_int)]
+
+
+_lib = None
+
+c_int, c_size_t, c_void_p = ctypes.c_int, ctypes.c_size_t, ctypes.c_void_p
+
+
+def load():
+    """Load libmeshprop.so once; raise MeshpropUnavailable if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise MeshpropUnavailable(
+            f"{LIB_PATH} is missing: build it with `make -C meshes_amd/csrc` (or __graft_entry__.build()).")
+    lib = ctypes.CDLL(LIB_PATH)
+    lib.meshprop_workspace_size.argtypes = [c_int, c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(Tile)]
+    lib.meshprop_workspace_size.restype = c_size_t
+    lib.meshprop_run.argtypes = ([c_int] * 6 + [c_void_p] * 3 + [c_void_p, c_int, c_void_p, c_int]
+                                 + [c_void_p] * 4 + [c_int, c_int] + [c_void_p, c_void_p, c_int]
+                                 + [ctypes.POINTER(Tile), c_int, c_void_p, c_size_t, c_void_p])
+    lib.meshprop_run.restype = c_int
+    lib.meshprop_resolve_tile.argtypes = [c_int, c_int, c_int, c_int, ctypes.POINTER(Tile)]
+    lib.meshprop_resolve_tile.restype = c_int
+    lib.meshprop_last_hip_error.restype = c_int
+    lib.meshprop_status_string.restype = ctypes.c_char_p
+    lib.meshprop_status_string.argtypes = [c_int]
+    lib.meshprop_version.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def check(rc: int, what: str = "meshprop"):
+    if rc != 0:
+        lib = load()
+        msg = lib.meshprop_status_string(rc).decode()
+        extra = f" (hipError {lib.meshprop_last_hip_error()})" if rc == 4 else ""
+        raise MeshpropError(f"{what}: {msg}{extra}")
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def tile_struct(tile):
+    if tile is None:
+        return None
+    if isinstance(tile, Tile):
+        return ctypes.byref(tile)
+    w, v, waves, chunk = (list(tile) + [0, 0, 0, 0])[:4]
+    return ctypes.byref(Tile(int(w), int(v), int(waves), int(chunk)))
+
+
+def workspace_size(op, X, N, L, B, fft=False, tile=None) -> int:
+    lib = load()
+    tp = None
+    if tile is not None:
+        w, v, waves, chunk = (list(tile) + [0, 0, 0, 0])[:4]
+        tp = ctypes.pointer(Tile(int(w), int(v), int(waves), int(chunk)))
+    return int(lib.meshprop_workspace_size(op, X_CODE[X] if isinstance(X, str) else X, N, L, B, int(fft), tp))
+
+
+def resolve_tile(op, X, N, B, tile=None):
+    lib = load()
+    w, v, waves, chunk = (list(tile or []) + [0, 0, 0, 0])[:4]
+    t = Tile(int(w), int(v), int(waves), int(chunk))
+    check(lib.meshprop_resolve_tile(op, X_CODE[X] if isinstance(X, str) else X, N, B, ctypes.byref(t)),
+          "resolve_tile")
+    return (t.w, t.v, t.waves, t.chunk)
+
+
+ONLY_PROLOGUE, SKIP_PROLOGUE, SKIP_REDUCE, ONLY_REDUCE = 1, 2, 4, 8
+
+
+def run(op, X, N, L, B, fft, g0, g1, p, dp, ldp, s, lds, u_a, du_a, u_b, du_b, ldu, target=None, ckpt=None,
+        ckpt_every=0, tile=None, workspace=None, ws_bytes=0, stream=None, flags=0):
+    """Thin call of meshprop_run on torch tensors (device pointers) and the current stream."""
+    lib = load()
+    if stream is None:
+        stream = torch.cuda.current_stream().cuda_stream
+    rc = lib.meshprop_run(op, X_CODE[X] if isinstance(X, str) else X, N, L, B, int(fft), _ptr(g0), _ptr(g1),
+                          _ptr(p), _ptr(dp), ldp, _ptr(s), lds, _ptr(u_a), _ptr(du_a), _ptr(u_b), _ptr(du_b), ldu,
+                          0, _ptr(target), _ptr(ckpt), int(ckpt_every), tile_struct(tile), int(flags), _ptr(workspace),
+                          ctypes.c_size_t(ws_bytes), ctypes.c_void_p(stream))
+    check(rc, ["fwdprop", "fwddiff", "backdiff"][op])

@@ -1,0 +1,478 @@
+// kernels.h -- device kernels of libmeshprop (prologue, propagation, dp reduction) and the
+// launch plan shared by the host translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "crossing.h"
+#include "meshprop.h"
+#include "propagate.h"
+
+namespace mp {
+
+// =======================================================================================
+// Prologue: row tables
+// =======================================================================================
+enum { TAB_FWD = 1, TAB_DIFF = 2, TAB_INV = 4, TAB_ADJ = 8, TAB_DTDP = 16 };
+
+// Slot and role of waveguide w in layer l.  Returns whether an active crossing acts on it.
+// NN: gpu/gmem.cu:236-240 (slot m couples 2m + shift%2, +1; active for
+// shift/2 <= m < shift/2 + len).  FFT: gpu/test.py:73-77.
+__device__ inline bool wg_role(const int fft, const int* g0, const int* g1, const int l, const int w, const int N,
+                               int& m, bool& upper) {
+    if (w >= N) return false;
+    if (fft) {
+        const int st = g0[l];
+        if (st < 0 || st > 20) return false;
+        const int S = 1 << st;
+        upper = (w & S) == 0;
+        const int i = upper ? w : w - S;
+        if (i + S >= N) return false;
+        m = (i / (2 * S)) * S + (i % (2 * S));
+        return true;
+    }
+    const int sh = g1[l], ln = g0[l];
+    const int par = sh & 1;
+    if (w < par) return false;
+    const int q = w - par;
+    m = q >> 1;
+    upper = (q & 1) == 0;
+    const int partner = upper ? w + 1 : w - 1;
+    return (m >= (sh >> 1)) && (m < (sh >> 1) + ln) && (partner < N) && (partner >= 0);
+}
+
+__device__ inline int layer_kind(const int fft, const int* g0, const int* g1, const int l, const int NP) {
+    if (fft) {
+        const int st = g0[l];
+        if (st < 0 || st > 20 || (2 << st) > NP) return KIND_IDENTITY;
+        return 2 + st;
+    }
+    return g1[l] & 1;
+}
+
+template <class R>
+__device__ __forceinline__ R make_row(const dc a, const dc b);
+template <>
+__device__ __forceinline__ float4 make_row<float4>(const dc a, const dc b) {
+    return make_float4((float)a.r, (float)a.i, (float)b.r, (float)b.i);
+}
+template <>
+__device__ __forceinline__ float2 make_row<float2>(const dc a, const dc b) {
+    return make_float2((float)a.r, (float)b.r);
+}
+
+// Row (coefficient on self, coefficient on partner) of matrix M for the upper / lower waveguide.
+__device__ __forceinline__ void row_of(const M2& M, const bool upper, dc& a, dc& b) {
+    if (upper) {
+        a = M.a00;
+        b = M.a01;
+    } else {
+        a = M.a11;
+        b = M.a10;
+    }
+}
+
+// One thread per (layer, waveguide) of the padded width NP.
+template <class R>
+__global__ void k_rows(int X, int N, int L, int NP, int fft, const int* __restrict__ g0, const int* __restrict__ g1,
+                       const float* __restrict__ p, int ldp, const float* __restrict__ s, int lds,
+                       const float* __restrict__ dp, int tabs, int* __restrict__ kind, R* __restrict__ tab_a,
+                       R* __restrict__ tab_b) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = blockIdx.y;
+    if (l >= L || w >= NP) return;
+    if (w == 0) kind[l] = layer_kind(fft, g0, g1, l, NP);
+    const size_t o = (size_t)l * NP + w;
+    int m = 0;
+    bool upper = true;
+    const bool act = wg_role(fft, g0, g1, l, w, N, m, upper);
+    const dc one = mk(1, 0), zero = mk(0, 0);
+    if (!act) {
+        if (tabs & (TAB_FWD | TAB_INV)) tab_a[o] = make_row<R>(one, zero);
+        if (tabs & TAB_DIFF) tab_b[o] = make_row<R>(zero, zero);
+        if (tabs & TAB_ADJ) tab_b[o] = make_row<R>(one, zero);
+        return;
+    }
+    const int np = n_phase_of(X), ns = n_split_of(X);
+    const float* pp = p + (size_t)l * ldp + (size_t)m * np;
+    const float* sp = (s && ns) ? s + (size_t)l * lds + (size_t)m * ns : nullptr;
+    M2 T, dT[2];
+    crossing_eval(X, pp, sp, &T, (tabs & TAB_DIFF) ? dT : nullptr);
+    dc a, b;
+    if (tabs & TAB_FWD) {
+        row_of(T, upper, a, b);
+        tab_a[o] = make_row<R>(a, b);
+    }
+    if (tabs & TAB_DIFF) {
+        const float* dpp = dp + (size_t)l * ldp + (size_t)m * np;
+        M2 D = mscale(dT[0], (double)dpp[0]);
+        if (np > 1) D = madd(D, mscale(dT[1], (double)dpp[1]));
+        row_of(D, upper, a, b);
+        tab_b[o] = make_row<R>(a, b);
+    }
+    if (tabs & TAB_INV) {
+        // Unitary crossings: T^-1 = T^H.  Otherwise the adjugate inverse.
+        const M2 Ti = (X == X_GENERIC) ? minv(T) : mherm(T);
+        row_of(Ti, upper, a, b);
+        tab_a[o] = make_row<R>(a, b);
+    }
+    if (tabs & TAB_ADJ) {
+        row_of(mherm(T), upper, a, b);
+        tab_b[o] = make_row<R>(a, b);
+    }
+}
+
+// dT/dp_k of every slot (zero for inactive slots), for the in-kernel dJdT -> dp step.
+// Layout: dtdp[((l*NPh + m)*np + k)*4 + e] with e = 00, 01, 10, 11, element type
+// float2 (complex) or float (real).
+template <class T>
+__device__ __forceinline__ T dc_to(const dc z);
+template <>
+__device__ __forceinline__ float2 dc_to<float2>(const dc z) {
+    return make_float2((float)z.r, (float)z.i);
+}
+template <>
+__device__ __forceinline__ float dc_to<float>(const dc z) {
+    return (float)z.r;
+}
+
+template <class T>
+__global__ void k_dtdp(int X, int N, int L, int NP, int fft, const int* __restrict__ g0, const int* __restrict__ g1,
+                       const float* __restrict__ p, int ldp, const float* __restrict__ s, int lds,
+                       T* __restrict__ dtdp) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = blockIdx.y;
+    const int NPh = NP / 2;
+    if (l >= L || m >= NPh) return;
+    const int np = n_phase_of(X), ns = n_split_of(X);
+    T* out = dtdp + ((size_t)l * NPh + m) * np * 4;
+    // Upper waveguide of slot m, then test whether the slot is active.
+    int i;
+    if (fft) {
+        const int st = g0[l];
+        bool ok = st >= 0 && st <= 20;
+        const int S = ok ? (1 << st) : 1;
+        i = 2 * S * (m / S) + (m % S);
+        ok = ok && (i + S < N);
+        if (!ok) i = -1;
+    } else {
+        i = 2 * m + (g1[l] & 1);
+    }
+    int mm = -1;
+    bool upper = false;
+    const bool act = (i >= 0) && wg_role(fft, g0, g1, l, i, N, mm, upper) && upper && mm == m;
+    if (!act) {
+        for (int k = 0; k < np * 4; ++k) out[k] = dc_to<T>(mk(0, 0));
+        return;
+    }
+    const float* pp = p + (size_t)l * ldp + (size_t)m * np;
+    const float* sp = (s && ns) ? s + (size_t)l * lds + (size_t)m * ns : nullptr;
+    M2 Tm, dT[2];
+    crossing_eval(X, pp, sp, &Tm, dT);
+    for (int k = 0; k < np; ++k) {
+        out[k * 4 + 0] = dc_to<T>(dT[k].a00);
+        out[k * 4 + 1] = dc_to<T>(dT[k].a01);
+        out[k * 4 + 2] = dc_to<T>(dT[k].a10);
+        out[k * 4 + 3] = dc_to<T>(dT[k].a11);
+    }
+}
+
+// =======================================================================================
+// Propagation kernels
+// =======================================================================================
+struct Lanes {
+    int lane, wave, grp, t;
+    long vbase;
+};
+template <int V>
+__device__ __forceinline__ Lanes lanes_of(const int LPV, const long tile) {
+    Lanes q;
+    q.lane = threadIdx.x & 63;
+    q.wave = threadIdx.x >> 6;
+    const int G = 64 / LPV;
+    q.grp = q.lane / LPV;
+    q.t = q.lane - q.grp * LPV;
+    const int nw = blockDim.x >> 6;
+    q.vbase = ((tile * nw + q.wave) * G + q.grp) * V;
+    return q;
+}
+
+template <class T, int W, int V>
+__device__ __forceinline__ void load_tile(T (&u)[V][W], const T* src, const Lanes& q, const int N, const int B,
+                                          const int ldu) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        const long b = q.vbase + v;
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            const int w = q.t * W + k;
+            u[v][k] = (b < B && w < N) ? src[b * ldu + w] : fzero(T());
+        }
+    }
+}
+
+template <class T, int W, int V>
+__device__ __forceinline__ void store_tile(const T (&u)[V][W], T* dst, const Lanes& q, const int N, const int B,
+                                           const int ldu) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        const long b = q.vbase + v;
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            const int w = q.t * W + k;
+            if (b < B && w < N) dst[b * ldu + w] = u[v][k];
+        }
+    }
+}
+
+template <class R, int W>
+__device__ __forceinline__ void load_rows(R (&r)[W], const R* base) {
+#pragma unroll
+    for (int k = 0; k < W; ++k) r[k] = base[k];
+}
+
+// ---- fwdprop --------------------------------------------------------------------------
+template <class T, int W, int V>
+__global__ void __launch_bounds__(256) k_fwd(int N, int L, int B, int LPV, int NP, const int* __restrict__ kind,
+                                             const typename RowOf<T>::type* __restrict__ rows, const T* u_in,
+                                             T* u_out, int ldu, T* ckpt, int ck_every) {
+    typedef typename RowOf<T>::type R;
+    const Lanes q = lanes_of<V>(LPV, blockIdx.x);
+    T u[V][W];
+    load_tile(u, u_in, q, N, B, ldu);
+    const R* rp = rows + q.t * W;
+    R r[W];
+    if (L > 0) load_rows(r, rp);
+    for (int l = 0; l < L; ++l) {
+        if (ckpt != nullptr && (l % ck_every) == 0) store_tile(u, ckpt + (size_t)(l / ck_every) * B * ldu, q, N, B, ldu);
+        const int kd = kind[l];
+        R rn[W];
+        if (l + 1 < L) load_rows(rn, rp + (size_t)(l + 1) * NP);
+        FwdStep<T, W, V> st{u, r, q.lane};
+        dispatch_kind<W>(kd, st);
+#pragma unroll
+        for (int k = 0; k < W; ++k) r[k] = rn[k];
+    }
+    store_tile(u, u_out, q, N, B, ldu);
+}
+
+// ---- fwddiff --------------------------------------------------------------------------
+template <class T, int W, int V>
+__global__ void __launch_bounds__(256) k_fwddiff(int N, int L, int B, int LPV, int NP, const int* __restrict__ kind,
+                                                 const typename RowOf<T>::type* __restrict__ rows,
+                                                 const typename RowOf<T>::type* __restrict__ drows, const T* u_in,
+                                                 const T* du_in, T* u_out, T* du_out, int ldu) {
+    typedef typename RowOf<T>::type R;
+    const Lanes q = lanes_of<V>(LPV, blockIdx.x);
+    T u[V][W], du[V][W];
+    load_tile(u, u_in, q, N, B, ldu);
+    if (du_in != nullptr) {
+        load_tile(du, du_in, q, N, B, ldu);
+    } else {
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+#pragma unroll
+            for (int k = 0; k < W; ++k) du[v][k] = fzero(T());
+    }
+    const R* rp = rows + q.t * W;
+    const R* dpp = drows + q.t * W;
+    R r[W], d[W];
+    if (L > 0) {
+        load_rows(r, rp);
+        load_rows(d, dpp);
+    }
+    for (int l = 0; l < L; ++l) {
+        const int kd = kind[l];
+        R rn[W], dn[W];
+        if (l + 1 < L) {
+            load_rows(rn, rp + (size_t)(l + 1) * NP);
+            load_rows(dn, dpp + (size_t)(l + 1) * NP);
+        }
+        FwdDiffStep<T, W, V> st{u, du, r, d, q.lane};
+        dispatch_kind<W>(kd, st);
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            r[k] = rn[k];
+            d[k] = dn[k];
+        }
+    }
+    store_tile(u, u_out, q, N, B, ldu);
+    if (du_out != nullptr) store_tile(du, du_out, q, N, B, ldu);
+}
+
+// ---- backdiff -------------------------------------------------------------------------
+// Workgroup = nw waves.  Per layer every lane accumulates its dJdT rows over its V
+// vectors in registers; lane groups are folded with cross-lane adds; the wave writes the
+// rows to an LDS stage of C layers.  At the end of each stage the workgroup sums the
+// waves in a fixed order, converts dJdT to dp with the dT/dp table and writes (or, from
+// its second vector tile on, adds) a per-workgroup partial.  k_dp_reduce then sums the
+// partials in workgroup order: the gradient is bitwise reproducible (the reference uses
+// shared and global float atomics, gpu/crossing.cu:179-180, 295-296).
+template <class T>
+__device__ __forceinline__ float re_dot(const T a, const T b);
+template <>
+__device__ __forceinline__ float re_dot<float2>(const float2 a, const float2 b) {
+    return fmaf(a.x, b.x, -a.y * b.y);
+}
+template <>
+__device__ __forceinline__ float re_dot<float>(const float a, const float b) {
+    return a * b;
+}
+
+template <class T, int W, int V, bool UNI>
+__global__ void __launch_bounds__(256) k_bwd(int N, int L, int B, int LPV, int NP, int C, int np,
+                                             const int* __restrict__ kind,
+                                             const typename RowOf<T>::type* __restrict__ rows_inv,
+                                             const typename RowOf<T>::type* __restrict__ rows_adj,
+                                             const T* __restrict__ dtdp, float* __restrict__ partial, long ntiles,
+                                             const T* u_out, const T* g_out, const T* target, T* u_in, T* g_in,
+                                             int ldu, const T* ckpt, int ck_every) {
+    typedef typename RowOf<T>::type R;
+    extern __shared__ unsigned char smem_raw[];
+    R* stage = reinterpret_cast<R*>(smem_raw);  // [nw][C][W][LPV]
+    const int nw = blockDim.x >> 6;
+    const int NPh = NP / 2;
+    const int G = 64 / LPV;
+    float* my_partial = partial + (size_t)blockIdx.x * L * NPh * np;
+    bool first = true;
+    for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const Lanes q = lanes_of<V>(LPV, tile);
+        T u[V][W], g[V][W];
+        load_tile(u, u_out, q, N, B, ldu);
+        if (target != nullptr) {
+            T y[V][W];
+            load_tile(y, target, q, N, B, ldu);
+#pragma unroll
+            for (int v = 0; v < V; ++v)
+#pragma unroll
+                for (int k = 0; k < W; ++k) g[v][k] = fsub2(u[v][k], y[v][k]);
+        } else {
+            load_tile(g, g_out, q, N, B, ldu);
+        }
+        const R* rip = rows_inv + q.t * W;
+        const R* rap = rows_adj + q.t * W;
+        R ri[W], ra[W];
+        if (L > 0) {
+            load_rows(ri, rip + (size_t)(L - 1) * NP);
+            if constexpr (!UNI) load_rows(ra, rap + (size_t)(L - 1) * NP);
+        }
+        for (int c0 = L - 1; c0 >= 0; c0 -= C) {
+            const int cl = (C < c0 + 1) ? C : c0 + 1;
+            for (int lc = 0; lc < cl; ++lc) {
+                const int l = c0 - lc;
+                const int kd = kind[l];
+                R rin[W], ran[W];
+                if (l > 0) {
+                    load_rows(rin, rip + (size_t)(l - 1) * NP);
+                    if constexpr (!UNI) load_rows(ran, rap + (size_t)(l - 1) * NP);
+                }
+                T accA[W], accB[W];
+#pragma unroll
+                for (int k = 0; k < W; ++k) {
+                    accA[k] = fzero(T());
+                    accB[k] = fzero(T());
+                }
+                if constexpr (UNI) {
+                    BwdStep<T, W, V> st{u, g, ri, ri, accA, accB, q.lane};
+                    dispatch_kind<W>(kd, st);
+                } else {
+                    BwdStep<T, W, V> st{u, g, ri, ra, accA, accB, q.lane};
+                    dispatch_kind<W>(kd, st);
+                }
+                // Fold lane groups (groups hold the same waveguides of different vectors).
+                for (int off = LPV; off < 64; off <<= 1) {
+#pragma unroll
+                    for (int k = 0; k < W; ++k) {
+                        accA[k] = fadd(accA[k], lane_read(accA[k], q.lane ^ off));
+                        accB[k] = fadd(accB[k], lane_read(accB[k], q.lane ^ off));
+                    }
+                }
+                if (q.grp == 0) {
+#pragma unroll
+                    for (int k = 0; k < W; ++k)
+                        stage[(((size_t)q.wave * C + lc) * W + k) * LPV + q.t] = pack_row(accA[k], accB[k]);
+                }
+                if (ckpt != nullptr && (l % ck_every) == 0)
+                    load_tile(u, ckpt + (size_t)(l / ck_every) * B * ldu, q, N, B, ldu);
+#pragma unroll
+                for (int k = 0; k < W; ++k) {
+                    ri[k] = rin[k];
+                    if constexpr (!UNI) ra[k] = ran[k];
+                }
+            }
+            __syncthreads();
+            // dJdT -> dp for the cl layers of this stage.
+            for (int idx = threadIdx.x; idx < cl * NPh; idx += blockDim.x) {
+                const int lc = idx / NPh, m = idx - lc * NPh;
+                const int l = c0 - lc;
+                const int kd = kind[l];
+                int i = -1;
+                if (kd == 0)
+                    i = 2 * m;
+                else if (kd == 1)
+                    i = 2 * m + 1;
+                else if (kd != KIND_IDENTITY) {
+                    const int S = 1 << (kd - 2);
+                    i = 2 * S * (m / S) + (m % S);
+                }
+                const int j = (kd >= 2) ? i + (1 << (kd - 2)) : i + 1;
+                float* outp = my_partial + ((size_t)l * NPh + m) * np;
+                if (i < 0 || j >= NP) {
+                    for (int k = 0; k < np; ++k)
+                        if (first) outp[k] = 0.f;
+                    continue;
+                }
+                R si = stage[(((size_t)0 * C + lc) * W + (i % W)) * LPV + i / W];
+                R sj = stage[(((size_t)0 * C + lc) * W + (j % W)) * LPV + j / W];
+                for (int wv = 1; wv < nw; ++wv) {
+                    si = row_add(si, stage[(((size_t)wv * C + lc) * W + (i % W)) * LPV + i / W]);
+                    sj = row_add(sj, stage[(((size_t)wv * C + lc) * W + (j % W)) * LPV + j / W]);
+                }
+                // dJdT00 = A_i, dJdT01 = B_i, dJdT11 = A_j, dJdT10 = B_j
+                const T* dd = dtdp + ((size_t)l * NPh + m) * np * 4;
+                for (int k = 0; k < np; ++k) {
+                    float val = re_dot(row_a(si), dd[k * 4 + 0]);
+                    val += re_dot(row_b(si), dd[k * 4 + 1]);
+                    val += re_dot(row_b(sj), dd[k * 4 + 2]);
+                    val += re_dot(row_a(sj), dd[k * 4 + 3]);
+                    outp[k] = first ? val : outp[k] + val;
+                }
+            }
+            __syncthreads();
+        }
+        store_tile(u, u_in, q, N, B, ldu);
+        if (g_in != nullptr) store_tile(g, g_in, q, N, B, ldu);
+        first = false;
+    }
+    (void)G;
+}
+
+// =======================================================================================
+// Launch plan (host)
+// =======================================================================================
+struct Plan {
+    int op, X, N, L, B, fft;
+    int W, V, LPV, NP, waves, C;
+    int np, ns, real;
+    long ntiles;
+    int grid;         // workgroups of the propagation kernel
+    size_t rowbytes;  // sizeof(row) = 16 (complex) or 8 (real)
+    size_t off_kind, off_a, off_b, off_dtdp, off_partial, total;  // workspace layout
+    int uni;          // crossings are unitary: T^-1 = T^H
+};
+
+// Tiles compiled into the library.  fwdprop keeps u[V][W]; fwddiff and backdiff keep two
+// field tiles plus row tables, so their W*V is capped lower.
+__host__ __device__ constexpr bool fwd_tile_ok(int W, int V) { return W * V <= 32 && V <= 8; }
+__host__ __device__ constexpr bool diff_tile_ok(int W, int V) { return W * V <= 16 && V <= 4; }
+
+int launch_fwd_any(const Plan& P, const int* kind, const void* rows, const void* u_in, void* u_out, int ldu,
+                   void* ckpt, int ck, hipStream_t st);
+int launch_fwddiff_any(const Plan& P, const int* kind, const void* rows, const void* drows, const void* u_in,
+                       const void* du_in, void* u_out, void* du_out, int ldu, hipStream_t st);
+int launch_bwd_any(const Plan& P, const int* kind, const void* ri, const void* ra, const void* dtdp, float* partial,
+                   const void* u_out, const void* g_out, const void* target, void* u_in, void* g_in, int ldu,
+                   const void* ckpt, int ck, hipStream_t st);
+
+}  // namespace mp
+

@@ -1,0 +1,334 @@
+// meshprop.hip -- libmeshprop.so: MI355X (gfx950) batched propagation through meshes of
+// 2x2 crossings, and its C ABI (include/meshprop.h).
+//
+// One call = (1) a prologue kernel that turns slot-layout parameters and the layer
+// geometry into per-waveguide row tables (FP64 maths, FP32 storage) plus one kind code
+// per layer, (2) one propagation kernel that keeps each field tile in VGPRs across all L
+// layers, and (3) for backdiff, a fixed-order reduction of per-workgroup dp partials.
+// Kernels never allocate: everything lives in the caller's workspace.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "crossing.h"
+#include "meshprop.h"
+#include "propagate.h"
+
+#include "kernels.h"
+
+namespace mp {
+
+// dp[l*ldp + m*np + k] += sum_wg partial[wg][l][m][k], workgroups in fixed order.
+__global__ void k_dp_reduce(int L, int Nh, int NPh, int np, int nwg, const float* __restrict__ partial,
+                            float* __restrict__ dp, int ldp) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long total = (long)L * Nh * np;
+    if (idx >= total) return;
+    const int k = idx % np;
+    const long lm = idx / np;
+    const int m = lm % Nh;
+    const int l = lm / Nh;
+    const size_t stride = (size_t)L * NPh * np;
+    const float* src = partial + ((size_t)l * NPh + m) * np + k;
+    float acc = 0.f;
+    for (int wg = 0; wg < nwg; ++wg) acc += src[(size_t)wg * stride];
+    dp[(size_t)l * ldp + (size_t)m * np + k] += acc;
+}
+
+
+}  // namespace mp
+
+// =======================================================================================
+// Host side
+// =======================================================================================
+using namespace mp;
+
+namespace {
+
+thread_local int g_last_hip_error = 0;
+
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int num_cus() {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) return 256;
+    return n;
+}
+
+int pow2ceil(int x) {
+    int p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+bool valid_w(int w) { return w == 2 || w == 4 || w == 8 || w == 16; }
+
+bool tile_ok(int op, int W, int V) {
+    return op == MESHPROP_OP_FWDPROP ? fwd_tile_ok(W, V) : diff_tile_ok(W, V);
+}
+
+// Default tile: narrowest W that keeps a vector inside one wave; the largest compiled V
+// that still gives at least as many waves as SIMDs (4 per CU).
+void resolve_tile(int op, int N, int B, const meshprop_tile* t, int ncu, int& W, int& V, int& LPV, int& waves,
+                  int& C) {
+    W = N <= 128 ? 2 : N <= 256 ? 4 : N <= 512 ? 8 : 16;
+    if (t && valid_w(t->w) && (long)t->w * 64 >= N) W = t->w;
+    LPV = pow2ceil((N + W - 1) / W);
+    const int G = 64 / LPV;
+    waves = (t && t->waves >= 1 && t->waves <= 4) ? t->waves : 4;
+    if (t && t->v >= 1 && tile_ok(op, W, t->v) && (t->v & (t->v - 1)) == 0) {
+        V = t->v;
+    } else {
+        V = 1;
+        const long target = 4L * ncu;
+        for (int v = 8; v >= 1; v >>= 1) {
+            if (!tile_ok(op, W, v)) continue;
+            const long wv = ((long)B + (long)G * v - 1) / ((long)G * v);
+            if (wv >= target) {
+                V = v;
+                break;
+            }
+        }
+    }
+    C = (t && t->chunk >= 1) ? t->chunk : 0;
+}
+
+int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* tile, Plan& P) {
+    if (X < 0 || X > 3) return MESHPROP_ERR_ARG;
+    if (op < 0 || op > 2) return MESHPROP_ERR_ARG;
+    if (N <= 0 || (N & 1) || L < 0 || B < 0) return MESHPROP_ERR_ARG;
+    if (N > 1024) return MESHPROP_ERR_UNSUPPORTED;
+    P.op = op;
+    P.X = X;
+    P.N = N;
+    P.L = L;
+    P.B = B;
+    P.fft = fft;
+    P.np = n_phase_of(X);
+    P.ns = n_split_of(X);
+    P.real = (X == X_ORTH);
+    P.uni = (X != X_GENERIC);
+    const int ncu = num_cus();
+    resolve_tile(op, N, B, tile, ncu, P.W, P.V, P.LPV, P.waves, P.C);
+    P.NP = P.LPV * P.W;
+    P.rowbytes = P.real ? sizeof(float2) : sizeof(float4);
+    const int G = 64 / P.LPV;
+    const long per_tile = (long)P.waves * G * P.V;
+    P.ntiles = ((long)B + per_tile - 1) / per_tile;
+    if (P.ntiles < 1) P.ntiles = 1;
+    P.grid = (int)P.ntiles;
+    if (op == MESHPROP_OP_BACKDIFF) {
+        // Cap the number of dp partials; each workgroup then walks several vector tiles.
+        const long cap = 2L * ncu;
+        if (P.ntiles > cap) P.grid = (int)cap;
+        // LDS stage depth: up to 64 KiB per workgroup.
+        const size_t per_layer = (size_t)P.waves * P.NP * P.rowbytes;
+        int C = (int)((64 * 1024) / per_layer);
+        if (C < 1) C = 1;
+        if (C > 16) C = 16;
+        if (P.C >= 1 && (size_t)P.C * per_layer <= 64 * 1024) C = P.C;
+        P.C = C;
+    } else {
+        P.C = 0;
+    }
+    size_t off = 0;
+    P.off_kind = off;
+    off = align256(off + (size_t)L * sizeof(int));
+    P.off_a = off;
+    off = align256(off + (size_t)L * P.NP * P.rowbytes);
+    P.off_b = off;
+    const bool need_b = (op == MESHPROP_OP_FWDDIFF) || (op == MESHPROP_OP_BACKDIFF && !P.uni);
+    if (need_b) off = align256(off + (size_t)L * P.NP * P.rowbytes);
+    P.off_dtdp = off;
+    P.off_partial = off;
+    if (op == MESHPROP_OP_BACKDIFF) {
+        const size_t elem = P.real ? sizeof(float) : sizeof(float2);
+        off = align256(off + (size_t)L * (P.NP / 2) * P.np * 4 * elem);
+        P.off_partial = off;
+        off = align256(off + (size_t)P.grid * L * (P.NP / 2) * P.np * sizeof(float));
+    }
+    P.total = off;
+    return MESHPROP_OK;
+}
+
+int check_launch() {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        g_last_hip_error = (int)e;
+        return MESHPROP_ERR_HIP;
+    }
+    return MESHPROP_OK;
+}
+
+int run_impl(int op, int X, int N, int L, int B, int fft, const int* g0, const int* g1, const float* p, float* dp,
+             int ldp, const float* s, int lds, const void* u_a, const void* du_a, void* u_b, void* du_b, int ldu,
+             int mode, const void* target, void* ckpt, int ck_every, const meshprop_tile* tile, int flags, void* ws,
+             size_t ws_bytes, hipStream_t st) {
+    if (mode != 0) return MESHPROP_ERR_UNSUPPORTED;  // gpu/crossing.cu:122-125: only mode 0 exists
+    Plan P;
+    int rc = make_plan(op, X, N, L, B, fft, tile, P);
+    if (rc) return rc;
+    if (B == 0) return MESHPROP_OK;
+    if (ldu < N) return MESHPROP_ERR_ARG;
+    if (!u_a || !u_b) return MESHPROP_ERR_ARG;
+    if (L > 0 && (!g0 || (!fft && !g1) || !p)) return MESHPROP_ERR_ARG;
+    if (ldp < (N / 2) * P.np) return MESHPROP_ERR_ARG;
+    if (s && P.ns && lds < (N / 2) * P.ns) return MESHPROP_ERR_ARG;
+    if (op == MESHPROP_OP_FWDDIFF && L > 0 && !dp) return MESHPROP_ERR_ARG;
+    if (op == MESHPROP_OP_BACKDIFF && !dp) return MESHPROP_ERR_ARG;
+    if (op == MESHPROP_OP_BACKDIFF && !du_a && !target) return MESHPROP_ERR_ARG;
+    if (ckpt && ck_every < 1) return MESHPROP_ERR_ARG;
+    if (!ckpt) ck_every = 1;
+    if (P.total > 0 && (!ws || ws_bytes < P.total)) return MESHPROP_ERR_WORKSPACE;
+    if (((uintptr_t)ws) & 255) return MESHPROP_ERR_WORKSPACE;
+
+    unsigned char* base = (unsigned char*)ws;
+    int* kind = (int*)(base + P.off_kind);
+    void* tab_a = base + P.off_a;
+    void* tab_b = base + P.off_b;
+    void* dtdp = base + P.off_dtdp;
+    float* partial = (float*)(base + P.off_partial);
+    const float* sp = (P.ns > 0) ? s : nullptr;
+
+    const bool do_pro = !(flags & (MESHPROP_SKIP_PROLOGUE | MESHPROP_ONLY_REDUCE));
+    const bool do_main = !(flags & (MESHPROP_ONLY_PROLOGUE | MESHPROP_ONLY_REDUCE));
+    const bool do_red = !(flags & (MESHPROP_ONLY_PROLOGUE | MESHPROP_SKIP_REDUCE));
+    if (L > 0 && do_pro) {
+        int tabs = 0;
+        if (op == MESHPROP_OP_FWDPROP) tabs = TAB_FWD;
+        if (op == MESHPROP_OP_FWDDIFF) tabs = TAB_FWD | TAB_DIFF;
+        if (op == MESHPROP_OP_BACKDIFF) tabs = TAB_INV | (P.uni ? 0 : TAB_ADJ);
+        dim3 grid((P.NP + 255) / 256, L);
+        if (P.real)
+            hipLaunchKernelGGL((k_rows<float2>), grid, dim3(256), 0, st, X, N, L, P.NP, fft, g0, g1, p, ldp, sp, lds,
+                               (const float*)dp, tabs, kind, (float2*)tab_a, (float2*)tab_b);
+        else
+            hipLaunchKernelGGL((k_rows<float4>), grid, dim3(256), 0, st, X, N, L, P.NP, fft, g0, g1, p, ldp, sp, lds,
+                               (const float*)dp, tabs, kind, (float4*)tab_a, (float4*)tab_b);
+        if ((rc = check_launch())) return rc;
+        if (op == MESHPROP_OP_BACKDIFF) {
+            dim3 g2((P.NP / 2 + 255) / 256, L);
+            if (P.real)
+                hipLaunchKernelGGL((k_dtdp<float>), g2, dim3(256), 0, st, X, N, L, P.NP, fft, g0, g1, p, ldp, sp, lds,
+                                   (float*)dtdp);
+            else
+                hipLaunchKernelGGL((k_dtdp<float2>), g2, dim3(256), 0, st, X, N, L, P.NP, fft, g0, g1, p, ldp, sp,
+                                   lds, (float2*)dtdp);
+            if ((rc = check_launch())) return rc;
+        }
+    }
+
+    const void* rows_b = P.uni ? tab_a : tab_b;
+    if (!do_main) {
+        rc = MESHPROP_OK;
+    } else if (op == MESHPROP_OP_FWDPROP)
+        rc = launch_fwd_any(P, kind, tab_a, u_a, u_b, ldu, ckpt, ck_every, st);
+    else if (op == MESHPROP_OP_FWDDIFF)
+        rc = launch_fwddiff_any(P, kind, tab_a, tab_b, u_a, du_a, u_b, du_b, ldu, st);
+    else
+        rc = launch_bwd_any(P, kind, tab_a, rows_b, dtdp, partial, u_a, du_a, target, u_b, du_b, ldu, ckpt,
+                            ck_every, st);
+    if (rc) return rc;
+    if ((rc = check_launch())) return rc;
+    if (op == MESHPROP_OP_BACKDIFF && L > 0 && do_red) {
+        const long total = (long)L * (N / 2) * P.np;
+        hipLaunchKernelGGL(k_dp_reduce, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, L, N / 2, P.NP / 2,
+                           P.np, P.grid, (const float*)partial, dp, ldp);
+        if ((rc = check_launch())) return rc;
+    }
+    return MESHPROP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t meshprop_workspace_size(int op, int X, int N, int L, int B, int fft, const meshprop_tile* tile) {
+    Plan P;
+    if (make_plan(op, X, N, L, B, fft, tile, P) != MESHPROP_OK) return 0;
+    return P.total;
+}
+
+int meshprop_resolve_tile(int op, int X, int N, int B, meshprop_tile* out) {
+    Plan P;
+    int rc = make_plan(op, X, N, 1, B, 0, out, P);
+    if (rc) return rc;
+    out->w = P.W;
+    out->v = P.V;
+    out->waves = P.waves;
+    out->chunk = P.C;
+    return MESHPROP_OK;
+}
+
+int meshprop_run(int op, int X, int N, int L, int B, int fft, const int* geom0, const int* geom1, const float* p,
+                 float* dp, int ldp, const float* s, int lds, const void* u_a, const void* du_a, void* u_b, void* du_b,
+                 int ldu, int mode, const void* target, void* ckpt, int ckpt_every, const meshprop_tile* tile,
+                 int flags, void* workspace, size_t workspace_bytes, void* stream) {
+    return run_impl(op, X, N, L, B, fft, geom0, geom1, p, dp, ldp, s, lds, u_a, du_a, u_b, du_b, ldu, mode, target,
+                    ckpt, ckpt_every, tile, flags, workspace, workspace_bytes, (hipStream_t)stream);
+}
+
+int meshprop_fwdprop(int X, int N, int L, int B, const int* lens, const int* shifts, const float* p, int ldp,
+                     const float* s, int lds, const void* u_in, void* u_out, int ldu, int mode, void* workspace,
+                     size_t workspace_bytes, void* stream) {
+    return run_impl(MESHPROP_OP_FWDPROP, X, N, L, B, 0, lens, shifts, p, nullptr, ldp, s, lds, u_in, nullptr, u_out,
+                    nullptr, ldu, mode, nullptr, nullptr, 0, nullptr, 0, workspace, workspace_bytes,
+                    (hipStream_t)stream);
+}
+
+int meshprop_fwdprop_ft(int X, int N, int L, int B, const int* strides, const float* p, int ldp, const float* s,
+                        int lds, const void* u_in, void* u_out, int ldu, int mode, void* workspace,
+                        size_t workspace_bytes, void* stream) {
+    return run_impl(MESHPROP_OP_FWDPROP, X, N, L, B, 1, strides, nullptr, p, nullptr, ldp, s, lds, u_in, nullptr,
+                    u_out, nullptr, ldu, mode, nullptr, nullptr, 0, nullptr, 0, workspace, workspace_bytes,
+                    (hipStream_t)stream);
+}
+
+int meshprop_fwddiff(int X, int N, int L, int B, const int* lens, const int* shifts, const float* p, const float* dp,
+                     int ldp, const float* s, int lds, const void* u_in, const void* du_in, void* u_out, void* du_out,
+                     int ldu, int mode, void* workspace, size_t workspace_bytes, void* stream) {
+    return run_impl(MESHPROP_OP_FWDDIFF, X, N, L, B, 0, lens, shifts, p, (float*)dp, ldp, s, lds, u_in, du_in, u_out,
+                    du_out, ldu, mode, nullptr, nullptr, 0, nullptr, 0, workspace, workspace_bytes, (hipStream_t)stream);
+}
+
+int meshprop_fwddiff_ft(int X, int N, int L, int B, const int* strides, const float* p, const float* dp, int ldp,
+                        const float* s, int lds, const void* u_in, const void* du_in, void* u_out, void* du_out,
+                        int ldu, int mode, void* workspace, size_t workspace_bytes, void* stream) {
+    return run_impl(MESHPROP_OP_FWDDIFF, X, N, L, B, 1, strides, nullptr, p, (float*)dp, ldp, s, lds, u_in, du_in,
+                    u_out, du_out, ldu, mode, nullptr, nullptr, 0, nullptr, 0, workspace, workspace_bytes,
+                    (hipStream_t)stream);
+}
+
+int meshprop_backdiff(int X, int N, int L, int B, const int* lens, const int* shifts, const float* p, float* dp,
+                      int ldp, const float* s, int lds, const void* u_out, const void* dJdu_out, void* u_in,
+                      void* dJdu_in, int ldu, int mode, void* workspace, size_t workspace_bytes, void* stream) {
+    return run_impl(MESHPROP_OP_BACKDIFF, X, N, L, B, 0, lens, shifts, p, dp, ldp, s, lds, u_out, dJdu_out, u_in,
+                    dJdu_in, ldu, mode, nullptr, nullptr, 0, nullptr, 0, workspace, workspace_bytes,
+                    (hipStream_t)stream);
+}
+
+int meshprop_backdiff_ft(int X, int N, int L, int B, const int* strides, const float* p, float* dp, int ldp,
+                         const float* s, int lds, const void* u_out, const void* dJdu_out, void* u_in, void* dJdu_in,
+                         int ldu, int mode, void* workspace, size_t workspace_bytes, void* stream) {
+    return run_impl(MESHPROP_OP_BACKDIFF, X, N, L, B, 1, strides, nullptr, p, dp, ldp, s, lds, u_out, dJdu_out, u_in,
+                    dJdu_in, ldu, mode, nullptr, nullptr, 0, nullptr, 0, workspace, workspace_bytes,
+                    (hipStream_t)stream);
+}
+
+int meshprop_last_hip_error(void) { return g_last_hip_error; }
+
+const char* meshprop_status_string(int status) {
+    switch (status) {
+        case MESHPROP_OK: return "ok";
+        case MESHPROP_ERR_ARG: return "invalid argument";
+        case MESHPROP_ERR_WORKSPACE: return "workspace missing, too small or not 256-byte aligned";
+        case MESHPROP_ERR_UNSUPPORTED: return "unsupported configuration";
+        case MESHPROP_ERR_HIP: return "HIP launch error";
+        default: return "unknown status";
+    }
+}
+
+const char* meshprop_version(void) { return "meshprop-gfx950 0.1"; }
+
+}  // extern "C"

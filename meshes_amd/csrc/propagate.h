@@ -1,0 +1,269 @@
+// propagate.h -- register-resident layer kernels for gfx950 (wave64).
+//
+// Lane mapping.  A wave64 is split into G = 64/LPV lane groups; each group owns V field
+// vectors and each of its LPV lanes holds W consecutive waveguides of every one of those
+// vectors in VGPRs for the whole depth of the mesh ("field tile").  A layer is applied
+// through per-waveguide *rows*: waveguide w of layer l becomes
+//     u'_w = a_w u_w + b_w u_partner(w)
+// where (a_w, b_w) is row w of the 2x2 crossing that acts on w (identity a=1, b=0 for
+// waveguides no active crossing touches).  Rows are built once per call in FP64 by the
+// prologue and streamed per layer from L2 (W rows per lane, coalesced), and every row load
+// is reused for the V vectors the lane holds.
+//
+// Partners by layer kind (computed by the prologue, one code per layer):
+//   0  NN, even shift : w ^ 1                      (in-lane)
+//   1  NN, odd shift  : w+1 for odd w, w-1 for even w; the two ends of each lane's block
+//                       pair with the neighbouring lanes (ds_bpermute)
+//   2+st FFT, stride S = 2^st : w ^ S  (in-lane for S < W, lane ^ (S/W) otherwise)
+//   255 identity layer (invalid stride): skipped
+// These are the NN and FFT geometries of gpu/fwdprop.cu:48-112 (slot pairing of
+// gpu/gmem.cu:236-240), restated for a row-per-waveguide layout.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace mp {
+
+enum { KIND_IDENTITY = 255 };
+
+// ---------------------------------------------------------------------------------------
+// Scalar algebra: complex fields are float2 with rows float4 = (a.re, a.im, b.re, b.im);
+// real fields are float with rows float2 = (a, b).  Every output costs 4 (complex) or 1
+// (real) FMA-class instructions per term, i.e. 8 / 2 per waveguide and 16 / 4 per crossing.
+// ---------------------------------------------------------------------------------------
+template <class T>
+struct RowOf;
+template <>
+struct RowOf<float2> {
+    typedef float4 type;
+};
+template <>
+struct RowOf<float> {
+    typedef float2 type;
+};
+
+__device__ __forceinline__ float2 fzero(float2) { return make_float2(0.f, 0.f); }
+__device__ __forceinline__ float fzero(float) { return 0.f; }
+
+// a*x + b*y
+__device__ __forceinline__ float2 rapply(const float4 r, const float2 x, const float2 y) {
+    float2 o;
+    o.x = fmaf(-r.w, y.y, fmaf(r.z, y.x, fmaf(-r.y, x.y, r.x * x.x)));
+    o.y = fmaf(r.w, y.x, fmaf(r.z, y.y, fmaf(r.y, x.x, r.x * x.y)));
+    return o;
+}
+__device__ __forceinline__ float rapply(const float2 r, const float x, const float y) {
+    return fmaf(r.y, y, r.x * x);
+}
+// a*dx + b*dy + da*x + db*y
+__device__ __forceinline__ float2 rapply_d(const float4 r, const float4 d, const float2 x, const float2 y,
+                                           const float2 dx, const float2 dy) {
+    float2 o;
+    o.x = r.x * dx.x;
+    o.x = fmaf(-r.y, dx.y, o.x);
+    o.x = fmaf(r.z, dy.x, o.x);
+    o.x = fmaf(-r.w, dy.y, o.x);
+    o.x = fmaf(d.x, x.x, o.x);
+    o.x = fmaf(-d.y, x.y, o.x);
+    o.x = fmaf(d.z, y.x, o.x);
+    o.x = fmaf(-d.w, y.y, o.x);
+    o.y = r.x * dx.y;
+    o.y = fmaf(r.y, dx.x, o.y);
+    o.y = fmaf(r.z, dy.y, o.y);
+    o.y = fmaf(r.w, dy.x, o.y);
+    o.y = fmaf(d.x, x.y, o.y);
+    o.y = fmaf(d.y, x.x, o.y);
+    o.y = fmaf(d.z, y.y, o.y);
+    o.y = fmaf(d.w, y.x, o.y);
+    return o;
+}
+__device__ __forceinline__ float rapply_d(const float2 r, const float2 d, const float x, const float y,
+                                          const float dx, const float dy) {
+    return fmaf(d.y, y, fmaf(d.x, x, fmaf(r.y, dy, r.x * dx)));
+}
+// acc + conj(g) * x
+__device__ __forceinline__ float2 cjfma(const float2 g, const float2 x, float2 acc) {
+    acc.x = fmaf(g.y, x.y, fmaf(g.x, x.x, acc.x));
+    acc.y = fmaf(-g.y, x.x, fmaf(g.x, x.y, acc.y));
+    return acc;
+}
+__device__ __forceinline__ float cjfma(const float g, const float x, float acc) { return fmaf(g, x, acc); }
+
+__device__ __forceinline__ float4 pack_row(const float2 a, const float2 b) { return make_float4(a.x, a.y, b.x, b.y); }
+__device__ __forceinline__ float2 pack_row(const float a, const float b) { return make_float2(a, b); }
+__device__ __forceinline__ float2 row_a(const float4 r) { return make_float2(r.x, r.y); }
+__device__ __forceinline__ float2 row_b(const float4 r) { return make_float2(r.z, r.w); }
+__device__ __forceinline__ float row_a(const float2 r) { return r.x; }
+__device__ __forceinline__ float row_b(const float2 r) { return r.y; }
+__device__ __forceinline__ float4 row_add(const float4 a, const float4 b) {
+    return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+}
+__device__ __forceinline__ float2 row_add(const float2 a, const float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 fadd(const float2 a, const float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float fadd(const float a, const float b) { return a + b; }
+__device__ __forceinline__ float2 fsub2(const float2 a, const float2 b) {
+    return make_float2(2.f * (a.x - b.x), 2.f * (a.y - b.y));
+}
+__device__ __forceinline__ float fsub2(const float a, const float b) { return 2.f * (a - b); }
+
+// Cross-lane read through the LDS crossbar (no LDS storage).  src is taken modulo 64.
+__device__ __forceinline__ float lane_read(const float x, const int src) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(x)));
+}
+__device__ __forceinline__ float2 lane_read(const float2 x, const int src) {
+    return make_float2(lane_read(x.x, src), lane_read(x.y, src));
+}
+
+template <int W>
+struct Log2 {
+    static constexpr int value = W == 1 ? 0 : W == 2 ? 1 : W == 4 ? 2 : W == 8 ? 3 : W == 16 ? 4 : -1;
+};
+
+// ---------------------------------------------------------------------------------------
+// Partner gather: pn[v][k] = value of the partner of waveguide k (of this lane) in a.
+// KC: 0 in-lane stride S, 1 misaligned NN, 2 cross-lane xor xm.
+// ---------------------------------------------------------------------------------------
+template <int KC, int S, class T, int W, int V>
+__device__ __forceinline__ void gather(const T (&a)[V][W], T (&pn)[V][W], const int lane, const int xm) {
+    if constexpr (KC == 0) {
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+#pragma unroll
+            for (int k = 0; k < W; ++k) pn[v][k] = a[v][k ^ S];
+    } else if constexpr (KC == 1) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            const T hi = lane_read(a[v][0], lane + 1);
+            const T lo = lane_read(a[v][W - 1], lane - 1);
+#pragma unroll
+            for (int k = 0; k < W; ++k) {
+                if (k == W - 1)
+                    pn[v][k] = hi;
+                else if (k == 0)
+                    pn[v][k] = lo;
+                else
+                    pn[v][k] = a[v][(k & 1) ? k + 1 : k - 1];
+            }
+        }
+    } else {
+        const int src = lane ^ xm;
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+#pragma unroll
+            for (int k = 0; k < W; ++k) pn[v][k] = lane_read(a[v][k], src);
+    }
+}
+
+// Calls op.template run<KC, S>(xm) for layer kind kd.
+template <int W, class Op>
+__device__ __forceinline__ void dispatch_kind(const int kd, Op& op) {
+    if (kd == 0 || kd == 2) {
+        op.template run<0, 1>(0);
+        return;
+    }
+    if (kd == 1) {
+        op.template run<1, 1>(0);
+        return;
+    }
+    if (kd == KIND_IDENTITY) return;
+    const int st = kd - 2;
+    if constexpr (W > 2) {
+        if (st == 1) {
+            op.template run<0, 2>(0);
+            return;
+        }
+    }
+    if constexpr (W > 4) {
+        if (st == 2) {
+            op.template run<0, 4>(0);
+            return;
+        }
+    }
+    if constexpr (W > 8) {
+        if (st == 3) {
+            op.template run<0, 8>(0);
+            return;
+        }
+    }
+    op.template run<2, 1>(1 << (st - Log2<W>::value));
+}
+
+// ---------------------------------------------------------------------------------------
+// Per-layer steps
+// ---------------------------------------------------------------------------------------
+template <class T, int W, int V>
+struct FwdStep {
+    typedef typename RowOf<T>::type R;
+    T (&u)[V][W];
+    const R (&r)[W];
+    int lane;
+    template <int KC, int S>
+    __device__ __forceinline__ void run(const int xm) {
+        T pn[V][W];
+        gather<KC, S>(u, pn, lane, xm);
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+#pragma unroll
+            for (int k = 0; k < W; ++k) u[v][k] = rapply(r[k], u[v][k], pn[v][k]);
+    }
+};
+
+template <class T, int W, int V>
+struct FwdDiffStep {
+    typedef typename RowOf<T>::type R;
+    T (&u)[V][W];
+    T (&du)[V][W];
+    const R (&r)[W];
+    const R (&d)[W];
+    int lane;
+    template <int KC, int S>
+    __device__ __forceinline__ void run(const int xm) {
+        T up[V][W], dup[V][W];
+        gather<KC, S>(u, up, lane, xm);
+        gather<KC, S>(du, dup, lane, xm);
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+#pragma unroll
+            for (int k = 0; k < W; ++k) {
+                du[v][k] = rapply_d(r[k], d[k], u[v][k], up[v][k], du[v][k], dup[v][k]);
+                u[v][k] = rapply(r[k], u[v][k], up[v][k]);
+            }
+    }
+};
+
+// Reverse step through one layer: x = Rinv u (reconstructed layer input),
+// dJdT rows: A_k += conj(g_k) x_k, B_k += conj(g_k) x_partner(k), then g <- Radj g.
+template <class T, int W, int V>
+struct BwdStep {
+    typedef typename RowOf<T>::type R;
+    T (&u)[V][W];
+    T (&g)[V][W];
+    const R (&ri)[W];
+    const R (&ra)[W];
+    T (&accA)[W];
+    T (&accB)[W];
+    int lane;
+    template <int KC, int S>
+    __device__ __forceinline__ void run(const int xm) {
+        T up[V][W], gp[V][W], xp[V][W];
+        gather<KC, S>(u, up, lane, xm);
+        gather<KC, S>(g, gp, lane, xm);
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+#pragma unroll
+            for (int k = 0; k < W; ++k) u[v][k] = rapply(ri[k], u[v][k], up[v][k]);
+        gather<KC, S>(u, xp, lane, xm);
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+#pragma unroll
+            for (int k = 0; k < W; ++k) {
+                accA[k] = cjfma(g[v][k], u[v][k], accA[k]);
+                accB[k] = cjfma(g[v][k], xp[v][k], accB[k]);
+            }
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+#pragma unroll
+            for (int k = 0; k < W; ++k) g[v][k] = rapply(ra[k], g[v][k], gp[v][k]);
+    }
+};
+
+}  // namespace mp

@@ -1,0 +1,78 @@
+"""C ABI checks that need no GPU: the library loads, exports every symbol the header declares,
+and its host-side planning (workspace sizes, status strings, argument validation) behaves."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "meshprop.h")
+LIB = os.path.join(ROOT, "meshes_amd", "libmeshprop.so")
+
+
+def _declared():
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(meshprop_[a-z_0-9]+)\s*\(", txt)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(LIB):
+        pytest.fail("libmeshprop.so is not built (run __graft_entry__.build())")
+    import meshes_amd._lib as L
+    return L.load()
+
+
+def test_exports_every_declared_symbol(lib):
+    names = _declared()
+    assert len(names) >= 12
+    for n in names:
+        assert hasattr(lib, n), n
+    import meshes_amd._lib as L
+    assert set(L.EXPORTS) == set(names)
+
+
+def test_status_strings(lib):
+    for rc in range(5):
+        assert lib.meshprop_status_string(rc)
+    assert b"gfx950" in lib.meshprop_version()
+
+
+def test_workspace_sizes(lib):
+    import meshes_amd._lib as L
+    # fwdprop: kind array + one row table (L x NP x 16 B) for complex crossings
+    n = L.workspace_size(L.OP_FWDPROP, "mzi", 256, 256, 4096)
+    assert n >= 256 * 256 * 16
+    # fwddiff needs a second table
+    assert L.workspace_size(L.OP_FWDDIFF, "mzi", 256, 256, 4096) >= 2 * 256 * 256 * 16
+    # backdiff adds the dT/dp table and the dp partials
+    assert L.workspace_size(L.OP_BACKDIFF, "mzi", 256, 256, 4096) > n
+    # orth uses real rows (8 B)
+    assert L.workspace_size(L.OP_FWDPROP, "orth", 256, 256, 4096) < n
+    # invalid sizes give 0
+    assert L.workspace_size(L.OP_FWDPROP, "mzi", 255, 4, 4) == 0
+    assert L.workspace_size(L.OP_FWDPROP, "mzi", 2048, 4, 4) == 0
+
+
+def test_argument_validation_without_gpu(lib):
+    """Bad arguments are rejected on the host before anything is launched."""
+    import meshes_amd._lib as L
+    fn = lib.meshprop_fwdprop
+    fn.argtypes = [ctypes.c_int] * 4 + [ctypes.c_void_p] * 3 + [ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                                                  ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t,
+                                                                  ctypes.c_void_p]
+    fake = ctypes.c_void_p(256)
+    # odd N
+    assert fn(L.MZI, 7, 2, 1, fake, fake, fake, 8, None, 0, fake, fake, 7, 0, fake, 1 << 20, None) == 1
+    # unsupported mode (only mode 0 exists, gpu/crossing.cu:122-125)
+    assert fn(L.MZI, 8, 2, 1, fake, fake, fake, 8, None, 0, fake, fake, 8, 1, fake, 1 << 20, None) == 3
+    # missing workspace
+    assert fn(L.MZI, 8, 2, 1, fake, fake, fake, 8, None, 0, fake, fake, 8, 0, None, 0, None) == 2
+    # ldu < N
+    assert fn(L.MZI, 8, 2, 1, fake, fake, fake, 8, None, 0, fake, fake, 6, 0, fake, 1 << 20, None) == 1
+    # N beyond 1024
+    assert fn(L.MZI, 2048, 2, 1, fake, fake, fake, 2048, None, 0, fake, fake, 2048, 0, fake, 1 << 20, None) == 3
+    # B = 0 is a no-op
+    assert fn(L.MZI, 8, 2, 0, fake, fake, fake, 8, None, 0, fake, fake, 8, 0, fake, 1 << 20, None) == 0
