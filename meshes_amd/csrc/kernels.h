@@ -232,28 +232,48 @@ __device__ __forceinline__ void load_rows(R (&r)[W], const R* base) {
 }
 
 // ---- fwdprop --------------------------------------------------------------------------
+// Layers [l0, l0 + nl) of the row tables.  Rows of two consecutive layers live in two
+// register sets used in turn (ping-pong), so the load for layer l+1 is in flight while
+// layer l is applied and no register copies are needed between iterations.
 template <class T, int W, int V>
-__global__ void __launch_bounds__(256) k_fwd(int N, int L, int B, int LPV, int NP, const int* __restrict__ kind,
+__global__ void __launch_bounds__(256) k_fwd(int N, int l0, int nl, int B, int LPV, int NP,
+                                             const int* __restrict__ kind,
                                              const typename RowOf<T>::type* __restrict__ rows, const T* u_in,
-                                             T* u_out, int ldu, T* ckpt, int ck_every) {
+                                             T* u_out, int ldu) {
     typedef typename RowOf<T>::type R;
     const Lanes q = lanes_of<V>(LPV, blockIdx.x);
-    T u[V][W];
-    load_tile(u, u_in, q, N, B, ldu);
-    const R* rp = rows + q.t * W;
-    R r[W];
-    if (L > 0) load_rows(r, rp);
-    for (int l = 0; l < L; ++l) {
-        if (ckpt != nullptr && (l % ck_every) == 0) store_tile(u, ckpt + (size_t)(l / ck_every) * B * ldu, q, N, B, ldu);
-        const int kd = kind[l];
-        R rn[W];
-        if (l + 1 < L) load_rows(rn, rp + (size_t)(l + 1) * NP);
-        FwdStep<T, W, V> st{u, r, q.lane};
-        dispatch_kind<W>(kd, st);
-#pragma unroll
-        for (int k = 0; k < W; ++k) r[k] = rn[k];
+    T ua[V][W], ub[V][W];
+    load_tile(ua, u_in, q, N, B, ldu);
+    if (nl > 0) {
+        kind += l0;
+        const R* rp = rows + (size_t)l0 * NP + q.t * W;
+        R ra[W], rb[W];
+        int ka = kind[0];
+        load_rows(ra, rp);
+        int l = 0;
+        for (; l + 1 < nl; l += 2) {
+            const int kb = kind[l + 1];
+            load_rows(rb, rp + (size_t)(l + 1) * NP);
+            {
+                FwdStep<T, W, V> st{ua, ub, ra, q.lane};
+                dispatch_kind<W>(ka, st);
+            }
+            const int ln = (l + 2 < nl) ? l + 2 : nl - 1;
+            ka = kind[ln];
+            load_rows(ra, rp + (size_t)ln * NP);
+            {
+                FwdStep<T, W, V> st{ub, ua, rb, q.lane};
+                dispatch_kind<W>(kb, st);
+            }
+        }
+        if (l < nl) {
+            FwdStep<T, W, V> st{ua, ub, ra, q.lane};
+            dispatch_kind<W>(ka, st);
+            store_tile(ub, u_out, q, N, B, ldu);
+            return;
+        }
     }
-    store_tile(u, u_out, q, N, B, ldu);
+    store_tile(ua, u_out, q, N, B, ldu);
 }
 
 // ---- fwddiff --------------------------------------------------------------------------
@@ -264,7 +284,7 @@ __global__ void __launch_bounds__(256) k_fwddiff(int N, int L, int B, int LPV, i
                                                  const T* du_in, T* u_out, T* du_out, int ldu) {
     typedef typename RowOf<T>::type R;
     const Lanes q = lanes_of<V>(LPV, blockIdx.x);
-    T u[V][W], du[V][W];
+    T u[V][W], du[V][W], ub[V][W], dub[V][W];
     load_tile(u, u_in, q, N, B, ldu);
     if (du_in != nullptr) {
         load_tile(du, du_in, q, N, B, ldu);
@@ -274,26 +294,37 @@ __global__ void __launch_bounds__(256) k_fwddiff(int N, int L, int B, int LPV, i
 #pragma unroll
             for (int k = 0; k < W; ++k) du[v][k] = fzero(T());
     }
-    const R* rp = rows + q.t * W;
-    const R* dpp = drows + q.t * W;
-    R r[W], d[W];
     if (L > 0) {
-        load_rows(r, rp);
-        load_rows(d, dpp);
-    }
-    for (int l = 0; l < L; ++l) {
-        const int kd = kind[l];
-        R rn[W], dn[W];
-        if (l + 1 < L) {
-            load_rows(rn, rp + (size_t)(l + 1) * NP);
-            load_rows(dn, dpp + (size_t)(l + 1) * NP);
+        const R* rp = rows + q.t * W;
+        const R* dq = drows + q.t * W;
+        R ra[W], da[W], rb[W], db[W];
+        int ka = kind[0];
+        load_rows(ra, rp);
+        load_rows(da, dq);
+        int l = 0;
+        for (; l + 1 < L; l += 2) {
+            const int kb = kind[l + 1];
+            load_rows(rb, rp + (size_t)(l + 1) * NP);
+            load_rows(db, dq + (size_t)(l + 1) * NP);
+            {
+                FwdDiffStep<T, W, V> st{u, du, ub, dub, ra, da, q.lane};
+                dispatch_kind<W>(ka, st);
+            }
+            const int ln = (l + 2 < L) ? l + 2 : L - 1;
+            ka = kind[ln];
+            load_rows(ra, rp + (size_t)ln * NP);
+            load_rows(da, dq + (size_t)ln * NP);
+            {
+                FwdDiffStep<T, W, V> st{ub, dub, u, du, rb, db, q.lane};
+                dispatch_kind<W>(kb, st);
+            }
         }
-        FwdDiffStep<T, W, V> st{u, du, r, d, q.lane};
-        dispatch_kind<W>(kd, st);
-#pragma unroll
-        for (int k = 0; k < W; ++k) {
-            r[k] = rn[k];
-            d[k] = dn[k];
+        if (l < L) {
+            FwdDiffStep<T, W, V> st{u, du, ub, dub, ra, da, q.lane};
+            dispatch_kind<W>(ka, st);
+            store_tile(ub, u_out, q, N, B, ldu);
+            if (du_out != nullptr) store_tile(dub, du_out, q, N, B, ldu);
+            return;
         }
     }
     store_tile(u, u_out, q, N, B, ldu);
@@ -303,11 +334,13 @@ __global__ void __launch_bounds__(256) k_fwddiff(int N, int L, int B, int LPV, i
 // ---- backdiff -------------------------------------------------------------------------
 // Workgroup = nw waves.  Per layer every lane accumulates its dJdT rows over its V
 // vectors in registers; lane groups are folded with cross-lane adds; the wave writes the
-// rows to an LDS stage of C layers.  At the end of each stage the workgroup sums the
-// waves in a fixed order, converts dJdT to dp with the dT/dp table and writes (or, from
-// its second vector tile on, adds) a per-workgroup partial.  k_dp_reduce then sums the
-// partials in workgroup order: the gradient is bitwise reproducible (the reference uses
-// shared and global float atomics, gpu/crossing.cu:179-180, 295-296).
+// rows to an LDS stage of C layers (C even).  At the end of each stage the workgroup sums
+// the waves in a fixed order, converts dJdT to dp with the dT/dp table and writes (or, from
+// its second vector tile on, adds) a per-workgroup partial.  The dp reduction kernels then
+// sum the partials in a fixed order: the gradient is bitwise reproducible (the reference
+// uses shared and global float atomics, gpu/crossing.cu:179-180, 295-296).
+// Layers [l0, l0 + nl) are walked from the last to the first; checkpointed adjoints run
+// one launch per segment.
 template <class T>
 __device__ __forceinline__ float re_dot(const T a, const T b);
 template <>
@@ -320,24 +353,50 @@ __device__ __forceinline__ float re_dot<float>(const float a, const float b) {
 }
 
 template <class T, int W, int V, bool UNI>
-__global__ void __launch_bounds__(256) k_bwd(int N, int L, int B, int LPV, int NP, int C, int np,
+__device__ __forceinline__ void bwd_layer(const int kd, const T (&u)[V][W], const T (&g)[V][W], T (&uo)[V][W],
+                                          T (&go)[V][W], const typename RowOf<T>::type (&ri)[W],
+                                          const typename RowOf<T>::type (&ra)[W], typename RowOf<T>::type* stage,
+                                          const Lanes& q, const int LPV, const int C, const int lc) {
+    T accA[W], accB[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        accA[k] = fzero(T());
+        accB[k] = fzero(T());
+    }
+    BwdStep<T, W, V> st{u, g, uo, go, ri, ra, accA, accB, q.lane};
+    dispatch_kind<W>(kd, st);
+    // Fold lane groups (groups hold the same waveguides of different vectors).
+    for (int off = LPV; off < 64; off <<= 1) {
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            accA[k] = fadd(accA[k], lane_read(accA[k], q.lane ^ off));
+            accB[k] = fadd(accB[k], lane_read(accB[k], q.lane ^ off));
+        }
+    }
+    if (q.grp == 0) {
+#pragma unroll
+        for (int k = 0; k < W; ++k) stage[(((size_t)q.wave * C + lc) * W + k) * LPV + q.t] = pack_row(accA[k], accB[k]);
+    }
+}
+
+template <class T, int W, int V, bool UNI>
+__global__ void __launch_bounds__(256) k_bwd(int N, int l0, int nl, int B, int LPV, int NP, int C, int np,
                                              const int* __restrict__ kind,
                                              const typename RowOf<T>::type* __restrict__ rows_inv,
                                              const typename RowOf<T>::type* __restrict__ rows_adj,
-                                             const T* __restrict__ dtdp, float* __restrict__ partial, long ntiles,
-                                             const T* u_out, const T* g_out, const T* target, T* u_in, T* g_in,
-                                             int ldu, const T* ckpt, int ck_every) {
+                                             const T* __restrict__ dtdp, float* __restrict__ partial,
+                                             size_t part_stride, long ntiles, const T* u_out, const T* g_out,
+                                             const T* target, T* u_in, T* g_in, int ldu) {
     typedef typename RowOf<T>::type R;
     extern __shared__ unsigned char smem_raw[];
     R* stage = reinterpret_cast<R*>(smem_raw);  // [nw][C][W][LPV]
     const int nw = blockDim.x >> 6;
     const int NPh = NP / 2;
-    const int G = 64 / LPV;
-    float* my_partial = partial + (size_t)blockIdx.x * L * NPh * np;
+    float* my_partial = partial + (size_t)blockIdx.x * part_stride;
     bool first = true;
     for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const Lanes q = lanes_of<V>(LPV, tile);
-        T u[V][W], g[V][W];
+        T u[V][W], g[V][W], ub[V][W], gb[V][W];
         load_tile(u, u_out, q, N, B, ldu);
         if (target != nullptr) {
             T y[V][W];
@@ -349,87 +408,77 @@ __global__ void __launch_bounds__(256) k_bwd(int N, int L, int B, int LPV, int N
         } else {
             load_tile(g, g_out, q, N, B, ldu);
         }
-        const R* rip = rows_inv + q.t * W;
-        const R* rap = rows_adj + q.t * W;
-        R ri[W], ra[W];
-        if (L > 0) {
-            load_rows(ri, rip + (size_t)(L - 1) * NP);
-            if constexpr (!UNI) load_rows(ra, rap + (size_t)(L - 1) * NP);
+        const R* rip = rows_inv + (size_t)l0 * NP + q.t * W;
+        const R* rap = rows_adj + (size_t)l0 * NP + q.t * W;
+        const int* kd_p = kind + l0;
+        R ria[W], raa[W], rib[W], rab[W];
+        int ka = 0;
+        if (nl > 0) {
+            ka = kd_p[nl - 1];
+            load_rows(ria, rip + (size_t)(nl - 1) * NP);
+            if constexpr (!UNI) load_rows(raa, rap + (size_t)(nl - 1) * NP);
         }
-        for (int c0 = L - 1; c0 >= 0; c0 -= C) {
-            const int cl = (C < c0 + 1) ? C : c0 + 1;
-            for (int lc = 0; lc < cl; ++lc) {
-                const int l = c0 - lc;
-                const int kd = kind[l];
-                R rin[W], ran[W];
-                if (l > 0) {
-                    load_rows(rin, rip + (size_t)(l - 1) * NP);
-                    if constexpr (!UNI) load_rows(ran, rap + (size_t)(l - 1) * NP);
-                }
-                T accA[W], accB[W];
-#pragma unroll
-                for (int k = 0; k < W; ++k) {
-                    accA[k] = fzero(T());
-                    accB[k] = fzero(T());
-                }
-                if constexpr (UNI) {
-                    BwdStep<T, W, V> st{u, g, ri, ri, accA, accB, q.lane};
-                    dispatch_kind<W>(kd, st);
-                } else {
-                    BwdStep<T, W, V> st{u, g, ri, ra, accA, accB, q.lane};
-                    dispatch_kind<W>(kd, st);
-                }
-                // Fold lane groups (groups hold the same waveguides of different vectors).
-                for (int off = LPV; off < 64; off <<= 1) {
-#pragma unroll
-                    for (int k = 0; k < W; ++k) {
-                        accA[k] = fadd(accA[k], lane_read(accA[k], q.lane ^ off));
-                        accB[k] = fadd(accB[k], lane_read(accB[k], q.lane ^ off));
-                    }
-                }
-                if (q.grp == 0) {
-#pragma unroll
-                    for (int k = 0; k < W; ++k)
-                        stage[(((size_t)q.wave * C + lc) * W + k) * LPV + q.t] = pack_row(accA[k], accB[k]);
-                }
-                if (ckpt != nullptr && (l % ck_every) == 0)
-                    load_tile(u, ckpt + (size_t)(l / ck_every) * B * ldu, q, N, B, ldu);
-#pragma unroll
-                for (int k = 0; k < W; ++k) {
-                    ri[k] = rin[k];
-                    if constexpr (!UNI) ra[k] = ran[k];
-                }
+        for (int c0 = 0; c0 < nl; c0 += C) {
+            const int cl = (C < nl - c0) ? C : nl - c0;
+            int lc = 0;
+            for (; lc + 1 < cl; lc += 2) {
+                const int la = nl - 1 - (c0 + lc);  // this layer, then la - 1
+                const int kb = kd_p[la - 1];
+                load_rows(rib, rip + (size_t)(la - 1) * NP);
+                if constexpr (!UNI) load_rows(rab, rap + (size_t)(la - 1) * NP);
+                if constexpr (UNI)
+                    bwd_layer<T, W, V, UNI>(ka, u, g, ub, gb, ria, ria, stage, q, LPV, C, lc);
+                else
+                    bwd_layer<T, W, V, UNI>(ka, u, g, ub, gb, ria, raa, stage, q, LPV, C, lc);
+                const int ln = (la - 2 >= 0) ? la - 2 : 0;
+                ka = kd_p[ln];
+                load_rows(ria, rip + (size_t)ln * NP);
+                if constexpr (!UNI) load_rows(raa, rap + (size_t)ln * NP);
+                if constexpr (UNI)
+                    bwd_layer<T, W, V, UNI>(kb, ub, gb, u, g, rib, rib, stage, q, LPV, C, lc + 1);
+                else
+                    bwd_layer<T, W, V, UNI>(kb, ub, gb, u, g, rib, rab, stage, q, LPV, C, lc + 1);
+            }
+            if (lc < cl) {  // odd length: only the last stage, so nothing further to prefetch
+                if constexpr (UNI)
+                    bwd_layer<T, W, V, UNI>(ka, u, g, ub, gb, ria, ria, stage, q, LPV, C, lc);
+                else
+                    bwd_layer<T, W, V, UNI>(ka, u, g, ub, gb, ria, raa, stage, q, LPV, C, lc);
+                copy_state(ub, u);
+                copy_state(gb, g);
             }
             __syncthreads();
             // dJdT -> dp for the cl layers of this stage.
             for (int idx = threadIdx.x; idx < cl * NPh; idx += blockDim.x) {
-                const int lc = idx / NPh, m = idx - lc * NPh;
-                const int l = c0 - lc;
-                const int kd = kind[l];
-                int i = -1;
-                if (kd == 0)
+                const int lcc = idx / NPh, m = idx - lcc * NPh;
+                const int l = nl - 1 - (c0 + lcc);
+                const int kd = kd_p[l];
+                int i = -1, j = -1;
+                if (kd == 0) {
                     i = 2 * m;
-                else if (kd == 1)
+                    j = i + 1;
+                } else if (kd == 1) {
                     i = 2 * m + 1;
-                else if (kd != KIND_IDENTITY) {
+                    j = i + 1;
+                } else if (kd != KIND_IDENTITY) {
                     const int S = 1 << (kd - 2);
                     i = 2 * S * (m / S) + (m % S);
+                    j = i + S;
                 }
-                const int j = (kd >= 2) ? i + (1 << (kd - 2)) : i + 1;
-                float* outp = my_partial + ((size_t)l * NPh + m) * np;
+                float* outp = my_partial + ((size_t)(l0 + l) * NPh + m) * np;
                 if (i < 0 || j >= NP) {
-                    for (int k = 0; k < np; ++k)
-                        if (first) outp[k] = 0.f;
+                    if (first)
+                        for (int k = 0; k < np; ++k) outp[k] = 0.f;
                     continue;
                 }
-                R si = stage[(((size_t)0 * C + lc) * W + (i % W)) * LPV + i / W];
-                R sj = stage[(((size_t)0 * C + lc) * W + (j % W)) * LPV + j / W];
+                R si = stage[(((size_t)0 * C + lcc) * W + (i % W)) * LPV + i / W];
+                R sj = stage[(((size_t)0 * C + lcc) * W + (j % W)) * LPV + j / W];
                 for (int wv = 1; wv < nw; ++wv) {
-                    si = row_add(si, stage[(((size_t)wv * C + lc) * W + (i % W)) * LPV + i / W]);
-                    sj = row_add(sj, stage[(((size_t)wv * C + lc) * W + (j % W)) * LPV + j / W]);
+                    si = row_add(si, stage[(((size_t)wv * C + lcc) * W + (i % W)) * LPV + i / W]);
+                    sj = row_add(sj, stage[(((size_t)wv * C + lcc) * W + (j % W)) * LPV + j / W]);
                 }
                 // dJdT00 = A_i, dJdT01 = B_i, dJdT11 = A_j, dJdT10 = B_j
-                const T* dd = dtdp + ((size_t)l * NPh + m) * np * 4;
+                const T* dd = dtdp + ((size_t)(l0 + l) * NPh + m) * np * 4;
                 for (int k = 0; k < np; ++k) {
                     float val = re_dot(row_a(si), dd[k * 4 + 0]);
                     val += re_dot(row_b(si), dd[k * 4 + 1]);
@@ -440,11 +489,10 @@ __global__ void __launch_bounds__(256) k_bwd(int N, int L, int B, int LPV, int N
             }
             __syncthreads();
         }
-        store_tile(u, u_in, q, N, B, ldu);
+        if (u_in != nullptr) store_tile(u, u_in, q, N, B, ldu);
         if (g_in != nullptr) store_tile(g, g_in, q, N, B, ldu);
         first = false;
     }
-    (void)G;
 }
 
 // =======================================================================================
@@ -456,8 +504,9 @@ struct Plan {
     int np, ns, real;
     long ntiles;
     int grid;         // workgroups of the propagation kernel
+    int nseg;         // first-stage segments of the dp reduction
     size_t rowbytes;  // sizeof(row) = 16 (complex) or 8 (real)
-    size_t off_kind, off_a, off_b, off_dtdp, off_partial, total;  // workspace layout
+    size_t off_kind, off_a, off_b, off_dtdp, off_partial, off_seg, total;  // workspace layout
     int uni;          // crossings are unitary: T^-1 = T^H
 };
 
@@ -466,13 +515,13 @@ struct Plan {
 __host__ __device__ constexpr bool fwd_tile_ok(int W, int V) { return W * V <= 32 && V <= 8; }
 __host__ __device__ constexpr bool diff_tile_ok(int W, int V) { return W * V <= 16 && V <= 4; }
 
-int launch_fwd_any(const Plan& P, const int* kind, const void* rows, const void* u_in, void* u_out, int ldu,
-                   void* ckpt, int ck, hipStream_t st);
+int launch_fwd_any(const Plan& P, const int* kind, const void* rows, int l0, int nl, const void* u_in, void* u_out,
+                   int ldu, hipStream_t st);
 int launch_fwddiff_any(const Plan& P, const int* kind, const void* rows, const void* drows, const void* u_in,
                        const void* du_in, void* u_out, void* du_out, int ldu, hipStream_t st);
 int launch_bwd_any(const Plan& P, const int* kind, const void* ri, const void* ra, const void* dtdp, float* partial,
-                   const void* u_out, const void* g_out, const void* target, void* u_in, void* g_in, int ldu,
-                   const void* ckpt, int ck, hipStream_t st);
+                   size_t part_stride, int l0, int nl, const void* u_out, const void* g_out, const void* target,
+                   void* u_in, void* g_in, int ldu, hipStream_t st);
 
 }  // namespace mp
 

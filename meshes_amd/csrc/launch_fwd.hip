@@ -6,22 +6,22 @@
 namespace mp {
 
 template <class T, int W, int V>
-static void go(const Plan& P, const int* kind, const void* rows, const void* u_in, void* u_out, int ldu, void* ckpt,
-               int ck, hipStream_t st) {
+static void go(const Plan& P, const int* kind, const void* rows, int l0, int nl, const void* u_in, void* u_out,
+               int ldu, hipStream_t st) {
     typedef typename RowOf<T>::type R;
-    hipLaunchKernelGGL((k_fwd<T, W, V>), dim3(P.grid), dim3(64 * P.waves), 0, st, P.N, P.L, P.B, P.LPV, P.NP, kind,
-                       (const R*)rows, (const T*)u_in, (T*)u_out, ldu, (T*)ckpt, ck);
+    hipLaunchKernelGGL((k_fwd<T, W, V>), dim3(P.grid), dim3(64 * P.waves), 0, st, P.N, l0, nl, P.B, P.LPV, P.NP,
+                       kind, (const R*)rows, (const T*)u_in, (T*)u_out, ldu);
 }
 
-int launch_fwd_any(const Plan& P, const int* kind, const void* rows, const void* u_in, void* u_out, int ldu,
-                   void* ckpt, int ck, hipStream_t st) {
-#define MP_CALL(W_, V_)                                                          \
-    if constexpr (fwd_tile_ok(W_, V_)) {                                         \
-        if (P.real)                                                              \
-            go<float, W_, V_>(P, kind, rows, u_in, u_out, ldu, ckpt, ck, st);    \
-        else                                                                     \
-            go<float2, W_, V_>(P, kind, rows, u_in, u_out, ldu, ckpt, ck, st);   \
-        return MESHPROP_OK;                                                      \
+int launch_fwd_any(const Plan& P, const int* kind, const void* rows, int l0, int nl, const void* u_in, void* u_out,
+                   int ldu, hipStream_t st) {
+#define MP_CALL(W_, V_)                                                        \
+    if constexpr (fwd_tile_ok(W_, V_)) {                                       \
+        if (P.real)                                                            \
+            go<float, W_, V_>(P, kind, rows, l0, nl, u_in, u_out, ldu, st);    \
+        else                                                                   \
+            go<float2, W_, V_>(P, kind, rows, l0, nl, u_in, u_out, ldu, st);   \
+        return MESHPROP_OK;                                                    \
     }
     MP_FOR_WV(P.W, P.V, MP_CALL)
 #undef MP_CALL

@@ -18,9 +18,31 @@
 
 namespace mp {
 
-// dp[l*ldp + m*np + k] += sum_wg partial[wg][l][m][k], workgroups in fixed order.
-__global__ void k_dp_reduce(int L, int Nh, int NPh, int np, int nwg, const float* __restrict__ partial,
-                            float* __restrict__ dp, int ldp) {
+// dp reduction, stage 1: seg[s][e] = sum of partial[wg][e] over the s-th contiguous run of
+// workgroups (e indexes the partial layout [L][NP/2][np]).
+__global__ void __launch_bounds__(256) k_dp_seg(long E, int nwg, int nseg, const float* __restrict__ partial,
+                                                float* __restrict__ seg) {
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    if (e >= E) return;
+    const int per = (nwg + nseg - 1) / nseg;
+    const int w0 = s * per, w1 = (w0 + per < nwg) ? w0 + per : nwg;
+    const float* src = partial + e;
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+    int w = w0;
+    for (; w + 3 < w1; w += 4) {
+        a0 += src[(size_t)w * E];
+        a1 += src[(size_t)(w + 1) * E];
+        a2 += src[(size_t)(w + 2) * E];
+        a3 += src[(size_t)(w + 3) * E];
+    }
+    for (; w < w1; ++w) a0 += src[(size_t)w * E];
+    seg[(size_t)s * E + e] = (a0 + a1) + (a2 + a3);
+}
+
+// dp reduction, stage 2: dp[l*ldp + m*np + k] += sum_s seg[s][(l*NPh + m)*np + k], m < N/2.
+__global__ void __launch_bounds__(256) k_dp_final(int L, int Nh, int NPh, int np, int nseg,
+                                                  const float* __restrict__ seg, float* __restrict__ dp, int ldp) {
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long total = (long)L * Nh * np;
     if (idx >= total) return;
@@ -28,15 +50,15 @@ __global__ void k_dp_reduce(int L, int Nh, int NPh, int np, int nwg, const float
     const long lm = idx / np;
     const int m = lm % Nh;
     const int l = lm / Nh;
-    const size_t stride = (size_t)L * NPh * np;
-    const float* src = partial + ((size_t)l * NPh + m) * np + k;
+    const size_t E = (size_t)L * NPh * np;
+    const float* src = seg + ((size_t)l * NPh + m) * np + k;
     float acc = 0.f;
-    for (int wg = 0; wg < nwg; ++wg) acc += src[(size_t)wg * stride];
+    for (int s = 0; s < nseg; ++s) acc += src[(size_t)s * E];
     dp[(size_t)l * ldp + (size_t)m * np + k] += acc;
 }
 
-
 }  // namespace mp
+
 
 // =======================================================================================
 // Host side
@@ -125,12 +147,16 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
         // LDS stage depth: up to 64 KiB per workgroup.
         const size_t per_layer = (size_t)P.waves * P.NP * P.rowbytes;
         int C = (int)((64 * 1024) / per_layer);
-        if (C < 1) C = 1;
         if (C > 16) C = 16;
         if (P.C >= 1 && (size_t)P.C * per_layer <= 64 * 1024) C = P.C;
+        C &= ~1;  // the backdiff stage walks layers in pairs
+        if (C < 2) C = 2;
+        if ((size_t)C * per_layer > 64 * 1024) return MESHPROP_ERR_UNSUPPORTED;
         P.C = C;
+        P.nseg = P.grid < 16 ? P.grid : 16;
     } else {
         P.C = 0;
+        P.nseg = 0;
     }
     size_t off = 0;
     P.off_kind = off;
@@ -147,7 +173,10 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
         off = align256(off + (size_t)L * (P.NP / 2) * P.np * 4 * elem);
         P.off_partial = off;
         off = align256(off + (size_t)P.grid * L * (P.NP / 2) * P.np * sizeof(float));
+        P.off_seg = off;
+        off = align256(off + (size_t)P.nseg * L * (P.NP / 2) * P.np * sizeof(float));
     }
+    P.off_seg = (op == MESHPROP_OP_BACKDIFF) ? P.off_seg : off;
     P.total = off;
     return MESHPROP_OK;
 }
@@ -179,7 +208,9 @@ int run_impl(int op, int X, int N, int L, int B, int fft, const int* g0, const i
     if (op == MESHPROP_OP_BACKDIFF && !dp) return MESHPROP_ERR_ARG;
     if (op == MESHPROP_OP_BACKDIFF && !du_a && !target) return MESHPROP_ERR_ARG;
     if (ckpt && ck_every < 1) return MESHPROP_ERR_ARG;
-    if (!ckpt) ck_every = 1;
+    if (ckpt && op == MESHPROP_OP_FWDDIFF) return MESHPROP_ERR_UNSUPPORTED;
+    if (ckpt && op == MESHPROP_OP_BACKDIFF && !du_b) return MESHPROP_ERR_ARG;  // carries g between segments
+    if (!ckpt || ck_every > L) ck_every = L > 0 ? L : 1;
     if (P.total > 0 && (!ws || ws_bytes < P.total)) return MESHPROP_ERR_WORKSPACE;
     if (((uintptr_t)ws) & 255) return MESHPROP_ERR_WORKSPACE;
 
@@ -220,21 +251,54 @@ int run_impl(int op, int X, int N, int L, int B, int fft, const int* g0, const i
     }
 
     const void* rows_b = P.uni ? tab_a : tab_b;
-    if (!do_main) {
-        rc = MESHPROP_OK;
-    } else if (op == MESHPROP_OP_FWDPROP)
-        rc = launch_fwd_any(P, kind, tab_a, u_a, u_b, ldu, ckpt, ck_every, st);
-    else if (op == MESHPROP_OP_FWDDIFF)
-        rc = launch_fwddiff_any(P, kind, tab_a, tab_b, u_a, du_a, u_b, du_b, ldu, st);
-    else
-        rc = launch_bwd_any(P, kind, tab_a, rows_b, dtdp, partial, u_a, du_a, target, u_b, du_b, ldu, ckpt,
-                            ck_every, st);
-    if (rc) return rc;
-    if ((rc = check_launch())) return rc;
+    const int K = ck_every;
+    const int nck = L > 0 ? (L + K - 1) / K : 0;
+    const size_t fstride = (size_t)B * ldu * (P.real ? sizeof(float) : 2 * sizeof(float));
+    unsigned char* ck = (unsigned char*)ckpt;
+    if (do_main) {
+        if (op == MESHPROP_OP_FWDPROP) {
+            if (!ckpt) {
+                rc = launch_fwd_any(P, kind, tab_a, 0, L, u_a, u_b, ldu, st);
+            } else {
+                // Checkpointed forward: ckpt[c] receives the field entering layer c*K.
+                rc = launch_fwd_any(P, kind, tab_a, 0, 0, u_a, ck, ldu, st);
+                for (int c = 0; c < nck && !rc; ++c) {
+                    const int l0 = c * K, nl = (L - l0 < K) ? L - l0 : K;
+                    void* out = (c + 1 < nck) ? (void*)(ck + (size_t)(c + 1) * fstride) : u_b;
+                    rc = launch_fwd_any(P, kind, tab_a, l0, nl, ck + (size_t)c * fstride, out, ldu, st);
+                }
+            }
+        } else if (op == MESHPROP_OP_FWDDIFF) {
+            rc = launch_fwddiff_any(P, kind, tab_a, tab_b, u_a, du_a, u_b, du_b, ldu, st);
+        } else {
+            const size_t ps = (size_t)L * (P.NP / 2) * P.np;
+            if (!ckpt) {
+                rc = launch_bwd_any(P, kind, tab_a, rows_b, dtdp, partial, ps, 0, L, u_a, du_a, target, u_b, du_b,
+                                    ldu, st);
+            } else {
+                // Checkpointed adjoint: segment c starts from the exact forward field ckpt[c+1]
+                // (u_out for the last one); the adjoint field is carried in dJdu_in.
+                for (int c = nck - 1; c >= 0 && !rc; --c) {
+                    const int l0 = c * K, nl = (L - l0 < K) ? L - l0 : K;
+                    const bool last = (c == nck - 1);
+                    const void* us = last ? u_a : (const void*)(ck + (size_t)(c + 1) * fstride);
+                    rc = launch_bwd_any(P, kind, tab_a, rows_b, dtdp, partial, ps, l0, nl, us, last ? du_a : du_b,
+                                        last ? target : nullptr, c == 0 ? u_b : nullptr, du_b, ldu, st);
+                }
+            }
+        }
+        if (rc) return rc;
+        if ((rc = check_launch())) return rc;
+    }
     if (op == MESHPROP_OP_BACKDIFF && L > 0 && do_red) {
+        float* seg = (float*)(base + P.off_seg);
+        const long E = (long)L * (P.NP / 2) * P.np;
+        hipLaunchKernelGGL(k_dp_seg, dim3((unsigned)((E + 255) / 256), P.nseg), dim3(256), 0, st, E, P.grid, P.nseg,
+                           (const float*)partial, seg);
+        if ((rc = check_launch())) return rc;
         const long total = (long)L * (N / 2) * P.np;
-        hipLaunchKernelGGL(k_dp_reduce, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, L, N / 2, P.NP / 2,
-                           P.np, P.grid, (const float*)partial, dp, ldp);
+        hipLaunchKernelGGL(k_dp_final, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, L, N / 2, P.NP / 2,
+                           P.np, P.nseg, (const float*)seg, dp, ldp);
         if ((rc = check_launch())) return rc;
     }
     return MESHPROP_OK;

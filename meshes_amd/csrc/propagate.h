@@ -119,41 +119,9 @@ struct Log2 {
 };
 
 // ---------------------------------------------------------------------------------------
-// Partner gather: pn[v][k] = value of the partner of waveguide k (of this lane) in a.
-// KC: 0 in-lane stride S, 1 misaligned NN, 2 cross-lane xor xm.
+// Layer kinds -> step variants.  Calls op.template run<KC, S>(xm) where KC = 0: in-lane
+// pairs (k, k+S); KC = 1: odd-shift NN layer; KC = 2: cross-lane partner lane ^ xm.
 // ---------------------------------------------------------------------------------------
-template <int KC, int S, class T, int W, int V>
-__device__ __forceinline__ void gather(const T (&a)[V][W], T (&pn)[V][W], const int lane, const int xm) {
-    if constexpr (KC == 0) {
-#pragma unroll
-        for (int v = 0; v < V; ++v)
-#pragma unroll
-            for (int k = 0; k < W; ++k) pn[v][k] = a[v][k ^ S];
-    } else if constexpr (KC == 1) {
-#pragma unroll
-        for (int v = 0; v < V; ++v) {
-            const T hi = lane_read(a[v][0], lane + 1);
-            const T lo = lane_read(a[v][W - 1], lane - 1);
-#pragma unroll
-            for (int k = 0; k < W; ++k) {
-                if (k == W - 1)
-                    pn[v][k] = hi;
-                else if (k == 0)
-                    pn[v][k] = lo;
-                else
-                    pn[v][k] = a[v][(k & 1) ? k + 1 : k - 1];
-            }
-        }
-    } else {
-        const int src = lane ^ xm;
-#pragma unroll
-        for (int v = 0; v < V; ++v)
-#pragma unroll
-            for (int k = 0; k < W; ++k) pn[v][k] = lane_read(a[v][k], src);
-    }
-}
-
-// Calls op.template run<KC, S>(xm) for layer kind kd.
 template <int W, class Op>
 __device__ __forceinline__ void dispatch_kind(const int kd, Op& op) {
     if (kd == 0 || kd == 2) {
@@ -164,7 +132,10 @@ __device__ __forceinline__ void dispatch_kind(const int kd, Op& op) {
         op.template run<1, 1>(0);
         return;
     }
-    if (kd == KIND_IDENTITY) return;
+    if (kd == KIND_IDENTITY) {
+        op.identity();
+        return;
+    }
     const int st = kd - 2;
     if constexpr (W > 2) {
         if (st == 1) {
@@ -188,81 +159,196 @@ __device__ __forceinline__ void dispatch_kind(const int kd, Op& op) {
 }
 
 // ---------------------------------------------------------------------------------------
-// Per-layer steps
+// Per-layer steps, written pair by pair and in place so that no partner copies are
+// materialised.  Odd-shift layers pair the last waveguide of each lane with the first of
+// the next lane; those two values travel through ds_bpermute.
 // ---------------------------------------------------------------------------------------
+template <class T, int W, int V>
+__device__ __forceinline__ void copy_state(const T (&a)[V][W], T (&b)[V][W]) {
+#pragma unroll
+    for (int v = 0; v < V; ++v)
+#pragma unroll
+        for (int k = 0; k < W; ++k) b[v][k] = a[v][k];
+}
+
+// The state arrays come in pairs (in -> out): the kernels unroll the layer loop by two
+// and alternate the roles, so every result is written to a register that is not an input
+// of the same layer and no loop-carried copies are needed.
 template <class T, int W, int V>
 struct FwdStep {
     typedef typename RowOf<T>::type R;
-    T (&u)[V][W];
+    const T (&u)[V][W];
+    T (&o)[V][W];
     const R (&r)[W];
     int lane;
+    __device__ __forceinline__ void identity() { copy_state(u, o); }
     template <int KC, int S>
     __device__ __forceinline__ void run(const int xm) {
-        T pn[V][W];
-        gather<KC, S>(u, pn, lane, xm);
+        if constexpr (KC == 0) {
 #pragma unroll
-        for (int v = 0; v < V; ++v)
+            for (int v = 0; v < V; ++v)
 #pragma unroll
-            for (int k = 0; k < W; ++k) u[v][k] = rapply(r[k], u[v][k], pn[v][k]);
+                for (int k = 0; k < W; ++k)
+                    if ((k & S) == 0) {
+                        o[v][k] = rapply(r[k], u[v][k], u[v][k + S]);
+                        o[v][k + S] = rapply(r[k + S], u[v][k + S], u[v][k]);
+                    }
+        } else if constexpr (KC == 1) {
+            T hi[V], lo[V];
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                hi[v] = lane_read(u[v][0], lane + 1);
+                lo[v] = lane_read(u[v][W - 1], lane - 1);
+            }
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+#pragma unroll
+                for (int k = 1; k + 1 < W; k += 2) {
+                    o[v][k] = rapply(r[k], u[v][k], u[v][k + 1]);
+                    o[v][k + 1] = rapply(r[k + 1], u[v][k + 1], u[v][k]);
+                }
+                o[v][W - 1] = rapply(r[W - 1], u[v][W - 1], hi[v]);
+                o[v][0] = rapply(r[0], u[v][0], lo[v]);
+            }
+        } else {
+            const int src = lane ^ xm;
+#pragma unroll
+            for (int v = 0; v < V; ++v)
+#pragma unroll
+                for (int k = 0; k < W; ++k) o[v][k] = rapply(r[k], u[v][k], lane_read(u[v][k], src));
+        }
     }
 };
 
 template <class T, int W, int V>
 struct FwdDiffStep {
     typedef typename RowOf<T>::type R;
-    T (&u)[V][W];
-    T (&du)[V][W];
+    const T (&u)[V][W];
+    const T (&du)[V][W];
+    T (&o)[V][W];
+    T (&dout)[V][W];
     const R (&r)[W];
     const R (&d)[W];
     int lane;
+    __device__ __forceinline__ void identity() {
+        copy_state(u, o);
+        copy_state(du, dout);
+    }
+    __device__ __forceinline__ void one(const int v, const int k, const T y, const T dy) {
+        dout[v][k] = rapply_d(r[k], d[k], u[v][k], y, du[v][k], dy);
+        o[v][k] = rapply(r[k], u[v][k], y);
+    }
     template <int KC, int S>
     __device__ __forceinline__ void run(const int xm) {
-        T up[V][W], dup[V][W];
-        gather<KC, S>(u, up, lane, xm);
-        gather<KC, S>(du, dup, lane, xm);
+        if constexpr (KC == 0) {
 #pragma unroll
-        for (int v = 0; v < V; ++v)
+            for (int v = 0; v < V; ++v)
 #pragma unroll
-            for (int k = 0; k < W; ++k) {
-                du[v][k] = rapply_d(r[k], d[k], u[v][k], up[v][k], du[v][k], dup[v][k]);
-                u[v][k] = rapply(r[k], u[v][k], up[v][k]);
+                for (int k = 0; k < W; ++k) one(v, k, u[v][k ^ S], du[v][k ^ S]);
+        } else if constexpr (KC == 1) {
+            T hu[V], hd[V], lu[V], ld[V];
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                hu[v] = lane_read(u[v][0], lane + 1);
+                hd[v] = lane_read(du[v][0], lane + 1);
+                lu[v] = lane_read(u[v][W - 1], lane - 1);
+                ld[v] = lane_read(du[v][W - 1], lane - 1);
             }
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+#pragma unroll
+                for (int k = 1; k + 1 < W; ++k) {
+                    const int j = (k & 1) ? k + 1 : k - 1;
+                    one(v, k, u[v][j], du[v][j]);
+                }
+                one(v, W - 1, hu[v], hd[v]);
+                one(v, 0, lu[v], ld[v]);
+            }
+        } else {
+            const int src = lane ^ xm;
+#pragma unroll
+            for (int v = 0; v < V; ++v)
+#pragma unroll
+                for (int k = 0; k < W; ++k) one(v, k, lane_read(u[v][k], src), lane_read(du[v][k], src));
+        }
     }
 };
 
 // Reverse step through one layer: x = Rinv u (reconstructed layer input),
-// dJdT rows: A_k += conj(g_k) x_k, B_k += conj(g_k) x_partner(k), then g <- Radj g.
+// dJdT rows A_k += conj(g_k) x_k, B_k += conj(g_k) x_partner(k), then g <- Radj g.
 template <class T, int W, int V>
 struct BwdStep {
     typedef typename RowOf<T>::type R;
-    T (&u)[V][W];
-    T (&g)[V][W];
+    const T (&u)[V][W];
+    const T (&g)[V][W];
+    T (&uo)[V][W];
+    T (&go)[V][W];
     const R (&ri)[W];
     const R (&ra)[W];
     T (&accA)[W];
     T (&accB)[W];
     int lane;
+    __device__ __forceinline__ void identity() {
+        // Identity layer: fields pass through; its rows meet a zero dT/dp.
+        copy_state(u, uo);
+        copy_state(g, go);
+    }
+    __device__ __forceinline__ void pair(const int v, const int k, const int j) {
+        const T xk = rapply(ri[k], u[v][k], u[v][j]), xj = rapply(ri[j], u[v][j], u[v][k]);
+        accA[k] = cjfma(g[v][k], xk, accA[k]);
+        accB[k] = cjfma(g[v][k], xj, accB[k]);
+        accA[j] = cjfma(g[v][j], xj, accA[j]);
+        accB[j] = cjfma(g[v][j], xk, accB[j]);
+        go[v][k] = rapply(ra[k], g[v][k], g[v][j]);
+        go[v][j] = rapply(ra[j], g[v][j], g[v][k]);
+        uo[v][k] = xk;
+        uo[v][j] = xj;
+    }
     template <int KC, int S>
     __device__ __forceinline__ void run(const int xm) {
-        T up[V][W], gp[V][W], xp[V][W];
-        gather<KC, S>(u, up, lane, xm);
-        gather<KC, S>(g, gp, lane, xm);
+        if constexpr (KC == 0) {
 #pragma unroll
-        for (int v = 0; v < V; ++v)
+            for (int v = 0; v < V; ++v)
 #pragma unroll
-            for (int k = 0; k < W; ++k) u[v][k] = rapply(ri[k], u[v][k], up[v][k]);
-        gather<KC, S>(u, xp, lane, xm);
+                for (int k = 0; k < W; ++k)
+                    if ((k & S) == 0) pair(v, k, k + S);
+        } else if constexpr (KC == 1) {
 #pragma unroll
-        for (int v = 0; v < V; ++v)
-#pragma unroll
-            for (int k = 0; k < W; ++k) {
-                accA[k] = cjfma(g[v][k], u[v][k], accA[k]);
-                accB[k] = cjfma(g[v][k], xp[v][k], accB[k]);
+            for (int v = 0; v < V; ++v) {
+                const T hu = lane_read(u[v][0], lane + 1), hg = lane_read(g[v][0], lane + 1);
+                const T lu = lane_read(u[v][W - 1], lane - 1), lg = lane_read(g[v][W - 1], lane - 1);
+                uo[v][W - 1] = rapply(ri[W - 1], u[v][W - 1], hu);
+                uo[v][0] = rapply(ri[0], u[v][0], lu);
+                go[v][W - 1] = rapply(ra[W - 1], g[v][W - 1], hg);
+                go[v][0] = rapply(ra[0], g[v][0], lg);
             }
 #pragma unroll
-        for (int v = 0; v < V; ++v)
+            for (int v = 0; v < V; ++v) {
+                const T hx = lane_read(uo[v][0], lane + 1), lx = lane_read(uo[v][W - 1], lane - 1);
 #pragma unroll
-            for (int k = 0; k < W; ++k) g[v][k] = rapply(ra[k], g[v][k], gp[v][k]);
+                for (int k = 1; k + 1 < W; k += 2) pair(v, k, k + 1);
+                accA[W - 1] = cjfma(g[v][W - 1], uo[v][W - 1], accA[W - 1]);
+                accB[W - 1] = cjfma(g[v][W - 1], hx, accB[W - 1]);
+                accA[0] = cjfma(g[v][0], uo[v][0], accA[0]);
+                accB[0] = cjfma(g[v][0], lx, accB[0]);
+            }
+        } else {
+            const int src = lane ^ xm;
+#pragma unroll
+            for (int v = 0; v < V; ++v)
+#pragma unroll
+                for (int k = 0; k < W; ++k) {
+                    uo[v][k] = rapply(ri[k], u[v][k], lane_read(u[v][k], src));
+                    go[v][k] = rapply(ra[k], g[v][k], lane_read(g[v][k], src));
+                }
+#pragma unroll
+            for (int v = 0; v < V; ++v)
+#pragma unroll
+                for (int k = 0; k < W; ++k) {
+                    accA[k] = cjfma(g[v][k], uo[v][k], accA[k]);
+                    accB[k] = cjfma(g[v][k], lane_read(uo[v][k], src), accB[k]);
+                }
+        }
     }
 };
 
