@@ -105,7 +105,18 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--tile-fwd", type=str, default="")
     ap.add_argument("--tile-bwd", type=str, default="")
+    ap.add_argument("--probe-copy", type=int, default=0,
+                    help="only copy a buffer of this many MiB a few times (calibrates HBM byte counters)")
     args = ap.parse_args()
+    if args.probe_copy:
+        import torch
+        a = torch.empty(args.probe_copy << 20, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"probe_copy_MiB": args.probe_copy, "copies": 5}))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the workload runs on a single MI355X (batch sharding and the dp "
               "all-reduce across GPUs are described in DESIGN.md, not run).", file=sys.stderr)

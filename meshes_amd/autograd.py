@@ -1,0 +1,81 @@
+"""torch.autograd integration: the mesh as a differentiable function of its phases and inputs.
+
+``mesh_apply(mesh, p, x)`` returns ``y = U(p) x`` for a batch ``x`` of shape (B, N) (one vector
+per row) and differentiates through the hand-written adjoint kernel (backdiff) instead of
+recording the layer loop.  The interface follows the reference's custom-VJP wrapper
+(jax.py:15-36); torch's complex gradient convention (dL/dRe + i dL/dIm) is the meshes
+convention, so no conjugation is needed at the boundary (jax.py:31-33 conjugates for JAX).
+
+Adjoint.  For unitary crossings (mzi, sym, orth) backdiff reconstructs the layer inputs by
+inverse propagation, so the forward saves only its output.  For lossy crossings
+(mzi_generic with gains) the forward stores the field entering every ``ckpt_every``-th
+layer and the backward restarts each segment from the stored field (checkpointed adjoint),
+with T^-1 for the reconstruction and T^H for the adjoint.
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _lib
+from .mesh import MeshNetworkGPU
+
+
+def _launch(mesh, op, B, p, dp, u_a, du_a, u_b, du_b, target=None, ckpt=None, ckpt_every=0):
+    ldp, lds = mesh.N // 2 * mesh.X_np, mesh.N // 2 * mesh.X_ns
+    g0, g1 = mesh._geom()
+    s = mesh.p_splitter if mesh.p_splitter.numel() else None
+    ws, nb = mesh._workspace(op, B)
+    _lib.run(op, mesh.X, mesh.N, mesh.L, B, mesh.fft, g0, g1, p, dp, ldp, s, lds, u_a, du_a, u_b, du_b, mesh.N,
+             target=target, ckpt=ckpt, ckpt_every=ckpt_every, tile=mesh.tile, workspace=ws, ws_bytes=nb)
+
+
+class MeshFunction(torch.autograd.Function):
+    """y = U(p) x with x, y of shape (B, N); p flat (n_phase,) float32."""
+
+    @staticmethod
+    def forward(ctx, p, x, mesh: MeshNetworkGPU, ckpt_every: int):
+        p = p.detach().to(torch.float32).contiguous()
+        xb = x.detach().to(mesh.dtype).contiguous()
+        B = xb.shape[0]
+        y = torch.empty_like(xb)
+        ckpt = None
+        if ckpt_every > 0 and mesh.L > 0:
+            nck = (mesh.L + ckpt_every - 1) // ckpt_every
+            ckpt = torch.empty((nck, B, mesh.N), dtype=xb.dtype, device=xb.device)
+        _launch(mesh, _lib.OP_FWDPROP, B, p, None, xb, None, y, None, ckpt=ckpt,
+                ckpt_every=ckpt_every if ckpt is not None else 0)
+        ctx.mesh, ctx.ckpt_every = mesh, ckpt_every if ckpt is not None else 0
+        ctx.save_for_backward(p, y, ckpt if ckpt is not None else torch.empty(0, device=xb.device))
+        return y
+
+    @staticmethod
+    def backward(ctx, gy):
+        p, y, ckpt = ctx.saved_tensors
+        mesh = ctx.mesh
+        B = y.shape[0]
+        g = gy.to(mesh.dtype).contiguous()
+        dp = torch.zeros_like(p)
+        gx = torch.empty_like(y)
+        xr = torch.empty_like(y)
+        _launch(mesh, _lib.OP_BACKDIFF, B, p, dp, y, g, xr, gx, ckpt=ckpt if ckpt.numel() else None,
+                ckpt_every=ctx.ckpt_every)
+        return dp, gx, None, None
+
+
+def mesh_apply(mesh: MeshNetworkGPU, p: torch.Tensor, x: torch.Tensor, ckpt_every: int = 0) -> torch.Tensor:
+    """Differentiable y = U(p) x (x: (B, N)).  ckpt_every > 0 stores forward checkpoints for a
+    lossy mesh; for unitary crossings the adjoint reconstructs fields by inverse propagation."""
+    return MeshFunction.apply(p, x, mesh, int(ckpt_every))
+
+
+class MeshLayer(torch.nn.Module):
+    """A mesh with trainable phases as a torch module: forward(x) = U(p) x for x of shape (B, N)."""
+
+    def __init__(self, mesh: MeshNetworkGPU, ckpt_every: int = 0):
+        super().__init__()
+        self.mesh = mesh
+        self.ckpt_every = ckpt_every
+        self.p = torch.nn.Parameter(mesh.p_phase.clone())
+
+    def forward(self, x):
+        return mesh_apply(self.mesh, self.p, x, self.ckpt_every)

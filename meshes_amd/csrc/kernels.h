@@ -393,6 +393,57 @@ __global__ void __launch_bounds__(256) k_bwd(int N, int l0, int nl, int B, int L
     const int nw = blockDim.x >> 6;
     const int NPh = NP / 2;
     float* my_partial = partial + (size_t)blockIdx.x * part_stride;
+    constexpr int PF = 2;  // stage-end slots per thread whose dT/dp is prefetched
+    const int* kd_all = kind + l0;
+    // Slot idx of the current stage -> (layer, slot) and its dT/dp values.
+    auto fetch_dtdp = [&](const int idx, const int c0, T (&dd)[8]) {
+        const int lcc = idx / NPh, m = idx - lcc * NPh;
+        const int l = nl - 1 - (c0 + lcc);
+        const T* src = dtdp + ((size_t)(l0 + l) * NPh + m) * np * 4;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) dd[e] = (e < np * 4) ? src[e] : fzero(T());
+    };
+    int stage_c0 = 0;
+    auto stage_slot = [&](const int idx, const T (&dd)[8], const bool first_tile) {
+        const int lcc = idx / NPh, m = idx - lcc * NPh;
+        const int l = nl - 1 - (stage_c0 + lcc);
+        const int kd = kd_all[l];
+        int i = -1, j = -1;
+        if (kd == 0) {
+            i = 2 * m;
+            j = i + 1;
+        } else if (kd == 1) {
+            i = 2 * m + 1;
+            j = i + 1;
+        } else if (kd != KIND_IDENTITY) {
+            const int S = 1 << (kd - 2);
+            i = 2 * S * (m / S) + (m % S);
+            j = i + S;
+        }
+        float* outp = my_partial + ((size_t)(l0 + l) * NPh + m) * np;
+        if (i < 0 || j >= NP) {
+            if (first_tile)
+                for (int k = 0; k < np; ++k) outp[k] = 0.f;
+            return;
+        }
+        R si = stage[(((size_t)0 * C + lcc) * W + (i % W)) * LPV + i / W];
+        R sj = stage[(((size_t)0 * C + lcc) * W + (j % W)) * LPV + j / W];
+        for (int wv = 1; wv < nw; ++wv) {
+            si = row_add(si, stage[(((size_t)wv * C + lcc) * W + (i % W)) * LPV + i / W]);
+            sj = row_add(sj, stage[(((size_t)wv * C + lcc) * W + (j % W)) * LPV + j / W]);
+        }
+        // dJdT00 = A_i, dJdT01 = B_i, dJdT11 = A_j, dJdT10 = B_j
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            if (k < np) {
+                float val = re_dot(row_a(si), dd[k * 4 + 0]);
+                val += re_dot(row_b(si), dd[k * 4 + 1]);
+                val += re_dot(row_b(sj), dd[k * 4 + 2]);
+                val += re_dot(row_a(sj), dd[k * 4 + 3]);
+                outp[k] = first_tile ? val : outp[k] + val;
+            }
+        }
+    };
     bool first = true;
     for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const Lanes q = lanes_of<V>(LPV, tile);
@@ -420,6 +471,12 @@ __global__ void __launch_bounds__(256) k_bwd(int N, int l0, int nl, int B, int L
         }
         for (int c0 = 0; c0 < nl; c0 += C) {
             const int cl = (C < nl - c0) ? C : nl - c0;
+            T pre[PF][8];
+#pragma unroll
+            for (int r = 0; r < PF; ++r) {
+                const int idx = threadIdx.x + r * blockDim.x;
+                if (idx < cl * NPh) fetch_dtdp(idx, c0, pre[r]);
+            }
             int lc = 0;
             for (; lc + 1 < cl; lc += 2) {
                 const int la = nl - 1 - (c0 + lc);  // this layer, then la - 1
@@ -448,44 +505,17 @@ __global__ void __launch_bounds__(256) k_bwd(int N, int l0, int nl, int B, int L
                 copy_state(gb, g);
             }
             __syncthreads();
-            // dJdT -> dp for the cl layers of this stage.
-            for (int idx = threadIdx.x; idx < cl * NPh; idx += blockDim.x) {
-                const int lcc = idx / NPh, m = idx - lcc * NPh;
-                const int l = nl - 1 - (c0 + lcc);
-                const int kd = kd_p[l];
-                int i = -1, j = -1;
-                if (kd == 0) {
-                    i = 2 * m;
-                    j = i + 1;
-                } else if (kd == 1) {
-                    i = 2 * m + 1;
-                    j = i + 1;
-                } else if (kd != KIND_IDENTITY) {
-                    const int S = 1 << (kd - 2);
-                    i = 2 * S * (m / S) + (m % S);
-                    j = i + S;
-                }
-                float* outp = my_partial + ((size_t)(l0 + l) * NPh + m) * np;
-                if (i < 0 || j >= NP) {
-                    if (first)
-                        for (int k = 0; k < np; ++k) outp[k] = 0.f;
-                    continue;
-                }
-                R si = stage[(((size_t)0 * C + lcc) * W + (i % W)) * LPV + i / W];
-                R sj = stage[(((size_t)0 * C + lcc) * W + (j % W)) * LPV + j / W];
-                for (int wv = 1; wv < nw; ++wv) {
-                    si = row_add(si, stage[(((size_t)wv * C + lcc) * W + (i % W)) * LPV + i / W]);
-                    sj = row_add(sj, stage[(((size_t)wv * C + lcc) * W + (j % W)) * LPV + j / W]);
-                }
-                // dJdT00 = A_i, dJdT01 = B_i, dJdT11 = A_j, dJdT10 = B_j
-                const T* dd = dtdp + ((size_t)(l0 + l) * NPh + m) * np * 4;
-                for (int k = 0; k < np; ++k) {
-                    float val = re_dot(row_a(si), dd[k * 4 + 0]);
-                    val += re_dot(row_b(si), dd[k * 4 + 1]);
-                    val += re_dot(row_b(sj), dd[k * 4 + 2]);
-                    val += re_dot(row_a(sj), dd[k * 4 + 3]);
-                    outp[k] = first ? val : outp[k] + val;
-                }
+            stage_c0 = c0;
+            // dJdT -> dp for the cl layers of this stage (dT/dp of the first slots prefetched).
+#pragma unroll
+            for (int r = 0; r < PF; ++r) {
+                const int idx = threadIdx.x + r * blockDim.x;
+                if (idx < cl * NPh) stage_slot(idx, pre[r], first);
+            }
+            for (int idx = threadIdx.x + PF * blockDim.x; idx < cl * NPh; idx += blockDim.x) {
+                T dd[8];
+                fetch_dtdp(idx, c0, dd);
+                stage_slot(idx, dd, first);
             }
             __syncthreads();
         }

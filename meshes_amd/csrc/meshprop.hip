@@ -135,6 +135,8 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
     resolve_tile(op, N, B, tile, ncu, P.W, P.V, P.LPV, P.waves, P.C);
     P.NP = P.LPV * P.W;
     P.rowbytes = P.real ? sizeof(float2) : sizeof(float4);
+    if (op == MESHPROP_OP_BACKDIFF)  // the LDS stage must hold two layers of every wave
+        while (P.waves > 1 && (size_t)2 * P.waves * P.NP * P.rowbytes > 64 * 1024) P.waves >>= 1;
     const int G = 64 / P.LPV;
     const long per_tile = (long)P.waves * G * P.V;
     P.ntiles = ((long)B + per_tile - 1) / per_tile;
@@ -144,7 +146,7 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
         // Cap the number of dp partials; each workgroup then walks several vector tiles.
         const long cap = 2L * ncu;
         if (P.ntiles > cap) P.grid = (int)cap;
-        // LDS stage depth: up to 64 KiB per workgroup.
+        // LDS stage depth: up to 64 KiB per workgroup, at least two layers.
         const size_t per_layer = (size_t)P.waves * P.NP * P.rowbytes;
         int C = (int)((64 * 1024) / per_layer);
         if (C > 16) C = 16;

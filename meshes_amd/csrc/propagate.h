@@ -113,6 +113,18 @@ __device__ __forceinline__ float2 lane_read(const float2 x, const int src) {
     return make_float2(lane_read(x.x, src), lane_read(x.y, src));
 }
 
+// Neighbour reads through DPP (GFX9 wavefront shifts; no LDS round trip).  Lanes with no
+// neighbour (lane 63 for next, lane 0 for prev) read 0; the rows there are the identity
+// (b = 0), since they sit at the edge of the padded mesh.
+__device__ __forceinline__ float lane_next(const float x) {  // value held by lane + 1
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float lane_prev(const float x) {  // value held by lane - 1
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float2 lane_next(const float2 x) { return make_float2(lane_next(x.x), lane_next(x.y)); }
+__device__ __forceinline__ float2 lane_prev(const float2 x) { return make_float2(lane_prev(x.x), lane_prev(x.y)); }
+
 template <int W>
 struct Log2 {
     static constexpr int value = W == 1 ? 0 : W == 2 ? 1 : W == 4 ? 2 : W == 8 ? 3 : W == 16 ? 4 : -1;
@@ -159,9 +171,8 @@ __device__ __forceinline__ void dispatch_kind(const int kd, Op& op) {
 }
 
 // ---------------------------------------------------------------------------------------
-// Per-layer steps, written pair by pair and in place so that no partner copies are
-// materialised.  Odd-shift layers pair the last waveguide of each lane with the first of
-// the next lane; those two values travel through ds_bpermute.
+// Per-layer steps.  Odd-shift layers pair the last waveguide of each lane with the first
+// of the next lane; those two values travel through DPP wavefront shifts.
 // ---------------------------------------------------------------------------------------
 template <class T, int W, int V>
 __device__ __forceinline__ void copy_state(const T (&a)[V][W], T (&b)[V][W]) {
@@ -197,8 +208,8 @@ struct FwdStep {
             T hi[V], lo[V];
 #pragma unroll
             for (int v = 0; v < V; ++v) {
-                hi[v] = lane_read(u[v][0], lane + 1);
-                lo[v] = lane_read(u[v][W - 1], lane - 1);
+                hi[v] = lane_next(u[v][0]);
+                lo[v] = lane_prev(u[v][W - 1]);
             }
 #pragma unroll
             for (int v = 0; v < V; ++v) {
@@ -249,10 +260,10 @@ struct FwdDiffStep {
             T hu[V], hd[V], lu[V], ld[V];
 #pragma unroll
             for (int v = 0; v < V; ++v) {
-                hu[v] = lane_read(u[v][0], lane + 1);
-                hd[v] = lane_read(du[v][0], lane + 1);
-                lu[v] = lane_read(u[v][W - 1], lane - 1);
-                ld[v] = lane_read(du[v][W - 1], lane - 1);
+                hu[v] = lane_next(u[v][0]);
+                hd[v] = lane_next(du[v][0]);
+                lu[v] = lane_prev(u[v][W - 1]);
+                ld[v] = lane_prev(du[v][W - 1]);
             }
 #pragma unroll
             for (int v = 0; v < V; ++v) {
@@ -315,8 +326,8 @@ struct BwdStep {
         } else if constexpr (KC == 1) {
 #pragma unroll
             for (int v = 0; v < V; ++v) {
-                const T hu = lane_read(u[v][0], lane + 1), hg = lane_read(g[v][0], lane + 1);
-                const T lu = lane_read(u[v][W - 1], lane - 1), lg = lane_read(g[v][W - 1], lane - 1);
+                const T hu = lane_next(u[v][0]), hg = lane_next(g[v][0]);
+                const T lu = lane_prev(u[v][W - 1]), lg = lane_prev(g[v][W - 1]);
                 uo[v][W - 1] = rapply(ri[W - 1], u[v][W - 1], hu);
                 uo[v][0] = rapply(ri[0], u[v][0], lu);
                 go[v][W - 1] = rapply(ra[W - 1], g[v][W - 1], hg);
@@ -324,7 +335,7 @@ struct BwdStep {
             }
 #pragma unroll
             for (int v = 0; v < V; ++v) {
-                const T hx = lane_read(uo[v][0], lane + 1), lx = lane_read(uo[v][W - 1], lane - 1);
+                const T hx = lane_next(uo[v][0]), lx = lane_prev(uo[v][W - 1]);
 #pragma unroll
                 for (int k = 1; k + 1 < W; k += 2) pair(v, k, k + 1);
                 accA[W - 1] = cjfma(g[v][W - 1], uo[v][W - 1], accA[W - 1]);
