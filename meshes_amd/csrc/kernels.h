@@ -232,9 +232,10 @@ __device__ __forceinline__ void load_rows(R (&r)[W], const R* base) {
 }
 
 // ---- fwdprop --------------------------------------------------------------------------
-// Layers [l0, l0 + nl) of the row tables.  Rows of two consecutive layers live in two
-// register sets used in turn (ping-pong), so the load for layer l+1 is in flight while
-// layer l is applied and no register copies are needed between iterations.
+// Layers [l0, l0 + nl) of the row tables.  The layer loop is unrolled by four with four
+// row register sets: the rows of layer l+2 are requested while layer l is applied (two
+// layers of latency cover), and the field state alternates between two register sets so
+// that no result overwrites an input of its own layer.
 template <class T, int W, int V>
 __global__ void __launch_bounds__(256) k_fwd(int N, int l0, int nl, int B, int LPV, int NP,
                                              const int* __restrict__ kind,
@@ -245,30 +246,58 @@ __global__ void __launch_bounds__(256) k_fwd(int N, int l0, int nl, int B, int L
     T ua[V][W], ub[V][W];
     load_tile(ua, u_in, q, N, B, ldu);
     if (nl > 0) {
-        kind += l0;
+        const int* kp = kind + l0;
         const R* rp = rows + (size_t)l0 * NP + q.t * W;
-        R ra[W], rb[W];
-        int ka = kind[0];
-        load_rows(ra, rp);
+        const int last = nl - 1;
+        auto ld = [&](R (&r)[W], const int l) { load_rows(r, rp + (size_t)(l < last ? l : last) * NP); };
+        auto kd = [&](const int l) { return kp[l < last ? l : last]; };
+        R r0[W], r1[W], r2[W], r3[W];
+        int k0 = kd(0), k1 = kd(1), k2, k3;
+        ld(r0, 0);
+        ld(r1, 1);
         int l = 0;
-        for (; l + 1 < nl; l += 2) {
-            const int kb = kind[l + 1];
-            load_rows(rb, rp + (size_t)(l + 1) * NP);
+        for (; l + 4 <= nl; l += 4) {
+            k2 = kd(l + 2);
+            ld(r2, l + 2);
             {
-                FwdStep<T, W, V> st{ua, ub, ra, q.lane};
-                dispatch_kind<W>(ka, st);
+                FwdStep<T, W, V> st{ua, ub, r0, q.lane};
+                dispatch_kind<W>(k0, st);
             }
-            const int ln = (l + 2 < nl) ? l + 2 : nl - 1;
-            ka = kind[ln];
-            load_rows(ra, rp + (size_t)ln * NP);
+            k3 = kd(l + 3);
+            ld(r3, l + 3);
             {
-                FwdStep<T, W, V> st{ub, ua, rb, q.lane};
-                dispatch_kind<W>(kb, st);
+                FwdStep<T, W, V> st{ub, ua, r1, q.lane};
+                dispatch_kind<W>(k1, st);
+            }
+            k0 = kd(l + 4);
+            ld(r0, l + 4);
+            {
+                FwdStep<T, W, V> st{ua, ub, r2, q.lane};
+                dispatch_kind<W>(k2, st);
+            }
+            k1 = kd(l + 5);
+            ld(r1, l + 5);
+            {
+                FwdStep<T, W, V> st{ub, ua, r3, q.lane};
+                dispatch_kind<W>(k3, st);
             }
         }
-        if (l < nl) {
-            FwdStep<T, W, V> st{ua, ub, ra, q.lane};
-            dispatch_kind<W>(ka, st);
+        const int rem = nl - l;  // 0..3 layers left; rows of l and l+1 are in r0, r1
+        if (rem >= 1) {
+            ld(r2, l + 2);
+            k2 = kd(l + 2);
+            FwdStep<T, W, V> st{ua, ub, r0, q.lane};
+            dispatch_kind<W>(k0, st);
+        }
+        if (rem >= 2) {
+            FwdStep<T, W, V> st{ub, ua, r1, q.lane};
+            dispatch_kind<W>(k1, st);
+        }
+        if (rem >= 3) {
+            FwdStep<T, W, V> st{ua, ub, r2, q.lane};
+            dispatch_kind<W>(k2, st);
+        }
+        if (rem & 1) {
             store_tile(ub, u_out, q, N, B, ldu);
             return;
         }
@@ -392,33 +421,30 @@ __global__ void __launch_bounds__(256) k_bwd(int N, int l0, int nl, int B, int L
     R* stage = reinterpret_cast<R*>(smem_raw);  // [nw][C][W][LPV]
     const int nw = blockDim.x >> 6;
     const int NPh = NP / 2;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     float* my_partial = partial + (size_t)blockIdx.x * part_stride;
-    constexpr int PF = 2;  // stage-end slots per thread whose dT/dp is prefetched
-    const int* kd_all = kind + l0;
-    // Slot idx of the current stage -> (layer, slot) and its dT/dp values.
-    auto fetch_dtdp = [&](const int idx, const int c0, T (&dd)[8]) {
-        const int lcc = idx / NPh, m = idx - lcc * NPh;
-        const int l = nl - 1 - (c0 + lcc);
+    const int* kp = kind + l0;
+    const int last = nl - 1;
+    constexpr int PF = 2;  // stage-end slots per lane whose dT/dp is prefetched
+
+    // dT/dp of slot m of local layer l.
+    auto fetch_dtdp = [&](const int l, const int m, T (&dd)[8]) {
         const T* src = dtdp + ((size_t)(l0 + l) * NPh + m) * np * 4;
 #pragma unroll
         for (int e = 0; e < 8; ++e) dd[e] = (e < np * 4) ? src[e] : fzero(T());
     };
-    int stage_c0 = 0;
-    auto stage_slot = [&](const int idx, const T (&dd)[8], const bool first_tile) {
-        const int lcc = idx / NPh, m = idx - lcc * NPh;
-        const int l = nl - 1 - (stage_c0 + lcc);
-        const int kd = kd_all[l];
+    // Sum the waves' dJdT rows of slot m of stage layer lcc (local layer l, kind kd) and
+    // write its dp contribution into this workgroup's partial.
+    auto convert = [&](const int lcc, const int l, const int kd, const int m, const T (&dd)[8],
+                       const bool first_tile) {
         int i = -1, j = -1;
-        if (kd == 0) {
-            i = 2 * m;
-            j = i + 1;
-        } else if (kd == 1) {
-            i = 2 * m + 1;
+        if (kd <= 1) {
+            i = 2 * m + kd;
             j = i + 1;
         } else if (kd != KIND_IDENTITY) {
-            const int S = 1 << (kd - 2);
-            i = 2 * S * (m / S) + (m % S);
-            j = i + S;
+            const int st = kd - 2;
+            i = ((m >> st) << (st + 1)) + (m & ((1 << st) - 1));
+            j = i + (1 << st);
         }
         float* outp = my_partial + ((size_t)(l0 + l) * NPh + m) * np;
         if (i < 0 || j >= NP) {
@@ -426,11 +452,13 @@ __global__ void __launch_bounds__(256) k_bwd(int N, int l0, int nl, int B, int L
                 for (int k = 0; k < np; ++k) outp[k] = 0.f;
             return;
         }
-        R si = stage[(((size_t)0 * C + lcc) * W + (i % W)) * LPV + i / W];
-        R sj = stage[(((size_t)0 * C + lcc) * W + (j % W)) * LPV + j / W];
+        const R* s0 = stage + (size_t)lcc * W * LPV;
+        const size_t wstride = (size_t)C * W * LPV;
+        const int oi = (i % W) * LPV + i / W, oj = (j % W) * LPV + j / W;
+        R si = s0[oi], sj = s0[oj];
         for (int wv = 1; wv < nw; ++wv) {
-            si = row_add(si, stage[(((size_t)wv * C + lcc) * W + (i % W)) * LPV + i / W]);
-            sj = row_add(sj, stage[(((size_t)wv * C + lcc) * W + (j % W)) * LPV + j / W]);
+            si = row_add(si, s0[wv * wstride + oi]);
+            sj = row_add(sj, s0[wv * wstride + oj]);
         }
         // dJdT00 = A_i, dJdT01 = B_i, dJdT11 = A_j, dJdT10 = B_j
 #pragma unroll
@@ -444,83 +472,117 @@ __global__ void __launch_bounds__(256) k_bwd(int N, int l0, int nl, int B, int L
             }
         }
     };
+
     bool first = true;
     for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const Lanes q = lanes_of<V>(LPV, tile);
-        T u[V][W], g[V][W], ub[V][W], gb[V][W];
-        load_tile(u, u_out, q, N, B, ldu);
+        T ua[V][W], ga[V][W], ub[V][W], gb[V][W];
+        load_tile(ua, u_out, q, N, B, ldu);
         if (target != nullptr) {
             T y[V][W];
             load_tile(y, target, q, N, B, ldu);
 #pragma unroll
             for (int v = 0; v < V; ++v)
 #pragma unroll
-                for (int k = 0; k < W; ++k) g[v][k] = fsub2(u[v][k], y[v][k]);
+                for (int k = 0; k < W; ++k) ga[v][k] = fsub2(ua[v][k], y[v][k]);
         } else {
-            load_tile(g, g_out, q, N, B, ldu);
+            load_tile(ga, g_out, q, N, B, ldu);
         }
+        // Local layers are visited in the order s = 0, 1, ... as l = last - s.
         const R* rip = rows_inv + (size_t)l0 * NP + q.t * W;
         const R* rap = rows_adj + (size_t)l0 * NP + q.t * W;
-        const int* kd_p = kind + l0;
-        R ria[W], raa[W], rib[W], rab[W];
-        int ka = 0;
+        auto ldi = [&](R (&r)[W], const int s) { load_rows(r, rip + (size_t)(s < last ? last - s : 0) * NP); };
+        auto lda = [&](R (&r)[W], const int s) {
+            if constexpr (!UNI) load_rows(r, rap + (size_t)(s < last ? last - s : 0) * NP);
+        };
+        auto kd = [&](const int s) { return kp[s < last ? last - s : 0]; };
+        R i0[W], i1[W], i2[W], i3[W], a0[W], a1[W], a2[W], a3[W];
+        int k0 = 0, k1 = 0, k2 = 0, k3 = 0;
         if (nl > 0) {
-            ka = kd_p[nl - 1];
-            load_rows(ria, rip + (size_t)(nl - 1) * NP);
-            if constexpr (!UNI) load_rows(raa, rap + (size_t)(nl - 1) * NP);
+            k0 = kd(0);
+            k1 = kd(1);
+            ldi(i0, 0);
+            lda(a0, 0);
+            ldi(i1, 1);
+            lda(a1, 1);
         }
+#define MP_BWD_LAYER(KD, U, G, UO, GO, RI, RA, LC)                                                  \
+    {                                                                                               \
+        if constexpr (UNI)                                                                          \
+            bwd_layer<T, W, V, UNI>(KD, U, G, UO, GO, RI, RI, stage, q, LPV, C, LC);                \
+        else                                                                                        \
+            bwd_layer<T, W, V, UNI>(KD, U, G, UO, GO, RI, RA, stage, q, LPV, C, LC);                \
+    }
         for (int c0 = 0; c0 < nl; c0 += C) {
             const int cl = (C < nl - c0) ? C : nl - c0;
+            // Prefetch the dT/dp this wave converts at the end of the stage.
             T pre[PF][8];
+            const int lcw = wave;  // first stage layer converted by this wave
+            if (lcw < cl) {
 #pragma unroll
-            for (int r = 0; r < PF; ++r) {
-                const int idx = threadIdx.x + r * blockDim.x;
-                if (idx < cl * NPh) fetch_dtdp(idx, c0, pre[r]);
+                for (int r = 0; r < PF; ++r)
+                    if (lane + 64 * r < NPh) fetch_dtdp(last - (c0 + lcw), lane + 64 * r, pre[r]);
             }
             int lc = 0;
-            for (; lc + 1 < cl; lc += 2) {
-                const int la = nl - 1 - (c0 + lc);  // this layer, then la - 1
-                const int kb = kd_p[la - 1];
-                load_rows(rib, rip + (size_t)(la - 1) * NP);
-                if constexpr (!UNI) load_rows(rab, rap + (size_t)(la - 1) * NP);
-                if constexpr (UNI)
-                    bwd_layer<T, W, V, UNI>(ka, u, g, ub, gb, ria, ria, stage, q, LPV, C, lc);
-                else
-                    bwd_layer<T, W, V, UNI>(ka, u, g, ub, gb, ria, raa, stage, q, LPV, C, lc);
-                const int ln = (la - 2 >= 0) ? la - 2 : 0;
-                ka = kd_p[ln];
-                load_rows(ria, rip + (size_t)ln * NP);
-                if constexpr (!UNI) load_rows(raa, rap + (size_t)ln * NP);
-                if constexpr (UNI)
-                    bwd_layer<T, W, V, UNI>(kb, ub, gb, u, g, rib, rib, stage, q, LPV, C, lc + 1);
-                else
-                    bwd_layer<T, W, V, UNI>(kb, ub, gb, u, g, rib, rab, stage, q, LPV, C, lc + 1);
+            for (; lc + 4 <= cl; lc += 4) {
+                const int s = c0 + lc;
+                k2 = kd(s + 2);
+                ldi(i2, s + 2);
+                lda(a2, s + 2);
+                MP_BWD_LAYER(k0, ua, ga, ub, gb, i0, a0, lc);
+                k3 = kd(s + 3);
+                ldi(i3, s + 3);
+                lda(a3, s + 3);
+                MP_BWD_LAYER(k1, ub, gb, ua, ga, i1, a1, lc + 1);
+                k0 = kd(s + 4);
+                ldi(i0, s + 4);
+                lda(a0, s + 4);
+                MP_BWD_LAYER(k2, ua, ga, ub, gb, i2, a2, lc + 2);
+                k1 = kd(s + 5);
+                ldi(i1, s + 5);
+                lda(a1, s + 5);
+                MP_BWD_LAYER(k3, ub, gb, ua, ga, i3, a3, lc + 3);
             }
-            if (lc < cl) {  // odd length: only the last stage, so nothing further to prefetch
-                if constexpr (UNI)
-                    bwd_layer<T, W, V, UNI>(ka, u, g, ub, gb, ria, ria, stage, q, LPV, C, lc);
-                else
-                    bwd_layer<T, W, V, UNI>(ka, u, g, ub, gb, ria, raa, stage, q, LPV, C, lc);
-                copy_state(ub, u);
-                copy_state(gb, g);
+            const int rem = cl - lc;  // only in the last stage
+            if (rem >= 1) {
+                k2 = kd(c0 + lc + 2);
+                ldi(i2, c0 + lc + 2);
+                lda(a2, c0 + lc + 2);
+                MP_BWD_LAYER(k0, ua, ga, ub, gb, i0, a0, lc);
+            }
+            if (rem >= 2) MP_BWD_LAYER(k1, ub, gb, ua, ga, i1, a1, lc + 1);
+            if (rem >= 3) MP_BWD_LAYER(k2, ua, ga, ub, gb, i2, a2, lc + 2);
+            if (rem & 1) {
+                copy_state(ub, ua);
+                copy_state(gb, ga);
             }
             __syncthreads();
-            stage_c0 = c0;
-            // dJdT -> dp for the cl layers of this stage (dT/dp of the first slots prefetched).
+            // dJdT -> dp: wave w converts stage layers w, w + nw, ...
+            for (int lcc = wave; lcc < cl; lcc += nw) {
+                const int l = last - (c0 + lcc);
+                const int kdl = kp[l];
+                if (lcc == lcw) {
 #pragma unroll
-            for (int r = 0; r < PF; ++r) {
-                const int idx = threadIdx.x + r * blockDim.x;
-                if (idx < cl * NPh) stage_slot(idx, pre[r], first);
-            }
-            for (int idx = threadIdx.x + PF * blockDim.x; idx < cl * NPh; idx += blockDim.x) {
-                T dd[8];
-                fetch_dtdp(idx, c0, dd);
-                stage_slot(idx, dd, first);
+                    for (int r = 0; r < PF; ++r)
+                        if (lane + 64 * r < NPh) convert(lcc, l, kdl, lane + 64 * r, pre[r], first);
+                    for (int m = lane + 64 * PF; m < NPh; m += 64) {
+                        T dd[8];
+                        fetch_dtdp(l, m, dd);
+                        convert(lcc, l, kdl, m, dd, first);
+                    }
+                } else {
+                    for (int m = lane; m < NPh; m += 64) {
+                        T dd[8];
+                        fetch_dtdp(l, m, dd);
+                        convert(lcc, l, kdl, m, dd, first);
+                    }
+                }
             }
             __syncthreads();
         }
-        if (u_in != nullptr) store_tile(u, u_in, q, N, B, ldu);
-        if (g_in != nullptr) store_tile(g, g_in, q, N, B, ldu);
+#undef MP_BWD_LAYER
+        if (u_in != nullptr) store_tile(ua, u_in, q, N, B, ldu);
+        if (g_in != nullptr) store_tile(ga, g_in, q, N, B, ldu);
         first = false;
     }
 }

@@ -135,8 +135,8 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
     resolve_tile(op, N, B, tile, ncu, P.W, P.V, P.LPV, P.waves, P.C);
     P.NP = P.LPV * P.W;
     P.rowbytes = P.real ? sizeof(float2) : sizeof(float4);
-    if (op == MESHPROP_OP_BACKDIFF)  // the LDS stage must hold two layers of every wave
-        while (P.waves > 1 && (size_t)2 * P.waves * P.NP * P.rowbytes > 64 * 1024) P.waves >>= 1;
+    if (op == MESHPROP_OP_BACKDIFF)  // the LDS stage must hold four layers of every wave
+        while (P.waves > 1 && (size_t)4 * P.waves * P.NP * P.rowbytes > 64 * 1024) P.waves >>= 1;
     const int G = 64 / P.LPV;
     const long per_tile = (long)P.waves * G * P.V;
     P.ntiles = ((long)B + per_tile - 1) / per_tile;
@@ -151,8 +151,8 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
         int C = (int)((64 * 1024) / per_layer);
         if (C > 16) C = 16;
         if (P.C >= 1 && (size_t)P.C * per_layer <= 64 * 1024) C = P.C;
-        C &= ~1;  // the backdiff stage walks layers in pairs
-        if (C < 2) C = 2;
+        C &= ~3;  // the backdiff loop is unrolled by four layers
+        if (C < 4) C = 4;
         if ((size_t)C * per_layer > 64 * 1024) return MESHPROP_ERR_UNSUPPORTED;
         P.C = C;
         P.nseg = P.grid < 16 ? P.grid : 16;
