@@ -73,8 +73,12 @@ __device__ __forceinline__ void row_of(const M2& M, const bool upper, dc& a, dc&
 }
 
 // One thread per (layer, waveguide) of the padded width NP.
+// Row tables are stored k-major inside each layer: row of waveguide w = t*W + k (lane t of a
+// lane group, k-th waveguide of that lane) sits at [l][k][t], so the k-th row load of all
+// lanes reads one contiguous block.
 template <class R>
-__global__ void k_rows(int X, int N, int L, int NP, int fft, const int* __restrict__ g0, const int* __restrict__ g1,
+__global__ void k_rows(int X, int N, int L, int NP, int LPV, int fft, const int* __restrict__ g0,
+                       const int* __restrict__ g1,
                        const float* __restrict__ p, int ldp, const float* __restrict__ s, int lds,
                        const float* __restrict__ dp, int tabs, int* __restrict__ kind, R* __restrict__ tab_a,
                        R* __restrict__ tab_b) {
@@ -82,7 +86,8 @@ __global__ void k_rows(int X, int N, int L, int NP, int fft, const int* __restri
     const int l = blockIdx.y;
     if (l >= L || w >= NP) return;
     if (w == 0) kind[l] = layer_kind(fft, g0, g1, l, NP);
-    const size_t o = (size_t)l * NP + w;
+    const int Wd = NP / LPV;
+    const size_t o = (size_t)l * NP + (size_t)(w % Wd) * LPV + w / Wd;
     int m = 0;
     bool upper = true;
     const bool act = wg_role(fft, g0, g1, l, w, N, m, upper);
@@ -123,8 +128,8 @@ __global__ void k_rows(int X, int N, int L, int NP, int fft, const int* __restri
 }
 
 // dT/dp_k of every slot (zero for inactive slots), for the in-kernel dJdT -> dp step.
-// Layout: dtdp[((l*NPh + m)*np + k)*4 + e] with e = 00, 01, 10, 11, element type
-// float2 (complex) or float (real).
+// Layout: dtdp[(l*np*4 + k*4 + e)*NPh + m] with e = 00, 01, 10, 11 (slot-contiguous), element
+// type float2 (complex) or float (real).
 template <class T>
 __device__ __forceinline__ T dc_to(const dc z);
 template <>
@@ -145,7 +150,8 @@ __global__ void k_dtdp(int X, int N, int L, int NP, int fft, const int* __restri
     const int NPh = NP / 2;
     if (l >= L || m >= NPh) return;
     const int np = n_phase_of(X), ns = n_split_of(X);
-    T* out = dtdp + ((size_t)l * NPh + m) * np * 4;
+    // Layout [l][e][m] (e = param*4 + entry) so that consecutive slots are contiguous.
+    T* out = dtdp + (size_t)l * NPh * np * 4 + m;
     // Upper waveguide of slot m, then test whether the slot is active.
     int i;
     if (fft) {
@@ -162,7 +168,7 @@ __global__ void k_dtdp(int X, int N, int L, int NP, int fft, const int* __restri
     bool upper = false;
     const bool act = (i >= 0) && wg_role(fft, g0, g1, l, i, N, mm, upper) && upper && mm == m;
     if (!act) {
-        for (int k = 0; k < np * 4; ++k) out[k] = dc_to<T>(mk(0, 0));
+        for (int k = 0; k < np * 4; ++k) out[(size_t)k * NPh] = dc_to<T>(mk(0, 0));
         return;
     }
     const float* pp = p + (size_t)l * ldp + (size_t)m * np;
@@ -170,10 +176,10 @@ __global__ void k_dtdp(int X, int N, int L, int NP, int fft, const int* __restri
     M2 Tm, dT[2];
     crossing_eval(X, pp, sp, &Tm, dT);
     for (int k = 0; k < np; ++k) {
-        out[k * 4 + 0] = dc_to<T>(dT[k].a00);
-        out[k * 4 + 1] = dc_to<T>(dT[k].a01);
-        out[k * 4 + 2] = dc_to<T>(dT[k].a10);
-        out[k * 4 + 3] = dc_to<T>(dT[k].a11);
+        out[(size_t)(k * 4 + 0) * NPh] = dc_to<T>(dT[k].a00);
+        out[(size_t)(k * 4 + 1) * NPh] = dc_to<T>(dT[k].a01);
+        out[(size_t)(k * 4 + 2) * NPh] = dc_to<T>(dT[k].a10);
+        out[(size_t)(k * 4 + 3) * NPh] = dc_to<T>(dT[k].a11);
     }
 }
 
@@ -226,9 +232,9 @@ __device__ __forceinline__ void store_tile(const T (&u)[V][W], T* dst, const Lan
 }
 
 template <class R, int W>
-__device__ __forceinline__ void load_rows(R (&r)[W], const R* base) {
+__device__ __forceinline__ void load_rows(R (&r)[W], const R* base, const int LPV) {
 #pragma unroll
-    for (int k = 0; k < W; ++k) r[k] = base[k];
+    for (int k = 0; k < W; ++k) r[k] = base[(size_t)k * LPV];
 }
 
 // ---- fwdprop --------------------------------------------------------------------------
@@ -247,9 +253,9 @@ __global__ void __launch_bounds__(256) k_fwd(int N, int l0, int nl, int B, int L
     load_tile(ua, u_in, q, N, B, ldu);
     if (nl > 0) {
         const int* kp = kind + l0;
-        const R* rp = rows + (size_t)l0 * NP + q.t * W;
+        const R* rp = rows + (size_t)l0 * NP + q.t;
         const int last = nl - 1;
-        auto ld = [&](R (&r)[W], const int l) { load_rows(r, rp + (size_t)(l < last ? l : last) * NP); };
+        auto ld = [&](R (&r)[W], const int l) { load_rows(r, rp + (size_t)(l < last ? l : last) * NP, LPV); };
         auto kd = [&](const int l) { return kp[l < last ? l : last]; };
         R r0[W], r1[W], r2[W], r3[W];
         int k0 = kd(0), k1 = kd(1), k2, k3;
@@ -324,25 +330,25 @@ __global__ void __launch_bounds__(256) k_fwddiff(int N, int L, int B, int LPV, i
             for (int k = 0; k < W; ++k) du[v][k] = fzero(T());
     }
     if (L > 0) {
-        const R* rp = rows + q.t * W;
-        const R* dq = drows + q.t * W;
+        const R* rp = rows + q.t;
+        const R* dq = drows + q.t;
         R ra[W], da[W], rb[W], db[W];
         int ka = kind[0];
-        load_rows(ra, rp);
-        load_rows(da, dq);
+        load_rows(ra, rp, LPV);
+        load_rows(da, dq, LPV);
         int l = 0;
         for (; l + 1 < L; l += 2) {
             const int kb = kind[l + 1];
-            load_rows(rb, rp + (size_t)(l + 1) * NP);
-            load_rows(db, dq + (size_t)(l + 1) * NP);
+            load_rows(rb, rp + (size_t)(l + 1) * NP, LPV);
+            load_rows(db, dq + (size_t)(l + 1) * NP, LPV);
             {
                 FwdDiffStep<T, W, V> st{u, du, ub, dub, ra, da, q.lane};
                 dispatch_kind<W>(ka, st);
             }
             const int ln = (l + 2 < L) ? l + 2 : L - 1;
             ka = kind[ln];
-            load_rows(ra, rp + (size_t)ln * NP);
-            load_rows(da, dq + (size_t)ln * NP);
+            load_rows(ra, rp + (size_t)ln * NP, LPV);
+            load_rows(da, dq + (size_t)ln * NP, LPV);
             {
                 FwdDiffStep<T, W, V> st{ub, dub, u, du, rb, db, q.lane};
                 dispatch_kind<W>(kb, st);
@@ -429,9 +435,9 @@ __global__ void __launch_bounds__(256) k_bwd(int N, int l0, int nl, int B, int L
 
     // dT/dp of slot m of local layer l.
     auto fetch_dtdp = [&](const int l, const int m, T (&dd)[8]) {
-        const T* src = dtdp + ((size_t)(l0 + l) * NPh + m) * np * 4;
+        const T* src = dtdp + (size_t)(l0 + l) * NPh * np * 4 + m;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) dd[e] = (e < np * 4) ? src[e] : fzero(T());
+        for (int e = 0; e < 8; ++e) dd[e] = (e < np * 4) ? src[(size_t)e * NPh] : fzero(T());
     };
     // Sum the waves' dJdT rows of slot m of stage layer lcc (local layer l, kind kd) and
     // write its dp contribution into this workgroup's partial.
@@ -468,7 +474,7 @@ __global__ void __launch_bounds__(256) k_bwd(int N, int l0, int nl, int B, int L
                 val += re_dot(row_b(si), dd[k * 4 + 1]);
                 val += re_dot(row_b(sj), dd[k * 4 + 2]);
                 val += re_dot(row_a(sj), dd[k * 4 + 3]);
-                outp[k] = first_tile ? val : outp[k] + val;
+                outp[k] = first_tile ? val : outp[k] + val;  // np <= 2: adjacent floats
             }
         }
     };
@@ -489,11 +495,11 @@ __global__ void __launch_bounds__(256) k_bwd(int N, int l0, int nl, int B, int L
             load_tile(ga, g_out, q, N, B, ldu);
         }
         // Local layers are visited in the order s = 0, 1, ... as l = last - s.
-        const R* rip = rows_inv + (size_t)l0 * NP + q.t * W;
-        const R* rap = rows_adj + (size_t)l0 * NP + q.t * W;
-        auto ldi = [&](R (&r)[W], const int s) { load_rows(r, rip + (size_t)(s < last ? last - s : 0) * NP); };
+        const R* rip = rows_inv + (size_t)l0 * NP + q.t;
+        const R* rap = rows_adj + (size_t)l0 * NP + q.t;
+        auto ldi = [&](R (&r)[W], const int s) { load_rows(r, rip + (size_t)(s < last ? last - s : 0) * NP, LPV); };
         auto lda = [&](R (&r)[W], const int s) {
-            if constexpr (!UNI) load_rows(r, rap + (size_t)(s < last ? last - s : 0) * NP);
+            if constexpr (!UNI) load_rows(r, rap + (size_t)(s < last ? last - s : 0) * NP, LPV);
         };
         auto kd = [&](const int s) { return kp[s < last ? last - s : 0]; };
         R i0[W], i1[W], i2[W], i3[W], a0[W], a1[W], a2[W], a3[W];

@@ -234,10 +234,10 @@ int run_impl(int op, int X, int N, int L, int B, int fft, const int* g0, const i
         if (op == MESHPROP_OP_BACKDIFF) tabs = TAB_INV | (P.uni ? 0 : TAB_ADJ);
         dim3 grid((P.NP + 255) / 256, L);
         if (P.real)
-            hipLaunchKernelGGL((k_rows<float2>), grid, dim3(256), 0, st, X, N, L, P.NP, fft, g0, g1, p, ldp, sp, lds,
+            hipLaunchKernelGGL((k_rows<float2>), grid, dim3(256), 0, st, X, N, L, P.NP, P.LPV, fft, g0, g1, p, ldp, sp, lds,
                                (const float*)dp, tabs, kind, (float2*)tab_a, (float2*)tab_b);
         else
-            hipLaunchKernelGGL((k_rows<float4>), grid, dim3(256), 0, st, X, N, L, P.NP, fft, g0, g1, p, ldp, sp, lds,
+            hipLaunchKernelGGL((k_rows<float4>), grid, dim3(256), 0, st, X, N, L, P.NP, P.LPV, fft, g0, g1, p, ldp, sp, lds,
                                (const float*)dp, tabs, kind, (float4*)tab_a, (float4*)tab_b);
         if ((rc = check_launch())) return rc;
         if (op == MESHPROP_OP_BACKDIFF) {
