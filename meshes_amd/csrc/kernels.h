@@ -238,10 +238,9 @@ __device__ __forceinline__ void load_rows(R (&r)[W], const R* base, const int LP
 }
 
 // ---- fwdprop --------------------------------------------------------------------------
-// Layers [l0, l0 + nl) of the row tables.  The layer loop is unrolled by four with four
-// row register sets: the rows of layer l+2 are requested while layer l is applied (two
-// layers of latency cover), and the field state alternates between two register sets so
-// that no result overwrites an input of its own layer.
+// Layers [l0, l0 + nl) of the row tables.  The layer loop is unrolled by two with two row
+// register sets (rows of layer l+1 are in flight while layer l is applied) and two field
+// register sets, so that no result overwrites an input of its own layer.
 template <class T, int W, int V>
 __global__ void __launch_bounds__(256) k_fwd(int N, int l0, int nl, int B, int LPV, int NP,
                                              const int* __restrict__ kind,
@@ -254,56 +253,28 @@ __global__ void __launch_bounds__(256) k_fwd(int N, int l0, int nl, int B, int L
     if (nl > 0) {
         const int* kp = kind + l0;
         const R* rp = rows + (size_t)l0 * NP + q.t;
-        const int last = nl - 1;
-        auto ld = [&](R (&r)[W], const int l) { load_rows(r, rp + (size_t)(l < last ? l : last) * NP, LPV); };
-        auto kd = [&](const int l) { return kp[l < last ? l : last]; };
-        R r0[W], r1[W], r2[W], r3[W];
-        int k0 = kd(0), k1 = kd(1), k2, k3;
-        ld(r0, 0);
-        ld(r1, 1);
+        R ra[W], rb[W];
+        int ka = kp[0];
+        load_rows(ra, rp, LPV);
         int l = 0;
-        for (; l + 4 <= nl; l += 4) {
-            k2 = kd(l + 2);
-            ld(r2, l + 2);
+        for (; l + 1 < nl; l += 2) {
+            const int kb = kp[l + 1];
+            load_rows(rb, rp + (size_t)(l + 1) * NP, LPV);
             {
-                FwdStep<T, W, V> st{ua, ub, r0, q.lane};
-                dispatch_kind<W>(k0, st);
+                FwdStep<T, W, V> st{ua, ub, ra, q.lane};
+                dispatch_kind<W>(ka, st);
             }
-            k3 = kd(l + 3);
-            ld(r3, l + 3);
+            const int ln = (l + 2 < nl) ? l + 2 : nl - 1;
+            ka = kp[ln];
+            load_rows(ra, rp + (size_t)ln * NP, LPV);
             {
-                FwdStep<T, W, V> st{ub, ua, r1, q.lane};
-                dispatch_kind<W>(k1, st);
-            }
-            k0 = kd(l + 4);
-            ld(r0, l + 4);
-            {
-                FwdStep<T, W, V> st{ua, ub, r2, q.lane};
-                dispatch_kind<W>(k2, st);
-            }
-            k1 = kd(l + 5);
-            ld(r1, l + 5);
-            {
-                FwdStep<T, W, V> st{ub, ua, r3, q.lane};
-                dispatch_kind<W>(k3, st);
+                FwdStep<T, W, V> st{ub, ua, rb, q.lane};
+                dispatch_kind<W>(kb, st);
             }
         }
-        const int rem = nl - l;  // 0..3 layers left; rows of l and l+1 are in r0, r1
-        if (rem >= 1) {
-            ld(r2, l + 2);
-            k2 = kd(l + 2);
-            FwdStep<T, W, V> st{ua, ub, r0, q.lane};
-            dispatch_kind<W>(k0, st);
-        }
-        if (rem >= 2) {
-            FwdStep<T, W, V> st{ub, ua, r1, q.lane};
-            dispatch_kind<W>(k1, st);
-        }
-        if (rem >= 3) {
-            FwdStep<T, W, V> st{ua, ub, r2, q.lane};
-            dispatch_kind<W>(k2, st);
-        }
-        if (rem & 1) {
+        if (l < nl) {
+            FwdStep<T, W, V> st{ua, ub, ra, q.lane};
+            dispatch_kind<W>(ka, st);
             store_tile(ub, u_out, q, N, B, ldu);
             return;
         }
@@ -414,70 +385,90 @@ __device__ __forceinline__ void bwd_layer(const int kd, const T (&u)[V][W], cons
     }
 }
 
+// Stage-end conversion of one slot: sum the waves' dJdT rows of slot m (layer kind kd,
+// wave-uniform) from the stage layer s0 and add its dp contribution to pout[m*np ...].
+template <class T, int W>
+__device__ __forceinline__ void convert_slot(const typename RowOf<T>::type* s0, const int wstride, const int nw,
+                                             const int LPV, const int NP, const int np, float* pout, const int kd,
+                                             const int m, const T (&dd)[8], const bool first_tile) {
+    typedef typename RowOf<T>::type R;
+    int i, j;
+    if (kd <= 1) {
+        i = 2 * m + kd;
+        j = i + 1;
+    } else if (kd != KIND_IDENTITY) {
+        const int st = kd - 2;
+        i = ((m >> st) << (st + 1)) + (m & ((1 << st) - 1));
+        j = i + (1 << st);
+    } else {
+        i = -1;
+        j = 0;
+    }
+    float* outp = pout + m * np;
+    if (i < 0 || j >= NP) {
+        if (first_tile)
+            for (int k = 0; k < np; ++k) outp[k] = 0.f;
+        return;
+    }
+    const int oi = (i % W) * LPV + i / W, oj = (j % W) * LPV + j / W;
+    R si = s0[oi], sj = s0[oj];
+    for (int wv = 1; wv < nw; ++wv) {
+        si = row_add(si, s0[wv * wstride + oi]);
+        sj = row_add(sj, s0[wv * wstride + oj]);
+    }
+    // dJdT00 = A_i, dJdT01 = B_i, dJdT11 = A_j, dJdT10 = B_j
+    float val[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        val[k] = re_dot(row_a(si), dd[k * 4 + 0]);
+        val[k] += re_dot(row_b(si), dd[k * 4 + 1]);
+        val[k] += re_dot(row_b(sj), dd[k * 4 + 2]);
+        val[k] += re_dot(row_a(sj), dd[k * 4 + 3]);
+    }
+    if (np == 2) {
+        float2* o2 = reinterpret_cast<float2*>(outp);
+        if (first_tile) {
+            *o2 = make_float2(val[0], val[1]);
+        } else {
+            const float2 old = *o2;
+            *o2 = make_float2(old.x + val[0], old.y + val[1]);
+        }
+    } else {
+        outp[0] = first_tile ? val[0] : outp[0] + val[0];
+    }
+}
+
+// dT/dp of slot m of (absolute) layer la; table layout [layer][param*4 + entry][slot].
+template <class T>
+__device__ __forceinline__ void fetch_dtdp(const T* dtdp, const int NPh, const int np, const int la, const int m,
+                                           T (&dd)[8]) {
+    const T* src = dtdp + (size_t)la * NPh * np * 4 + m;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) dd[e] = (e < np * 4) ? src[e * NPh] : fzero(T());
+}
+
+// Backdiff workgroups may hold up to eight waves when the tile is small enough to leave
+// 256 VGPRs per wave spill-free; larger tiles keep four.
+__host__ __device__ constexpr int bwd_max_waves(int W, int V, bool uni) { return (uni && W * V <= 8) ? 8 : 4; }
+
 template <class T, int W, int V, bool UNI>
-__global__ void __launch_bounds__(256) k_bwd(int N, int l0, int nl, int B, int LPV, int NP, int C, int np,
-                                             const int* __restrict__ kind,
-                                             const typename RowOf<T>::type* __restrict__ rows_inv,
-                                             const typename RowOf<T>::type* __restrict__ rows_adj,
-                                             const T* __restrict__ dtdp, float* __restrict__ partial,
-                                             size_t part_stride, long ntiles, const T* u_out, const T* g_out,
-                                             const T* target, T* u_in, T* g_in, int ldu) {
+__global__ void __launch_bounds__(64 * bwd_max_waves(W, V, UNI)) k_bwd(
+    int N, int l0, int nl, int B, int LPV, int NP, int C, int np, const int* __restrict__ kind,
+    const typename RowOf<T>::type* __restrict__ rows_inv, const typename RowOf<T>::type* __restrict__ rows_adj,
+    const T* __restrict__ dtdp, float* __restrict__ partial, size_t part_stride, long ntiles, const T* u_out,
+    const T* g_out, const T* target, T* u_in, T* g_in, int ldu) {
     typedef typename RowOf<T>::type R;
     extern __shared__ unsigned char smem_raw[];
     R* stage = reinterpret_cast<R*>(smem_raw);  // [nw][C][W][LPV]
     const int nw = blockDim.x >> 6;
     const int NPh = NP / 2;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int wave_u = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave index as a scalar
+    const int wstride = C * W * LPV;
     float* my_partial = partial + (size_t)blockIdx.x * part_stride;
     const int* kp = kind + l0;
     const int last = nl - 1;
     constexpr int PF = 2;  // stage-end slots per lane whose dT/dp is prefetched
-
-    // dT/dp of slot m of local layer l.
-    auto fetch_dtdp = [&](const int l, const int m, T (&dd)[8]) {
-        const T* src = dtdp + (size_t)(l0 + l) * NPh * np * 4 + m;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) dd[e] = (e < np * 4) ? src[(size_t)e * NPh] : fzero(T());
-    };
-    // Sum the waves' dJdT rows of slot m of stage layer lcc (local layer l, kind kd) and
-    // write its dp contribution into this workgroup's partial.
-    auto convert = [&](const int lcc, const int l, const int kd, const int m, const T (&dd)[8],
-                       const bool first_tile) {
-        int i = -1, j = -1;
-        if (kd <= 1) {
-            i = 2 * m + kd;
-            j = i + 1;
-        } else if (kd != KIND_IDENTITY) {
-            const int st = kd - 2;
-            i = ((m >> st) << (st + 1)) + (m & ((1 << st) - 1));
-            j = i + (1 << st);
-        }
-        float* outp = my_partial + ((size_t)(l0 + l) * NPh + m) * np;
-        if (i < 0 || j >= NP) {
-            if (first_tile)
-                for (int k = 0; k < np; ++k) outp[k] = 0.f;
-            return;
-        }
-        const R* s0 = stage + (size_t)lcc * W * LPV;
-        const size_t wstride = (size_t)C * W * LPV;
-        const int oi = (i % W) * LPV + i / W, oj = (j % W) * LPV + j / W;
-        R si = s0[oi], sj = s0[oj];
-        for (int wv = 1; wv < nw; ++wv) {
-            si = row_add(si, s0[wv * wstride + oi]);
-            sj = row_add(sj, s0[wv * wstride + oj]);
-        }
-        // dJdT00 = A_i, dJdT01 = B_i, dJdT11 = A_j, dJdT10 = B_j
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            if (k < np) {
-                float val = re_dot(row_a(si), dd[k * 4 + 0]);
-                val += re_dot(row_b(si), dd[k * 4 + 1]);
-                val += re_dot(row_b(sj), dd[k * 4 + 2]);
-                val += re_dot(row_a(sj), dd[k * 4 + 3]);
-                outp[k] = first_tile ? val : outp[k] + val;  // np <= 2: adjacent floats
-            }
-        }
-    };
 
     bool first = true;
     for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -494,23 +485,15 @@ __global__ void __launch_bounds__(256) k_bwd(int N, int l0, int nl, int B, int L
         } else {
             load_tile(ga, g_out, q, N, B, ldu);
         }
-        // Local layers are visited in the order s = 0, 1, ... as l = last - s.
+        // Local layers are visited in the order s = 0, 1, ..., as l = last - s.
         const R* rip = rows_inv + (size_t)l0 * NP + q.t;
         const R* rap = rows_adj + (size_t)l0 * NP + q.t;
-        auto ldi = [&](R (&r)[W], const int s) { load_rows(r, rip + (size_t)(s < last ? last - s : 0) * NP, LPV); };
-        auto lda = [&](R (&r)[W], const int s) {
-            if constexpr (!UNI) load_rows(r, rap + (size_t)(s < last ? last - s : 0) * NP, LPV);
-        };
-        auto kd = [&](const int s) { return kp[s < last ? last - s : 0]; };
-        R i0[W], i1[W], i2[W], i3[W], a0[W], a1[W], a2[W], a3[W];
-        int k0 = 0, k1 = 0, k2 = 0, k3 = 0;
+        R ia[W], ib[W], aa[W], ab[W];
+        int ka = 0;
         if (nl > 0) {
-            k0 = kd(0);
-            k1 = kd(1);
-            ldi(i0, 0);
-            lda(a0, 0);
-            ldi(i1, 1);
-            lda(a1, 1);
+            ka = kp[last];
+            load_rows(ia, rip + (size_t)last * NP, LPV);
+            if constexpr (!UNI) load_rows(aa, rap + (size_t)last * NP, LPV);
         }
 #define MP_BWD_LAYER(KD, U, G, UO, GO, RI, RA, LC)                                                  \
     {                                                                                               \
@@ -521,66 +504,53 @@ __global__ void __launch_bounds__(256) k_bwd(int N, int l0, int nl, int B, int L
     }
         for (int c0 = 0; c0 < nl; c0 += C) {
             const int cl = (C < nl - c0) ? C : nl - c0;
-            // Prefetch the dT/dp this wave converts at the end of the stage.
+            // Prefetch the dT/dp this wave converts first at the end of the stage.
             T pre[PF][8];
-            const int lcw = wave;  // first stage layer converted by this wave
-            if (lcw < cl) {
+            if (wave_u < cl) {
 #pragma unroll
                 for (int r = 0; r < PF; ++r)
-                    if (lane + 64 * r < NPh) fetch_dtdp(last - (c0 + lcw), lane + 64 * r, pre[r]);
+                    if (lane + 64 * r < NPh) fetch_dtdp(dtdp, NPh, np, l0 + last - (c0 + wave_u), lane + 64 * r, pre[r]);
             }
             int lc = 0;
-            for (; lc + 4 <= cl; lc += 4) {
-                const int s = c0 + lc;
-                k2 = kd(s + 2);
-                ldi(i2, s + 2);
-                lda(a2, s + 2);
-                MP_BWD_LAYER(k0, ua, ga, ub, gb, i0, a0, lc);
-                k3 = kd(s + 3);
-                ldi(i3, s + 3);
-                lda(a3, s + 3);
-                MP_BWD_LAYER(k1, ub, gb, ua, ga, i1, a1, lc + 1);
-                k0 = kd(s + 4);
-                ldi(i0, s + 4);
-                lda(a0, s + 4);
-                MP_BWD_LAYER(k2, ua, ga, ub, gb, i2, a2, lc + 2);
-                k1 = kd(s + 5);
-                ldi(i1, s + 5);
-                lda(a1, s + 5);
-                MP_BWD_LAYER(k3, ub, gb, ua, ga, i3, a3, lc + 3);
+            for (; lc + 1 < cl; lc += 2) {
+                const int s = c0 + lc;  // layers last - s and last - s - 1
+                const int kb = kp[last - s - 1];
+                load_rows(ib, rip + (size_t)(last - s - 1) * NP, LPV);
+                if constexpr (!UNI) load_rows(ab, rap + (size_t)(last - s - 1) * NP, LPV);
+                MP_BWD_LAYER(ka, ua, ga, ub, gb, ia, aa, lc);
+                const int ln = (last - s - 2 >= 0) ? last - s - 2 : 0;
+                ka = kp[ln];
+                load_rows(ia, rip + (size_t)ln * NP, LPV);
+                if constexpr (!UNI) load_rows(aa, rap + (size_t)ln * NP, LPV);
+                MP_BWD_LAYER(kb, ub, gb, ua, ga, ib, ab, lc + 1);
             }
-            const int rem = cl - lc;  // only in the last stage
-            if (rem >= 1) {
-                k2 = kd(c0 + lc + 2);
-                ldi(i2, c0 + lc + 2);
-                lda(a2, c0 + lc + 2);
-                MP_BWD_LAYER(k0, ua, ga, ub, gb, i0, a0, lc);
-            }
-            if (rem >= 2) MP_BWD_LAYER(k1, ub, gb, ua, ga, i1, a1, lc + 1);
-            if (rem >= 3) MP_BWD_LAYER(k2, ua, ga, ub, gb, i2, a2, lc + 2);
-            if (rem & 1) {
+            if (lc < cl) {  // odd length: only the last stage
+                MP_BWD_LAYER(ka, ua, ga, ub, gb, ia, aa, lc);
                 copy_state(ub, ua);
                 copy_state(gb, ga);
             }
             __syncthreads();
             // dJdT -> dp: wave w converts stage layers w, w + nw, ...
-            for (int lcc = wave; lcc < cl; lcc += nw) {
-                const int l = last - (c0 + lcc);
-                const int kdl = kp[l];
-                if (lcc == lcw) {
+            for (int lcc = wave_u; lcc < cl; lcc += nw) {
+                const int la = l0 + last - (c0 + lcc);
+                const int kdl = kind[la];
+                const R* s0 = stage + lcc * W * LPV;
+                float* pout = my_partial + (size_t)la * NPh * np;
+                if (lcc == wave_u) {
 #pragma unroll
                     for (int r = 0; r < PF; ++r)
-                        if (lane + 64 * r < NPh) convert(lcc, l, kdl, lane + 64 * r, pre[r], first);
+                        if (lane + 64 * r < NPh)
+                            convert_slot<T, W>(s0, wstride, nw, LPV, NP, np, pout, kdl, lane + 64 * r, pre[r], first);
                     for (int m = lane + 64 * PF; m < NPh; m += 64) {
                         T dd[8];
-                        fetch_dtdp(l, m, dd);
-                        convert(lcc, l, kdl, m, dd, first);
+                        fetch_dtdp(dtdp, NPh, np, la, m, dd);
+                        convert_slot<T, W>(s0, wstride, nw, LPV, NP, np, pout, kdl, m, dd, first);
                     }
                 } else {
                     for (int m = lane; m < NPh; m += 64) {
                         T dd[8];
-                        fetch_dtdp(l, m, dd);
-                        convert(lcc, l, kdl, m, dd, first);
+                        fetch_dtdp(dtdp, NPh, np, la, m, dd);
+                        convert_slot<T, W>(s0, wstride, nw, LPV, NP, np, pout, kdl, m, dd, first);
                     }
                 }
             }

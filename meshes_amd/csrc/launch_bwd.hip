@@ -10,6 +10,14 @@ static void go(const Plan& P, const int* kind, const void* ri, const void* ra, c
                void* g_in, int ldu, hipStream_t st) {
     typedef typename RowOf<T>::type R;
     const size_t shmem = (size_t)P.waves * P.C * P.NP * P.rowbytes;
+    if (shmem > 64 * 1024) {  // beyond the default dynamic LDS limit of a launch
+        if (P.uni)
+            hipFuncSetAttribute((const void*)k_bwd<T, W, V, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)shmem);
+        else
+            hipFuncSetAttribute((const void*)k_bwd<T, W, V, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)shmem);
+    }
     if (P.uni)
         hipLaunchKernelGGL((k_bwd<T, W, V, true>), dim3(P.grid), dim3(64 * P.waves), shmem, st, P.N, l0, nl, P.B,
                            P.LPV, P.NP, P.C, P.np, kind, (const R*)ri, (const R*)ra, (const T*)dtdp, partial, ps,

@@ -71,6 +71,12 @@ thread_local int g_last_hip_error = 0;
 
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// LDS of the backdiff stage.  gfx950 gives a workgroup up to 160 KiB; workgroups of up to
+// four waves keep to 64 KiB so that two of them share a CU.
+constexpr size_t kStageBytes = 64 * 1024;
+constexpr size_t kStageBytesWide = 128 * 1024;
+inline size_t stage_budget(int waves) { return waves > 4 ? kStageBytesWide : kStageBytes; }
+
 int num_cus() {
     int dev = 0, n = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 256;
@@ -98,7 +104,7 @@ void resolve_tile(int op, int N, int B, const meshprop_tile* t, int ncu, int& W,
     if (t && valid_w(t->w) && (long)t->w * 64 >= N) W = t->w;
     LPV = pow2ceil((N + W - 1) / W);
     const int G = 64 / LPV;
-    waves = (t && t->waves >= 1 && t->waves <= 4) ? t->waves : 4;
+    waves = (t && t->waves >= 1 && t->waves <= 8) ? t->waves : 4;
     if (t && t->v >= 1 && tile_ok(op, W, t->v) && (t->v & (t->v - 1)) == 0) {
         V = t->v;
     } else {
@@ -135,8 +141,12 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
     resolve_tile(op, N, B, tile, ncu, P.W, P.V, P.LPV, P.waves, P.C);
     P.NP = P.LPV * P.W;
     P.rowbytes = P.real ? sizeof(float2) : sizeof(float4);
-    if (op == MESHPROP_OP_BACKDIFF)  // the LDS stage must hold four layers of every wave
-        while (P.waves > 1 && (size_t)4 * P.waves * P.NP * P.rowbytes > 64 * 1024) P.waves >>= 1;
+    {
+        const int cap = (op == MESHPROP_OP_BACKDIFF) ? bwd_max_waves(P.W, P.V, P.uni) : 4;
+        if (P.waves > cap) P.waves = cap;
+    }
+    if (op == MESHPROP_OP_BACKDIFF)  // the LDS stage must hold two layers of every wave
+        while (P.waves > 1 && (size_t)2 * P.waves * P.NP * P.rowbytes > stage_budget(P.waves)) P.waves >>= 1;
     const int G = 64 / P.LPV;
     const long per_tile = (long)P.waves * G * P.V;
     P.ntiles = ((long)B + per_tile - 1) / per_tile;
@@ -146,14 +156,15 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
         // Cap the number of dp partials; each workgroup then walks several vector tiles.
         const long cap = 2L * ncu;
         if (P.ntiles > cap) P.grid = (int)cap;
-        // LDS stage depth: up to 64 KiB per workgroup, at least two layers.
+        // LDS stage depth: at least two layers; the stage fits the budget.
         const size_t per_layer = (size_t)P.waves * P.NP * P.rowbytes;
-        int C = (int)((64 * 1024) / per_layer);
+        const size_t budget = stage_budget(P.waves);
+        int C = (int)(budget / per_layer);
         if (C > 16) C = 16;
-        if (P.C >= 1 && (size_t)P.C * per_layer <= 64 * 1024) C = P.C;
-        C &= ~3;  // the backdiff loop is unrolled by four layers
-        if (C < 4) C = 4;
-        if ((size_t)C * per_layer > 64 * 1024) return MESHPROP_ERR_UNSUPPORTED;
+        if (P.C >= 1 && (size_t)P.C * per_layer <= budget) C = P.C;
+        C &= ~1;  // the backdiff loop walks layers in pairs
+        if (C < 2) C = 2;
+        if ((size_t)C * per_layer > budget) return MESHPROP_ERR_UNSUPPORTED;
         P.C = C;
         P.nseg = P.grid < 16 ? P.grid : 16;
     } else {
