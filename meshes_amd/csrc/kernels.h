@@ -387,10 +387,12 @@ __device__ __forceinline__ void bwd_layer(const int kd, const T (&u)[V][W], cons
 
 // Stage-end conversion of one slot: sum the waves' dJdT rows of slot m (layer kind kd,
 // wave-uniform) from the stage layer s0 and add its dp contribution to pout[m*np ...].
-template <class T, int W>
+__device__ __forceinline__ int s_clamp(const int last, const int s) { return s < last ? last - s : 0; }
+
+template <class T, int W, int NPAR>
 __device__ __forceinline__ void convert_slot(const typename RowOf<T>::type* s0, const int wstride, const int nw,
-                                             const int LPV, const int NP, const int np, float* pout, const int kd,
-                                             const int m, const T (&dd)[8], const bool first_tile) {
+                                             const int LPV, const int NP, float* pout, const int kd, const int m,
+                                             const T (&dd)[4 * NPAR], const bool first_tile) {
     typedef typename RowOf<T>::type R;
     int i, j;
     if (kd <= 1) {
@@ -404,10 +406,11 @@ __device__ __forceinline__ void convert_slot(const typename RowOf<T>::type* s0, 
         i = -1;
         j = 0;
     }
-    float* outp = pout + m * np;
+    float* outp = pout + m * NPAR;
     if (i < 0 || j >= NP) {
         if (first_tile)
-            for (int k = 0; k < np; ++k) outp[k] = 0.f;
+#pragma unroll
+            for (int k = 0; k < NPAR; ++k) outp[k] = 0.f;
         return;
     }
     const int oi = (i % W) * LPV + i / W, oj = (j % W) * LPV + j / W;
@@ -417,15 +420,15 @@ __device__ __forceinline__ void convert_slot(const typename RowOf<T>::type* s0, 
         sj = row_add(sj, s0[wv * wstride + oj]);
     }
     // dJdT00 = A_i, dJdT01 = B_i, dJdT11 = A_j, dJdT10 = B_j
-    float val[2];
+    float val[NPAR];
 #pragma unroll
-    for (int k = 0; k < 2; ++k) {
+    for (int k = 0; k < NPAR; ++k) {
         val[k] = re_dot(row_a(si), dd[k * 4 + 0]);
         val[k] += re_dot(row_b(si), dd[k * 4 + 1]);
         val[k] += re_dot(row_b(sj), dd[k * 4 + 2]);
         val[k] += re_dot(row_a(sj), dd[k * 4 + 3]);
     }
-    if (np == 2) {
+    if constexpr (NPAR == 2) {
         float2* o2 = reinterpret_cast<float2*>(outp);
         if (first_tile) {
             *o2 = make_float2(val[0], val[1]);
@@ -451,13 +454,34 @@ __device__ __forceinline__ void fetch_dtdp(const T* dtdp, const int NPh, const i
 // 256 VGPRs per wave spill-free; larger tiles keep four.
 __host__ __device__ constexpr int bwd_max_waves(int W, int V, bool uni) { return (uni && W * V <= 8) ? 8 : 4; }
 
+// Number of phase parameters per crossing, fixed by the field type: real fields come only
+// from orthogonal crossings (one parameter), complex fields from the MZI family (two).
+template <class T>
+struct NPhase {
+    static constexpr int value = 2;
+};
+template <>
+struct NPhase<float> {
+    static constexpr int value = 1;
+};
+
+// Row pipeline depth of backdiff: four register sets (rows requested two layers ahead) for
+// narrow tiles, two (one layer ahead) when the tile's registers do not allow more.
+template <int W>
+struct BwdDepth {
+    static constexpr int value = W <= 4 ? 4 : 2;
+};
+
 template <class T, int W, int V, bool UNI>
 __global__ void __launch_bounds__(64 * bwd_max_waves(W, V, UNI)) k_bwd(
-    int N, int l0, int nl, int B, int LPV, int NP, int C, int np, const int* __restrict__ kind,
+    int N, int l0, int nl, int B, int LPV, int NP, int C, int np_unused, const int* __restrict__ kind,
     const typename RowOf<T>::type* __restrict__ rows_inv, const typename RowOf<T>::type* __restrict__ rows_adj,
     const T* __restrict__ dtdp, float* __restrict__ partial, size_t part_stride, long ntiles, const T* u_out,
     const T* g_out, const T* target, T* u_in, T* g_in, int ldu) {
     typedef typename RowOf<T>::type R;
+    constexpr int np = NPhase<T>::value;
+    constexpr int D = BwdDepth<W>::value;  // row register sets
+    constexpr int PD = D / 2;              // prefetch distance in layers
     extern __shared__ unsigned char smem_raw[];
     R* stage = reinterpret_cast<R*>(smem_raw);  // [nw][C][W][LPV]
     const int nw = blockDim.x >> 6;
@@ -469,6 +493,7 @@ __global__ void __launch_bounds__(64 * bwd_max_waves(W, V, UNI)) k_bwd(
     const int* kp = kind + l0;
     const int last = nl - 1;
     constexpr int PF = 2;  // stage-end slots per lane whose dT/dp is prefetched
+    (void)np_unused;
 
     bool first = true;
     for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -488,12 +513,16 @@ __global__ void __launch_bounds__(64 * bwd_max_waves(W, V, UNI)) k_bwd(
         // Local layers are visited in the order s = 0, 1, ..., as l = last - s.
         const R* rip = rows_inv + (size_t)l0 * NP + q.t;
         const R* rap = rows_adj + (size_t)l0 * NP + q.t;
-        R ia[W], ib[W], aa[W], ab[W];
-        int ka = 0;
-        if (nl > 0) {
-            ka = kp[last];
-            load_rows(ia, rip + (size_t)last * NP, LPV);
-            if constexpr (!UNI) load_rows(aa, rap + (size_t)last * NP, LPV);
+        R ri[D][W], ra[D][W];
+        int kk[D];
+        auto lrow = [&](const int s) { return (size_t)(s < last ? last - s : 0) * NP; };
+#pragma unroll
+        for (int d = 0; d < PD; ++d) {
+            if (nl > 0) {
+                kk[d] = kp[s_clamp(last, d)];
+                load_rows(ri[d], rip + lrow(d), LPV);
+                if constexpr (!UNI) load_rows(ra[d], rap + lrow(d), LPV);
+            }
         }
 #define MP_BWD_LAYER(KD, U, G, UO, GO, RI, RA, LC)                                                  \
     {                                                                                               \
@@ -505,27 +534,51 @@ __global__ void __launch_bounds__(64 * bwd_max_waves(W, V, UNI)) k_bwd(
         for (int c0 = 0; c0 < nl; c0 += C) {
             const int cl = (C < nl - c0) ? C : nl - c0;
             // Prefetch the dT/dp this wave converts first at the end of the stage.
-            T pre[PF][8];
+            T pre[PF][2 * np];
             if (wave_u < cl) {
+                const T* src = dtdp + (size_t)(l0 + last - (c0 + wave_u)) * NPh * np * 4 + lane;
 #pragma unroll
                 for (int r = 0; r < PF; ++r)
-                    if (lane + 64 * r < NPh) fetch_dtdp(dtdp, NPh, np, l0 + last - (c0 + wave_u), lane + 64 * r, pre[r]);
+                    if (lane + 64 * r < NPh) {
+#pragma unroll
+                        for (int e = 0; e < 4 * np; ++e) pre[r][e] = src[e * NPh + 64 * r];
+                    }
             }
             int lc = 0;
-            for (; lc + 1 < cl; lc += 2) {
-                const int s = c0 + lc;  // layers last - s and last - s - 1
-                const int kb = kp[last - s - 1];
-                load_rows(ib, rip + (size_t)(last - s - 1) * NP, LPV);
-                if constexpr (!UNI) load_rows(ab, rap + (size_t)(last - s - 1) * NP, LPV);
-                MP_BWD_LAYER(ka, ua, ga, ub, gb, ia, aa, lc);
-                const int ln = (last - s - 2 >= 0) ? last - s - 2 : 0;
-                ka = kp[ln];
-                load_rows(ia, rip + (size_t)ln * NP, LPV);
-                if constexpr (!UNI) load_rows(aa, rap + (size_t)ln * NP, LPV);
-                MP_BWD_LAYER(kb, ub, gb, ua, ga, ib, ab, lc + 1);
+            for (; lc + D <= cl; lc += D) {
+                const int s = c0 + lc;
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    constexpr int dummy = 0;
+                    (void)dummy;
+                    const int dn = (d + PD) % D;  // register set receiving layer s + d + PD
+                    kk[dn] = kp[s_clamp(last, s + d + PD)];
+                    load_rows(ri[dn], rip + lrow(s + d + PD), LPV);
+                    if constexpr (!UNI) load_rows(ra[dn], rap + lrow(s + d + PD), LPV);
+                    if ((d & 1) == 0)
+                        MP_BWD_LAYER(kk[d], ua, ga, ub, gb, ri[d], ra[d], lc + d)
+                    else
+                        MP_BWD_LAYER(kk[d], ub, gb, ua, ga, ri[d], ra[d], lc + d)
+                }
             }
-            if (lc < cl) {  // odd length: only the last stage
-                MP_BWD_LAYER(ka, ua, ga, ub, gb, ia, aa, lc);
+            // Remainder of the last stage (cl - lc < D layers): register sets 0.. hold them.
+            int left = cl - lc;
+#pragma unroll
+            for (int d = 0; d < D - 1; ++d) {
+                if (d < left) {
+                    if (d + PD < D) {
+                        const int dn = (d + PD) % D;
+                        kk[dn] = kp[s_clamp(last, c0 + lc + d + PD)];
+                        load_rows(ri[dn], rip + lrow(c0 + lc + d + PD), LPV);
+                        if constexpr (!UNI) load_rows(ra[dn], rap + lrow(c0 + lc + d + PD), LPV);
+                    }
+                    if ((d & 1) == 0)
+                        MP_BWD_LAYER(kk[d], ua, ga, ub, gb, ri[d], ra[d], lc + d)
+                    else
+                        MP_BWD_LAYER(kk[d], ub, gb, ua, ga, ri[d], ra[d], lc + d)
+                }
+            }
+            if (left & 1) {
                 copy_state(ub, ua);
                 copy_state(gb, ga);
             }
@@ -536,22 +589,17 @@ __global__ void __launch_bounds__(64 * bwd_max_waves(W, V, UNI)) k_bwd(
                 const int kdl = kind[la];
                 const R* s0 = stage + lcc * W * LPV;
                 float* pout = my_partial + (size_t)la * NPh * np;
-                if (lcc == wave_u) {
+                const T* src = dtdp + (size_t)la * NPh * np * 4;
+                for (int m = lane, r = 0; m < NPh; m += 64, ++r) {
+                    T dd[2 * np * 2];
+                    if (lcc == wave_u && r < PF) {
 #pragma unroll
-                    for (int r = 0; r < PF; ++r)
-                        if (lane + 64 * r < NPh)
-                            convert_slot<T, W>(s0, wstride, nw, LPV, NP, np, pout, kdl, lane + 64 * r, pre[r], first);
-                    for (int m = lane + 64 * PF; m < NPh; m += 64) {
-                        T dd[8];
-                        fetch_dtdp(dtdp, NPh, np, la, m, dd);
-                        convert_slot<T, W>(s0, wstride, nw, LPV, NP, np, pout, kdl, m, dd, first);
+                        for (int e = 0; e < 4 * np; ++e) dd[e] = (r == 0) ? pre[0][e] : pre[PF - 1][e];
+                    } else {
+#pragma unroll
+                        for (int e = 0; e < 4 * np; ++e) dd[e] = src[e * NPh + m];
                     }
-                } else {
-                    for (int m = lane; m < NPh; m += 64) {
-                        T dd[8];
-                        fetch_dtdp(dtdp, NPh, np, la, m, dd);
-                        convert_slot<T, W>(s0, wstride, nw, LPV, NP, np, pout, kdl, m, dd, first);
-                    }
+                    convert_slot<T, W, np>(s0, wstride, nw, LPV, NP, pout, kdl, m, dd, first);
                 }
             }
             __syncthreads();

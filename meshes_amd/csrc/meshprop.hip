@@ -97,7 +97,7 @@ bool tile_ok(int op, int W, int V) {
 }
 
 // Default tile: narrowest W that keeps a vector inside one wave; the largest compiled V
-// that still gives at least as many waves as SIMDs (4 per CU).
+// that still gives two waves per SIMD (8 per CU).
 void resolve_tile(int op, int N, int B, const meshprop_tile* t, int ncu, int& W, int& V, int& LPV, int& waves,
                   int& C) {
     W = N <= 128 ? 2 : N <= 256 ? 4 : N <= 512 ? 8 : 16;
@@ -109,7 +109,7 @@ void resolve_tile(int op, int N, int B, const meshprop_tile* t, int ncu, int& W,
         V = t->v;
     } else {
         V = 1;
-        const long target = 4L * ncu;
+        const long target = 8L * ncu;  // two waves per SIMD hide more latency than wider tiles
         for (int v = 8; v >= 1; v >>= 1) {
             if (!tile_ok(op, W, v)) continue;
             const long wv = ((long)B + (long)G * v - 1) / ((long)G * v);
@@ -145,8 +145,8 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
         const int cap = (op == MESHPROP_OP_BACKDIFF) ? bwd_max_waves(P.W, P.V, P.uni) : 4;
         if (P.waves > cap) P.waves = cap;
     }
-    if (op == MESHPROP_OP_BACKDIFF)  // the LDS stage must hold two layers of every wave
-        while (P.waves > 1 && (size_t)2 * P.waves * P.NP * P.rowbytes > stage_budget(P.waves)) P.waves >>= 1;
+    if (op == MESHPROP_OP_BACKDIFF)  // the LDS stage must hold one unrolled group of layers
+        while (P.waves > 1 && (size_t)(P.W <= 4 ? 4 : 2) * P.waves * P.NP * P.rowbytes > stage_budget(P.waves)) P.waves >>= 1;
     const int G = 64 / P.LPV;
     const long per_tile = (long)P.waves * G * P.V;
     P.ntiles = ((long)B + per_tile - 1) / per_tile;
@@ -162,8 +162,9 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
         int C = (int)(budget / per_layer);
         if (C > 16) C = 16;
         if (P.C >= 1 && (size_t)P.C * per_layer <= budget) C = P.C;
-        C &= ~1;  // the backdiff loop walks layers in pairs
-        if (C < 2) C = 2;
+        const int depth = P.W <= 4 ? 4 : 2;  // layers per unrolled group of the backdiff loop
+        C -= C % depth;
+        if (C < depth) C = depth;
         if ((size_t)C * per_layer > budget) return MESHPROP_ERR_UNSUPPORTED;
         P.C = C;
         P.nseg = P.grid < 16 ? P.grid : 16;
