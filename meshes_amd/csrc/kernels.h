@@ -534,7 +534,7 @@ __global__ void __launch_bounds__(64 * bwd_max_waves(W, V, UNI)) k_bwd(
         for (int c0 = 0; c0 < nl; c0 += C) {
             const int cl = (C < nl - c0) ? C : nl - c0;
             // Prefetch the dT/dp this wave converts first at the end of the stage.
-            T pre[PF][2 * np];
+            T pre[PF][4 * np];
             if (wave_u < cl) {
                 const T* src = dtdp + (size_t)(l0 + last - (c0 + wave_u)) * NPh * np * 4 + lane;
 #pragma unroll
@@ -591,7 +591,7 @@ __global__ void __launch_bounds__(64 * bwd_max_waves(W, V, UNI)) k_bwd(
                 float* pout = my_partial + (size_t)la * NPh * np;
                 const T* src = dtdp + (size_t)la * NPh * np * 4;
                 for (int m = lane, r = 0; m < NPh; m += 64, ++r) {
-                    T dd[2 * np * 2];
+                    T dd[4 * np];
                     if (lcc == wave_u && r < PF) {
 #pragma unroll
                         for (int e = 0; e < 4 * np; ++e) dd[e] = (r == 0) ? pre[0][e] : pre[PF - 1][e];
