@@ -450,9 +450,9 @@ __device__ __forceinline__ void fetch_dtdp(const T* dtdp, const int NPh, const i
     for (int e = 0; e < 8; ++e) dd[e] = (e < np * 4) ? src[e * NPh] : fzero(T());
 }
 
-// Backdiff workgroups may hold up to eight waves when the tile is small enough to leave
-// 256 VGPRs per wave spill-free; larger tiles keep four.
-__host__ __device__ constexpr int bwd_max_waves(int W, int V, bool uni) { return (uni && W * V <= 8) ? 8 : 4; }
+// Backdiff workgroups hold at most four waves: measured on MI355X, eight-wave workgroups
+// (with the wider LDS stage) did not beat two four-wave workgroups per CU.
+__host__ __device__ constexpr int bwd_max_waves(int, int, bool) { return 4; }
 
 // Number of phase parameters per crossing, fixed by the field type: real fields come only
 // from orthogonal crossings (one parameter), complex fields from the MZI family (two).
@@ -583,22 +583,25 @@ __global__ void __launch_bounds__(64 * bwd_max_waves(W, V, UNI)) k_bwd(
                 copy_state(gb, ga);
             }
             __syncthreads();
-            // dJdT -> dp: wave w converts stage layers w, w + nw, ...
+            // dJdT -> dp: wave w converts stage layers w, w + nw, ... (the first one prefetched).
             for (int lcc = wave_u; lcc < cl; lcc += nw) {
                 const int la = l0 + last - (c0 + lcc);
                 const int kdl = kind[la];
                 const R* s0 = stage + lcc * W * LPV;
                 float* pout = my_partial + (size_t)la * NPh * np;
                 const T* src = dtdp + (size_t)la * NPh * np * 4;
-                for (int m = lane, r = 0; m < NPh; m += 64, ++r) {
+                int m0 = lane;
+                if (lcc == wave_u) {
+#pragma unroll
+                    for (int r = 0; r < PF; ++r)
+                        if (lane + 64 * r < NPh)
+                            convert_slot<T, W, np>(s0, wstride, nw, LPV, NP, pout, kdl, lane + 64 * r, pre[r], first);
+                    m0 = lane + 64 * PF;
+                }
+                for (int m = m0; m < NPh; m += 64) {
                     T dd[4 * np];
-                    if (lcc == wave_u && r < PF) {
 #pragma unroll
-                        for (int e = 0; e < 4 * np; ++e) dd[e] = (r == 0) ? pre[0][e] : pre[PF - 1][e];
-                    } else {
-#pragma unroll
-                        for (int e = 0; e < 4 * np; ++e) dd[e] = src[e * NPh + m];
-                    }
+                    for (int e = 0; e < 4 * np; ++e) dd[e] = src[e * NPh + m];
                     convert_slot<T, W, np>(s0, wstride, nw, LPV, NP, pout, kdl, m, dd, first);
                 }
             }
