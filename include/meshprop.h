@@ -58,7 +58,8 @@ enum meshprop_status {
 enum meshprop_op {
     MESHPROP_OP_FWDPROP = 0,
     MESHPROP_OP_FWDDIFF = 1,
-    MESHPROP_OP_BACKDIFF = 2
+    MESHPROP_OP_BACKDIFF = 2,
+    MESHPROP_OP_FWDGRAD = 3    /* group entries only: fwdprop, then backdiff of its own output */
 };
 
 /* Optional tile choice.  Zero fields mean "library default".  w: waveguides per lane
@@ -142,6 +143,74 @@ enum meshprop_flags {
     MESHPROP_SKIP_REDUCE = 4,
     MESHPROP_ONLY_REDUCE = 8
 };
+
+/*
+ * Grouped launch: many meshes of different width, depth and geometry in one persistent
+ * propagation kernel (no reference counterpart: the reference launches one kernel per mesh
+ * and call, gpu/mesh.py:158-231).  Work items are (mesh, partial slot) pairs taken from a
+ * device-side queue by resident workgroups, longest first; an item runs each of its vector
+ * tiles through the whole depth (forward, then the adjoint for gradient entries).
+ * A launch is a prologue (row tables of every entry that needs them), the persistent
+ * kernel and the dp reduction, all grid-stride kernels whose launch shape depends only on
+ * the device: every plan uploaded into the same workspace replays one captured hipGraph.
+ *
+ * Entry fields.  op: MESHPROP_OP_FWDPROP (u_in -> u_out), MESHPROP_OP_BACKDIFF (u_out,
+ * dJdu_out or target -> dp, dJdu_in and, if u_in is set, the reconstructed input written
+ * to u_in) or MESHPROP_OP_FWDGRAD (u_in -> u_out, then the adjoint of that output).
+ * Gradient entries need a unitary crossing (MESHPROP_MZI, MESHPROP_SYM); forward entries
+ * take any complex crossing.  MESHPROP_ORTH (real fields) is not grouped.  target, when
+ * set, replaces dJdu_out by 2 (u_out - target).  dp is accumulated into (or, with
+ * MESHPROP_GROUP_DP_ASSIGN in flags, overwritten on its N/2 slots per layer).  tile.w, tile.v
+ * and tile.chunk pick the tile (0 = default); tile.waves is ignored.  tables, when set,
+ * is a caller-owned buffer of meshprop_group_tables_size() bytes holding the entry's
+ * row tables across calls (device-resident phases): they are rebuilt from p only when
+ * build_tables is nonzero.  Otherwise the tables live in the workspace and are rebuilt on
+ * every launch.
+ */
+typedef struct meshprop_group_entry {
+    int op, X, N, L, B, fft;
+    const int *geom0, *geom1;
+    const float *p;
+    int ldp;
+    const float *s;
+    int lds;
+    float *dp;
+    const void *u_in;
+    void *u_out;
+    const void *target;
+    const void *dJdu_out;
+    void *dJdu_in;
+    int ldu;
+    meshprop_tile tile;
+    void *tables;
+    int build_tables;
+    int flags;             /* MESHPROP_GROUP_DP_ASSIGN: write dp instead of accumulating */
+} meshprop_group_entry;
+
+enum meshprop_group_flags {
+    MESHPROP_GROUP_DP_ASSIGN = 1
+};
+
+typedef struct meshprop_group meshprop_group; /* host-side plan of a grouped launch */
+
+/* Plans a grouped launch of n entries (copied; pointers are kept, not dereferenced). */
+int meshprop_group_create(const meshprop_group_entry *entries, int n, meshprop_group **out);
+void meshprop_group_destroy(meshprop_group *g);
+/* Device workspace bytes of the plan (descriptors, queue, tables, dp partials). */
+size_t meshprop_group_workspace_size(const meshprop_group *g);
+/* Bytes of a caller-owned table buffer for one entry of this crossing, size and tile. */
+size_t meshprop_group_tables_size(int X, int N, int L, int fft, const meshprop_tile *tile);
+/* Writes the plan's descriptors into the workspace (one host-to-device copy on stream).
+ * The plan must stay alive until the copy has completed. */
+int meshprop_group_upload(const meshprop_group *g, void *workspace, size_t workspace_bytes, void *stream);
+/* Launches prologue, persistent kernel and reduction on stream for the descriptors last
+ * uploaded into the workspace.  The launch shape depends on the device and on
+ * meshprop_group_signature() only. */
+int meshprop_group_launch(const meshprop_group *g, void *workspace, size_t workspace_bytes, void *stream);
+/* Launch-shape signature: two launches with equal signatures replay the same graph. */
+int meshprop_group_signature(const meshprop_group *g);
+/* Resolved tile and number of work items of entry i. */
+int meshprop_group_entry_info(const meshprop_group *g, int i, meshprop_tile *tile, int *items);
 
 /* The tile the library would use for a call (after defaults and clamping). */
 int meshprop_resolve_tile(int op, int X, int N, int B, meshprop_tile *out);

@@ -15,7 +15,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("MESHPROP_LIB", os.path.join(_HERE, "libmeshprop.so"))
 
 MZI, SYM, ORTH, MZI_GENERIC = 0, 1, 2, 3
-OP_FWDPROP, OP_FWDDIFF, OP_BACKDIFF = 0, 1, 2
+OP_FWDPROP, OP_FWDDIFF, OP_BACKDIFF, OP_FWDGRAD = 0, 1, 2, 3
+GROUP_DP_ASSIGN = 1
 X_CODE = {"mzi": MZI, "sym": SYM, "orth": ORTH, "mzi_generic": MZI_GENERIC}
 N_PHASE = {"mzi": 2, "sym": 2, "orth": 1, "mzi_generic": 2}
 N_SPLIT = {"mzi": 2, "sym": 1, "orth": 0, "mzi_generic": 14}
@@ -28,6 +29,9 @@ EXPORTS = (
     "meshprop_workspace_size", "meshprop_fwdprop", "meshprop_fwdprop_ft", "meshprop_fwddiff",
     "meshprop_fwddiff_ft", "meshprop_backdiff", "meshprop_backdiff_ft", "meshprop_run",
     "meshprop_resolve_tile", "meshprop_last_hip_error", "meshprop_status_string", "meshprop_version",
+    "meshprop_group_create", "meshprop_group_destroy", "meshprop_group_workspace_size",
+    "meshprop_group_tables_size", "meshprop_group_upload", "meshprop_group_launch", "meshprop_group_signature",
+    "meshprop_group_entry_info",
 )
 
 
@@ -41,6 +45,17 @@ class MeshpropError(RuntimeError):
 
 class Tile(ctypes.Structure):
     _fields_ = [("w", ctypes.c_int), ("v", ctypes.c_int), ("waves", ctypes.c_int), ("chunk", ctypes.c_int)]
+
+
+class GroupEntry(ctypes.Structure):
+    """meshprop_group_entry (include/meshprop.h)."""
+    _fields_ = [("op", ctypes.c_int), ("X", ctypes.c_int), ("N", ctypes.c_int), ("L", ctypes.c_int),
+                ("B", ctypes.c_int), ("fft", ctypes.c_int), ("geom0", ctypes.c_void_p), ("geom1", ctypes.c_void_p),
+                ("p", ctypes.c_void_p), ("ldp", ctypes.c_int), ("s", ctypes.c_void_p), ("lds", ctypes.c_int),
+                ("dp", ctypes.c_void_p), ("u_in", ctypes.c_void_p), ("u_out", ctypes.c_void_p),
+                ("target", ctypes.c_void_p), ("dJdu_out", ctypes.c_void_p), ("dJdu_in", ctypes.c_void_p),
+                ("ldu", ctypes.c_int), ("tile", Tile), ("tables", ctypes.c_void_p), ("build_tables", ctypes.c_int),
+                ("flags", ctypes.c_int)]
 
 
 _lib = None
@@ -69,6 +84,23 @@ def load():
     lib.meshprop_status_string.restype = ctypes.c_char_p
     lib.meshprop_status_string.argtypes = [c_int]
     lib.meshprop_version.restype = ctypes.c_char_p
+    vp = c_void_p
+    lib.meshprop_group_create.argtypes = [ctypes.POINTER(GroupEntry), c_int, ctypes.POINTER(vp)]
+    lib.meshprop_group_create.restype = c_int
+    lib.meshprop_group_destroy.argtypes = [vp]
+    lib.meshprop_group_destroy.restype = None
+    lib.meshprop_group_workspace_size.argtypes = [vp]
+    lib.meshprop_group_workspace_size.restype = c_size_t
+    lib.meshprop_group_tables_size.argtypes = [c_int, c_int, c_int, c_int, ctypes.POINTER(Tile)]
+    lib.meshprop_group_tables_size.restype = c_size_t
+    lib.meshprop_group_upload.argtypes = [vp, vp, c_size_t, vp]
+    lib.meshprop_group_upload.restype = c_int
+    lib.meshprop_group_launch.argtypes = [vp, vp, c_size_t, vp]
+    lib.meshprop_group_launch.restype = c_int
+    lib.meshprop_group_signature.argtypes = [vp]
+    lib.meshprop_group_signature.restype = c_int
+    lib.meshprop_group_entry_info.argtypes = [vp, c_int, ctypes.POINTER(Tile), ctypes.POINTER(c_int)]
+    lib.meshprop_group_entry_info.restype = c_int
     _lib = lib
     return lib
 

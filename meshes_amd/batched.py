@@ -1,0 +1,253 @@
+"""Grouped propagation: many meshes of uneven width, depth and topology in one persistent launch.
+
+A ``MeshGroup`` holds a list of ``GroupJob`` (one mesh and its batch each) and runs all of them
+with the library's grouped launch (``meshprop_group_*`` in include/meshprop.h): a prologue that
+builds every mesh's row tables, ONE persistent propagation kernel whose resident workgroups take
+(mesh, partial slot) items from a device queue longest-first, and the dp reduction.  The plan
+lives in the workspace, so the three launches can be captured once as a hipGraph
+(``MeshGroup.capture``) and replayed for every later plan uploaded into the same workspace.
+
+Jobs: ``op='fwd'`` computes y = U x; ``op='grad'`` computes y = U x and then the gradient of
+J = sum |y - target|^2 (or of a given dJ/dy) with respect to the phases (dp) and, optionally, the
+input (gx); ``op='backdiff'`` takes y and dJ/dy (or target) as the reference's backdiff does
+(gpu/mesh.py:196-231).  Fields are (B, N) tensors, one vector per row (the (N, B) Fortran arrays
+of the reference, transposed).  Gradient jobs need unitary crossings (mzi, sym).
+
+The per-mesh reference path, one plain launch sequence per mesh (gpu/mesh.py:158-231, as the
+reference runs each mesh), is ``run_plain``.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import torch
+
+from . import _lib
+from .mesh import MeshNetworkGPU
+
+OPS = {"fwd": _lib.OP_FWDPROP, "backdiff": _lib.OP_BACKDIFF, "grad": _lib.OP_FWDGRAD}
+
+
+@dataclass
+class GroupJob:
+    mesh: MeshNetworkGPU
+    op: str
+    x: Optional[torch.Tensor] = None        # (B, N) input (fwd, grad); reconstructed input (backdiff, optional)
+    y: Optional[torch.Tensor] = None        # (B, N) output (allocated if None)
+    target: Optional[torch.Tensor] = None   # (B, N): dJ/dy = 2 (y - target)
+    dy: Optional[torch.Tensor] = None       # (B, N) dJ/dy when target is None
+    gx: Optional[torch.Tensor] = None       # (B, N) dJ/dx output (optional)
+    dp: Optional[torch.Tensor] = None       # (n_phase,) float32 gradient (allocated if None)
+    accumulate: bool = False                # dp += gradient instead of dp = gradient
+    tile: Optional[Sequence[int]] = None    # (w, v, waves, chunk); None = library default
+    tables: Optional[torch.Tensor] = None   # resident row tables (uint8), see tables_for()
+    build_tables: bool = True
+    B: int = field(init=False, default=0)
+
+    def __post_init__(self):
+        assert self.op in OPS, self.op
+        m = self.mesh
+        assert m.X != "orth", "real (orth) meshes are not grouped"
+        if self.op != "fwd":
+            assert m.X in ("mzi", "sym"), "gradient jobs need unitary crossings"
+        ref = self.x if self.op != "backdiff" else self.y
+        assert ref is not None and ref.ndim == 2 and ref.shape[1] == m.N
+        self.B = int(ref.shape[0])
+        dev = m.device
+        for name in ("x", "y", "target", "dy", "gx"):
+            t = getattr(self, name)
+            if t is not None:
+                assert t.shape == (self.B, m.N) and t.dtype == torch.complex64 and t.is_contiguous(), name
+                assert t.device == dev, name
+        if self.y is None:
+            self.y = torch.empty((self.B, m.N), dtype=torch.complex64, device=dev)
+        if self.op != "fwd":
+            assert self.target is not None or self.dy is not None, "gradient jobs need target or dy"
+            if self.dp is None:
+                self.dp = torch.zeros(m.n_phase, dtype=torch.float32, device=dev)
+
+
+def tables_for(mesh: MeshNetworkGPU, tile=None) -> torch.Tensor:
+    """A device buffer for a mesh's resident row tables (kept across group launches)."""
+    lib = _lib.load()
+    t = _lib.Tile(*((list(tile or []) + [0, 0, 0, 0])[:4]))
+    n = int(lib.meshprop_group_tables_size(_lib.X_CODE[mesh.X], mesh.N, mesh.L, int(mesh.fft), ctypes.byref(t)))
+    if n == 0:
+        raise _lib.MeshpropError("tables_size: unsupported mesh")
+    return torch.empty(n + 256, dtype=torch.uint8, device=mesh.device)
+
+
+def _aligned_ptr(t: torch.Tensor) -> int:
+    p = t.data_ptr()
+    return (p + 255) & ~255
+
+
+def _entry(job: GroupJob) -> _lib.GroupEntry:
+    m = job.mesh
+    e = _lib.GroupEntry()
+    e.op = OPS[job.op]
+    e.X = _lib.X_CODE[m.X]
+    e.N, e.L, e.B, e.fft = m.N, m.L, job.B, int(m.fft)
+    g0, g1 = m._geom()
+    e.geom0 = g0.data_ptr() if (g0 is not None and g0.numel()) else None
+    e.geom1 = g1.data_ptr() if (g1 is not None and g1.numel()) else None
+    e.p = m.p_phase.data_ptr() if m.p_phase.numel() else None
+    e.ldp = m.N // 2 * m.X_np
+    e.s = m.p_splitter.data_ptr() if m.p_splitter.numel() else None
+    e.lds = m.N // 2 * m.X_ns
+    e.dp = job.dp.data_ptr() if job.dp is not None else None
+    if job.op == "backdiff":
+        e.u_in = job.x.data_ptr() if job.x is not None else None
+    else:
+        e.u_in = job.x.data_ptr()
+    e.u_out = job.y.data_ptr()
+    e.target = job.target.data_ptr() if job.target is not None else None
+    e.dJdu_out = job.dy.data_ptr() if job.dy is not None else None
+    e.dJdu_in = job.gx.data_ptr() if job.gx is not None else None
+    e.ldu = m.N
+    w, v, waves, chunk = (list(job.tile or []) + [0, 0, 0, 0])[:4]
+    e.tile = _lib.Tile(int(w), int(v), int(waves), int(chunk))
+    e.tables = _aligned_ptr(job.tables) if job.tables is not None else None
+    e.build_tables = int(bool(job.build_tables))
+    e.flags = 0 if job.accumulate else _lib.GROUP_DP_ASSIGN
+    return e
+
+
+class MeshGroup:
+    """Plan, workspace and launches of a grouped propagation of ``jobs``.
+
+    ``run()`` launches eagerly on the current stream; ``capture()`` records the launches as a
+    hipGraph (through torch.cuda.CUDAGraph) and ``run()`` then replays it.  ``replan(jobs)``
+    uploads a new plan into the same workspace (growing it if needed) and keeps the graph when
+    the launch signature is unchanged.
+    """
+
+    def __init__(self, jobs: List[GroupJob], workspace: Optional[torch.Tensor] = None):
+        self._lib = _lib.load()
+        self._plan = ctypes.c_void_p()
+        self._graphs = {}  # launch signature -> captured graph
+        self._ws = workspace
+        self.jobs: List[GroupJob] = []
+        self.replan(jobs)
+
+    # -- plan ------------------------------------------------------------------------------
+    def replan(self, jobs: List[GroupJob]):
+        self.jobs = list(jobs)
+        ents = (_lib.GroupEntry * max(1, len(self.jobs)))(*[_entry(j) for j in self.jobs])
+        plan = ctypes.c_void_p()
+        _lib.check(self._lib.meshprop_group_create(ents, len(self.jobs), ctypes.byref(plan)), "group_create")
+        self._free_plan()
+        self._plan = plan
+        self._entries = ents
+        need = int(self._lib.meshprop_group_workspace_size(self._plan))
+        if self._ws is None or self._ws.numel() < need + 256:
+            dev = self.jobs[0].mesh.device if self.jobs else torch.device("cuda", torch.cuda.current_device())
+            self._ws = torch.empty(need + 256, dtype=torch.uint8, device=dev)
+            self._graphs = {}  # captured graphs hold the old workspace address
+        self.ws_bytes = need
+        self.signature = int(self._lib.meshprop_group_signature(self._plan))
+        self.upload()
+
+    @property
+    def workspace_bytes(self) -> int:
+        return self._ws.numel() - 256
+
+    def upload(self):
+        s = torch.cuda.current_stream().cuda_stream
+        _lib.check(self._lib.meshprop_group_upload(self._plan, _aligned_ptr(self._ws), self.ws_bytes,
+                                                   ctypes.c_void_p(s)), "group_upload")
+
+    def info(self, i: int):
+        """(tile (w, v, waves, chunk), number of work items) of job i."""
+        t = _lib.Tile()
+        n = ctypes.c_int()
+        _lib.check(self._lib.meshprop_group_entry_info(self._plan, i, ctypes.byref(t), ctypes.byref(n)), "info")
+        return (t.w, t.v, t.waves, t.chunk), n.value
+
+    # -- launches --------------------------------------------------------------------------
+    def launch(self):
+        s = torch.cuda.current_stream().cuda_stream
+        _lib.check(self._lib.meshprop_group_launch(self._plan, _aligned_ptr(self._ws), self.ws_bytes,
+                                                   ctypes.c_void_p(s)), "group_launch")
+
+    def capture(self, warm: bool = True):
+        """Record prologue, persistent kernel and reduction as one replayable graph (kept per
+        launch signature; later plans uploaded into this workspace replay it)."""
+        if warm:
+            self.launch()  # loads the code objects before capture
+            torch.cuda.synchronize()
+            self.upload()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.launch()
+        torch.cuda.synchronize()
+        self._graphs[self.signature] = g
+
+    @property
+    def graph(self):
+        return self._graphs.get(self.signature)
+
+    def run(self):
+        g = self._graphs.get(self.signature)
+        if g is not None:
+            g.replay()
+        else:
+            self.launch()
+
+    def _free_plan(self):
+        if self._plan:
+            self._lib.meshprop_group_destroy(self._plan)
+            self._plan = ctypes.c_void_p()
+
+    def close(self):
+        self._graphs = {}
+        self._free_plan()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PlainRunner:
+    """The per-mesh path: each job as its own plain launch sequence (prologue, propagation,
+    reduction) through meshprop_run, one mesh after another on one stream, with workspaces
+    allocated once up front."""
+
+    def __init__(self, jobs: List[GroupJob]):
+        self.jobs = list(jobs)
+        self.tiles = [j.tile for j in self.jobs]
+        self.ws = []
+        for j in self.jobs:
+            m = j.mesh
+            ops = {"fwd": [_lib.OP_FWDPROP], "grad": [_lib.OP_FWDPROP, _lib.OP_BACKDIFF],
+                   "backdiff": [_lib.OP_BACKDIFF]}[j.op]
+            n = max(_lib.workspace_size(op, m.X, m.N, m.L, j.B, m.fft, j.tile) for op in ops)
+            # meshprop_run's backdiff always writes the reconstructed input: scratch for jobs without x
+            rec = j.x if j.op == "backdiff" and j.x is not None else (
+                torch.empty_like(j.y) if j.op != "fwd" else None)
+            self.ws.append((torch.empty(max(n, 1), dtype=torch.uint8, device=m.device), n, rec))
+
+    def run(self):
+        for j, tile, (ws, nb, rec) in zip(self.jobs, self.tiles, self.ws):
+            m = j.mesh
+            g0, g1 = m._geom()
+            ldp, lds = m.N // 2 * m.X_np, m.N // 2 * m.X_ns
+            s = m.p_splitter if m.p_splitter.numel() else None
+            if j.B == 0:
+                continue
+            if j.op in ("fwd", "grad"):
+                _lib.run(_lib.OP_FWDPROP, m.X, m.N, m.L, j.B, m.fft, g0, g1, m.p_phase, None, ldp, s, lds, j.x, None,
+                         j.y, None, m.N, tile=tile, workspace=ws, ws_bytes=nb)
+            if j.op in ("grad", "backdiff"):
+                if not j.accumulate:
+                    j.dp.zero_()
+                _lib.run(_lib.OP_BACKDIFF, m.X, m.N, m.L, j.B, m.fft, g0, g1, m.p_phase, j.dp, ldp, s, lds, j.y,
+                         j.dy, rec, j.gx, m.N, target=j.target, tile=tile, workspace=ws, ws_bytes=nb)
+
+
+def run_plain(jobs: List[GroupJob]):
+    PlainRunner(jobs).run()

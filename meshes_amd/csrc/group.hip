@@ -1,0 +1,574 @@
+// group.hip -- grouped launch: many meshes of different width, depth and geometry propagated
+// by one persistent kernel that takes (mesh, partial slot) work items from a device queue.
+//
+// The plan lives at the start of the workspace (header, per-mesh descriptors, item lists);
+// the prologue, the persistent kernel and the dp reduction read it from there and walk their
+// work grid-stride, so their launch shapes depend on the device alone and one captured
+// hipGraph replays every plan uploaded into the same workspace.
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "kernels.h"
+
+namespace mp {
+
+// Kernel families: family 0 holds the tiles with W <= 8 in four-wave workgroups (two per CU),
+// family 1 the W = 16 tile in two-wave workgroups, whose registers allow one wave per SIMD.
+// Variant v of the persistent kernels: (W, V) = kGroupTiles[v].
+struct GTile {
+    int w, v, fam;
+};
+constexpr GTile kGroupTiles[] = {{2, 1, 0}, {2, 2, 0}, {2, 4, 0}, {4, 1, 0}, {4, 2, 0}, {8, 1, 0}, {16, 1, 1}};
+constexpr int kGroupVariants = sizeof(kGroupTiles) / sizeof(kGroupTiles[0]);
+constexpr int kFamWaves[2] = {4, 2};
+constexpr size_t kGroupStage = 64 * 1024;  // LDS of one workgroup (backdiff stage)
+
+enum { G_FWD = 1, G_GRAD = 2 };  // what an item runs
+enum { B_KIND = 1, B_FWD = 2, B_INV = 4, B_DTDP = 8 };
+
+struct GMesh {
+    int what, var, X, N, L, B, fft, LPV, NP, C, np, ns, ldp, lds, ldu, S, build, assign;
+    long ntiles, E;
+    long pro_chunks;  // prologue blocks per layer
+    const int *g0, *g1;
+    const float *p, *s;
+    float* dp;
+    int* kind;
+    float4 *rows_fwd, *rows_inv;
+    float2* dtdp;
+    float* partial;
+    const float2 *u_in, *target, *g_out;
+    float2 *u_out, *g_in, *u_rec;
+};
+
+struct GHeader {
+    int nmesh, pad0;
+    int nitems[2];
+    int grab[2], done[2];  // queue counters; the last workgroup out of a launch resets them
+    long pro_total, red_total;
+    long off_pro, off_red, off_mesh, off_items[2];
+};
+
+__device__ __forceinline__ int find_mesh(const long* begin, const int nmesh, const long u) {
+    // Largest e with begin[e] <= u (entries without units have begin[e] == begin[e + 1]).
+    int lo = 0, hi = nmesh - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (begin[mid] <= u)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(256) k_group_prologue(const unsigned char* __restrict__ ws) {
+    const GHeader* H = reinterpret_cast<const GHeader*>(ws);
+    const long* begin = reinterpret_cast<const long*>(ws + H->off_pro);
+    const GMesh* meshes = reinterpret_cast<const GMesh*>(ws + H->off_mesh);
+    for (long u = blockIdx.x; u < H->pro_total; u += gridDim.x) {
+        const int e = find_mesh(begin, H->nmesh, u);
+        const GMesh& M = meshes[e];
+        const long local = u - begin[e];
+        const int l = (int)(local / M.pro_chunks);
+        const int w = (int)(local % M.pro_chunks) * 256 + threadIdx.x;
+        if (w >= M.NP) continue;
+        if (M.build & B_FWD)
+            rows_at<float4>(M.X, M.N, M.NP, M.LPV, M.fft, M.g0, M.g1, M.p, M.ldp, M.s, M.lds, nullptr, TAB_FWD,
+                            M.kind, M.rows_fwd, nullptr, l, w);
+        if (M.build & B_INV)
+            rows_at<float4>(M.X, M.N, M.NP, M.LPV, M.fft, M.g0, M.g1, M.p, M.ldp, M.s, M.lds, nullptr, TAB_INV,
+                            M.kind, M.rows_inv, nullptr, l, w);
+        if ((M.build & B_DTDP) && w < M.NP / 2)
+            dtdp_at<float2>(M.X, M.N, M.NP, M.fft, M.g0, M.g1, M.p, M.ldp, M.s, M.lds, M.dtdp, l, w);
+    }
+}
+
+// dp[l*ldp + m*np + k] += sum over slots s of partial[s*E + (l*NPh + m)*np + k], m < N/2,
+// summed in slot order (four interleaved partial sums), so the result is deterministic.
+__global__ void __launch_bounds__(256) k_group_reduce(const unsigned char* __restrict__ ws) {
+    const GHeader* H = reinterpret_cast<const GHeader*>(ws);
+    const long* begin = reinterpret_cast<const long*>(ws + H->off_red);
+    const GMesh* meshes = reinterpret_cast<const GMesh*>(ws + H->off_mesh);
+    for (long u = blockIdx.x; u < H->red_total; u += gridDim.x) {
+        const int e = find_mesh(begin, H->nmesh, u);
+        const GMesh& M = meshes[e];
+        const long idx = (u - begin[e]) * 256 + threadIdx.x;
+        const int Nh = M.N / 2;
+        if (idx >= (long)M.L * Nh * M.np) continue;
+        const int k = (int)(idx % M.np);
+        const long lm = idx / M.np;
+        const int m = (int)(lm % Nh);
+        const int l = (int)(lm / Nh);
+        const float* src = M.partial + ((size_t)l * (M.NP / 2) + m) * M.np + k;
+        const size_t E = (size_t)M.E;
+        float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+        int s = 0;
+        for (; s + 3 < M.S; s += 4) {
+            a0 += src[(size_t)s * E];
+            a1 += src[(size_t)(s + 1) * E];
+            a2 += src[(size_t)(s + 2) * E];
+            a3 += src[(size_t)(s + 3) * E];
+        }
+        for (; s < M.S; ++s) a0 += src[(size_t)s * E];
+        float* d = M.dp + (size_t)l * M.ldp + (size_t)m * M.np + k;
+        const float acc = (a0 + a1) + (a2 + a3);
+        *d = M.assign ? acc : *d + acc;
+    }
+}
+
+// One work item: the vector tiles slot, slot + S, ... of one mesh, each through the whole
+// depth: forward, then (gradient items) the adjoint with inverse reconstruction; the adjoint
+// of a tile reads back the output its own threads stored.
+template <class P>
+__device__ __forceinline__ P* uniform_ptr(P* p) {
+    const unsigned long long v = reinterpret_cast<unsigned long long>(p);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return reinterpret_cast<P*>(((unsigned long long)hi << 32) | lo);
+}
+// The forward tile runs as a called function: inlined beside the adjoint it raises the
+// kernel's register allocation above 256 (one wave per SIMD instead of two).  Its arguments
+// arrive in vector registers and are made scalar again with readfirstlane.
+template <int W, int V>
+__device__ __noinline__ void group_fwd_tile(const GMesh* Mp, const long tile_v) {
+    const GMesh& M = *uniform_ptr(Mp);
+    const long tile = (long)__builtin_amdgcn_readfirstlane((unsigned)tile_v);
+    fwd_tile<float2, W, V>(M.N, 0, M.L, M.B, M.LPV, M.NP, M.kind, M.rows_fwd, M.u_in, M.u_out, M.ldu, tile);
+}
+
+template <int W, int V>
+__device__ __forceinline__ void group_item(const GMesh& M, const int slot, float4* stage) {
+    float* part = (M.what & G_GRAD) ? M.partial + (size_t)slot * M.E : nullptr;
+    bool first = true;
+    for (long tile = slot; tile < M.ntiles; tile += M.S) {
+        if (M.what & G_FWD)
+            group_fwd_tile<W, V>(&M, tile);
+        if (M.what & G_GRAD)
+            bwd_tile<float2, W, V, true>(M.N, 0, M.L, M.B, M.LPV, M.NP, M.C, M.kind, M.rows_inv, M.rows_inv, M.dtdp,
+                                         part, M.u_out, M.g_out, M.target, M.u_rec, M.g_in, M.ldu, tile, first,
+                                         stage);
+        first = false;
+    }
+}
+
+template <int FAM>
+__global__ void __launch_bounds__(64 * kFamWaves[FAM]) k_group(unsigned char* __restrict__ ws) {
+    GHeader* H = reinterpret_cast<GHeader*>(ws);
+    const GMesh* meshes = reinterpret_cast<const GMesh*>(ws + H->off_mesh);
+    const int2* items = reinterpret_cast<const int2*>(ws + H->off_items[FAM]);
+    const int n = H->nitems[FAM];
+    extern __shared__ unsigned char smem_raw[];
+    float4* stage = reinterpret_cast<float4*>(smem_raw);
+    __shared__ int s_next;
+    for (;;) {
+        if (threadIdx.x == 0) s_next = atomicAdd(&H->grab[FAM], 1);
+        __syncthreads();
+        const int it = __builtin_amdgcn_readfirstlane(s_next);
+        __syncthreads();
+        if (it >= n) break;
+        const int2 wk = items[it];
+        const GMesh& M = meshes[wk.x];
+        if constexpr (FAM == 0) {
+            switch (M.var) {
+                case 0: group_item<2, 1>(M, wk.y, stage); break;
+                case 1: group_item<2, 2>(M, wk.y, stage); break;
+                case 2: group_item<2, 4>(M, wk.y, stage); break;
+                case 3: group_item<4, 1>(M, wk.y, stage); break;
+                case 4: group_item<4, 2>(M, wk.y, stage); break;
+                case 5: group_item<8, 1>(M, wk.y, stage); break;
+                default: break;
+            }
+        } else {
+            group_item<16, 1>(M, wk.y, stage);
+        }
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&H->done[FAM], 1) == (int)gridDim.x - 1) {
+            // Every workgroup has taken its last item: reset the queue for the next launch.
+            atomicExch(&H->grab[FAM], 0);
+            atomicExch(&H->done[FAM], 0);
+        }
+    }
+}
+
+}  // namespace mp
+
+using namespace mp;
+
+struct meshprop_group {
+    struct Info {
+        int W, V, LPV, NP, C, S, var, fam, what, build, np, ns;
+        long ntiles, E, pro_units, red_units;
+        bool own_tables;
+        size_t off_tab, off_partial;  // workspace offsets (tables when own_tables)
+        size_t tab_kind, tab_fwd, tab_inv, tab_dtdp;  // offsets inside the table block
+    };
+    std::vector<meshprop_group_entry> ent;
+    std::vector<Info> info;
+    std::vector<int2> items[2];
+    size_t header_bytes = 0, total = 0;
+    long pro_total = 0, red_total = 0;
+    int ncu = 256, grid[2] = {0, 0}, dev = 0;
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    mutable std::vector<unsigned char> image;
+};
+
+namespace {
+
+inline size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int group_var(int W, int V) {
+    for (int i = 0; i < kGroupVariants; ++i)
+        if (kGroupTiles[i].w == W && kGroupTiles[i].v == V) return i;
+    return -1;
+}
+
+int group_default_w(int N) { return N <= 128 ? 2 : N <= 256 ? 4 : N <= 512 ? 8 : 16; }
+
+int group_w(int N, const meshprop_tile& t) {
+    const int w = t.w;
+    if ((w == 2 || w == 4 || w == 8 || w == 16) && (long)w * 64 >= N) return w;
+    return group_default_w(N);
+}
+
+int lpv_of(int N, int W) {
+    int p = 1;
+    while (p < (N + W - 1) / W) p <<= 1;
+    return p;
+}
+
+// Table block of one entry: kind[L] | fwd rows [L][NP] | inverse rows [L][NP] | dT/dp.
+void table_layout(int N, int L, int W, size_t& kind, size_t& fwd, size_t& inv, size_t& dtdp, size_t& total) {
+    const int NP = lpv_of(N, W) * W;
+    kind = 0;
+    fwd = a256((size_t)L * sizeof(int));
+    inv = a256(fwd + (size_t)L * NP * sizeof(float4));
+    dtdp = a256(inv + (size_t)L * NP * sizeof(float4));
+    total = a256(dtdp + (size_t)L * (NP / 2) * 2 * 4 * sizeof(float2));
+}
+
+int cu_count() {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) return 256;
+    return n;
+}
+
+int check_entry(const meshprop_group_entry& e) {
+    if (e.op != MESHPROP_OP_FWDPROP && e.op != MESHPROP_OP_BACKDIFF && e.op != MESHPROP_OP_FWDGRAD)
+        return MESHPROP_ERR_ARG;
+    if (e.X < 0 || e.X > 3) return MESHPROP_ERR_ARG;
+    if (e.X == X_ORTH) return MESHPROP_ERR_UNSUPPORTED;
+    if (e.op != MESHPROP_OP_FWDPROP && e.X == X_GENERIC) return MESHPROP_ERR_UNSUPPORTED;
+    if (e.N <= 0 || (e.N & 1) || e.L < 0 || e.B < 0) return MESHPROP_ERR_ARG;
+    if (e.N > 1024) return MESHPROP_ERR_UNSUPPORTED;
+    const int np = n_phase_of(e.X), ns = n_split_of(e.X);
+    if (e.L > 0 && (!e.geom0 || (!e.fft && !e.geom1) || !e.p)) return MESHPROP_ERR_ARG;
+    if (e.ldp < (e.N / 2) * np) return MESHPROP_ERR_ARG;
+    if (e.s && ns && e.lds < (e.N / 2) * ns) return MESHPROP_ERR_ARG;
+    if (e.B > 0) {
+        if (e.ldu < e.N || !e.u_out) return MESHPROP_ERR_ARG;
+        if (e.op != MESHPROP_OP_BACKDIFF && !e.u_in) return MESHPROP_ERR_ARG;
+        if (e.op != MESHPROP_OP_FWDPROP && (!e.dp || (!e.target && !e.dJdu_out))) return MESHPROP_ERR_ARG;
+    }
+    if (((uintptr_t)e.tables) & 255) return MESHPROP_ERR_ARG;
+    return MESHPROP_OK;
+}
+
+int plan_group(meshprop_group& g) {
+    g.ncu = cu_count();
+    hipGetDevice(&g.dev);
+    const int n = (int)g.ent.size();
+    g.info.resize(n);
+    struct Cand {
+        double cost;
+        int e, s, fam;
+    };
+    std::vector<Cand> cand;
+    size_t off = 0;
+    g.pro_total = g.red_total = 0;
+    for (int i = 0; i < n; ++i) {
+        const meshprop_group_entry& e = g.ent[i];
+        meshprop_group::Info& f = g.info[i];
+        f.W = group_w(e.N, e.tile);
+        f.LPV = lpv_of(e.N, f.W);
+        f.NP = f.LPV * f.W;
+        f.np = n_phase_of(e.X);
+        f.ns = n_split_of(e.X);
+        f.what = (e.op == MESHPROP_OP_FWDPROP) ? G_FWD : (e.op == MESHPROP_OP_BACKDIFF) ? G_GRAD : (G_FWD | G_GRAD);
+        const int G = 64 / f.LPV;
+        f.V = 1;
+        if (e.tile.v >= 1 && group_var(f.W, e.tile.v) >= 0) {
+            f.V = e.tile.v;
+        } else {
+            for (int v = 4; v >= 1; v >>= 1) {  // largest tile that still gives two waves per SIMD
+                if (group_var(f.W, v) < 0) continue;
+                if (((long)e.B + (long)G * v - 1) / ((long)G * v) >= 8L * g.ncu) {
+                    f.V = v;
+                    break;
+                }
+            }
+        }
+        f.var = group_var(f.W, f.V);
+        f.fam = kGroupTiles[f.var].fam;
+        const int nw = kFamWaves[f.fam];
+        f.ntiles = ((long)e.B + (long)nw * G * f.V - 1) / ((long)nw * G * f.V);
+        // Layers per LDS stage: at least one unrolled group of the adjoint loop.
+        const int depth = f.W <= 4 ? 4 : 2;
+        const size_t per_layer = (size_t)nw * f.NP * sizeof(float4);
+        int C = (int)(kGroupStage / per_layer);
+        if (C > 16) C = 16;
+        if (e.tile.chunk >= 1 && (size_t)e.tile.chunk * per_layer <= kGroupStage) C = e.tile.chunk;
+        C -= C % depth;
+        if (C < depth) C = depth;
+        if ((size_t)C * per_layer > kGroupStage) return MESHPROP_ERR_UNSUPPORTED;
+        f.C = C;
+        const bool grad = f.what & G_GRAD;
+        f.S = (e.B == 0) ? 0 : grad ? (int)std::min<long>(f.ntiles, 2L * g.ncu) : (int)f.ntiles;
+        f.E = (long)e.L * (f.NP / 2) * f.np;
+        // Tables: caller-owned (built on request) or in the workspace (built every launch).
+        size_t tk, tf, ti, td, tt;
+        table_layout(e.N, e.L, f.W, tk, tf, ti, td, tt);
+        f.tab_kind = tk;
+        f.tab_fwd = tf;
+        f.tab_inv = ti;
+        f.tab_dtdp = td;
+        f.own_tables = (e.tables == nullptr);
+        f.build = 0;
+        if (e.L > 0) {
+            if (f.own_tables)
+                f.build = B_KIND | ((f.what & G_FWD) ? B_FWD : 0) | (grad ? (B_INV | B_DTDP) : 0);
+            else if (e.build_tables)
+                f.build = B_KIND | B_FWD | B_INV | B_DTDP;
+        }
+        f.off_tab = off;
+        if (f.own_tables) off = a256(off + tt);
+        f.off_partial = off;
+        if (grad && e.L > 0 && f.S > 0) off = a256(off + (size_t)f.S * f.E * sizeof(float));
+        f.pro_units = f.build ? (long)e.L * ((f.NP + 255) / 256) : 0;
+        const bool assign = (e.flags & MESHPROP_GROUP_DP_ASSIGN) != 0;
+        f.red_units = (grad && e.L > 0 && (f.S > 0 || assign)) ? ((long)e.L * (e.N / 2) * f.np + 255) / 256 : 0;
+        g.pro_total += f.pro_units;
+        g.red_total += f.red_units;
+        // Work items, costed by layers x tile work x tiles per item.
+        if (f.S > 0) {
+            const long per_item = (f.ntiles + f.S - 1) / f.S;
+            const double w = ((f.what & G_FWD) ? 1.0 : 0.0) + (grad ? 3.0 : 0.0);
+            const double cost = (double)per_item * ((double)e.L * f.W * f.V * w + 64.0);
+            for (int s = 0; s < f.S; ++s) cand.push_back({cost, i, s, f.fam});
+        }
+    }
+    std::stable_sort(cand.begin(), cand.end(), [](const Cand& a, const Cand& b) { return a.cost > b.cost; });
+    g.items[0].clear();
+    g.items[1].clear();
+    for (const Cand& c : cand) g.items[c.fam].push_back(make_int2(c.e, c.s));
+    // Device plan at the start of the workspace, data after it.
+    const size_t hdr = a256(sizeof(GHeader));
+    const size_t pro = hdr, red = a256(pro + (size_t)(n + 1) * sizeof(long));
+    const size_t mesh = a256(red + (size_t)(n + 1) * sizeof(long));
+    const size_t it0 = a256(mesh + (size_t)n * sizeof(GMesh));
+    const size_t it1 = a256(it0 + g.items[0].size() * sizeof(int2));
+    g.header_bytes = a256(it1 + g.items[1].size() * sizeof(int2));
+    g.total = g.header_bytes + off;
+    // Resident workgroups of each family: the persistent grids.
+    hipFuncSetAttribute((const void*)k_group<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGroupStage);
+    hipFuncSetAttribute((const void*)k_group<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGroupStage);
+    for (int fam = 0; fam < 2; ++fam) {
+        int nb = 0;
+        hipError_t rc = fam == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_group<0>, 64 * kFamWaves[0],
+                                                                                   kGroupStage)
+                                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_group<1>, 64 * kFamWaves[1],
+                                                                                   kGroupStage);
+        if (rc != hipSuccess || nb < 1) nb = 1;
+        g.grid[fam] = nb * g.ncu;
+    }
+    return MESHPROP_OK;
+}
+
+void build_image(const meshprop_group& g, unsigned char* ws) {
+    const int n = (int)g.ent.size();
+    const size_t hdr = a256(sizeof(GHeader));
+    const size_t pro = hdr, red = a256(pro + (size_t)(n + 1) * sizeof(long));
+    const size_t mesh = a256(red + (size_t)(n + 1) * sizeof(long));
+    const size_t it0 = a256(mesh + (size_t)n * sizeof(GMesh));
+    const size_t it1 = a256(it0 + g.items[0].size() * sizeof(int2));
+    g.image.assign(g.header_bytes, 0);
+    unsigned char* im = g.image.data();
+    GHeader* H = reinterpret_cast<GHeader*>(im);
+    H->nmesh = n;
+    H->nitems[0] = (int)g.items[0].size();
+    H->nitems[1] = (int)g.items[1].size();
+    H->pro_total = g.pro_total;
+    H->red_total = g.red_total;
+    H->off_pro = (long)pro;
+    H->off_red = (long)red;
+    H->off_mesh = (long)mesh;
+    H->off_items[0] = (long)it0;
+    H->off_items[1] = (long)it1;
+    long* pb = reinterpret_cast<long*>(im + pro);
+    long* rb = reinterpret_cast<long*>(im + red);
+    GMesh* M = reinterpret_cast<GMesh*>(im + mesh);
+    unsigned char* data = ws + g.header_bytes;
+    long pacc = 0, racc = 0;
+    for (int i = 0; i < n; ++i) {
+        const meshprop_group_entry& e = g.ent[i];
+        const meshprop_group::Info& f = g.info[i];
+        pb[i] = pacc;
+        rb[i] = racc;
+        pacc += f.pro_units;
+        racc += f.red_units;
+        GMesh& m = M[i];
+        m.what = f.what;
+        m.var = f.var;
+        m.X = e.X;
+        m.N = e.N;
+        m.L = e.L;
+        m.B = e.B;
+        m.fft = e.fft;
+        m.LPV = f.LPV;
+        m.NP = f.NP;
+        m.C = f.C;
+        m.np = f.np;
+        m.ns = f.ns;
+        m.ldp = e.ldp;
+        m.lds = e.lds;
+        m.ldu = e.ldu;
+        m.S = f.S;
+        m.build = f.build;
+        m.assign = (e.flags & MESHPROP_GROUP_DP_ASSIGN) ? 1 : 0;
+        m.ntiles = f.ntiles;
+        m.E = f.E;
+        m.pro_chunks = (f.NP + 255) / 256;
+        m.g0 = e.geom0;
+        m.g1 = e.geom1;
+        m.p = e.p;
+        m.s = (f.ns > 0) ? e.s : nullptr;
+        m.dp = e.dp;
+        unsigned char* tb = f.own_tables ? data + f.off_tab : (unsigned char*)e.tables;
+        m.kind = reinterpret_cast<int*>(tb + f.tab_kind);
+        m.rows_fwd = reinterpret_cast<float4*>(tb + f.tab_fwd);
+        m.rows_inv = reinterpret_cast<float4*>(tb + f.tab_inv);
+        m.dtdp = reinterpret_cast<float2*>(tb + f.tab_dtdp);
+        m.partial = reinterpret_cast<float*>(data + f.off_partial);
+        m.u_in = (const float2*)e.u_in;
+        m.u_out = (float2*)e.u_out;
+        m.target = (const float2*)e.target;
+        m.g_out = (const float2*)e.dJdu_out;
+        m.g_in = (float2*)e.dJdu_in;
+        m.u_rec = (e.op == MESHPROP_OP_BACKDIFF) ? (float2*)e.u_in : nullptr;
+    }
+    pb[n] = pacc;
+    rb[n] = racc;
+    if (!g.items[0].empty()) std::memcpy(im + it0, g.items[0].data(), g.items[0].size() * sizeof(int2));
+    if (!g.items[1].empty()) std::memcpy(im + it1, g.items[1].data(), g.items[1].size() * sizeof(int2));
+}
+
+int hip_status(hipError_t e) {
+    if (e == hipSuccess) return MESHPROP_OK;
+    return MESHPROP_ERR_HIP;
+}
+
+}  // namespace
+
+extern "C" {
+
+int meshprop_group_create(const meshprop_group_entry* entries, int n, meshprop_group** out) {
+    if (!out || n < 0 || (n > 0 && !entries)) return MESHPROP_ERR_ARG;
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+        const int rc = check_entry(entries[i]);
+        if (rc) return rc;
+    }
+    meshprop_group* g = new (std::nothrow) meshprop_group();
+    if (!g) return MESHPROP_ERR_ARG;
+    g->ent.assign(entries, entries + n);
+    int rc = plan_group(*g);
+    if (rc) {
+        delete g;
+        return rc;
+    }
+    *out = g;
+    return MESHPROP_OK;
+}
+
+void meshprop_group_destroy(meshprop_group* g) {
+    if (!g) return;
+    if (g->ev_fork) hipEventDestroy(g->ev_fork);
+    if (g->ev_join) hipEventDestroy(g->ev_join);
+    if (g->side) hipStreamDestroy(g->side);
+    delete g;
+}
+
+size_t meshprop_group_workspace_size(const meshprop_group* g) { return g ? g->total : 0; }
+
+size_t meshprop_group_tables_size(int X, int N, int L, int fft, const meshprop_tile* tile) {
+    (void)fft;
+    if (X < 0 || X > 3 || X == X_ORTH || N <= 0 || (N & 1) || N > 1024 || L < 0) return 0;
+    meshprop_tile t = tile ? *tile : meshprop_tile{0, 0, 0, 0};
+    size_t k, f, i, d, total;
+    table_layout(N, L, group_w(N, t), k, f, i, d, total);
+    return total;
+}
+
+int meshprop_group_upload(const meshprop_group* g, void* ws, size_t bytes, void* stream) {
+    if (!g) return MESHPROP_ERR_ARG;
+    if (!ws || bytes < g->total || (((uintptr_t)ws) & 255)) return MESHPROP_ERR_WORKSPACE;
+    build_image(*g, (unsigned char*)ws);
+    return hip_status(hipMemcpyAsync(ws, g->image.data(), g->header_bytes, hipMemcpyHostToDevice,
+                                     (hipStream_t)stream));
+}
+
+int meshprop_group_signature(const meshprop_group* g) { return (g && !g->items[1].empty()) ? 1 : 0; }
+
+int meshprop_group_launch(const meshprop_group* gc, void* ws, size_t bytes, void* stream) {
+    meshprop_group* g = const_cast<meshprop_group*>(gc);
+    if (!g) return MESHPROP_ERR_ARG;
+    if (!ws || bytes < g->total || (((uintptr_t)ws) & 255)) return MESHPROP_ERR_WORKSPACE;
+    hipStream_t st = (hipStream_t)stream;
+    unsigned char* w = (unsigned char*)ws;
+    const int pro_grid = 4 * g->ncu;
+    hipLaunchKernelGGL(k_group_prologue, dim3(pro_grid), dim3(256), 0, st, (const unsigned char*)w);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return MESHPROP_ERR_HIP;
+    const bool wide = !g->items[1].empty();
+    if (wide) {
+        // The W = 16 family runs beside family 0 on a second stream (fork / join by events,
+        // which a graph capture records as two parallel branches).
+        if (!g->side) {
+            if ((e = hipStreamCreateWithFlags(&g->side, hipStreamNonBlocking)) != hipSuccess) return hip_status(e);
+            if ((e = hipEventCreateWithFlags(&g->ev_fork, hipEventDisableTiming)) != hipSuccess) return hip_status(e);
+            if ((e = hipEventCreateWithFlags(&g->ev_join, hipEventDisableTiming)) != hipSuccess) return hip_status(e);
+        }
+        if ((e = hipEventRecord(g->ev_fork, st)) != hipSuccess) return hip_status(e);
+        if ((e = hipStreamWaitEvent(g->side, g->ev_fork, 0)) != hipSuccess) return hip_status(e);
+        hipLaunchKernelGGL((k_group<1>), dim3(g->grid[1]), dim3(64 * kFamWaves[1]), kGroupStage, g->side, w);
+        if ((e = hipGetLastError()) != hipSuccess) return hip_status(e);
+        if ((e = hipEventRecord(g->ev_join, g->side)) != hipSuccess) return hip_status(e);
+    }
+    hipLaunchKernelGGL((k_group<0>), dim3(g->grid[0]), dim3(64 * kFamWaves[0]), kGroupStage, st, w);
+    if ((e = hipGetLastError()) != hipSuccess) return hip_status(e);
+    if (wide && (e = hipStreamWaitEvent(st, g->ev_join, 0)) != hipSuccess) return hip_status(e);
+    hipLaunchKernelGGL(k_group_reduce, dim3(4 * g->ncu), dim3(256), 0, st, (const unsigned char*)w);
+    if ((e = hipGetLastError()) != hipSuccess) return hip_status(e);
+    return MESHPROP_OK;
+}
+
+int meshprop_group_entry_info(const meshprop_group* g, int i, meshprop_tile* tile, int* items) {
+    if (!g || i < 0 || i >= (int)g->ent.size()) return MESHPROP_ERR_ARG;
+    const meshprop_group::Info& f = g->info[i];
+    if (tile) {
+        tile->w = f.W;
+        tile->v = f.V;
+        tile->waves = kFamWaves[f.fam];
+        tile->chunk = f.C;
+    }
+    if (items) *items = f.S;
+    return MESHPROP_OK;
+}
+
+}  // extern "C"

@@ -77,15 +77,12 @@ __device__ __forceinline__ void row_of(const M2& M, const bool upper, dc& a, dc&
 // lane group, k-th waveguide of that lane) sits at [l][k][t], so the k-th row load of all
 // lanes reads one contiguous block.
 template <class R>
-__global__ void k_rows(int X, int N, int L, int NP, int LPV, int fft, const int* __restrict__ g0,
-                       const int* __restrict__ g1,
-                       const float* __restrict__ p, int ldp, const float* __restrict__ s, int lds,
-                       const float* __restrict__ dp, int tabs, int* __restrict__ kind, R* __restrict__ tab_a,
-                       R* __restrict__ tab_b) {
-    const int w = blockIdx.x * blockDim.x + threadIdx.x;
-    const int l = blockIdx.y;
-    if (l >= L || w >= NP) return;
-    if (w == 0) kind[l] = layer_kind(fft, g0, g1, l, NP);
+__device__ void rows_at(const int X, const int N, const int NP, const int LPV, const int fft,
+                        const int* __restrict__ g0, const int* __restrict__ g1, const float* __restrict__ p,
+                        const int ldp, const float* __restrict__ s, const int lds, const float* __restrict__ dp,
+                        const int tabs, int* __restrict__ kind, R* __restrict__ tab_a, R* __restrict__ tab_b,
+                        const int l, const int w) {
+    if (w == 0 && kind) kind[l] = layer_kind(fft, g0, g1, l, NP);
     const int Wd = NP / LPV;
     const size_t o = (size_t)l * NP + (size_t)(w % Wd) * LPV + w / Wd;
     int m = 0;
@@ -127,6 +124,18 @@ __global__ void k_rows(int X, int N, int L, int NP, int LPV, int fft, const int*
     }
 }
 
+template <class R>
+__global__ void k_rows(int X, int N, int L, int NP, int LPV, int fft, const int* __restrict__ g0,
+                       const int* __restrict__ g1,
+                       const float* __restrict__ p, int ldp, const float* __restrict__ s, int lds,
+                       const float* __restrict__ dp, int tabs, int* __restrict__ kind, R* __restrict__ tab_a,
+                       R* __restrict__ tab_b) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = blockIdx.y;
+    if (l >= L || w >= NP) return;
+    rows_at<R>(X, N, NP, LPV, fft, g0, g1, p, ldp, s, lds, dp, tabs, kind, tab_a, tab_b, l, w);
+}
+
 // dT/dp_k of every slot (zero for inactive slots), for the in-kernel dJdT -> dp step.
 // Layout: dtdp[(l*np*4 + k*4 + e)*NPh + m] with e = 00, 01, 10, 11 (slot-contiguous), element
 // type float2 (complex) or float (real).
@@ -142,13 +151,10 @@ __device__ __forceinline__ float dc_to<float>(const dc z) {
 }
 
 template <class T>
-__global__ void k_dtdp(int X, int N, int L, int NP, int fft, const int* __restrict__ g0, const int* __restrict__ g1,
-                       const float* __restrict__ p, int ldp, const float* __restrict__ s, int lds,
-                       T* __restrict__ dtdp) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    const int l = blockIdx.y;
+__device__ void dtdp_at(const int X, const int N, const int NP, const int fft, const int* __restrict__ g0,
+                        const int* __restrict__ g1, const float* __restrict__ p, const int ldp,
+                        const float* __restrict__ s, const int lds, T* __restrict__ dtdp, const int l, const int m) {
     const int NPh = NP / 2;
-    if (l >= L || m >= NPh) return;
     const int np = n_phase_of(X), ns = n_split_of(X);
     // Layout [l][e][m] (e = param*4 + entry) so that consecutive slots are contiguous.
     T* out = dtdp + (size_t)l * NPh * np * 4 + m;
@@ -181,6 +187,16 @@ __global__ void k_dtdp(int X, int N, int L, int NP, int fft, const int* __restri
         out[(size_t)(k * 4 + 2) * NPh] = dc_to<T>(dT[k].a10);
         out[(size_t)(k * 4 + 3) * NPh] = dc_to<T>(dT[k].a11);
     }
+}
+
+template <class T>
+__global__ void k_dtdp(int X, int N, int L, int NP, int fft, const int* __restrict__ g0, const int* __restrict__ g1,
+                       const float* __restrict__ p, int ldp, const float* __restrict__ s, int lds,
+                       T* __restrict__ dtdp) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = blockIdx.y;
+    if (l >= L || m >= NP / 2) return;
+    dtdp_at<T>(X, N, NP, fft, g0, g1, p, ldp, s, lds, dtdp, l, m);
 }
 
 // =======================================================================================
@@ -242,12 +258,12 @@ __device__ __forceinline__ void load_rows(R (&r)[W], const R* base, const int LP
 // register sets (rows of layer l+1 are in flight while layer l is applied) and two field
 // register sets, so that no result overwrites an input of its own layer.
 template <class T, int W, int V>
-__global__ void __launch_bounds__(256) k_fwd(int N, int l0, int nl, int B, int LPV, int NP,
-                                             const int* __restrict__ kind,
-                                             const typename RowOf<T>::type* __restrict__ rows, const T* u_in,
-                                             T* u_out, int ldu) {
+__device__ __forceinline__ void fwd_tile(const int N, const int l0, const int nl, const int B, const int LPV,
+                                         const int NP, const int* __restrict__ kind,
+                                         const typename RowOf<T>::type* __restrict__ rows, const T* u_in, T* u_out,
+                                         const int ldu, const long tile) {
     typedef typename RowOf<T>::type R;
-    const Lanes q = lanes_of<V>(LPV, blockIdx.x);
+    const Lanes q = lanes_of<V>(LPV, tile);
     T ua[V][W], ub[V][W];
     load_tile(ua, u_in, q, N, B, ldu);
     if (nl > 0) {
@@ -280,6 +296,14 @@ __global__ void __launch_bounds__(256) k_fwd(int N, int l0, int nl, int B, int L
         }
     }
     store_tile(ua, u_out, q, N, B, ldu);
+}
+
+template <class T, int W, int V>
+__global__ void __launch_bounds__(256) k_fwd(int N, int l0, int nl, int B, int LPV, int NP,
+                                             const int* __restrict__ kind,
+                                             const typename RowOf<T>::type* __restrict__ rows, const T* u_in,
+                                             T* u_out, int ldu) {
+    fwd_tile<T, W, V>(N, l0, nl, B, LPV, NP, kind, rows, u_in, u_out, ldu, blockIdx.x);
 }
 
 // ---- fwddiff --------------------------------------------------------------------------
@@ -473,143 +497,154 @@ struct BwdDepth {
 };
 
 template <class T, int W, int V, bool UNI>
+__device__ __forceinline__ void bwd_tile(const int N, const int l0, const int nl, const int B, const int LPV,
+                                         const int NP, const int C, const int* __restrict__ kind,
+                                         const typename RowOf<T>::type* __restrict__ rows_inv,
+                                         const typename RowOf<T>::type* __restrict__ rows_adj,
+                                         const T* __restrict__ dtdp, float* __restrict__ my_partial, const T* u_out,
+                                         const T* g_out, const T* target, T* u_in, T* g_in, const int ldu,
+                                         const long tile, const bool first,
+                                         typename RowOf<T>::type* __restrict__ stage) {
+    typedef typename RowOf<T>::type R;
+    constexpr int np = NPhase<T>::value;
+    constexpr int D = BwdDepth<W>::value;  // row register sets
+    constexpr int PD = D / 2;              // prefetch distance in layers
+    const int nw = blockDim.x >> 6;
+    const int NPh = NP / 2;
+    const int lane = threadIdx.x & 63;
+    const int wave_u = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave index as a scalar
+    const int wstride = C * W * LPV;
+    const int* kp = kind + l0;
+    const int last = nl - 1;
+    constexpr int PF = 2;  // stage-end slots per lane whose dT/dp is prefetched
+    const Lanes q = lanes_of<V>(LPV, tile);
+    T ua[V][W], ga[V][W], ub[V][W], gb[V][W];
+    load_tile(ua, u_out, q, N, B, ldu);
+    if (target != nullptr) {
+        T y[V][W];
+        load_tile(y, target, q, N, B, ldu);
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+#pragma unroll
+            for (int k = 0; k < W; ++k) ga[v][k] = fsub2(ua[v][k], y[v][k]);
+    } else {
+        load_tile(ga, g_out, q, N, B, ldu);
+    }
+    // Local layers are visited in the order s = 0, 1, ..., as l = last - s.
+    const R* rip = rows_inv + (size_t)l0 * NP + q.t;
+    const R* rap = rows_adj + (size_t)l0 * NP + q.t;
+    R ri[D][W], ra[D][W];
+    int kk[D];
+    auto lrow = [&](const int s) { return (size_t)(s < last ? last - s : 0) * NP; };
+#pragma unroll
+    for (int d = 0; d < PD; ++d) {
+        if (nl > 0) {
+            kk[d] = kp[s_clamp(last, d)];
+            load_rows(ri[d], rip + lrow(d), LPV);
+            if constexpr (!UNI) load_rows(ra[d], rap + lrow(d), LPV);
+        }
+    }
+#define MP_BWD_LAYER(KD, U, G, UO, GO, RI, RA, LC)                                                  \
+    {                                                                                               \
+    if constexpr (UNI)                                                                          \
+        bwd_layer<T, W, V, UNI>(KD, U, G, UO, GO, RI, RI, stage, q, LPV, C, LC);                \
+    else                                                                                        \
+        bwd_layer<T, W, V, UNI>(KD, U, G, UO, GO, RI, RA, stage, q, LPV, C, LC);                \
+    }
+    for (int c0 = 0; c0 < nl; c0 += C) {
+        const int cl = (C < nl - c0) ? C : nl - c0;
+        // Prefetch the dT/dp this wave converts first at the end of the stage.
+        T pre[PF][4 * np];
+        if (wave_u < cl) {
+            const T* src = dtdp + (size_t)(l0 + last - (c0 + wave_u)) * NPh * np * 4 + lane;
+#pragma unroll
+            for (int r = 0; r < PF; ++r)
+                if (lane + 64 * r < NPh) {
+#pragma unroll
+                    for (int e = 0; e < 4 * np; ++e) pre[r][e] = src[e * NPh + 64 * r];
+                }
+        }
+        int lc = 0;
+        for (; lc + D <= cl; lc += D) {
+            const int s = c0 + lc;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const int dn = (d + PD) % D;  // register set receiving layer s + d + PD
+                kk[dn] = kp[s_clamp(last, s + d + PD)];
+                load_rows(ri[dn], rip + lrow(s + d + PD), LPV);
+                if constexpr (!UNI) load_rows(ra[dn], rap + lrow(s + d + PD), LPV);
+                if ((d & 1) == 0)
+                    MP_BWD_LAYER(kk[d], ua, ga, ub, gb, ri[d], ra[d], lc + d)
+                else
+                    MP_BWD_LAYER(kk[d], ub, gb, ua, ga, ri[d], ra[d], lc + d)
+            }
+        }
+        // Remainder of the last stage (cl - lc < D layers): register sets 0.. hold them.
+        int left = cl - lc;
+#pragma unroll
+        for (int d = 0; d < D - 1; ++d) {
+            if (d < left) {
+                if (d + PD < D) {
+                    const int dn = (d + PD) % D;
+                    kk[dn] = kp[s_clamp(last, c0 + lc + d + PD)];
+                    load_rows(ri[dn], rip + lrow(c0 + lc + d + PD), LPV);
+                    if constexpr (!UNI) load_rows(ra[dn], rap + lrow(c0 + lc + d + PD), LPV);
+                }
+                if ((d & 1) == 0)
+                    MP_BWD_LAYER(kk[d], ua, ga, ub, gb, ri[d], ra[d], lc + d)
+                else
+                    MP_BWD_LAYER(kk[d], ub, gb, ua, ga, ri[d], ra[d], lc + d)
+            }
+        }
+        if (left & 1) {
+            copy_state(ub, ua);
+            copy_state(gb, ga);
+        }
+        __syncthreads();
+        // dJdT -> dp: wave w converts stage layers w, w + nw, ... (the first one prefetched).
+        for (int lcc = wave_u; lcc < cl; lcc += nw) {
+            const int la = l0 + last - (c0 + lcc);
+            const int kdl = kind[la];
+            const R* s0 = stage + lcc * W * LPV;
+            float* pout = my_partial + (size_t)la * NPh * np;
+            const T* src = dtdp + (size_t)la * NPh * np * 4;
+            int m0 = lane;
+            if (lcc == wave_u) {
+#pragma unroll
+                for (int r = 0; r < PF; ++r)
+                    if (lane + 64 * r < NPh)
+                        convert_slot<T, W, np>(s0, wstride, nw, LPV, NP, pout, kdl, lane + 64 * r, pre[r], first);
+                m0 = lane + 64 * PF;
+            }
+            for (int m = m0; m < NPh; m += 64) {
+                T dd[4 * np];
+#pragma unroll
+                for (int e = 0; e < 4 * np; ++e) dd[e] = src[e * NPh + m];
+                convert_slot<T, W, np>(s0, wstride, nw, LPV, NP, pout, kdl, m, dd, first);
+            }
+        }
+        __syncthreads();
+    }
+#undef MP_BWD_LAYER
+    if (u_in != nullptr) store_tile(ua, u_in, q, N, B, ldu);
+    if (g_in != nullptr) store_tile(ga, g_in, q, N, B, ldu);
+}
+
+template <class T, int W, int V, bool UNI>
 __global__ void __launch_bounds__(64 * bwd_max_waves(W, V, UNI)) k_bwd(
     int N, int l0, int nl, int B, int LPV, int NP, int C, int np_unused, const int* __restrict__ kind,
     const typename RowOf<T>::type* __restrict__ rows_inv, const typename RowOf<T>::type* __restrict__ rows_adj,
     const T* __restrict__ dtdp, float* __restrict__ partial, size_t part_stride, long ntiles, const T* u_out,
     const T* g_out, const T* target, T* u_in, T* g_in, int ldu) {
     typedef typename RowOf<T>::type R;
-    constexpr int np = NPhase<T>::value;
-    constexpr int D = BwdDepth<W>::value;  // row register sets
-    constexpr int PD = D / 2;              // prefetch distance in layers
     extern __shared__ unsigned char smem_raw[];
     R* stage = reinterpret_cast<R*>(smem_raw);  // [nw][C][W][LPV]
-    const int nw = blockDim.x >> 6;
-    const int NPh = NP / 2;
-    const int lane = threadIdx.x & 63;
-    const int wave_u = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave index as a scalar
-    const int wstride = C * W * LPV;
-    float* my_partial = partial + (size_t)blockIdx.x * part_stride;
-    const int* kp = kind + l0;
-    const int last = nl - 1;
-    constexpr int PF = 2;  // stage-end slots per lane whose dT/dp is prefetched
     (void)np_unused;
-
+    float* my_partial = partial + (size_t)blockIdx.x * part_stride;
     bool first = true;
     for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const Lanes q = lanes_of<V>(LPV, tile);
-        T ua[V][W], ga[V][W], ub[V][W], gb[V][W];
-        load_tile(ua, u_out, q, N, B, ldu);
-        if (target != nullptr) {
-            T y[V][W];
-            load_tile(y, target, q, N, B, ldu);
-#pragma unroll
-            for (int v = 0; v < V; ++v)
-#pragma unroll
-                for (int k = 0; k < W; ++k) ga[v][k] = fsub2(ua[v][k], y[v][k]);
-        } else {
-            load_tile(ga, g_out, q, N, B, ldu);
-        }
-        // Local layers are visited in the order s = 0, 1, ..., as l = last - s.
-        const R* rip = rows_inv + (size_t)l0 * NP + q.t;
-        const R* rap = rows_adj + (size_t)l0 * NP + q.t;
-        R ri[D][W], ra[D][W];
-        int kk[D];
-        auto lrow = [&](const int s) { return (size_t)(s < last ? last - s : 0) * NP; };
-#pragma unroll
-        for (int d = 0; d < PD; ++d) {
-            if (nl > 0) {
-                kk[d] = kp[s_clamp(last, d)];
-                load_rows(ri[d], rip + lrow(d), LPV);
-                if constexpr (!UNI) load_rows(ra[d], rap + lrow(d), LPV);
-            }
-        }
-#define MP_BWD_LAYER(KD, U, G, UO, GO, RI, RA, LC)                                                  \
-    {                                                                                               \
-        if constexpr (UNI)                                                                          \
-            bwd_layer<T, W, V, UNI>(KD, U, G, UO, GO, RI, RI, stage, q, LPV, C, LC);                \
-        else                                                                                        \
-            bwd_layer<T, W, V, UNI>(KD, U, G, UO, GO, RI, RA, stage, q, LPV, C, LC);                \
-    }
-        for (int c0 = 0; c0 < nl; c0 += C) {
-            const int cl = (C < nl - c0) ? C : nl - c0;
-            // Prefetch the dT/dp this wave converts first at the end of the stage.
-            T pre[PF][4 * np];
-            if (wave_u < cl) {
-                const T* src = dtdp + (size_t)(l0 + last - (c0 + wave_u)) * NPh * np * 4 + lane;
-#pragma unroll
-                for (int r = 0; r < PF; ++r)
-                    if (lane + 64 * r < NPh) {
-#pragma unroll
-                        for (int e = 0; e < 4 * np; ++e) pre[r][e] = src[e * NPh + 64 * r];
-                    }
-            }
-            int lc = 0;
-            for (; lc + D <= cl; lc += D) {
-                const int s = c0 + lc;
-#pragma unroll
-                for (int d = 0; d < D; ++d) {
-                    constexpr int dummy = 0;
-                    (void)dummy;
-                    const int dn = (d + PD) % D;  // register set receiving layer s + d + PD
-                    kk[dn] = kp[s_clamp(last, s + d + PD)];
-                    load_rows(ri[dn], rip + lrow(s + d + PD), LPV);
-                    if constexpr (!UNI) load_rows(ra[dn], rap + lrow(s + d + PD), LPV);
-                    if ((d & 1) == 0)
-                        MP_BWD_LAYER(kk[d], ua, ga, ub, gb, ri[d], ra[d], lc + d)
-                    else
-                        MP_BWD_LAYER(kk[d], ub, gb, ua, ga, ri[d], ra[d], lc + d)
-                }
-            }
-            // Remainder of the last stage (cl - lc < D layers): register sets 0.. hold them.
-            int left = cl - lc;
-#pragma unroll
-            for (int d = 0; d < D - 1; ++d) {
-                if (d < left) {
-                    if (d + PD < D) {
-                        const int dn = (d + PD) % D;
-                        kk[dn] = kp[s_clamp(last, c0 + lc + d + PD)];
-                        load_rows(ri[dn], rip + lrow(c0 + lc + d + PD), LPV);
-                        if constexpr (!UNI) load_rows(ra[dn], rap + lrow(c0 + lc + d + PD), LPV);
-                    }
-                    if ((d & 1) == 0)
-                        MP_BWD_LAYER(kk[d], ua, ga, ub, gb, ri[d], ra[d], lc + d)
-                    else
-                        MP_BWD_LAYER(kk[d], ub, gb, ua, ga, ri[d], ra[d], lc + d)
-                }
-            }
-            if (left & 1) {
-                copy_state(ub, ua);
-                copy_state(gb, ga);
-            }
-            __syncthreads();
-            // dJdT -> dp: wave w converts stage layers w, w + nw, ... (the first one prefetched).
-            for (int lcc = wave_u; lcc < cl; lcc += nw) {
-                const int la = l0 + last - (c0 + lcc);
-                const int kdl = kind[la];
-                const R* s0 = stage + lcc * W * LPV;
-                float* pout = my_partial + (size_t)la * NPh * np;
-                const T* src = dtdp + (size_t)la * NPh * np * 4;
-                int m0 = lane;
-                if (lcc == wave_u) {
-#pragma unroll
-                    for (int r = 0; r < PF; ++r)
-                        if (lane + 64 * r < NPh)
-                            convert_slot<T, W, np>(s0, wstride, nw, LPV, NP, pout, kdl, lane + 64 * r, pre[r], first);
-                    m0 = lane + 64 * PF;
-                }
-                for (int m = m0; m < NPh; m += 64) {
-                    T dd[4 * np];
-#pragma unroll
-                    for (int e = 0; e < 4 * np; ++e) dd[e] = src[e * NPh + m];
-                    convert_slot<T, W, np>(s0, wstride, nw, LPV, NP, pout, kdl, m, dd, first);
-                }
-            }
-            __syncthreads();
-        }
-#undef MP_BWD_LAYER
-        if (u_in != nullptr) store_tile(ua, u_in, q, N, B, ldu);
-        if (g_in != nullptr) store_tile(ga, g_in, q, N, B, ldu);
+        bwd_tile<T, W, V, UNI>(N, l0, nl, B, LPV, NP, C, kind, rows_inv, rows_adj, dtdp, my_partial, u_out, g_out,
+                               target, u_in, g_in, ldu, tile, first, stage);
         first = false;
     }
 }

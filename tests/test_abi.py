@@ -76,3 +76,49 @@ def test_argument_validation_without_gpu(lib):
     assert fn(L.MZI, 2048, 2, 1, fake, fake, fake, 2048, None, 0, fake, fake, 2048, 0, fake, 1 << 20, None) == 3
     # B = 0 is a no-op
     assert fn(L.MZI, 8, 2, 0, fake, fake, fake, 8, None, 0, fake, fake, 8, 0, fake, 1 << 20, None) == 0
+
+
+def _entry(Lb, **kw):
+    e = Lb.GroupEntry()
+    fake = 4096
+    e.op, e.X, e.N, e.L, e.B, e.fft = Lb.OP_FWDGRAD, Lb.MZI, 64, 64, 100, 0
+    e.geom0 = e.geom1 = e.p = e.dp = e.u_in = e.u_out = e.target = fake
+    for k, v in kw.items():
+        setattr(e, k, v)
+    e.ldp = kw.get("ldp", e.N // 2 * 2)
+    e.ldu = kw.get("ldu", e.N)
+    return e
+
+
+def test_group_plan_without_gpu(lib):
+    """Grouped-launch planning and its argument checks run on the host alone."""
+    import meshes_amd._lib as L
+    plan = ctypes.c_void_p()
+    ents = (L.GroupEntry * 3)(_entry(L), _entry(L, N=512, L=512, B=2048), _entry(L, op=L.OP_FWDPROP, N=1024, L=10))
+    assert lib.meshprop_group_create(ents, 3, ctypes.byref(plan)) == 0
+    try:
+        # tables + partials of each entry fit in the workspace
+        assert lib.meshprop_group_workspace_size(plan) > 512 * 512 * 16 * 2
+        t, n = L.Tile(), ctypes.c_int()
+        assert lib.meshprop_group_entry_info(plan, 1, ctypes.byref(t), ctypes.byref(n)) == 0
+        assert (t.w, t.v) == (8, 1) and n.value >= 1
+        assert lib.meshprop_group_entry_info(plan, 2, ctypes.byref(t), ctypes.byref(n)) == 0
+        assert t.w == 16 and lib.meshprop_group_signature(plan) == 1
+        assert lib.meshprop_group_entry_info(plan, 3, ctypes.byref(t), ctypes.byref(n)) == 1
+        # missing or misaligned workspace
+        assert lib.meshprop_group_upload(plan, None, 0, None) == 2
+        assert lib.meshprop_group_launch(plan, ctypes.c_void_p(4096 + 8), 1 << 40, None) == 2
+    finally:
+        lib.meshprop_group_destroy(plan)
+    bad = [dict(N=63), dict(op=L.OP_FWDDIFF), dict(X=L.ORTH), dict(X=L.MZI_GENERIC), dict(N=2048), dict(ldu=32),
+           dict(target=None), dict(dp=None), dict(u_in=None), dict(ldp=10)]
+    want = [1, 1, 3, 3, 3, 1, 1, 1, 1, 1]
+    for kw, rc in zip(bad, want):
+        one = (L.GroupEntry * 1)(_entry(L, **kw))
+        assert lib.meshprop_group_create(one, 1, ctypes.byref(plan)) == rc, kw
+    # forward entries take the lossy crossing; tables size covers kind + two row tables + dT/dp
+    one = (L.GroupEntry * 1)(_entry(L, op=L.OP_FWDPROP, X=L.MZI_GENERIC, ldp=64))
+    assert lib.meshprop_group_create(one, 1, ctypes.byref(plan)) == 0
+    lib.meshprop_group_destroy(plan)
+    assert lib.meshprop_group_tables_size(L.MZI, 256, 256, 0, None) >= 256 * 256 * 16 * 2 + 256 * 128 * 64
+    assert lib.meshprop_group_tables_size(L.ORTH, 256, 256, 0, None) == 0
