@@ -154,6 +154,48 @@ def bench_batched(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_serving(args):
+    """Poisson request stream over 256 resident Clements meshes (N log-uniform in [16, 1024],
+    depth N), 70 % forward / 30 % gradient requests, lognormal batches (median 64, sigma 1.5):
+    the continuous-batching server against sequential plain launches, per arrival rate."""
+    import torch
+    from meshes_amd.serving import MeshServer, poisson_requests, resident_meshes
+
+    rates = [float(r) for r in args.rates.split(",")]
+    meshes = resident_meshes(256, seed=args.seed)
+    server = MeshServer(meshes, hbm_budget=args.hbm_gib << 30, slots=3)
+    warm = poisson_requests(meshes, rate=20000.0, n=200, seed=args.seed + 99)
+    server.serve(warm)
+    server.serve_sequential(warm[:20])
+    sweep = []
+    for k, rate in enumerate(rates):
+        n = int(min(max(rate * args.window, 150), args.max_requests))
+        reqs = poisson_requests(meshes, rate=rate, n=n, seed=args.seed + k)
+        _, st = server.serve(reqs)
+        seq_reqs = poisson_requests(meshes, rate=rate, n=n, seed=args.seed + k)
+        _, sq = server.serve_sequential(seq_reqs)
+        sweep.append({"rate_req_s": rate, "served": st, "sequential": sq})
+        print(json.dumps({"rate": rate, "served_p50_ms": st["p50_ms"], "served_p99_ms": st["p99_ms"],
+                          "seq_p50_ms": sq["p50_ms"], "seq_p99_ms": sq["p99_ms"]}), file=sys.stderr, flush=True)
+        del reqs, seq_reqs
+        torch.cuda.empty_cache()
+    top = sweep[-1]
+    out = {
+        "metric": "served request throughput and p50/p99 latency, Poisson fwd/grad stream over 256 resident meshes",
+        "value": top["served"]["throughput_req_s"], "unit": "requests/s", "n_gpus": 1,
+        "steps": sum(s["served"]["requests"] for s in sweep), "warmup": len(warm), "ms_per_step": None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 (complex64 fields, float32 phases)",
+        "data": "synthetic: seeded phases, Poisson arrivals, lognormal batches of CN(0,1) unit vectors on device",
+        "config": {"workload": "256 Clements MZI meshes, N log-uniform in [16,1024], depth N; 70/30 fwd/grad; "
+                               "batch lognormal median 64 sigma 1.5", "rates": rates, "slots": 3,
+                   "hbm_budget_GiB": args.hbm_gib},
+        "latency_ms": {"p50": top["served"]["p50_ms"], "p99": top["served"]["p99_ms"]},
+        "sweep": sweep,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -163,8 +205,13 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--tile-fwd", type=str, default="")
     ap.add_argument("--tile-bwd", type=str, default="")
-    ap.add_argument("--workload", choices=["fwdgrad", "batched"], default="fwdgrad",
-                    help="fwdgrad: the single-mesh step (default); batched: the heterogeneous grouped mix")
+    ap.add_argument("--workload", choices=["fwdgrad", "batched", "serving"], default="fwdgrad",
+                    help="fwdgrad: the single-mesh step (default); batched: the heterogeneous grouped mix; "
+                         "serving: the continuous-batching request loop")
+    ap.add_argument("--rates", type=str, default="100,1000,5000,20000")
+    ap.add_argument("--window", type=float, default=1.5, help="seconds of arrivals per rate (serving)")
+    ap.add_argument("--max-requests", type=int, default=3000)
+    ap.add_argument("--hbm-gib", type=int, default=24)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--probe-copy", type=int, default=0,
                     help="only copy a buffer of this many MiB a few times (calibrates HBM byte counters)")
@@ -185,6 +232,9 @@ def main():
 
     if args.workload == "batched":
         bench_batched(args)
+        return
+    if args.workload == "serving":
+        bench_serving(args)
         return
     import torch
     import meshes_amd._lib as Lb

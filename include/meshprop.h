@@ -152,7 +152,8 @@ enum meshprop_flags {
  * tiles through the whole depth (forward, then the adjoint for gradient entries).
  * A launch is a prologue (row tables of every entry that needs them), the persistent
  * kernel and the dp reduction, all grid-stride kernels whose launch shape depends only on
- * the device: every plan uploaded into the same workspace replays one captured hipGraph.
+ * the device and on which of them the plan needs (meshprop_group_signature): every plan
+ * with the same signature uploaded into the same workspace replays one captured hipGraph.
  *
  * Entry fields.  op: MESHPROP_OP_FWDPROP (u_in -> u_out), MESHPROP_OP_BACKDIFF (u_out,
  * dJdu_out or target -> dp, dJdu_in and, if u_in is set, the reconstructed input written
@@ -207,8 +208,16 @@ int meshprop_group_upload(const meshprop_group *g, void *workspace, size_t works
  * uploaded into the workspace.  The launch shape depends on the device and on
  * meshprop_group_signature() only. */
 int meshprop_group_launch(const meshprop_group *g, void *workspace, size_t workspace_bytes, void *stream);
-/* Launch-shape signature: two launches with equal signatures replay the same graph. */
+/* Launch-shape signature (MESHPROP_GROUP_SIG_* bits: which of the prologue, the two
+ * persistent kernel families and the reduction the plan launches).  Plans with equal
+ * signatures uploaded into the same workspace replay the same captured graph. */
 int meshprop_group_signature(const meshprop_group *g);
+enum meshprop_group_signature_bits {
+    MESHPROP_GROUP_SIG_NARROW = 1,    /* items of meshes with N <= 512 */
+    MESHPROP_GROUP_SIG_WIDE = 2,      /* items of meshes with N > 512 */
+    MESHPROP_GROUP_SIG_PROLOGUE = 4,  /* row tables to build */
+    MESHPROP_GROUP_SIG_REDUCE = 8     /* dp to reduce */
+};
 /* Resolved tile and number of work items of entry i. */
 int meshprop_group_entry_info(const meshprop_group *g, int i, meshprop_tile *tile, int *items);
 

@@ -136,14 +136,19 @@ class MeshGroup:
     def replan(self, jobs: List[GroupJob]):
         self.jobs = list(jobs)
         ents = (_lib.GroupEntry * max(1, len(self.jobs)))(*[_entry(j) for j in self.jobs])
+        dev = self.jobs[0].mesh.device if self.jobs else None
+        self.replan_entries(ents, len(self.jobs), dev)
+
+    def replan_entries(self, ents, n: int, device=None):
+        """Plan from a ctypes array of meshprop_group_entry (the fast path of the server)."""
         plan = ctypes.c_void_p()
-        _lib.check(self._lib.meshprop_group_create(ents, len(self.jobs), ctypes.byref(plan)), "group_create")
+        _lib.check(self._lib.meshprop_group_create(ents, n, ctypes.byref(plan)), "group_create")
         self._free_plan()
         self._plan = plan
         self._entries = ents
         need = int(self._lib.meshprop_group_workspace_size(self._plan))
         if self._ws is None or self._ws.numel() < need + 256:
-            dev = self.jobs[0].mesh.device if self.jobs else torch.device("cuda", torch.cuda.current_device())
+            dev = device if device is not None else torch.device("cuda", torch.cuda.current_device())
             self._ws = torch.empty(need + 256, dtype=torch.uint8, device=dev)
             self._graphs = {}  # captured graphs hold the old workspace address
         self.ws_bytes = need

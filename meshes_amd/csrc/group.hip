@@ -7,6 +7,7 @@
 // hipGraph replays every plan uploaded into the same workspace.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -24,6 +25,7 @@ constexpr GTile kGroupTiles[] = {{2, 1, 0}, {2, 2, 0}, {2, 4, 0}, {4, 1, 0}, {4,
 constexpr int kGroupVariants = sizeof(kGroupTiles) / sizeof(kGroupTiles[0]);
 constexpr int kFamWaves[2] = {4, 2};
 constexpr size_t kGroupStage = 64 * 1024;  // LDS of one workgroup (backdiff stage)
+constexpr size_t kGroupLds = kGroupStage + 16;  // stage + the queue slot
 
 enum { G_FWD = 1, G_GRAD = 2 };  // what an item runs
 enum { B_KIND = 1, B_FWD = 2, B_INV = 4, B_DTDP = 8 };
@@ -51,7 +53,15 @@ struct GHeader {
     long off_pro, off_red, off_mesh, off_items[2];
 };
 
-__device__ __forceinline__ int find_mesh(const long* begin, const int nmesh, const long u) {
+// Descriptors are read through the constant address space: they do not change while a launch
+// runs, their loads become scalar, and pointers loaded from constant memory are known to point
+// to global memory (generic pointers would compile to flat instructions, which also count
+// against the LDS wait counter and serialise the row prefetch with every LDS access).
+typedef __attribute__((address_space(4))) const GMesh KMesh;
+typedef __attribute__((address_space(4))) const long KLong;
+typedef __attribute__((address_space(4))) const int KInt;  // layer kinds: written by the prologue only
+
+__device__ __forceinline__ int find_mesh(KLong* begin, const int nmesh, const long u) {
     // Largest e with begin[e] <= u (entries without units have begin[e] == begin[e + 1]).
     int lo = 0, hi = nmesh - 1;
     while (lo < hi) {
@@ -66,23 +76,27 @@ __device__ __forceinline__ int find_mesh(const long* begin, const int nmesh, con
 
 __global__ void __launch_bounds__(256) k_group_prologue(const unsigned char* __restrict__ ws) {
     const GHeader* H = reinterpret_cast<const GHeader*>(ws);
-    const long* begin = reinterpret_cast<const long*>(ws + H->off_pro);
-    const GMesh* meshes = reinterpret_cast<const GMesh*>(ws + H->off_mesh);
+    KLong* begin = (KLong*)(ws + H->off_pro);
+    KMesh* meshes = (KMesh*)(ws + H->off_mesh);
     for (long u = blockIdx.x; u < H->pro_total; u += gridDim.x) {
         const int e = find_mesh(begin, H->nmesh, u);
-        const GMesh& M = meshes[e];
+        KMesh& M = meshes[e];
         const long local = u - begin[e];
         const int l = (int)(local / M.pro_chunks);
         const int w = (int)(local % M.pro_chunks) * 256 + threadIdx.x;
         if (w >= M.NP) continue;
+        const int* g0 = M.g0;
+        const int* g1 = M.g1;
+        const float* p = M.p;
+        const float* sp = M.s;
         if (M.build & B_FWD)
-            rows_at<float4>(M.X, M.N, M.NP, M.LPV, M.fft, M.g0, M.g1, M.p, M.ldp, M.s, M.lds, nullptr, TAB_FWD,
+            rows_at<float4>(M.X, M.N, M.NP, M.LPV, M.fft, g0, g1, p, M.ldp, sp, M.lds, nullptr, TAB_FWD,
                             M.kind, M.rows_fwd, nullptr, l, w);
         if (M.build & B_INV)
-            rows_at<float4>(M.X, M.N, M.NP, M.LPV, M.fft, M.g0, M.g1, M.p, M.ldp, M.s, M.lds, nullptr, TAB_INV,
+            rows_at<float4>(M.X, M.N, M.NP, M.LPV, M.fft, g0, g1, p, M.ldp, sp, M.lds, nullptr, TAB_INV,
                             M.kind, M.rows_inv, nullptr, l, w);
         if ((M.build & B_DTDP) && w < M.NP / 2)
-            dtdp_at<float2>(M.X, M.N, M.NP, M.fft, M.g0, M.g1, M.p, M.ldp, M.s, M.lds, M.dtdp, l, w);
+            dtdp_at<float2>(M.X, M.N, M.NP, M.fft, g0, g1, p, M.ldp, sp, M.lds, M.dtdp, l, w);
     }
 }
 
@@ -90,11 +104,11 @@ __global__ void __launch_bounds__(256) k_group_prologue(const unsigned char* __r
 // summed in slot order (four interleaved partial sums), so the result is deterministic.
 __global__ void __launch_bounds__(256) k_group_reduce(const unsigned char* __restrict__ ws) {
     const GHeader* H = reinterpret_cast<const GHeader*>(ws);
-    const long* begin = reinterpret_cast<const long*>(ws + H->off_red);
-    const GMesh* meshes = reinterpret_cast<const GMesh*>(ws + H->off_mesh);
+    KLong* begin = (KLong*)(ws + H->off_red);
+    KMesh* meshes = (KMesh*)(ws + H->off_mesh);
     for (long u = blockIdx.x; u < H->red_total; u += gridDim.x) {
         const int e = find_mesh(begin, H->nmesh, u);
-        const GMesh& M = meshes[e];
+        KMesh& M = meshes[e];
         const long idx = (u - begin[e]) * 256 + threadIdx.x;
         const int Nh = M.N / 2;
         if (idx >= (long)M.L * Nh * M.np) continue;
@@ -133,23 +147,24 @@ __device__ __forceinline__ P* uniform_ptr(P* p) {
 // kernel's register allocation above 256 (one wave per SIMD instead of two).  Its arguments
 // arrive in vector registers and are made scalar again with readfirstlane.
 template <int W, int V>
-__device__ __noinline__ void group_fwd_tile(const GMesh* Mp, const long tile_v) {
-    const GMesh& M = *uniform_ptr(Mp);
+__device__ __noinline__ void group_fwd_tile(KMesh* Mp, const long tile_v) {
+    KMesh& M = *uniform_ptr(Mp);
     const long tile = (long)__builtin_amdgcn_readfirstlane((unsigned)tile_v);
-    fwd_tile<float2, W, V>(M.N, 0, M.L, M.B, M.LPV, M.NP, M.kind, M.rows_fwd, M.u_in, M.u_out, M.ldu, tile);
+    fwd_tile<float2, W, V, KInt*>(M.N, 0, M.L, M.B, M.LPV, M.NP, (KInt*)M.kind, M.rows_fwd, M.u_in, M.u_out, M.ldu,
+                                  tile);
 }
 
 template <int W, int V>
-__device__ __forceinline__ void group_item(const GMesh& M, const int slot, float4* stage) {
+__device__ __forceinline__ void group_item(KMesh& M, const int slot, float4* stage) {
     float* part = (M.what & G_GRAD) ? M.partial + (size_t)slot * M.E : nullptr;
     bool first = true;
     for (long tile = slot; tile < M.ntiles; tile += M.S) {
         if (M.what & G_FWD)
             group_fwd_tile<W, V>(&M, tile);
         if (M.what & G_GRAD)
-            bwd_tile<float2, W, V, true>(M.N, 0, M.L, M.B, M.LPV, M.NP, M.C, M.kind, M.rows_inv, M.rows_inv, M.dtdp,
-                                         part, M.u_out, M.g_out, M.target, M.u_rec, M.g_in, M.ldu, tile, first,
-                                         stage);
+            bwd_tile<float2, W, V, true, KInt*>(M.N, 0, M.L, M.B, M.LPV, M.NP, M.C, (KInt*)M.kind, M.rows_inv,
+                                         M.rows_inv, M.dtdp, part, M.u_out, M.g_out,
+                                         M.target, M.u_rec, M.g_in, M.ldu, tile, first, stage);
         first = false;
     }
 }
@@ -157,33 +172,39 @@ __device__ __forceinline__ void group_item(const GMesh& M, const int slot, float
 template <int FAM>
 __global__ void __launch_bounds__(64 * kFamWaves[FAM]) k_group(unsigned char* __restrict__ ws) {
     GHeader* H = reinterpret_cast<GHeader*>(ws);
-    const GMesh* meshes = reinterpret_cast<const GMesh*>(ws + H->off_mesh);
-    const int2* items = reinterpret_cast<const int2*>(ws + H->off_items[FAM]);
+    KMesh* meshes = (KMesh*)(ws + H->off_mesh);
+    KInt* items = (KInt*)(ws + H->off_items[FAM]);  // (mesh, slot) pairs
     const int n = H->nitems[FAM];
     extern __shared__ unsigned char smem_raw[];
     float4* stage = reinterpret_cast<float4*>(smem_raw);
-    __shared__ int s_next;
+    // The queue slot lives behind the stage in the dynamic block: a static __shared__ variable
+    // would move the stage off LDS address 0, which made the adjoint about 1.4x slower.
+    int& s_next = *reinterpret_cast<int*>(smem_raw + kGroupStage);
+    // The first item of workgroup b is item b: the dispatcher spreads consecutive workgroups
+    // over distinct CUs, so the costliest items (the list is sorted by cost) start one per CU
+    // instead of wherever the first atomics land.  Later items come from the queue.
+    int it = blockIdx.x;
     for (;;) {
-        if (threadIdx.x == 0) s_next = atomicAdd(&H->grab[FAM], 1);
-        __syncthreads();
-        const int it = __builtin_amdgcn_readfirstlane(s_next);
-        __syncthreads();
         if (it >= n) break;
-        const int2 wk = items[it];
-        const GMesh& M = meshes[wk.x];
+        const int wm = items[2 * it], ws_slot = items[2 * it + 1];
+        KMesh& M = meshes[wm];
         if constexpr (FAM == 0) {
             switch (M.var) {
-                case 0: group_item<2, 1>(M, wk.y, stage); break;
-                case 1: group_item<2, 2>(M, wk.y, stage); break;
-                case 2: group_item<2, 4>(M, wk.y, stage); break;
-                case 3: group_item<4, 1>(M, wk.y, stage); break;
-                case 4: group_item<4, 2>(M, wk.y, stage); break;
-                case 5: group_item<8, 1>(M, wk.y, stage); break;
+                case 0: group_item<2, 1>(M, ws_slot, stage); break;
+                case 1: group_item<2, 2>(M, ws_slot, stage); break;
+                case 2: group_item<2, 4>(M, ws_slot, stage); break;
+                case 3: group_item<4, 1>(M, ws_slot, stage); break;
+                case 4: group_item<4, 2>(M, ws_slot, stage); break;
+                case 5: group_item<8, 1>(M, ws_slot, stage); break;
                 default: break;
             }
         } else {
-            group_item<16, 1>(M, wk.y, stage);
+            group_item<16, 1>(M, ws_slot, stage);
         }
+        if (threadIdx.x == 0) s_next = (int)gridDim.x + atomicAdd(&H->grab[FAM], 1);
+        __syncthreads();
+        it = __builtin_amdgcn_readfirstlane(s_next);
+        __syncthreads();
     }
     if (threadIdx.x == 0) {
         __threadfence();
@@ -213,8 +234,7 @@ struct meshprop_group {
     size_t header_bytes = 0, total = 0;
     long pro_total = 0, red_total = 0;
     int ncu = 256, grid[2] = {0, 0}, dev = 0;
-    hipStream_t side = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    int sig = 0;  // MESHPROP_GROUP_SIG_* bits: which launches the plan needs
     mutable std::vector<unsigned char> image;
 };
 
@@ -259,6 +279,42 @@ int cu_count() {
     return n;
 }
 
+// Per-device launch shape of the grouped kernels (resident workgroups per family) and the
+// second stream the W = 16 family runs on, set up once per device.
+struct DeviceGroupConfig {
+    std::once_flag once;
+    int ncu = 256, grid[2] = {256, 256};
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    std::mutex lock;  // orders fork / join sequences of concurrent host threads
+    bool ok = false;
+};
+constexpr int kMaxDevices = 64;
+DeviceGroupConfig g_dev_cfg[kMaxDevices];
+
+DeviceGroupConfig* device_config(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) return nullptr;
+    DeviceGroupConfig& c = g_dev_cfg[dev];
+    std::call_once(c.once, [&c]() {
+        c.ncu = cu_count();
+        (void)hipFuncSetAttribute((const void*)k_group<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGroupLds);
+        (void)hipFuncSetAttribute((const void*)k_group<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGroupLds);
+        for (int fam = 0; fam < 2; ++fam) {
+            int nb = 0;
+            hipError_t rc = fam == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_group<0>, 64 * kFamWaves[0],
+                                                                                       kGroupLds)
+                                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_group<1>, 64 * kFamWaves[1],
+                                                                                       kGroupLds);
+            if (rc != hipSuccess || nb < 1) nb = 1;
+            c.grid[fam] = nb * c.ncu;
+        }
+        c.ok = hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking) == hipSuccess &&
+               hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming) == hipSuccess &&
+               hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming) == hipSuccess;
+    });
+    return &c;
+}
+
 int check_entry(const meshprop_group_entry& e) {
     if (e.op != MESHPROP_OP_FWDPROP && e.op != MESHPROP_OP_BACKDIFF && e.op != MESHPROP_OP_FWDGRAD)
         return MESHPROP_ERR_ARG;
@@ -281,8 +337,9 @@ int check_entry(const meshprop_group_entry& e) {
 }
 
 int plan_group(meshprop_group& g) {
-    g.ncu = cu_count();
-    hipGetDevice(&g.dev);
+    if (hipGetDevice(&g.dev) != hipSuccess) g.dev = 0;
+    const DeviceGroupConfig* dc = device_config(g.dev);
+    g.ncu = dc ? dc->ncu : 256;
     const int n = (int)g.ent.size();
     g.info.resize(n);
     struct Cand {
@@ -375,18 +432,10 @@ int plan_group(meshprop_group& g) {
     const size_t it1 = a256(it0 + g.items[0].size() * sizeof(int2));
     g.header_bytes = a256(it1 + g.items[1].size() * sizeof(int2));
     g.total = g.header_bytes + off;
-    // Resident workgroups of each family: the persistent grids.
-    hipFuncSetAttribute((const void*)k_group<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGroupStage);
-    hipFuncSetAttribute((const void*)k_group<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGroupStage);
-    for (int fam = 0; fam < 2; ++fam) {
-        int nb = 0;
-        hipError_t rc = fam == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_group<0>, 64 * kFamWaves[0],
-                                                                                   kGroupStage)
-                                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_group<1>, 64 * kFamWaves[1],
-                                                                                   kGroupStage);
-        if (rc != hipSuccess || nb < 1) nb = 1;
-        g.grid[fam] = nb * g.ncu;
-    }
+    g.grid[0] = dc ? dc->grid[0] : g.ncu;
+    g.grid[1] = dc ? dc->grid[1] : g.ncu;
+    g.sig = (g.items[0].empty() ? 0 : MESHPROP_GROUP_SIG_NARROW) | (g.items[1].empty() ? 0 : MESHPROP_GROUP_SIG_WIDE) |
+            (g.pro_total > 0 ? MESHPROP_GROUP_SIG_PROLOGUE : 0) | (g.red_total > 0 ? MESHPROP_GROUP_SIG_REDUCE : 0);
     return MESHPROP_OK;
 }
 
@@ -496,13 +545,7 @@ int meshprop_group_create(const meshprop_group_entry* entries, int n, meshprop_g
     return MESHPROP_OK;
 }
 
-void meshprop_group_destroy(meshprop_group* g) {
-    if (!g) return;
-    if (g->ev_fork) hipEventDestroy(g->ev_fork);
-    if (g->ev_join) hipEventDestroy(g->ev_join);
-    if (g->side) hipStreamDestroy(g->side);
-    delete g;
-}
+void meshprop_group_destroy(meshprop_group* g) { delete g; }
 
 size_t meshprop_group_workspace_size(const meshprop_group* g) { return g ? g->total : 0; }
 
@@ -523,38 +566,45 @@ int meshprop_group_upload(const meshprop_group* g, void* ws, size_t bytes, void*
                                      (hipStream_t)stream));
 }
 
-int meshprop_group_signature(const meshprop_group* g) { return (g && !g->items[1].empty()) ? 1 : 0; }
+int meshprop_group_signature(const meshprop_group* g) { return g ? g->sig : 0; }
 
-int meshprop_group_launch(const meshprop_group* gc, void* ws, size_t bytes, void* stream) {
-    meshprop_group* g = const_cast<meshprop_group*>(gc);
+int meshprop_group_launch(const meshprop_group* g, void* ws, size_t bytes, void* stream) {
     if (!g) return MESHPROP_ERR_ARG;
     if (!ws || bytes < g->total || (((uintptr_t)ws) & 255)) return MESHPROP_ERR_WORKSPACE;
+    DeviceGroupConfig* dc = device_config(g->dev);
+    if (!dc) return MESHPROP_ERR_ARG;
     hipStream_t st = (hipStream_t)stream;
     unsigned char* w = (unsigned char*)ws;
-    const int pro_grid = 4 * g->ncu;
-    hipLaunchKernelGGL(k_group_prologue, dim3(pro_grid), dim3(256), 0, st, (const unsigned char*)w);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return MESHPROP_ERR_HIP;
-    const bool wide = !g->items[1].empty();
-    if (wide) {
+    hipError_t e;
+    if (g->sig & MESHPROP_GROUP_SIG_PROLOGUE) {
+        hipLaunchKernelGGL(k_group_prologue, dim3(4 * g->ncu), dim3(256), 0, st, (const unsigned char*)w);
+        if ((e = hipGetLastError()) != hipSuccess) return hip_status(e);
+    }
+    const bool narrow = g->sig & MESHPROP_GROUP_SIG_NARROW, wide = g->sig & MESHPROP_GROUP_SIG_WIDE;
+    if (wide && narrow) {
         // The W = 16 family runs beside family 0 on a second stream (fork / join by events,
         // which a graph capture records as two parallel branches).
-        if (!g->side) {
-            if ((e = hipStreamCreateWithFlags(&g->side, hipStreamNonBlocking)) != hipSuccess) return hip_status(e);
-            if ((e = hipEventCreateWithFlags(&g->ev_fork, hipEventDisableTiming)) != hipSuccess) return hip_status(e);
-            if ((e = hipEventCreateWithFlags(&g->ev_join, hipEventDisableTiming)) != hipSuccess) return hip_status(e);
-        }
-        if ((e = hipEventRecord(g->ev_fork, st)) != hipSuccess) return hip_status(e);
-        if ((e = hipStreamWaitEvent(g->side, g->ev_fork, 0)) != hipSuccess) return hip_status(e);
-        hipLaunchKernelGGL((k_group<1>), dim3(g->grid[1]), dim3(64 * kFamWaves[1]), kGroupStage, g->side, w);
+        if (!dc->ok) return MESHPROP_ERR_HIP;
+        std::lock_guard<std::mutex> hold(dc->lock);
+        if ((e = hipEventRecord(dc->ev_fork, st)) != hipSuccess) return hip_status(e);
+        if ((e = hipStreamWaitEvent(dc->side, dc->ev_fork, 0)) != hipSuccess) return hip_status(e);
+        hipLaunchKernelGGL((k_group<1>), dim3(g->grid[1]), dim3(64 * kFamWaves[1]), kGroupLds, dc->side, w);
         if ((e = hipGetLastError()) != hipSuccess) return hip_status(e);
-        if ((e = hipEventRecord(g->ev_join, g->side)) != hipSuccess) return hip_status(e);
+        if ((e = hipEventRecord(dc->ev_join, dc->side)) != hipSuccess) return hip_status(e);
+        hipLaunchKernelGGL((k_group<0>), dim3(g->grid[0]), dim3(64 * kFamWaves[0]), kGroupLds, st, w);
+        if ((e = hipGetLastError()) != hipSuccess) return hip_status(e);
+        if ((e = hipStreamWaitEvent(st, dc->ev_join, 0)) != hipSuccess) return hip_status(e);
+    } else if (wide) {
+        hipLaunchKernelGGL((k_group<1>), dim3(g->grid[1]), dim3(64 * kFamWaves[1]), kGroupLds, st, w);
+        if ((e = hipGetLastError()) != hipSuccess) return hip_status(e);
+    } else if (narrow) {
+        hipLaunchKernelGGL((k_group<0>), dim3(g->grid[0]), dim3(64 * kFamWaves[0]), kGroupLds, st, w);
+        if ((e = hipGetLastError()) != hipSuccess) return hip_status(e);
     }
-    hipLaunchKernelGGL((k_group<0>), dim3(g->grid[0]), dim3(64 * kFamWaves[0]), kGroupStage, st, w);
-    if ((e = hipGetLastError()) != hipSuccess) return hip_status(e);
-    if (wide && (e = hipStreamWaitEvent(st, g->ev_join, 0)) != hipSuccess) return hip_status(e);
-    hipLaunchKernelGGL(k_group_reduce, dim3(4 * g->ncu), dim3(256), 0, st, (const unsigned char*)w);
-    if ((e = hipGetLastError()) != hipSuccess) return hip_status(e);
+    if (g->sig & MESHPROP_GROUP_SIG_REDUCE) {
+        hipLaunchKernelGGL(k_group_reduce, dim3(4 * g->ncu), dim3(256), 0, st, (const unsigned char*)w);
+        if ((e = hipGetLastError()) != hipSuccess) return hip_status(e);
+    }
     return MESHPROP_OK;
 }
 

@@ -257,9 +257,11 @@ __device__ __forceinline__ void load_rows(R (&r)[W], const R* base, const int LP
 // Layers [l0, l0 + nl) of the row tables.  The layer loop is unrolled by two with two row
 // register sets (rows of layer l+1 are in flight while layer l is applied) and two field
 // register sets, so that no result overwrites an input of its own layer.
-template <class T, int W, int V>
+// KP: pointer type of the layer-kind array (a constant-address-space pointer where the caller
+// can promise it is read-only for the launch, so that its loads are scalar).
+template <class T, int W, int V, class KP>
 __device__ __forceinline__ void fwd_tile(const int N, const int l0, const int nl, const int B, const int LPV,
-                                         const int NP, const int* __restrict__ kind,
+                                         const int NP, KP kind,
                                          const typename RowOf<T>::type* __restrict__ rows, const T* u_in, T* u_out,
                                          const int ldu, const long tile) {
     typedef typename RowOf<T>::type R;
@@ -267,7 +269,7 @@ __device__ __forceinline__ void fwd_tile(const int N, const int l0, const int nl
     T ua[V][W], ub[V][W];
     load_tile(ua, u_in, q, N, B, ldu);
     if (nl > 0) {
-        const int* kp = kind + l0;
+        const KP kp = kind + l0;
         const R* rp = rows + (size_t)l0 * NP + q.t;
         R ra[W], rb[W];
         int ka = kp[0];
@@ -303,7 +305,7 @@ __global__ void __launch_bounds__(256) k_fwd(int N, int l0, int nl, int B, int L
                                              const int* __restrict__ kind,
                                              const typename RowOf<T>::type* __restrict__ rows, const T* u_in,
                                              T* u_out, int ldu) {
-    fwd_tile<T, W, V>(N, l0, nl, B, LPV, NP, kind, rows, u_in, u_out, ldu, blockIdx.x);
+    fwd_tile<T, W, V, const int*>(N, l0, nl, B, LPV, NP, kind, rows, u_in, u_out, ldu, blockIdx.x);
 }
 
 // ---- fwddiff --------------------------------------------------------------------------
@@ -496,9 +498,9 @@ struct BwdDepth {
     static constexpr int value = W <= 4 ? 4 : 2;
 };
 
-template <class T, int W, int V, bool UNI>
+template <class T, int W, int V, bool UNI, class KP>
 __device__ __forceinline__ void bwd_tile(const int N, const int l0, const int nl, const int B, const int LPV,
-                                         const int NP, const int C, const int* __restrict__ kind,
+                                         const int NP, const int C, KP kind,
                                          const typename RowOf<T>::type* __restrict__ rows_inv,
                                          const typename RowOf<T>::type* __restrict__ rows_adj,
                                          const T* __restrict__ dtdp, float* __restrict__ my_partial, const T* u_out,
@@ -514,7 +516,7 @@ __device__ __forceinline__ void bwd_tile(const int N, const int l0, const int nl
     const int lane = threadIdx.x & 63;
     const int wave_u = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave index as a scalar
     const int wstride = C * W * LPV;
-    const int* kp = kind + l0;
+    const KP kp = kind + l0;
     const int last = nl - 1;
     constexpr int PF = 2;  // stage-end slots per lane whose dT/dp is prefetched
     const Lanes q = lanes_of<V>(LPV, tile);
@@ -643,7 +645,8 @@ __global__ void __launch_bounds__(64 * bwd_max_waves(W, V, UNI)) k_bwd(
     float* my_partial = partial + (size_t)blockIdx.x * part_stride;
     bool first = true;
     for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        bwd_tile<T, W, V, UNI>(N, l0, nl, B, LPV, NP, C, kind, rows_inv, rows_adj, dtdp, my_partial, u_out, g_out,
+        bwd_tile<T, W, V, UNI, const int*>(N, l0, nl, B, LPV, NP, C, kind, rows_inv, rows_adj, dtdp, my_partial,
+                                           u_out, g_out,
                                target, u_in, g_in, ldu, tile, first, stage);
         first = false;
     }

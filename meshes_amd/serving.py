@@ -19,15 +19,16 @@ time with the plain per-call launches (gpu/mesh.py:158-231 call pattern), waitin
 from __future__ import annotations
 
 import collections
+import ctypes
 import time
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 from typing import List, Optional
 
 import numpy as np
 import torch
 
 from . import _lib
-from .batched import GroupJob, MeshGroup, tables_for
+from .batched import GroupJob, MeshGroup, _entry, tables_for
 from .mesh import MeshNetworkGPU
 
 
@@ -108,71 +109,83 @@ class MeshServer:
         g.run()
         torch.cuda.synchronize()
         g.close()
+        # Per-mesh entry templates (everything but the request's op, batch and buffers).
+        self._tmpl = []
+        for m, t in zip(self.meshes, self.tables):
+            e = _entry(GroupJob(m, "fwd", x=torch.empty((0, m.N), dtype=torch.complex64, device=self.device),
+                                tables=t, build_tables=False))
+            e.flags = _lib.GROUP_DP_ASSIGN
+            self._tmpl.append(e)
+        self._esize = ctypes.sizeof(_lib.GroupEntry)
         if graphs:
             self._capture_slots()
 
     # -- footprint --------------------------------------------------------------------------
     def footprint(self, mesh_id: int, op: str, B: int) -> int:
-        """HBM bytes a request holds while in flight: its field buffers (x, y and the target)
-        and, for gradients, its dp and dp partials (one slab per partial slot of the grouped
-        plan: min(vector tiles, 2 x CUs) slabs of L x N/2 x 2 floats)."""
-        m = self.meshes[mesh_id]
-        field_bytes = B * m.N * 8
-        W = _width(m.N)
-        lpv = _lpv(m.N, W)
-        waves = 2 if W == 16 else 4
-        per_tile = waves * (64 // lpv)
-        tiles = (B + per_tile - 1) // per_tile
-        plan = 2048  # descriptor, queue entries and alignment inside the workspace
-        if op == "fwd":
-            return 2 * field_bytes + plan + 8 * tiles
-        slabs = min(tiles, 2 * self.ncu)
-        partial = slabs * m.L * (lpv * W // 2) * 2 * 4
-        return 3 * field_bytes + m.n_phase * 4 + partial + plan + 8 * slabs
+        return request_footprint(self.meshes[mesh_id], op, B, self.ncu)
 
     # -- graphs -----------------------------------------------------------------------------
     def _capture_slots(self):
-        """Capture each slot's launch once per signature (with and without W = 16 meshes)."""
-        wide = [i for i, m in enumerate(self.meshes) if m.N > 512]
-        narrow = [i for i, m in enumerate(self.meshes) if m.N <= 512]
+        """Capture each slot's launch once per signature a batch can have: forward-only or
+        with gradients, with meshes up to N = 512, above it, or both."""
+        narrow = [i for i, m in enumerate(self.meshes) if m.N <= 512][:1]
+        wide = [i for i, m in enumerate(self.meshes) if m.N > 512][:1]
+        combos = []
+        for op in ("fwd", "grad"):
+            if narrow:
+                combos.append([(narrow[0], op)])
+            if wide:
+                combos.append([(wide[0], op)])
+            if narrow and wide:
+                combos.append([(narrow[0], op), (wide[0], op)])
         for s in self.slots:
             with torch.cuda.stream(s.stream):
-                for ids in (narrow[:1], wide[:1]):
-                    if not ids:
-                        continue
-                    m = self.meshes[ids[0]]
-                    x = torch.zeros((1, m.N), dtype=torch.complex64, device=self.device)
-                    job = GroupJob(m, "fwd", x=x, tables=self.tables[ids[0]], build_tables=False)
-                    if s.group is None:
-                        s.group = MeshGroup([job], workspace=s.workspace)
-                    else:
-                        s.group.replan([job])
+                for combo in combos:
+                    reqs = []
+                    for mid, op in combo:
+                        N = self.meshes[mid].N
+                        x = torch.zeros((1, N), dtype=torch.complex64, device=self.device)
+                        reqs.append(Request(-1, mid, op, x, torch.zeros_like(x) if op == "grad" else None))
+                    self._plan(s, reqs)
                     s.group.capture()
             torch.cuda.synchronize()
+            s.reqs, s.bytes = [], 0
 
     # -- request loop -----------------------------------------------------------------------
-    def _launch(self, slot: _Slot, batch: List[Request]):
-        jobs = []
+    def _plan(self, slot: _Slot, batch: List[Request]):
+        """Allocate the batch's outputs and upload its grouped plan into the slot's workspace
+        (on the slot's stream)."""
+        n = len(batch)
+        ents = (_lib.GroupEntry * n)()
+        base = ctypes.addressof(ents)
         with torch.cuda.stream(slot.stream):
-            for r in batch:
+            for k, r in enumerate(batch):
                 m = self.meshes[r.mesh_id]
+                ctypes.memmove(base + k * self._esize, ctypes.addressof(self._tmpl[r.mesh_id]), self._esize)
+                e = ents[k]
+                e.B = r.B
+                e.u_in = r.x.data_ptr()
                 r.y = torch.empty((r.B, m.N), dtype=torch.complex64, device=self.device)
+                e.u_out = r.y.data_ptr()
                 if r.op == "grad":
                     r.dp = torch.empty(m.n_phase, dtype=torch.float32, device=self.device)
-                    jobs.append(GroupJob(m, "grad", x=r.x, y=r.y, target=r.target, dp=r.dp,
-                                         tables=self.tables[r.mesh_id], build_tables=False))
+                    e.op = _lib.OP_FWDGRAD
+                    e.dp = r.dp.data_ptr()
+                    e.target = r.target.data_ptr()
                 else:
-                    jobs.append(GroupJob(m, "fwd", x=r.x, y=r.y, tables=self.tables[r.mesh_id],
-                                         build_tables=False))
+                    e.op = _lib.OP_FWDPROP
             if slot.group is None:
-                slot.group = MeshGroup(jobs, workspace=slot.workspace)
-            else:
-                slot.group.replan(jobs)
+                slot.group = MeshGroup([], workspace=slot.workspace)
+            slot.group.replan_entries(ents, n, self.device)
             if slot.group.workspace_bytes != self.slot_capacity:
                 raise RuntimeError("a batch outgrew its slot workspace")
+        slot.reqs = batch
+
+    def _launch(self, slot: _Slot, batch: List[Request]):
+        self._plan(slot, batch)
+        with torch.cuda.stream(slot.stream):
             slot.group.run()
             slot.event.record(slot.stream)
-        slot.reqs = batch
 
     def serve(self, requests: List[Request], clock=time.perf_counter):
         """Run the request stream in real time (arrival times relative to the start); returns
@@ -251,6 +264,25 @@ class MeshServer:
             torch.cuda.current_stream().synchronize()
             r.t_done = clock() - t0
         return reqs, _stats(reqs, self.meshes)
+
+
+def request_footprint(m: MeshNetworkGPU, op: str, B: int, ncu: int) -> int:
+    """HBM bytes a request holds while in flight: its field buffers (x, y and the target)
+    and, for gradients, its dp and dp partials (one slab per partial slot of the grouped
+    plan: min(vector tiles, 2 x CUs) slabs of L x N/2 x 2 floats), plus its share of the
+    plan (descriptor, queue entries, alignment)."""
+    field_bytes = B * m.N * 8
+    W = _width(m.N)
+    lpv = _lpv(m.N, W)
+    waves = 2 if W == 16 else 4
+    per_tile = waves * (64 // lpv)
+    tiles = (B + per_tile - 1) // per_tile
+    plan = 2048
+    if op == "fwd":
+        return 2 * field_bytes + plan + 8 * tiles
+    slabs = min(tiles, 2 * ncu)
+    partial = slabs * m.L * (lpv * W // 2) * 2 * 4
+    return 3 * field_bytes + m.n_phase * 4 + partial + plan + 8 * slabs
 
 
 def _stats(reqs, meshes, **extra):

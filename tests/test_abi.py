@@ -103,7 +103,7 @@ def test_group_plan_without_gpu(lib):
         assert lib.meshprop_group_entry_info(plan, 1, ctypes.byref(t), ctypes.byref(n)) == 0
         assert (t.w, t.v) == (8, 1) and n.value >= 1
         assert lib.meshprop_group_entry_info(plan, 2, ctypes.byref(t), ctypes.byref(n)) == 0
-        assert t.w == 16 and lib.meshprop_group_signature(plan) == 1
+        assert t.w == 16 and lib.meshprop_group_signature(plan) == 15  # narrow, wide, prologue, reduce
         assert lib.meshprop_group_entry_info(plan, 3, ctypes.byref(t), ctypes.byref(n)) == 1
         # missing or misaligned workspace
         assert lib.meshprop_group_upload(plan, None, 0, None) == 2

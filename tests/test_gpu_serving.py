@@ -1,0 +1,94 @@
+"""Serving loop: continuous batching over resident meshes.
+
+Each request's result is the same served in the mix as computed alone (bitwise: a request's
+arithmetic depends on its own mesh and batch only), agrees with the sequential plain launches
+and with the FP64 oracle within 1e-5 relative L2; admission never exceeds the HBM budget.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import meshprop_oracle as O
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+SIZES = [16, 24, 40, 64, 100, 130, 256, 300, 512, 700, 1024]
+
+
+def _meshes(seed=3):
+    from meshes_amd import MeshNetworkGPU, geometry
+    rng = np.random.RandomState(seed)
+    out = []
+    for N in SIZES:
+        lens, shifts = geometry.clements(N)
+        out.append(MeshNetworkGPU(N, N, lens, shifts, geometry.uniform_phases(rng, N, N), np.zeros(0), "mzi",
+                                  is_phase=False))
+    return out
+
+
+def _clone(reqs):
+    from meshes_amd.serving import Request
+    return [Request(r.rid, r.mesh_id, r.op, r.x, r.target, r.t_arrive) for r in reqs]
+
+
+def test_served_equals_alone_plain_and_oracle(gpu):
+    from meshes_amd.batched import GroupJob, MeshGroup
+    from meshes_amd.serving import MeshServer, poisson_requests
+    meshes = _meshes()
+    server = MeshServer(meshes, hbm_budget=3 << 30, slots=3)
+    reqs = poisson_requests(meshes, rate=50000.0, n=80, seed=5, b_max=300)
+    done, st = server.serve(reqs)
+    assert st["requests"] == 80 and st["mean_batch"] > 1.0
+    seq, _ = server.serve_sequential(_clone(reqs))
+    for r, q in zip(done, seq):
+        m = meshes[r.mesh_id]
+        if r.op == "grad":
+            job = GroupJob(m, "grad", x=r.x, target=r.target, tables=server.tables[r.mesh_id], build_tables=False)
+        else:
+            job = GroupJob(m, "fwd", x=r.x, tables=server.tables[r.mesh_id], build_tables=False)
+        MeshGroup([job]).run()
+        torch.cuda.synchronize()
+        assert torch.equal(job.y, r.y), (r.rid, m.N, r.op)
+        assert O.rel_l2(q.y.cpu().numpy(), r.y.cpu().numpy()) < TOL
+        if r.op == "grad":
+            assert torch.equal(job.dp, r.dp), (r.rid, m.N)
+            assert O.rel_l2(q.dp.cpu().numpy(), r.dp.cpu().numpy()) < TOL
+    # FP64 oracle on the smaller meshes
+    checked = 0
+    for r in done:
+        m = meshes[r.mesh_id]
+        if m.N > 130:
+            continue
+        g = O.clements(m.N)
+        p = m.p_phase.cpu().numpy().astype(np.float64).reshape(m.N, m.N // 2, 2)
+        x = r.x.cpu().numpy().astype(np.complex128)
+        y = O.mesh_fwd(g, "mzi", p, None, x)
+        assert O.rel_l2(r.y.cpu().numpy(), y) < TOL
+        if r.op == "grad":
+            yg = r.y.cpu().numpy().astype(np.complex128)
+            rdp, _, _ = O.mesh_vjp(g, "mzi", p, None, yg, 2 * (yg - r.target.cpu().numpy().astype(np.complex128)))
+            assert O.rel_l2(r.dp.cpu().numpy().reshape(rdp.shape), rdp) < TOL
+        checked += 1
+    assert checked >= 10
+
+
+def test_admission_respects_budget(gpu):
+    from meshes_amd.serving import MeshServer, poisson_requests, request_footprint
+    meshes = _meshes(seed=4)
+    reqs = poisson_requests(meshes, rate=50000.0, n=40, seed=6, b_max=256)
+    ncu = torch.cuda.get_device_properties(0).multi_processor_count
+    need = max(request_footprint(meshes[r.mesh_id], r.op, r.B, ncu) for r in reqs)
+    server = MeshServer(meshes, hbm_budget=2 * need, slots=2)
+    done, st = server.serve(reqs)
+    assert all(r.t_done >= r.t_arrive for r in done)
+    assert st["peak_inflight_bytes"] <= 2 * need
+    assert server.inflight_bytes == 0
+
+
+def test_oversized_request_is_rejected(gpu):
+    from meshes_amd.serving import MeshServer, poisson_requests
+    meshes = _meshes(seed=5)[-2:]
+    server = MeshServer(meshes, hbm_budget=1 << 20, slots=1, graphs=False)
+    reqs = poisson_requests(meshes, rate=1000.0, n=3, seed=7, b_median=512, b_sigma=0.0)
+    with pytest.raises(ValueError):
+        server.serve(reqs)
