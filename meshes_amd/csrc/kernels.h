@@ -257,13 +257,21 @@ __device__ __forceinline__ void load_rows(R (&r)[W], const R* base, const int LP
 // Layers [l0, l0 + nl) of the row tables.  The layer loop is unrolled by two with two row
 // register sets (rows of layer l+1 are in flight while layer l is applied) and two field
 // register sets, so that no result overwrites an input of its own layer.
+// Field checkpoints of the lossy adjoint: the field entering every layer l with
+// l % every == 0 lives at base + (l / every) * stride (base == nullptr: none).
+struct Ckpt {
+    void* base;
+    int every;
+    size_t stride;  // elements
+};
+
 // KP: pointer type of the layer-kind array (a constant-address-space pointer where the caller
 // can promise it is read-only for the launch, so that its loads are scalar).
 template <class T, int W, int V, class KP>
 __device__ __forceinline__ void fwd_tile(const int N, const int l0, const int nl, const int B, const int LPV,
                                          const int NP, KP kind,
                                          const typename RowOf<T>::type* __restrict__ rows, const T* u_in, T* u_out,
-                                         const int ldu, const long tile) {
+                                         const int ldu, const long tile, const Ckpt ck = Ckpt{nullptr, 0, 0}) {
     typedef typename RowOf<T>::type R;
     const Lanes q = lanes_of<V>(LPV, tile);
     T ua[V][W], ub[V][W];
@@ -274,10 +282,22 @@ __device__ __forceinline__ void fwd_tile(const int N, const int l0, const int nl
         R ra[W], rb[W];
         int ka = kp[0];
         load_rows(ra, rp, LPV);
+        // Next checkpointed layer (local index) and its slot; INT_MAX when there is none.
+        T* const ckb = reinterpret_cast<T*>(ck.base);
+        int ck_l = 0x7fffffff, ck_c = 0;
+        if (ckb) {
+            ck_c = (l0 + ck.every - 1) / ck.every;
+            ck_l = ck_c * ck.every - l0;
+        }
         int l = 0;
         for (; l + 1 < nl; l += 2) {
             const int kb = kp[l + 1];
             load_rows(rb, rp + (size_t)(l + 1) * NP, LPV);
+            if (l == ck_l) {
+                store_tile(ua, ckb + (size_t)ck_c * ck.stride, q, N, B, ldu);
+                ++ck_c;
+                ck_l += ck.every;
+            }
             {
                 FwdStep<T, W, V> st{ua, ub, ra, q.lane};
                 dispatch_kind<W>(ka, st);
@@ -285,12 +305,18 @@ __device__ __forceinline__ void fwd_tile(const int N, const int l0, const int nl
             const int ln = (l + 2 < nl) ? l + 2 : nl - 1;
             ka = kp[ln];
             load_rows(ra, rp + (size_t)ln * NP, LPV);
+            if (l + 1 == ck_l) {
+                store_tile(ub, ckb + (size_t)ck_c * ck.stride, q, N, B, ldu);
+                ++ck_c;
+                ck_l += ck.every;
+            }
             {
                 FwdStep<T, W, V> st{ub, ua, rb, q.lane};
                 dispatch_kind<W>(kb, st);
             }
         }
         if (l < nl) {
+            if (l == ck_l) store_tile(ua, ckb + (size_t)ck_c * ck.stride, q, N, B, ldu);
             FwdStep<T, W, V> st{ua, ub, ra, q.lane};
             dispatch_kind<W>(ka, st);
             store_tile(ub, u_out, q, N, B, ldu);
@@ -304,8 +330,8 @@ template <class T, int W, int V>
 __global__ void __launch_bounds__(256) k_fwd(int N, int l0, int nl, int B, int LPV, int NP,
                                              const int* __restrict__ kind,
                                              const typename RowOf<T>::type* __restrict__ rows, const T* u_in,
-                                             T* u_out, int ldu) {
-    fwd_tile<T, W, V, const int*>(N, l0, nl, B, LPV, NP, kind, rows, u_in, u_out, ldu, blockIdx.x);
+                                             T* u_out, int ldu, Ckpt ck) {
+    fwd_tile<T, W, V, const int*>(N, l0, nl, B, LPV, NP, kind, rows, u_in, u_out, ldu, blockIdx.x, ck);
 }
 
 // ---- fwddiff --------------------------------------------------------------------------
@@ -506,7 +532,8 @@ __device__ __forceinline__ void bwd_tile(const int N, const int l0, const int nl
                                          const T* __restrict__ dtdp, float* __restrict__ my_partial, const T* u_out,
                                          const T* g_out, const T* target, T* u_in, T* g_in, const int ldu,
                                          const long tile, const bool first,
-                                         typename RowOf<T>::type* __restrict__ stage) {
+                                         typename RowOf<T>::type* __restrict__ stage,
+                                         const Ckpt ck = Ckpt{nullptr, 0, 0}) {
     typedef typename RowOf<T>::type R;
     constexpr int np = NPhase<T>::value;
     constexpr int D = BwdDepth<W>::value;  // row register sets
@@ -555,6 +582,13 @@ __device__ __forceinline__ void bwd_tile(const int N, const int l0, const int nl
     }
     for (int c0 = 0; c0 < nl; c0 += C) {
         const int cl = (C < nl - c0) ? C : nl - c0;
+        if (ck.base != nullptr && c0 > 0) {
+            // The field entering layer e is stored: replace the reconstruction by it (the host
+            // only passes checkpoints whose layers fall on stage starts).
+            const int e = l0 + nl - c0;
+            if (e % ck.every == 0)
+                load_tile(ua, reinterpret_cast<const T*>(ck.base) + (size_t)(e / ck.every) * ck.stride, q, N, B, ldu);
+        }
         // Prefetch the dT/dp this wave converts first at the end of the stage.
         T pre[PF][4 * np];
         if (wave_u < cl) {
@@ -637,7 +671,7 @@ __global__ void __launch_bounds__(64 * bwd_max_waves(W, V, UNI)) k_bwd(
     int N, int l0, int nl, int B, int LPV, int NP, int C, int np_unused, const int* __restrict__ kind,
     const typename RowOf<T>::type* __restrict__ rows_inv, const typename RowOf<T>::type* __restrict__ rows_adj,
     const T* __restrict__ dtdp, float* __restrict__ partial, size_t part_stride, long ntiles, const T* u_out,
-    const T* g_out, const T* target, T* u_in, T* g_in, int ldu) {
+    const T* g_out, const T* target, T* u_in, T* g_in, int ldu, Ckpt ck) {
     typedef typename RowOf<T>::type R;
     extern __shared__ unsigned char smem_raw[];
     R* stage = reinterpret_cast<R*>(smem_raw);  // [nw][C][W][LPV]
@@ -647,7 +681,7 @@ __global__ void __launch_bounds__(64 * bwd_max_waves(W, V, UNI)) k_bwd(
     for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         bwd_tile<T, W, V, UNI, const int*>(N, l0, nl, B, LPV, NP, C, kind, rows_inv, rows_adj, dtdp, my_partial,
                                            u_out, g_out,
-                               target, u_in, g_in, ldu, tile, first, stage);
+                               target, u_in, g_in, ldu, tile, first, stage, ck);
         first = false;
     }
 }
@@ -673,12 +707,12 @@ __host__ __device__ constexpr bool fwd_tile_ok(int W, int V) { return W * V <= 3
 __host__ __device__ constexpr bool diff_tile_ok(int W, int V) { return W * V <= 16 && V <= 4; }
 
 int launch_fwd_any(const Plan& P, const int* kind, const void* rows, int l0, int nl, const void* u_in, void* u_out,
-                   int ldu, hipStream_t st);
+                   int ldu, hipStream_t st, Ckpt ck = Ckpt{nullptr, 0, 0});
 int launch_fwddiff_any(const Plan& P, const int* kind, const void* rows, const void* drows, const void* u_in,
                        const void* du_in, void* u_out, void* du_out, int ldu, hipStream_t st);
 int launch_bwd_any(const Plan& P, const int* kind, const void* ri, const void* ra, const void* dtdp, float* partial,
                    size_t part_stride, int l0, int nl, const void* u_out, const void* g_out, const void* target,
-                   void* u_in, void* g_in, int ldu, hipStream_t st);
+                   void* u_in, void* g_in, int ldu, hipStream_t st, Ckpt ck = Ckpt{nullptr, 0, 0});
 
 }  // namespace mp
 

@@ -7,20 +7,20 @@ namespace mp {
 
 template <class T, int W, int V>
 static void go(const Plan& P, const int* kind, const void* rows, int l0, int nl, const void* u_in, void* u_out,
-               int ldu, hipStream_t st) {
+               int ldu, hipStream_t st, Ckpt ck) {
     typedef typename RowOf<T>::type R;
     hipLaunchKernelGGL((k_fwd<T, W, V>), dim3(P.grid), dim3(64 * P.waves), 0, st, P.N, l0, nl, P.B, P.LPV, P.NP,
-                       kind, (const R*)rows, (const T*)u_in, (T*)u_out, ldu);
+                       kind, (const R*)rows, (const T*)u_in, (T*)u_out, ldu, ck);
 }
 
 int launch_fwd_any(const Plan& P, const int* kind, const void* rows, int l0, int nl, const void* u_in, void* u_out,
-                   int ldu, hipStream_t st) {
+                   int ldu, hipStream_t st, Ckpt ck) {
 #define MP_CALL(W_, V_)                                                        \
     if constexpr (fwd_tile_ok(W_, V_)) {                                       \
         if (P.real)                                                            \
-            go<float, W_, V_>(P, kind, rows, l0, nl, u_in, u_out, ldu, st);    \
+            go<float, W_, V_>(P, kind, rows, l0, nl, u_in, u_out, ldu, st, ck);  \
         else                                                                   \
-            go<float2, W_, V_>(P, kind, rows, l0, nl, u_in, u_out, ldu, st);   \
+            go<float2, W_, V_>(P, kind, rows, l0, nl, u_in, u_out, ldu, st, ck);  \
         return MESHPROP_OK;                                                    \
     }
     MP_FOR_WV(P.W, P.V, MP_CALL)

@@ -274,13 +274,8 @@ int run_impl(int op, int X, int N, int L, int B, int fft, const int* g0, const i
             if (!ckpt) {
                 rc = launch_fwd_any(P, kind, tab_a, 0, L, u_a, u_b, ldu, st);
             } else {
-                // Checkpointed forward: ckpt[c] receives the field entering layer c*K.
-                rc = launch_fwd_any(P, kind, tab_a, 0, 0, u_a, ck, ldu, st);
-                for (int c = 0; c < nck && !rc; ++c) {
-                    const int l0 = c * K, nl = (L - l0 < K) ? L - l0 : K;
-                    void* out = (c + 1 < nck) ? (void*)(ck + (size_t)(c + 1) * fstride) : u_b;
-                    rc = launch_fwd_any(P, kind, tab_a, l0, nl, ck + (size_t)c * fstride, out, ldu, st);
-                }
+                // Checkpointed forward, one launch: ckpt[c] receives the field entering layer c*K.
+                rc = launch_fwd_any(P, kind, tab_a, 0, L, u_a, u_b, ldu, st, Ckpt{ckpt, K, (size_t)B * ldu});
             }
         } else if (op == MESHPROP_OP_FWDDIFF) {
             rc = launch_fwddiff_any(P, kind, tab_a, tab_b, u_a, du_a, u_b, du_b, ldu, st);
@@ -289,9 +284,15 @@ int run_impl(int op, int X, int N, int L, int B, int fft, const int* g0, const i
             if (!ckpt) {
                 rc = launch_bwd_any(P, kind, tab_a, rows_b, dtdp, partial, ps, 0, L, u_a, du_a, target, u_b, du_b,
                                     ldu, st);
+            } else if (L % P.C == 0 && K % P.C == 0) {
+                // Checkpointed adjoint, one launch: every K layers the reconstructed field is
+                // replaced by the exact forward field ckpt[c] (checkpoints fall on LDS stage starts).
+                rc = launch_bwd_any(P, kind, tab_a, rows_b, dtdp, partial, ps, 0, L, u_a, du_a, target, u_b, du_b,
+                                    ldu, st, Ckpt{ckpt, K, (size_t)B * ldu});
             } else {
-                // Checkpointed adjoint: segment c starts from the exact forward field ckpt[c+1]
-                // (u_out for the last one); the adjoint field is carried in dJdu_in.
+                // Checkpointed adjoint, one launch per segment: segment c starts from the exact
+                // forward field ckpt[c+1] (u_out for the last one); the adjoint field is carried in
+                // dJdu_in.
                 for (int c = nck - 1; c >= 0 && !rc; --c) {
                     const int l0 = c * K, nl = (L - l0 < K) ? L - l0 : K;
                     const bool last = (c == nck - 1);

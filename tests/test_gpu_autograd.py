@@ -106,3 +106,44 @@ def test_adam_training_toward_haar_target(gpu):
         losses.append(loss.item())
     assert np.isfinite(losses).all()
     assert losses[-1] < 0.5 * losses[0], losses[::10]
+
+
+@pytest.mark.parametrize("K", [8, 16, 32])
+def test_c3_full_size_gradient(gpu, K):
+    """Lossy, imperfect Clements mesh at full size (N = L = 512; alpha, beta ~ N(0, 0.02),
+    0.1 dB per crossing) on a 32-vector slice: the checkpointed adjoint (checkpoints every K
+    layers, one launch each way) against the FP64 stored-field adjoint, and, for K = 16, central
+    finite differences (h = 1e-3) of the FP64 loss on 64 random active phases."""
+    from meshes_amd.autograd import mesh_apply
+    rng = np.random.RandomState(6)
+    N, B = 512, 32
+    g, p, s = _lossy(rng, N)
+    mesh = _mesh(g, "mzi_generic", p, s)
+    x = torch.as_tensor(O.unit_cn(rng, B, N).astype(np.complex64)).cuda()
+    t = torch.as_tensor(O.unit_cn(rng, B, N).astype(np.complex64)).cuda()
+    pt = mesh.p_phase.clone().requires_grad_(True)
+    y = mesh_apply(mesh, pt, x, ckpt_every=K)
+    torch.sum(torch.abs(y - t) ** 2).backward()
+    xn, tn = x.cpu().numpy().astype(np.complex128), t.cpu().numpy().astype(np.complex128)
+    p64, s64 = p.astype(np.float64), s.astype(np.float64)
+    T = O.slot_T(g, "mzi_generic", p64, s64)
+    yo = O.fwd(g, T, xn)
+    assert O.rel_l2(y.detach().cpu().numpy(), yo) < TOL
+    rdp, _, _ = O.mesh_vjp(g, "mzi_generic", p64, s64, yo, 2 * (yo - tn), u_in=xn)
+    gp = pt.grad.cpu().numpy().reshape(rdp.shape)
+    assert O.rel_l2(gp, rdp) < TOL
+    if K != 16:
+        return
+    act = np.argwhere(np.repeat(O.active_mask(g)[..., None], 2, axis=2))
+    pick = act[rng.choice(len(act), 64, replace=False)]
+    h = 1e-3
+    for (l, m, k) in pick:
+        vals = []
+        for sgn in (1, -1):
+            pp = p64[l, m].copy()
+            pp[k] += sgn * h
+            Tp = T.copy()
+            Tp[l, m] = O.crossing_T("mzi_generic", pp[None], s64[l, m][None])[0]
+            vals.append(np.sum(np.abs(O.fwd(g, Tp, xn) - tn) ** 2))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        assert abs(gp[l, m, k] - fd) <= 1e-4 * max(1.0, abs(fd)), (l, m, k, gp[l, m, k], fd)
