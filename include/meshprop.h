@@ -221,6 +221,18 @@ enum meshprop_group_signature_bits {
 /* Resolved tile and number of work items of entry i. */
 int meshprop_group_entry_info(const meshprop_group *g, int i, meshprop_tile *tile, int *items);
 
+/* Phase screen of is_phase meshes: u_w <- exp(i phi_w) u_w on every vector (no reference GPU
+ * counterpart: gpu/mesh.py:65 rejects is_phase; the CPU semantics are c/dot.cpp:97-143 and
+ * mesh.py:38-41).  In place on complex64 (N, B) fields with leading dimension ldu.
+ *   MESHPROP_OP_FWDPROP : u <- exp(i phi) u.
+ *   MESHPROP_OP_FWDDIFF : u <- exp(i phi) u;  du <- exp(i phi) du + i dphi (exp(i phi) u).
+ *   MESHPROP_OP_BACKDIFF: u, du = field and dJ/du after the screen: dphi += Im(sum_b du conj(u))
+ *                         (accumulated, fixed summation order), then u <- exp(-i phi) u and
+ *                         du <- exp(-i phi) du.  Needs meshprop_phase_screen_workspace_size bytes. */
+int meshprop_phase_screen(int op, int N, int B, const float *phi, float *dphi, void *u, void *du, int ldu,
+                          void *workspace, size_t workspace_bytes, void *stream);
+size_t meshprop_phase_screen_workspace_size(int N, int B);
+
 /* The tile the library would use for a call (after defaults and clamping). */
 int meshprop_resolve_tile(int op, int X, int N, int B, meshprop_tile *out);
 

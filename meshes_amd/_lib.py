@@ -31,7 +31,7 @@ EXPORTS = (
     "meshprop_resolve_tile", "meshprop_last_hip_error", "meshprop_status_string", "meshprop_version",
     "meshprop_group_create", "meshprop_group_destroy", "meshprop_group_workspace_size",
     "meshprop_group_tables_size", "meshprop_group_upload", "meshprop_group_launch", "meshprop_group_signature",
-    "meshprop_group_entry_info",
+    "meshprop_group_entry_info", "meshprop_phase_screen", "meshprop_phase_screen_workspace_size",
 )
 
 
@@ -101,6 +101,10 @@ def load():
     lib.meshprop_group_signature.restype = c_int
     lib.meshprop_group_entry_info.argtypes = [vp, c_int, ctypes.POINTER(Tile), ctypes.POINTER(c_int)]
     lib.meshprop_group_entry_info.restype = c_int
+    lib.meshprop_phase_screen.argtypes = [c_int, c_int, c_int, vp, vp, vp, vp, c_int, vp, c_size_t, vp]
+    lib.meshprop_phase_screen.restype = c_int
+    lib.meshprop_phase_screen_workspace_size.argtypes = [c_int, c_int]
+    lib.meshprop_phase_screen_workspace_size.restype = c_size_t
     _lib = lib
     return lib
 
@@ -116,6 +120,8 @@ def check(rc: int, what: str = "meshprop"):
 def _ptr(t):
     if t is None:
         return None
+    if t.is_conj() or t.is_neg():
+        raise ValueError("lazy conjugate / negative tensor views must be resolved before a kernel reads them")
     return ctypes.c_void_p(t.data_ptr())
 
 
@@ -160,3 +166,17 @@ def run(op, X, N, L, B, fft, g0, g1, p, dp, ldp, s, lds, u_a, du_a, u_b, du_b, l
                           0, _ptr(target), _ptr(ckpt), int(ckpt_every), tile_struct(tile), int(flags), _ptr(workspace),
                           ctypes.c_size_t(ws_bytes), ctypes.c_void_p(stream))
     check(rc, ["fwdprop", "fwddiff", "backdiff"][op])
+
+
+def phase_screen(op, N, B, phi, dphi, u, du, ldu, workspace=None, ws_bytes=0, stream=None):
+    """meshprop_phase_screen on torch tensors (in place on u, du)."""
+    lib = load()
+    if stream is None:
+        stream = torch.cuda.current_stream().cuda_stream
+    rc = lib.meshprop_phase_screen(op, N, B, _ptr(phi), _ptr(dphi), _ptr(u), _ptr(du), ldu, _ptr(workspace),
+                                   ctypes.c_size_t(ws_bytes), ctypes.c_void_p(stream))
+    check(rc, "phase_screen")
+
+
+def phase_screen_workspace_size(N, B) -> int:
+    return int(load().meshprop_phase_screen_workspace_size(N, B))

@@ -65,6 +65,7 @@ class MeshFunction(torch.autograd.Function):
 def mesh_apply(mesh: MeshNetworkGPU, p: torch.Tensor, x: torch.Tensor, ckpt_every: int = 0) -> torch.Tensor:
     """Differentiable y = U(p) x (x: (B, N)).  ckpt_every > 0 stores forward checkpoints for a
     lossy mesh; for unitary crossings the adjoint reconstructs fields by inverse propagation."""
+    assert not mesh.is_phase, "mesh_apply propagates through the crossings only (no phase screen)"
     return MeshFunction.apply(p, x, mesh, int(ckpt_every))
 
 

@@ -50,6 +50,7 @@ class GroupJob:
         assert self.op in OPS, self.op
         m = self.mesh
         assert m.X != "orth", "real (orth) meshes are not grouped"
+        assert not m.is_phase, "grouped jobs propagate through the crossings only (no phase screen)"
         if self.op != "fwd":
             assert m.X in ("mzi", "sym"), "gradient jobs need unitary crossings"
         ref = self.x if self.op != "backdiff" else self.y
@@ -60,6 +61,7 @@ class GroupJob:
             t = getattr(self, name)
             if t is not None:
                 assert t.shape == (self.B, m.N) and t.dtype == torch.complex64 and t.is_contiguous(), name
+                assert not (t.is_conj() or t.is_neg()), f"{name}: resolve lazy conjugate/negative views first"
                 assert t.device == dev, name
         if self.y is None:
             self.y = torch.empty((self.B, m.N), dtype=torch.complex64, device=dev)

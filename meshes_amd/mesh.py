@@ -35,7 +35,7 @@ def as_field(v, N, dtype, device) -> torch.Tensor:
     if v.ndim == 1:
         v = v.reshape(N, 1)
     assert v.ndim == 2 and v.shape[0] == N, f"field must be (N={N}, B), got {tuple(v.shape)}"
-    t = v.T
+    t = v.T.resolve_conj().resolve_neg()  # lazy conjugate/negative views hold the plain data
     if t.dtype != dtype:
         t = t.to(dtype)
     if t.device != device:
@@ -64,8 +64,10 @@ class MeshNetworkGPU:
     :param p_phase: crossing parameters in slot layout, size L*(N/2)*X_np.
     :param p_splitter: splitter imperfections, size L*(N/2)*X_ns or 0.
     :param X: crossing type: 'mzi', 'sym', 'orth' (the reference's three) or 'mzi_generic' (lossy MZI).
-    :param phi_pos, is_phase: as in the reference; the output phase screen is not supported on the
-        GPU path there either (gpu/mesh.py:65), so is_phase must be False.
+    :param phi_pos, is_phase: the phase screen exp(i phi) of the last N entries of p_phase, at the
+        output ('out') or input ('in') of the mesh (mesh.py:38-41, c/dot.cpp:97-143).  The reference
+        GPU path rejects is_phase (gpu/mesh.py:65); here it runs as a separate HIP kernel
+        (meshprop_phase_screen).
     :param strides: optional FFT/butterfly geometry (stride index per layer); the reference only
         reaches these kernels from gpu/test.py.
     :param tile: optional (w, v, waves, chunk) tile override passed to the library.
@@ -74,7 +76,8 @@ class MeshNetworkGPU:
     def __init__(self, N: int, L: int, lens, shifts, p_phase, p_splitter, X: str, phi_pos="out", is_phase=True,
                  strides=None, tile=None):
         _lib.load()
-        assert not is_phase  # the reference GPU path has no phase screen either (gpu/mesh.py:65)
+        assert phi_pos in ["in", "out"]
+        assert not (is_phase and X == "orth"), "the phase screen needs complex fields"
         assert X in ["mzi", "sym", "orth", "mzi_generic"]
         assert N % 2 == 0
         assert N <= 1024, "meshes wider than 1024 modes are not supported"
@@ -178,6 +181,14 @@ class MeshNetworkGPU:
     def matrix(self, p_phase=None, p_splitter=None) -> torch.Tensor:
         return self.dot(torch.eye(self.N, dtype=self.dtype, device=self.device), p_phase, p_splitter)
 
+    def _screen(self, op, B, phi, dphi, u, du):
+        ws = None
+        nb = 0
+        if op == _lib.OP_BACKDIFF:
+            nb = _lib.phase_screen_workspace_size(self.N, B)
+            ws = torch.empty(max(nb, 1), dtype=torch.uint8, device=self.device)
+        _lib.phase_screen(op, self.N, B, phi, dphi, u, du, self.N, workspace=ws, ws_bytes=nb)
+
     def dot(self, v, p_phase=None, p_splitter=None, dp=None, dv=None, out=None, dout=None):
         """out = U v (fwdprop), or (out, dout) with dout = U dv + (dU/dp . dp) v (fwddiff)."""
         vb = as_field(v, self.N, self.dtype, self.device)
@@ -186,18 +197,36 @@ class MeshNetworkGPU:
         ldp, lds = self._ld()
         g0, g1 = self._geom()
         ob = self._out_buffer(out, B)
+        n_cr_p = self.n_cr * self.X_np
+        screen_in = self.is_phase and self.phi_pos == "in"
         if dp is None and dv is None:
+            if screen_in:  # exp(i phi) on a copy of the input, then the mesh in place
+                ob.copy_(vb)
+                self._screen(_lib.OP_FWDPROP, B, p[n_cr_p:], None, ob, None)
+                vb = ob
             ws, nb = self._workspace(_lib.OP_FWDPROP, B)
             _lib.run(_lib.OP_FWDPROP, self.X, self.N, self.L, B, self.fft, g0, g1, p, None, ldp,
                      s if s.numel() else None, lds, vb, None, ob, None, self.N, tile=self.tile, workspace=ws,
                      ws_bytes=nb)
+            if self.is_phase and not screen_in:
+                self._screen(_lib.OP_FWDPROP, B, p[n_cr_p:], None, ob, None)
             return self._result(ob, out)
         dob = self._out_buffer(dout, B)
         dpt = as_params(dp, self.n_phase, self.device) if dp is not None else torch.zeros_like(p)
         dvb = as_field(dv, self.N, self.dtype, self.device) if dv is not None else None
+        if screen_in:
+            ob.copy_(vb)
+            if dvb is None:
+                dob.zero_()
+            else:
+                dob.copy_(dvb)
+            self._screen(_lib.OP_FWDDIFF, B, p[n_cr_p:], dpt[n_cr_p:], ob, dob)
+            vb, dvb = ob, dob
         ws, nb = self._workspace(_lib.OP_FWDDIFF, B)
         _lib.run(_lib.OP_FWDDIFF, self.X, self.N, self.L, B, self.fft, g0, g1, p, dpt, ldp,
                  s if s.numel() else None, lds, vb, dvb, ob, dob, self.N, tile=self.tile, workspace=ws, ws_bytes=nb)
+        if self.is_phase and not screen_in:
+            self._screen(_lib.OP_FWDDIFF, B, p[n_cr_p:], dpt[n_cr_p:], ob, dob)
         return self._result(ob, out), self._result(dob, dout)
 
     def _out_buffer(self, out, B):
@@ -245,7 +274,6 @@ class MeshNetworkGPU:
         dJdv: array dJ/dv_out (meshes convention dJ/dRe + i dJ/dIm) or a callable f(v_out) -> dJdv.
         Returns (out_dp, out_dJdv); out_dp (flat, size n_phase) is accumulated into when given.
         """
-        assert not self.is_phase
         assert vpos in ["in", "out"]
         p, s = self._defaults(p_phase, p_splitter)
         ldp, lds = self._ld()
@@ -265,14 +293,22 @@ class MeshNetworkGPU:
         assert out_dp.numel() == self.n_phase
         gin = self._out_buffer(out_dJdv, B)
         uin = torch.empty_like(vb)
+        n_cr_p = self.n_cr * self.X_np
+        if self.is_phase and self.phi_pos == "out":
+            # back through the output screen first (on copies: the caller's arrays stay intact)
+            vb, gb = vb.clone(), gb.clone()
+            self._screen(_lib.OP_BACKDIFF, B, p[n_cr_p:], out_dp[n_cr_p:], vb, gb)
         ws, nb = self._workspace(_lib.OP_BACKDIFF, B)
         _lib.run(_lib.OP_BACKDIFF, self.X, self.N, self.L, B, self.fft, g0, g1, p, out_dp, ldp,
                  s if s.numel() else None, lds, vb, gb, uin, gin, self.N, tile=self.tile, workspace=ws, ws_bytes=nb)
+        if self.is_phase and self.phi_pos == "in":
+            self._screen(_lib.OP_BACKDIFF, B, p[n_cr_p:], out_dp[n_cr_p:], uin, gin)
         return out_dp, self._result(gin, out_dJdv)
 
     def grad_target_l2(self, v, y, p_phase=None, p_splitter=None, out_dp=None):
         """Fused fwd+grad of J = sum |U v - y|^2 (the loss of gpu/test.py:131): returns
         (J, dp, u_out).  dJ/du_out = 2 (u_out - y) is formed inside the backdiff kernel."""
+        assert not self.is_phase, "the fused target form has no phase screen"
         p, s = self._defaults(p_phase, p_splitter)
         ldp, lds = self._ld()
         g0, g1 = self._geom()

@@ -192,6 +192,9 @@ class MeshServer:
         the requests with t_done set and a statistics dict."""
         reqs = sorted(requests, key=lambda r: r.t_arrive)
         for r in reqs:
+            r.x = r.x.resolve_conj().resolve_neg()
+            if r.target is not None:
+                r.target = r.target.resolve_conj().resolve_neg()
             r.footprint = self.footprint(r.mesh_id, r.op, r.B)
             if r.footprint > self.slot_capacity:
                 raise ValueError(f"request {r.rid} needs {r.footprint} B, above one slot's {self.slot_capacity} B")

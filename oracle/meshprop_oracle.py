@@ -445,6 +445,138 @@ def rel_l2(a, b):
     return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
 
 
+# --------------------------------------------------------------------------------------
+# Packed (StructuredMeshNetwork) form: permutations and the phase screen
+# --------------------------------------------------------------------------------------
+def _screen(v, ph):
+    return v * np.exp(1j * np.asarray(ph, np.float64))[None, :]
+
+
+def structured_run(N, lens, shifts, T, v, mode, dT=None, dv=None, ph=None, dph=None, perms=None, pos=-1):
+    """The reference CPU engine's three modes on the packed layout (c/dot.cpp:49-308,
+    mesh.py:29-70): crossings of layer n are T[:, :, inds[n]:inds[n]+lens[n]] acting on
+    waveguides (shifts[n] + 2k, shifts[n] + 2k + 1); ``perms[n]`` (gather, v <- v[p]) is applied
+    before layer n and ``perms[L]`` after the last; the phase screen exp(i ph) sits at the input
+    (pos 0) or the output (pos 1), none for pos -1.  Fields are (B, N) rows.
+
+    mode 0: returns v_out.  mode 1: returns (v_out, dv_out) with the tangent T' = dT, screen
+    tangent dph.  mode 2 (v = v_out, dv = dJ/dv_out, ``perms`` the inverse permutations as
+    mesh.py:381 passes them): returns (v_in, dJ/dv_in, dJdT (2, 2, nT), dJ/dph)."""
+    T = np.asarray(T, np.complex128)
+    v = np.array(v, dtype=np.complex128, copy=True)
+    L = len(lens)
+    inds = np.cumsum([0] + list(lens))
+    is_fd, is_bd = mode == 1, mode == 2
+    dv = (np.zeros_like(v) if dv is None else np.array(dv, dtype=np.complex128, copy=True)) if (is_fd or is_bd) else None
+    ph = np.zeros(N) if ph is None else np.asarray(ph, np.float64)
+    dph = np.zeros(N) if dph is None else np.asarray(dph, np.float64)
+    dph_out = np.zeros(N)
+    dJdT = np.zeros_like(T) if is_bd else None
+    is_perm = perms is not None and len(perms) > 0
+    # c/dot.cpp:82-87: the adjoint walks backwards with T^H (conjugate, transposed
+    # off-diagonals) and the screen at the mirrored position with -ph.
+    if is_bd:
+        pos = 1 - pos if pos >= 0 else pos
+        Tw = np.conj(T).transpose(1, 0, 2)
+        phw = -ph
+        order = range(L - 1, -1, -1)
+    else:
+        Tw = T
+        phw = ph
+        order = range(L)
+
+    def screen_step():
+        nonlocal v, dv
+        v = _screen(v, phw)
+        if is_fd:
+            dv = _screen(dv, phw) + 1j * v * dph[None, :]
+        if is_bd:
+            dv = _screen(dv, phw)
+            dph_out[:] += np.sum(v.real * dv.imag - v.imag * dv.real, axis=0)
+
+    if pos == 0:
+        screen_step()
+    for n in order:
+        if is_perm:
+            p = np.asarray(perms[n + int(is_bd)])
+            v = v[:, p]
+            if is_fd or is_bd:
+                dv = dv[:, p]
+        i1, ln, s = inds[n], lens[n], shifts[n]
+        a, b = np.arange(s, s + 2 * ln, 2), np.arange(s + 1, s + 2 * ln, 2)
+        t = Tw[:, :, i1:i1 + ln]
+        v1, v2 = v[:, a].copy(), v[:, b].copy()
+        if is_fd:
+            d = dT[:, :, i1:i1 + ln]
+            d1, d2 = dv[:, a].copy(), dv[:, b].copy()
+            dv[:, a] = d[0, 0] * v1 + d[0, 1] * v2 + t[0, 0] * d1 + t[0, 1] * d2
+            dv[:, b] = d[1, 0] * v1 + d[1, 1] * v2 + t[1, 0] * d1 + t[1, 1] * d2
+        v[:, a] = t[0, 0] * v1 + t[0, 1] * v2
+        v[:, b] = t[1, 0] * v1 + t[1, 1] * v2
+        if is_bd:
+            # dJ/dT_ij = sum_b conj(dJ/dy_i) x_j with x the reconstructed layer input
+            # (c/dot.cpp:240-285)
+            g1, g2 = dv[:, a].copy(), dv[:, b].copy()
+            x1, x2 = v[:, a], v[:, b]
+            dJdT[0, 0, i1:i1 + ln] += np.sum(np.conj(g1) * x1, axis=0)
+            dJdT[0, 1, i1:i1 + ln] += np.sum(np.conj(g1) * x2, axis=0)
+            dJdT[1, 0, i1:i1 + ln] += np.sum(np.conj(g2) * x1, axis=0)
+            dJdT[1, 1, i1:i1 + ln] += np.sum(np.conj(g2) * x2, axis=0)
+            dv[:, a] = t[0, 0] * g1 + t[0, 1] * g2
+            dv[:, b] = t[1, 0] * g1 + t[1, 1] * g2
+    if is_perm:
+        p = np.asarray(perms[0 if is_bd else L])
+        v = v[:, p]
+        if is_fd or is_bd:
+            dv = dv[:, p]
+    if pos == 1:
+        screen_step()
+    if mode == 0:
+        return v
+    if mode == 1:
+        return v, dv
+    return v, dv, dJdT, dph_out
+
+
+def structured_dense(N, lens, shifts, T, perms=None, ph=None, pos=-1):
+    """Dense FP64 transfer matrix of the packed form (brute force for small N):
+    U = S_out P_L M_{L-1} P_{L-1} ... M_0 P_0 S_in."""
+    T = np.asarray(T, np.complex128)
+    inds = np.cumsum([0] + list(lens))
+    U = np.eye(N, dtype=np.complex128)
+    S = np.diag(np.exp(1j * np.asarray(ph, np.float64))) if (ph is not None and pos >= 0) else np.eye(N)
+    if pos == 0:
+        U = S @ U
+    for n in range(len(lens)):
+        if perms is not None and len(perms):
+            U = np.eye(N)[np.asarray(perms[n])] @ U
+        M = np.eye(N, dtype=np.complex128)
+        for k in range(lens[n]):
+            i, j = shifts[n] + 2 * k, shifts[n] + 2 * k + 1
+            M[np.ix_([i, j], [i, j])] = T[:, :, inds[n] + k]
+        U = M @ U
+    if perms is not None and len(perms):
+        U = np.eye(N)[np.asarray(perms[len(lens)])] @ U
+    if pos == 1:
+        U = S @ U
+    return U
+
+
+def butterfly_perms(N):
+    """ButterflyNetwork geometry (butterfly.py:46-51): N-1 full layers; before layer i (odd)
+    the gather that makes stride-s pairs adjacent, s = 2^(trailing zeros of i+1), and before
+    layer i+1 its inverse."""
+    lens, shifts, perm = [N // 2] * (N - 1), [0] * (N - 1), [None] * N
+    for i in range(1, N - 1, 2):
+        s = 1 << ((i + 1) & -(i + 1)).bit_length() - 1
+        blocks = np.arange(0, N, 2 * s)
+        p1 = (blocks[:, None] + (np.arange(s)[:, None] + np.array([[0, s]])).reshape(-1)[None, :]).reshape(-1)
+        p2 = (blocks[:, None] + (np.arange(0, 2 * s, 2)[None, :] + np.array([[0], [1]])).reshape(-1)[None, :]).reshape(-1)
+        perm[i], perm[i + 1] = p1, p2
+    full = [np.arange(N) if p is None else np.asarray(p) for p in perm]
+    return lens, shifts, full
+
+
 PINS = {
     "crossing_T": "closed forms + unitarity/|det| invariants + ideal cross/bar special cases (tests/test_oracle.py)",
     "crossing_dT": "central finite differences of crossing_T in FP64 (tests/test_oracle.py)",
@@ -454,4 +586,8 @@ PINS = {
     "vjp_inverse": "golden vectors from c/dot.cpp mode 2 (threads=1) + adjoint identity + FD",
     "vjp_stored": "equals vjp_inverse on unitary meshes + FD on lossy meshes",
     "Geometry(fft)": "dense block-diagonal construction of gpu/test.py:73-77 restated + FD",
+    "structured_run": "golden vectors from c/dot.cpp modes 0/1/2 with permutations and the phase screen at "
+                      "either position + dense brute force (structured_dense) + FD of the screen phases",
+    "structured_dense": "agrees with structured_run mode 0 and with the reference engine's golden vectors",
+    "butterfly_perms": "permutation pairs checked against the stride-pairing definition (adjacent after gather)",
 }

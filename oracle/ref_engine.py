@@ -95,3 +95,44 @@ def run(geom, T_slot, v, mode, dT_slot=None, dv=None, threads=1, dtype=np.comple
     if mode == 1:
         return vv.T.copy(), dvv.T.copy()
     return vv.T.copy(), dvv.T.copy(), unpack_slots(geom, dT)
+
+
+def run_packed(N, lens, shifts, T, v, mode, dT=None, dv=None, ph=None, dph=None, perms=None, pos=-1, threads=1,
+               dtype=np.complex128):
+    """meshdot_helper on the packed layout with permutations and the phase screen
+    (mesh.py:343-392 argument preparation).  T, dT: (2, 2, nT); v, dv: (B, N) rows; perms:
+    (L+1, N) gathers (for mode 2, the inverse permutations, as mesh.py:381 passes them).
+    Returns what the engine leaves in (v, dv, dT, dph) for the mode, like structured_run."""
+    lib = _load()
+    is64 = dtype == np.complex128
+    fn = lib.meshdot_helper64 if is64 else lib.meshdot_helper32
+    ftype = np.float64 if is64 else np.float32
+    L = len(lens)
+    B = v.shape[0]
+    TT = np.ascontiguousarray(T, dtype=dtype)
+    nT = TT.shape[2]
+    if mode == 2:
+        dTT = np.zeros_like(TT)
+    elif mode == 1:
+        dTT = np.ascontiguousarray(dT, dtype=dtype)
+    else:
+        dTT = np.zeros((0, 0, 0), dtype=dtype)
+    vv = np.ascontiguousarray(np.asarray(v).T, dtype=dtype)
+    if mode == 0:
+        dvv = np.zeros((0, 0), dtype=dtype)
+    else:
+        dvv = np.zeros_like(vv) if dv is None else np.ascontiguousarray(np.asarray(dv).T, dtype=dtype)
+    phh = np.ascontiguousarray(np.zeros(N) if ph is None else ph, dtype=ftype)
+    dphh = np.ascontiguousarray(np.zeros(N) if dph is None else dph, dtype=ftype)
+    inds = np.ascontiguousarray(np.cumsum([0] + list(lens)), dtype=np.int32)
+    ln = np.ascontiguousarray(lens, dtype=np.int32)
+    sh = np.ascontiguousarray(shifts, dtype=np.int32)
+    pp = np.ascontiguousarray(np.zeros((0, 0)) if perms is None else perms, dtype=np.int32)
+    fn(N, B, nT, L, TT.ctypes.data, dTT.ctypes.data, phh.ctypes.data, dphh.ctypes.data, vv.ctypes.data,
+       dvv.ctypes.data, inds.ctypes.data, ln.ctypes.data, sh.ctypes.data, pp.ctypes.data, int(pp.size), int(pos),
+       int(mode), int(threads))
+    if mode == 0:
+        return vv.T.copy()
+    if mode == 1:
+        return vv.T.copy(), dvv.T.copy()
+    return vv.T.copy(), dvv.T.copy(), dTT, dphh

@@ -45,6 +45,30 @@ def _case(name, geom, X, p, s, u, dp, g_target_seed, threads=8):
     print("wrote", path)
 
 
+def _packed_case(name, N, lens, shifts, perms, X, p_packed, u, dp_packed, ph, dph, pos, seed):
+    """A packed-layout mesh (optionally with permutations and a phase screen) through the
+    reference engine, modes 0/1/2 (mode 2 with the inverse permutations, mesh.py:381)."""
+    T = O.crossing_T(X, p_packed).transpose(1, 2, 0)
+    dTk = O.crossing_dT(X, p_packed)  # (n, n_p, 2, 2)
+    dT = np.einsum("nkij,nk->ijn", dTk, dp_packed)
+    out0 = ref_engine.run_packed(N, lens, shifts, T, u, 0, ph=ph, perms=perms, pos=pos, threads=4)
+    out1, dout1 = ref_engine.run_packed(N, lens, shifts, T, u, 1, dT=dT, dv=None, ph=ph, dph=dph, perms=perms,
+                                        pos=pos, threads=4)
+    rng = np.random.RandomState(seed)
+    y = O.unit_cn(rng, u.shape[0], N)
+    g = 2 * (out0 - y)
+    perms_bk = None if perms is None else np.array([np.argsort(q) for q in perms])
+    rec2, gin2, dJdT2, dph2 = ref_engine.run_packed(N, lens, shifts, T, out0, 2, dv=g, ph=ph, perms=perms_bk,
+                                                    pos=pos, threads=1)
+    path = os.path.join(HERE, f"{name}.npz")
+    np.savez_compressed(
+        path, N=N, lens=np.asarray(lens), shifts=np.asarray(shifts),
+        perms=(np.zeros((0, 0), int) if perms is None else np.asarray(perms)), X=X, p=p_packed, dp=dp_packed,
+        ph=ph, dph=dph, pos=pos, u_in=u, y=y, fwd_out=out0, jvp_out=out1, jvp_dout=dout1, g_out=g,
+        vjp_u_in=rec2, vjp_g_in=gin2, vjp_dJdT=dJdT2, vjp_dph=dph2)
+    print("wrote", path)
+
+
 def main():
     if not ref_engine.available():
         raise SystemExit("build the reference engine first: make -C oracle")
@@ -77,6 +101,32 @@ def main():
     u = O.unit_cn(rng, 9, 12)
     dp = rng.standard_normal((12, 6, 2))
     _case("sym_clements12", geom, "sym", p, s, u, dp, 401)
+
+
+    # Packed layout with permutations and the phase screen: a ButterflyNetwork geometry
+    # (butterfly.py:46-51, N-1 layers with stride gathers) with the output screen, and a
+    # Clements mesh with the input screen.
+    rng = np.random.RandomState(500)
+    N = 16
+    lens, shifts, perms = O.butterfly_perms(N)
+    n_cr = sum(lens)
+    p = rng.uniform(0, 2 * np.pi, size=(n_cr, 2))
+    u = O.unit_cn(rng, 11, N)
+    dp = rng.standard_normal((n_cr, 2))
+    ph = rng.uniform(0, 2 * np.pi, N)
+    dph = rng.standard_normal(N)
+    _packed_case("butterfly16_screen_out", N, lens, shifts, perms, "mzi", p, u, dp, ph, dph, 1, 501)
+    rng = np.random.RandomState(600)
+    N = 12
+    geom = O.clements(N)
+    lens, shifts = list(geom.lens), list(geom.shifts)
+    n_cr = sum(lens)
+    p = rng.uniform(0, 2 * np.pi, size=(n_cr, 2))
+    u = O.unit_cn(rng, 7, N)
+    dp = rng.standard_normal((n_cr, 2))
+    ph = rng.uniform(0, 2 * np.pi, N)
+    dph = rng.standard_normal(N)
+    _packed_case("clements12_screen_in", N, lens, shifts, None, "mzi", p, u, dp, ph, dph, 0, 601)
 
 
 if __name__ == "__main__":

@@ -17,7 +17,9 @@ import pytest
 from oracle import meshprop_oracle as O
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-GOLDEN = sorted(glob.glob(os.path.join(HERE, "golden", "*.npz")))
+ALL_GOLDEN = sorted(glob.glob(os.path.join(HERE, "golden", "*.npz")))
+GOLDEN = [p for p in ALL_GOLDEN if "screen" not in os.path.basename(p)]   # slot layout, no screen
+PACKED = [p for p in ALL_GOLDEN if "screen" in os.path.basename(p)]   # packed, perms, phase screen
 
 
 def _geom(d):
@@ -217,3 +219,73 @@ def test_planted_mistake_is_caught():
     planted[:, 0, 0, 1] *= -1
     assert np.allclose(good[:, 0], fd, atol=1e-8)
     assert not np.allclose(planted[:, 0], fd, atol=1e-8)
+
+
+def _packed_setup(d):
+    N = int(d["N"])
+    lens, shifts = [int(x) for x in d["lens"]], [int(x) for x in d["shifts"]]
+    perms = d["perms"] if d["perms"].size else None
+    X = str(d["X"])
+    T = O.crossing_T(X, d["p"]).transpose(1, 2, 0)
+    dT = np.einsum("nkij,nk->ijn", O.crossing_dT(X, d["p"]), d["dp"])
+    return N, lens, shifts, perms, X, T, dT, int(d["pos"])
+
+
+@pytest.mark.parametrize("path", PACKED, ids=[os.path.basename(p)[:-4] for p in PACKED])
+def test_golden_packed_with_perms_and_screen(path):
+    """structured_run (permutations, phase screen at either end) against the compiled
+    reference engine c/dot.cpp, modes 0/1/2, and the dense brute-force product."""
+    d = np.load(path)
+    N, lens, shifts, perms, X, T, dT, pos = _packed_setup(d)
+    out = O.structured_run(N, lens, shifts, T, d["u_in"], 0, ph=d["ph"], perms=perms, pos=pos)
+    assert O.rel_l2(out, d["fwd_out"]) < 1e-13
+    U = O.structured_dense(N, lens, shifts, T, perms=perms, ph=d["ph"], pos=pos)
+    assert O.rel_l2(d["u_in"] @ U.T, d["fwd_out"]) < 1e-13
+    o1, do1 = O.structured_run(N, lens, shifts, T, d["u_in"], 1, dT=dT, ph=d["ph"], dph=d["dph"], perms=perms,
+                               pos=pos)
+    assert O.rel_l2(o1, d["jvp_out"]) < 1e-13 and O.rel_l2(do1, d["jvp_dout"]) < 1e-13
+    pbk = None if perms is None else np.array([np.argsort(q) for q in perms])
+    rec, gin, dJdT, dph = O.structured_run(N, lens, shifts, T, d["fwd_out"], 2, dv=d["g_out"], ph=d["ph"], perms=pbk,
+                                           pos=pos)
+    assert O.rel_l2(rec, d["vjp_u_in"]) < 1e-13 and O.rel_l2(gin, d["vjp_g_in"]) < 1e-13
+    assert O.rel_l2(dJdT, d["vjp_dJdT"]) < 1e-13 and O.rel_l2(dph, d["vjp_dph"]) < 1e-13
+
+
+@pytest.mark.parametrize("pos", [0, 1])
+def test_phase_screen_finite_difference(pos):
+    """dJ/dphi of the packed adjoint and the screen tangent against central differences."""
+    rng = np.random.RandomState(31 + pos)
+    N, B = 8, 5
+    g = O.clements(N)
+    lens, shifts = list(g.lens), list(g.shifts)
+    p = rng.uniform(0, 2 * np.pi, (sum(lens), 2))
+    T = O.crossing_T("mzi", p).transpose(1, 2, 0)
+    ph = rng.uniform(0, 2 * np.pi, N)
+    u = O.unit_cn(rng, B, N)
+    y = O.unit_cn(rng, B, N)
+    dph = rng.standard_normal(N)
+    h = 1e-6
+
+    def loss(phv):
+        return np.sum(np.abs(O.structured_run(N, lens, shifts, T, u, 0, ph=phv, pos=pos) - y) ** 2)
+    out, dout = O.structured_run(N, lens, shifts, T, u, 1, dT=np.zeros_like(T), ph=ph, dph=dph, pos=pos)
+    fd = (O.structured_run(N, lens, shifts, T, u, 0, ph=ph + h * dph, pos=pos)
+          - O.structured_run(N, lens, shifts, T, u, 0, ph=ph - h * dph, pos=pos)) / (2 * h)
+    assert O.rel_l2(dout, fd) < 1e-7
+    _, _, _, dJ = O.structured_run(N, lens, shifts, T, out, 2, dv=2 * (out - y), ph=ph, pos=pos)
+    fdJ = (loss(ph + h * dph) - loss(ph - h * dph)) / (2 * h)
+    assert np.isclose(np.sum(dJ * dph), fdJ, rtol=1e-6, atol=1e-9)
+
+
+def test_butterfly_perms_pair_stride_neighbours():
+    """After the gathers of butterfly_perms, layer i couples waveguides 2^tz(i+1) apart for odd i
+    (butterfly.py:46-51) and adjacent ones for even i; the last gather restores the order."""
+    N = 32
+    lens, shifts, perms = O.butterfly_perms(N)
+    pos = np.arange(N)
+    for i in range(N - 1):
+        pos = pos[perms[i]]
+        a, b = pos[0::2], pos[1::2]
+        s = 1 if i % 2 == 0 else ((i + 1) & -(i + 1))
+        assert np.all(b - a == s) and np.all((a & s) == 0)
+    assert np.array_equal(pos[perms[N - 1]], np.arange(N))
