@@ -233,6 +233,21 @@ int meshprop_phase_screen(int op, int N, int B, const float *phi, float *dphi, v
                           void *workspace, size_t workspace_bytes, void *stream);
 size_t meshprop_phase_screen_workspace_size(int N, int B);
 
+/* Exact Hessian-vector product along the parameter direction q, by forward-over-reverse
+ * differentiation (no reference counterpart: gpu/mesh.py:236-288 takes central differences of
+ * two backdiffs).  Inputs: the forward output u_out and its tangent du_out along q (from
+ * meshprop_fwddiff with dp = q), and either target (loss sum |u_out - target|^2: dJ/du_out =
+ * 2 (u_out - target), tangent 2 du_out) or dJdu_out with its tangent d_dJdu_out along q.
+ * Accumulates grad += dJ/dp (nullable) and hvp += (d2J/dp2) q in the slot layout of p.  Unitary
+ * complex crossings (MESHPROP_MZI, MESHPROP_SYM), N <= 512; other crossings return
+ * MESHPROP_ERR_UNSUPPORTED.  Workspace: meshprop_hvp_workspace_size bytes. */
+size_t meshprop_hvp_workspace_size(int X, int N, int L, int B, int fft);
+int meshprop_hvp(int X, int N, int L, int B, int fft, const int *geom0, const int *geom1,
+                 const float *p, const float *q, int ldp, const float *s, int lds,
+                 const void *u_out, const void *du_out, const void *dJdu_out, const void *d_dJdu_out,
+                 const void *target, float *grad, float *hvp, int ldu,
+                 void *workspace, size_t workspace_bytes, void *stream);
+
 /* The tile the library would use for a call (after defaults and clamping). */
 int meshprop_resolve_tile(int op, int X, int N, int B, meshprop_tile *out);
 

@@ -32,6 +32,7 @@ EXPORTS = (
     "meshprop_group_create", "meshprop_group_destroy", "meshprop_group_workspace_size",
     "meshprop_group_tables_size", "meshprop_group_upload", "meshprop_group_launch", "meshprop_group_signature",
     "meshprop_group_entry_info", "meshprop_phase_screen", "meshprop_phase_screen_workspace_size",
+    "meshprop_hvp", "meshprop_hvp_workspace_size",
 )
 
 
@@ -105,6 +106,10 @@ def load():
     lib.meshprop_phase_screen.restype = c_int
     lib.meshprop_phase_screen_workspace_size.argtypes = [c_int, c_int]
     lib.meshprop_phase_screen_workspace_size.restype = c_size_t
+    lib.meshprop_hvp_workspace_size.argtypes = [c_int] * 5
+    lib.meshprop_hvp_workspace_size.restype = c_size_t
+    lib.meshprop_hvp.argtypes = ([c_int] * 5 + [vp] * 4 + [c_int, vp, c_int] + [vp] * 7 + [c_int, vp, c_size_t, vp])
+    lib.meshprop_hvp.restype = c_int
     _lib = lib
     return lib
 
@@ -180,3 +185,20 @@ def phase_screen(op, N, B, phi, dphi, u, du, ldu, workspace=None, ws_bytes=0, st
 
 def phase_screen_workspace_size(N, B) -> int:
     return int(load().meshprop_phase_screen_workspace_size(N, B))
+
+
+def hvp_workspace_size(X, N, L, B, fft=False) -> int:
+    return int(load().meshprop_hvp_workspace_size(X_CODE[X] if isinstance(X, str) else X, N, L, B, int(fft)))
+
+
+def hvp(X, N, L, B, fft, g0, g1, p, q, ldp, s, lds, u_out, du_out, dJdu_out, d_dJdu_out, target, grad, hv, ldu,
+        workspace, ws_bytes, stream=None):
+    """meshprop_hvp on torch tensors."""
+    lib = load()
+    if stream is None:
+        stream = torch.cuda.current_stream().cuda_stream
+    rc = lib.meshprop_hvp(X_CODE[X] if isinstance(X, str) else X, N, L, B, int(fft), _ptr(g0), _ptr(g1), _ptr(p),
+                          _ptr(q), ldp, _ptr(s), lds, _ptr(u_out), _ptr(du_out), _ptr(dJdu_out), _ptr(d_dJdu_out),
+                          _ptr(target), _ptr(grad), _ptr(hv), ldu, _ptr(workspace), ctypes.c_size_t(ws_bytes),
+                          ctypes.c_void_p(stream))
+    check(rc, "hvp")

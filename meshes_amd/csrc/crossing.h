@@ -136,4 +136,27 @@ __device__ inline void crossing_eval(int X, const float* p, const float* s, M2* 
     }
 }
 
+// Second derivatives contracted with a direction q: S_k = sum_m q_m d2T/dp_k dp_m, from T and
+// dT/dp_k of the same crossing.  MZI and the generic MZI: d2T/dth2 = i dT/dth,
+// d2T/dth dph = dT/dth diag(i, 0), d2T/dph2 = T diag(-1, 0).  SYM: d2T/dth2 = -T/4,
+// d2T/dth dph = diag(i dT/dth_00, -i dT/dth_11), d2T/dph2 = diag(-T00, -T11).  ORTH: -T/4.
+__device__ inline void crossing_d2q(int X, const M2& T, const M2* dT, const double* q, M2* S) {
+    if (X == X_ORTH) {
+        S[0] = mscale(T, -0.25 * q[0]);
+        return;
+    }
+    M2 H00, H01, H11;
+    if (X == X_SYM) {
+        H00 = mscale(T, -0.25);
+        H01 = M2{cmuli(dT[0].a00), mk(0, 0), mk(0, 0), cscale(cmuli(dT[0].a11), -1.0)};
+        H11 = M2{cscale(T.a00, -1.0), mk(0, 0), mk(0, 0), cscale(T.a11, -1.0)};
+    } else {
+        H00 = M2{cmuli(dT[0].a00), cmuli(dT[0].a01), cmuli(dT[0].a10), cmuli(dT[0].a11)};
+        H01 = M2{cmuli(dT[0].a00), mk(0, 0), cmuli(dT[0].a10), mk(0, 0)};
+        H11 = M2{cscale(T.a00, -1.0), mk(0, 0), cscale(T.a10, -1.0), mk(0, 0)};
+    }
+    S[0] = madd(mscale(H00, q[0]), mscale(H01, q[1]));
+    S[1] = madd(mscale(H01, q[0]), mscale(H11, q[1]));
+}
+
 }  // namespace mp

@@ -1,0 +1,336 @@
+// hvp.hip -- exact Hessian-vector products by forward-over-reverse differentiation (no
+// reference counterpart: gpu/mesh.py:236-288 takes central differences of two backdiffs).
+//
+// Given the forward output and its tangent along the parameter direction q (u_out, du_out,
+// from fwddiff), and the loss gradient with its tangent (g, dg; or a target: g = 2 (u - t),
+// dg = 2 du), one reverse walk over the layers carries (u, du, g, dg):
+//   x  = T^H u,   dx = T^H du + dT_q^H u        (reconstructed input and its tangent)
+//   g' = T^H g,   dg' = T^H dg + dT_q^H g      (adjoint and its tangent)
+//   dJdT += conj(g) x,   d(dJdT) += conj(dg) x + conj(g) dx
+// with dT_q = sum_k q_k dT/dp_k.  At every LDS stage end the workgroup converts, per slot,
+//   grad_k += Re sum dJdT_ij dT_ij/dp_k
+//   hvp_k  += Re sum [d(dJdT)_ij dT_ij/dp_k + dJdT_ij S_k,ij],  S_k = sum_m q_m d2T/dp_k dp_m,
+// into per-workgroup partials, reduced afterwards in fixed order.  Unitary complex crossings
+// (MZI, SYM), one vector per lane group, N up to 512.
+#include <algorithm>
+
+#include "kernels.h"
+
+namespace mp {
+
+template <int W>
+struct HvpStep {
+    const float2 (&u)[W];
+    const float2 (&du)[W];
+    const float2 (&g)[W];
+    const float2 (&dg)[W];
+    float2 (&uo)[W];
+    float2 (&duo)[W];
+    float2 (&go)[W];
+    float2 (&dgo)[W];
+    const float4 (&r)[W];
+    const float4 (&d)[W];
+    float2 (&A)[W];
+    float2 (&Bq)[W];
+    float2 (&dA)[W];
+    float2 (&dB)[W];
+    int lane;
+
+    __device__ __forceinline__ void identity() {
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            uo[k] = u[k];
+            duo[k] = du[k];
+            go[k] = g[k];
+            dgo[k] = dg[k];
+        }
+    }
+    // Partner of waveguide k for layer class KC (in-lane stride S; odd NN shift; lane ^ xm).
+    template <int KC, int S>
+    __device__ __forceinline__ void partners(const float2 (&a)[W], float2 (&p)[W], const int xm) {
+        if constexpr (KC == 0) {
+#pragma unroll
+            for (int k = 0; k < W; ++k) p[k] = a[k ^ S];
+        } else if constexpr (KC == 1) {
+#pragma unroll
+            for (int k = 1; k + 1 < W; ++k) p[k] = a[(k & 1) ? k + 1 : k - 1];
+            p[W - 1] = lane_next(a[0]);
+            p[0] = lane_prev(a[W - 1]);
+        } else {
+            const int src = lane ^ xm;
+#pragma unroll
+            for (int k = 0; k < W; ++k) p[k] = lane_read(a[k], src);
+        }
+    }
+    template <int KC, int S>
+    __device__ __forceinline__ void run(const int xm) {
+        float2 pu[W], pdu[W], pg[W], pdg[W];
+        partners<KC, S>(u, pu, xm);
+        partners<KC, S>(du, pdu, xm);
+        partners<KC, S>(g, pg, xm);
+        partners<KC, S>(dg, pdg, xm);
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            uo[k] = rapply(r[k], u[k], pu[k]);
+            duo[k] = rapply_d(r[k], d[k], u[k], pu[k], du[k], pdu[k]);
+            go[k] = rapply(r[k], g[k], pg[k]);
+            dgo[k] = rapply_d(r[k], d[k], g[k], pg[k], dg[k], pdg[k]);
+        }
+        float2 px[W], pdx[W];
+        partners<KC, S>(uo, px, xm);
+        partners<KC, S>(duo, pdx, xm);
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            A[k] = cjfma(g[k], uo[k], A[k]);
+            Bq[k] = cjfma(g[k], px[k], Bq[k]);
+            dA[k] = cjfma(g[k], duo[k], cjfma(dg[k], uo[k], dA[k]));
+            dB[k] = cjfma(g[k], pdx[k], cjfma(dg[k], px[k], dB[k]));
+        }
+    }
+};
+
+constexpr int kHvpWaves = 4;
+constexpr size_t kHvpStage = 64 * 1024;
+
+template <int W>
+__global__ void __launch_bounds__(64 * kHvpWaves) k_hvp(
+    int N, int L, int B, int LPV, int NP, int C, const int* __restrict__ kind, const float4* __restrict__ rows_r,
+    const float4* __restrict__ rows_d, const float2* __restrict__ dtdp, const float2* __restrict__ d2,
+    float* __restrict__ part_g, float* __restrict__ part_h, size_t part_stride, long ntiles, const float2* u_out,
+    const float2* du_out, const float2* g_out, const float2* dg_out, const float2* target, int ldu) {
+    extern __shared__ unsigned char smem_raw[];
+    float4* stage = reinterpret_cast<float4*>(smem_raw);  // [nw][C][2][W][LPV]: (A,B) then (dA,dB)
+    const int nw = blockDim.x >> 6;
+    const int NPh = NP / 2;
+    const int lane = threadIdx.x & 63;
+    const int wave_u = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wstride = C * 2 * W * LPV;
+    float* pg = part_g + (size_t)blockIdx.x * part_stride;
+    float* ph = part_h + (size_t)blockIdx.x * part_stride;
+    bool first = true;
+    for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const Lanes q = lanes_of<1>(LPV, tile);
+        float2 u[1][W], du[1][W], g[1][W], dg[1][W];
+        load_tile(u, u_out, q, N, B, ldu);
+        load_tile(du, du_out, q, N, B, ldu);
+        if (target != nullptr) {
+            float2 t[1][W];
+            load_tile(t, target, q, N, B, ldu);
+#pragma unroll
+            for (int k = 0; k < W; ++k) {
+                g[0][k] = fsub2(u[0][k], t[0][k]);
+                dg[0][k] = make_float2(2.f * du[0][k].x, 2.f * du[0][k].y);
+            }
+        } else {
+            load_tile(g, g_out, q, N, B, ldu);
+            load_tile(dg, dg_out, q, N, B, ldu);
+        }
+        for (int c0 = 0; c0 < L; c0 += C) {
+            const int cl = (C < L - c0) ? C : L - c0;
+            for (int lc = 0; lc < cl; ++lc) {
+                const int la = L - 1 - (c0 + lc);
+                const int kd = kind[la];
+                float4 r[W], d[W];
+                load_rows(r, rows_r + (size_t)la * NP + q.t, LPV);
+                load_rows(d, rows_d + (size_t)la * NP + q.t, LPV);
+                float2 uo[W], duo[W], go[W], dgo[W], A[W], Bq[W], dA[W], dB[W];
+#pragma unroll
+                for (int k = 0; k < W; ++k) A[k] = Bq[k] = dA[k] = dB[k] = make_float2(0.f, 0.f);
+                HvpStep<W> st{u[0], du[0], g[0], dg[0], uo, duo, go, dgo, r, d, A, Bq, dA, dB, lane};
+                dispatch_kind<W>(kd, st);
+                // Fold lane groups (they hold the same waveguides of different vectors).
+                for (int off = LPV; off < 64; off <<= 1) {
+#pragma unroll
+                    for (int k = 0; k < W; ++k) {
+                        A[k] = fadd(A[k], lane_read(A[k], lane ^ off));
+                        Bq[k] = fadd(Bq[k], lane_read(Bq[k], lane ^ off));
+                        dA[k] = fadd(dA[k], lane_read(dA[k], lane ^ off));
+                        dB[k] = fadd(dB[k], lane_read(dB[k], lane ^ off));
+                    }
+                }
+                if (q.grp == 0) {
+                    float4* s0 = stage + (((size_t)q.wave * C + lc) * 2) * W * LPV;
+#pragma unroll
+                    for (int k = 0; k < W; ++k) {
+                        s0[k * LPV + q.t] = pack_row(A[k], Bq[k]);
+                        s0[(W + k) * LPV + q.t] = pack_row(dA[k], dB[k]);
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < W; ++k) {
+                    u[0][k] = uo[k];
+                    du[0][k] = duo[k];
+                    g[0][k] = go[k];
+                    dg[0][k] = dgo[k];
+                }
+            }
+            __syncthreads();
+            for (int lcc = wave_u; lcc < cl; lcc += nw) {
+                const int la = L - 1 - (c0 + lcc);
+                const int kdl = kind[la];
+                const float4* s0 = stage + (size_t)lcc * 2 * W * LPV;
+                const float4* s1 = s0 + W * LPV;
+                const float2* t1 = dtdp + (size_t)la * NPh * 8;
+                const float2* t2 = d2 + (size_t)la * NPh * 8;
+                float* og = pg + (size_t)la * NPh * 2;
+                float* oh = ph + (size_t)la * NPh * 2;
+                for (int m = lane; m < NPh; m += 64) {
+                    float2 e1[8], e2[8];
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        e1[e] = t1[e * NPh + m];
+                        e2[e] = t2[e * NPh + m];
+                    }
+                    convert_slot<float2, W, 2>(s0, wstride, nw, LPV, NP, og, kdl, m, e1, first);
+                    convert_slot<float2, W, 2>(s1, wstride, nw, LPV, NP, oh, kdl, m, e1, first);
+                    convert_slot<float2, W, 2>(s0, wstride, nw, LPV, NP, oh, kdl, m, e2, false);
+                }
+            }
+            __syncthreads();
+        }
+        first = false;
+    }
+}
+
+// out[l*ldp + m*np + k] += sum over workgroups (in order) of partial[wg][(l*NPh + m)*np + k], m < N/2.
+__global__ void __launch_bounds__(256) k_hvp_reduce(int L, int Nh, int NPh, int np, int nwg, size_t stride,
+                                                    const float* __restrict__ partial, float* __restrict__ out,
+                                                    int ldp) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)L * Nh * np) return;
+    const int k = idx % np;
+    const long lm = idx / np;
+    const int m = lm % Nh;
+    const int l = lm / Nh;
+    const float* src = partial + ((size_t)l * NPh + m) * np + k;
+    float a0 = 0.f, a1 = 0.f;
+    int w = 0;
+    for (; w + 1 < nwg; w += 2) {
+        a0 += src[(size_t)w * stride];
+        a1 += src[(size_t)(w + 1) * stride];
+    }
+    if (w < nwg) a0 += src[(size_t)w * stride];
+    out[(size_t)l * ldp + (size_t)m * np + k] += a0 + a1;
+}
+
+}  // namespace mp
+
+using namespace mp;
+
+namespace {
+
+struct HvpPlan {
+    int W, LPV, NP, C, grid;
+    long ntiles;
+    size_t E, off_kind, off_r, off_d, off_t1, off_t2, off_pg, off_ph, total;
+};
+
+inline size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int hvp_plan(int X, int N, int L, int B, HvpPlan& P) {
+    if (X != X_MZI && X != X_SYM) return (X < 0 || X > 3) ? MESHPROP_ERR_ARG : MESHPROP_ERR_UNSUPPORTED;
+    if (N <= 0 || (N & 1) || L < 0 || B < 0) return MESHPROP_ERR_ARG;
+    if (N > 512) return MESHPROP_ERR_UNSUPPORTED;
+    P.W = N <= 128 ? 2 : N <= 256 ? 4 : 8;
+    int lpv = 1;
+    while (lpv < (N + P.W - 1) / P.W) lpv <<= 1;
+    P.LPV = lpv;
+    P.NP = lpv * P.W;
+    const int G = 64 / lpv;
+    P.ntiles = std::max<long>(1, ((long)B + kHvpWaves * G - 1) / (kHvpWaves * G));
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0) ncu = 256;
+    P.grid = (int)std::min<long>(P.ntiles, 2L * ncu);
+    const size_t per_layer = (size_t)kHvpWaves * 2 * P.NP * sizeof(float4);
+    P.C = (int)std::max<size_t>(1, std::min<size_t>(8, kHvpStage / per_layer));
+    if ((size_t)P.C * per_layer > kHvpStage) return MESHPROP_ERR_UNSUPPORTED;
+    const size_t NPh = P.NP / 2;
+    P.E = (size_t)L * NPh * 2;
+    size_t off = 0;
+    P.off_kind = off;
+    off = al256(off + (size_t)L * sizeof(int));
+    P.off_r = off;
+    off = al256(off + (size_t)L * P.NP * sizeof(float4));
+    P.off_d = off;
+    off = al256(off + (size_t)L * P.NP * sizeof(float4));
+    P.off_t1 = off;
+    off = al256(off + (size_t)L * NPh * 8 * sizeof(float2));
+    P.off_t2 = off;
+    off = al256(off + (size_t)L * NPh * 8 * sizeof(float2));
+    P.off_pg = off;
+    off = al256(off + (size_t)P.grid * P.E * sizeof(float));
+    P.off_ph = off;
+    off = al256(off + (size_t)P.grid * P.E * sizeof(float));
+    P.total = off;
+    return MESHPROP_OK;
+}
+
+template <int W>
+void launch_hvp(const HvpPlan& P, int N, int L, int B, unsigned char* base, const void* u_out, const void* du_out,
+                const void* g, const void* dg, const void* target, int ldu, hipStream_t st) {
+    const size_t shmem = (size_t)kHvpWaves * P.C * 2 * P.NP * sizeof(float4);
+    hipLaunchKernelGGL((k_hvp<W>), dim3(P.grid), dim3(64 * kHvpWaves), shmem, st, N, L, B, P.LPV, P.NP, P.C,
+                       (const int*)(base + P.off_kind), (const float4*)(base + P.off_r),
+                       (const float4*)(base + P.off_d), (const float2*)(base + P.off_t1),
+                       (const float2*)(base + P.off_t2), (float*)(base + P.off_pg), (float*)(base + P.off_ph), P.E,
+                       P.ntiles, (const float2*)u_out, (const float2*)du_out, (const float2*)g, (const float2*)dg,
+                       (const float2*)target, ldu);
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t meshprop_hvp_workspace_size(int X, int N, int L, int B, int fft) {
+    (void)fft;
+    HvpPlan P;
+    return hvp_plan(X, N, L, B, P) == MESHPROP_OK ? P.total : 0;
+}
+
+int meshprop_hvp(int X, int N, int L, int B, int fft, const int* geom0, const int* geom1, const float* p,
+                 const float* q, int ldp, const float* s, int lds, const void* u_out, const void* du_out,
+                 const void* dJdu_out, const void* d_dJdu_out, const void* target, float* grad, float* hvp, int ldu,
+                 void* workspace, size_t workspace_bytes, void* stream) {
+    HvpPlan P;
+    int rc = hvp_plan(X, N, L, B, P);
+    if (rc) return rc;
+    if (B == 0 || L == 0) return MESHPROP_OK;
+    if (ldu < N || !u_out || !du_out || !hvp || !p || !q || !geom0 || (!fft && !geom1)) return MESHPROP_ERR_ARG;
+    if (!target && (!dJdu_out || !d_dJdu_out)) return MESHPROP_ERR_ARG;
+    if (ldp < (N / 2) * 2) return MESHPROP_ERR_ARG;
+    const int ns = n_split_of(X);
+    if (s && ns && lds < (N / 2) * ns) return MESHPROP_ERR_ARG;
+    if (!workspace || workspace_bytes < P.total || (((uintptr_t)workspace) & 255)) return MESHPROP_ERR_WORKSPACE;
+    hipStream_t st = (hipStream_t)stream;
+    unsigned char* base = (unsigned char*)workspace;
+    const float* sp = ns ? s : nullptr;
+    hipLaunchKernelGGL((k_rows<float4>), dim3((P.NP + 255) / 256, L), dim3(256), 0, st, X, N, L, P.NP, P.LPV, fft,
+                       geom0, geom1, p, ldp, sp, lds, q, (int)(TAB_INV | TAB_DIFF_ADJ), (int*)(base + P.off_kind),
+                       (float4*)(base + P.off_r), (float4*)(base + P.off_d));
+    if (hipGetLastError() != hipSuccess) return MESHPROP_ERR_HIP;
+    const dim3 g2((P.NP / 2 + 255) / 256, L);
+    hipLaunchKernelGGL((k_dtdp<float2>), g2, dim3(256), 0, st, X, N, L, P.NP, fft, geom0, geom1, p, ldp, sp, lds,
+                       (float2*)(base + P.off_t1));
+    hipLaunchKernelGGL((k_d2tdp<float2>), g2, dim3(256), 0, st, X, N, L, P.NP, fft, geom0, geom1, p, q, ldp, sp, lds,
+                       (float2*)(base + P.off_t2));
+    if (hipGetLastError() != hipSuccess) return MESHPROP_ERR_HIP;
+    switch (P.W) {
+        case 2: launch_hvp<2>(P, N, L, B, base, u_out, du_out, dJdu_out, d_dJdu_out, target, ldu, st); break;
+        case 4: launch_hvp<4>(P, N, L, B, base, u_out, du_out, dJdu_out, d_dJdu_out, target, ldu, st); break;
+        default: launch_hvp<8>(P, N, L, B, base, u_out, du_out, dJdu_out, d_dJdu_out, target, ldu, st); break;
+    }
+    if (hipGetLastError() != hipSuccess) return MESHPROP_ERR_HIP;
+    const long total = (long)L * (N / 2) * 2;
+    const dim3 gr((unsigned)((total + 255) / 256));
+    if (grad) {
+        hipLaunchKernelGGL(k_hvp_reduce, gr, dim3(256), 0, st, L, N / 2, P.NP / 2, 2, P.grid, P.E,
+                           (const float*)(base + P.off_pg), grad, ldp);
+    }
+    hipLaunchKernelGGL(k_hvp_reduce, gr, dim3(256), 0, st, L, N / 2, P.NP / 2, 2, P.grid, P.E,
+                       (const float*)(base + P.off_ph), hvp, ldp);
+    if (hipGetLastError() != hipSuccess) return MESHPROP_ERR_HIP;
+    return MESHPROP_OK;
+}
+
+}  // extern "C"

@@ -13,7 +13,7 @@ namespace mp {
 // =======================================================================================
 // Prologue: row tables
 // =======================================================================================
-enum { TAB_FWD = 1, TAB_DIFF = 2, TAB_INV = 4, TAB_ADJ = 8, TAB_DTDP = 16 };
+enum { TAB_FWD = 1, TAB_DIFF = 2, TAB_INV = 4, TAB_ADJ = 8, TAB_DTDP = 16, TAB_DIFF_ADJ = 32 };
 
 // Slot and role of waveguide w in layer l.  Returns whether an active crossing acts on it.
 // NN: gpu/gmem.cu:236-240 (slot m couples 2m + shift%2, +1; active for
@@ -91,7 +91,7 @@ __device__ void rows_at(const int X, const int N, const int NP, const int LPV, c
     const dc one = mk(1, 0), zero = mk(0, 0);
     if (!act) {
         if (tabs & (TAB_FWD | TAB_INV)) tab_a[o] = make_row<R>(one, zero);
-        if (tabs & TAB_DIFF) tab_b[o] = make_row<R>(zero, zero);
+        if (tabs & (TAB_DIFF | TAB_DIFF_ADJ)) tab_b[o] = make_row<R>(zero, zero);
         if (tabs & TAB_ADJ) tab_b[o] = make_row<R>(one, zero);
         return;
     }
@@ -99,7 +99,7 @@ __device__ void rows_at(const int X, const int N, const int NP, const int LPV, c
     const float* pp = p + (size_t)l * ldp + (size_t)m * np;
     const float* sp = (s && ns) ? s + (size_t)l * lds + (size_t)m * ns : nullptr;
     M2 T, dT[2];
-    crossing_eval(X, pp, sp, &T, (tabs & TAB_DIFF) ? dT : nullptr);
+    crossing_eval(X, pp, sp, &T, (tabs & (TAB_DIFF | TAB_DIFF_ADJ)) ? dT : nullptr);
     dc a, b;
     if (tabs & TAB_FWD) {
         row_of(T, upper, a, b);
@@ -120,6 +120,13 @@ __device__ void rows_at(const int X, const int N, const int NP, const int LPV, c
     }
     if (tabs & TAB_ADJ) {
         row_of(mherm(T), upper, a, b);
+        tab_b[o] = make_row<R>(a, b);
+    }
+    if (tabs & TAB_DIFF_ADJ) {  // rows of (sum_k dp_k dT/dp_k)^H: the tangent of T^H along dp
+        const float* dpp = dp + (size_t)l * ldp + (size_t)m * np;
+        M2 D = mscale(dT[0], (double)dpp[0]);
+        if (np > 1) D = madd(D, mscale(dT[1], (double)dpp[1]));
+        row_of(mherm(D), upper, a, b);
         tab_b[o] = make_row<R>(a, b);
     }
 }
@@ -186,6 +193,51 @@ __device__ void dtdp_at(const int X, const int N, const int NP, const int fft, c
         out[(size_t)(k * 4 + 1) * NPh] = dc_to<T>(dT[k].a01);
         out[(size_t)(k * 4 + 2) * NPh] = dc_to<T>(dT[k].a10);
         out[(size_t)(k * 4 + 3) * NPh] = dc_to<T>(dT[k].a11);
+    }
+}
+
+// Contracted second derivatives S_k = sum_m q_m d2T/dp_k dp_m of every slot, same layout as
+// dT/dp (zero for inactive slots).
+template <class T>
+__global__ void k_d2tdp(int X, int N, int L, int NP, int fft, const int* __restrict__ g0, const int* __restrict__ g1,
+                        const float* __restrict__ p, const float* __restrict__ q, int ldp, const float* __restrict__ s,
+                        int lds, T* __restrict__ d2) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = blockIdx.y;
+    const int NPh = NP / 2;
+    if (l >= L || m >= NPh) return;
+    const int np = n_phase_of(X), ns = n_split_of(X);
+    T* out = d2 + (size_t)l * NPh * np * 4 + m;
+    int i;
+    if (fft) {
+        const int st = g0[l];
+        bool ok = st >= 0 && st <= 20;
+        const int S = ok ? (1 << st) : 1;
+        i = 2 * S * (m / S) + (m % S);
+        ok = ok && (i + S < N);
+        if (!ok) i = -1;
+    } else {
+        i = 2 * m + (g1[l] & 1);
+    }
+    int mm = -1;
+    bool upper = false;
+    const bool act = (i >= 0) && wg_role(fft, g0, g1, l, i, N, mm, upper) && upper && mm == m;
+    if (!act) {
+        for (int k = 0; k < np * 4; ++k) out[(size_t)k * NPh] = dc_to<T>(mk(0, 0));
+        return;
+    }
+    const float* pp = p + (size_t)l * ldp + (size_t)m * np;
+    const float* qp = q + (size_t)l * ldp + (size_t)m * np;
+    const float* sp = (s && ns) ? s + (size_t)l * lds + (size_t)m * ns : nullptr;
+    M2 Tm, dT[2], S2[2];
+    crossing_eval(X, pp, sp, &Tm, dT);
+    const double qq[2] = {qp[0], np > 1 ? (double)qp[1] : 0.0};
+    crossing_d2q(X, Tm, dT, qq, S2);
+    for (int k = 0; k < np; ++k) {
+        out[(size_t)(k * 4 + 0) * NPh] = dc_to<T>(S2[k].a00);
+        out[(size_t)(k * 4 + 1) * NPh] = dc_to<T>(S2[k].a01);
+        out[(size_t)(k * 4 + 2) * NPh] = dc_to<T>(S2[k].a10);
+        out[(size_t)(k * 4 + 3) * NPh] = dc_to<T>(S2[k].a11);
     }
 }
 

@@ -330,19 +330,64 @@ class MeshNetworkGPU:
         J = torch.sum(torch.abs(uout - yb) ** 2)
         return J, out_dp, uout.T
 
-    # ---- Hessians by finite differences (gpu/mesh.py:236-288) -------------------------
+    # ---- Hessian-vector products (gpu/mesh.py:236-288) --------------------------------
+    def exact_hvp_supported(self) -> bool:
+        """Forward-over-reverse HVP kernel: unitary complex crossings, N <= 512, no phase screen."""
+        return self.X in ("mzi", "sym") and self.N <= 512 and not self.is_phase
+
+    def _hvp_exact(self, q, out, dout, g, dg, p, s):
+        B = out.shape[0]
+        hv = torch.zeros(self.n_phase, dtype=torch.float32, device=self.device)
+        nb = _lib.hvp_workspace_size(self.X, self.N, self.L, B, self.fft)
+        ws = torch.empty(max(nb, 1), dtype=torch.uint8, device=self.device)
+        ldp, lds = self._ld()
+        g0, g1 = self._geom()
+        _lib.hvp(self.X, self.N, self.L, B, self.fft, g0, g1, p, q, ldp, s if s.numel() else None, lds, out, dout,
+                 g, dg, None, None, hv, self.N, ws, nb)
+        return hv
+
     def hvp(self, vec, v, w: Callable, p_phase=None, p_splitter=None, eps=1e-3):
+        """H vec for the objective whose output gradient is w(v_out).
+
+        Where exact_hvp_supported(): forward-over-reverse on the GPU (fwddiff along vec, then the
+        HVP adjoint kernel); the tangent of w along the output tangent is torch.func.jvp of w, or
+        a unit-step central difference when w cannot be traced (exact for the affine gradients of
+        quadratic losses).  Otherwise central differences of grad_phi with step eps, as the
+        reference does (gpu/mesh.py:236-251)."""
         p = self.p_phase if p_phase is None else as_params(p_phase, self.n_phase, self.device)
         vec = as_params(vec, self.n_phase, self.device)
-        return (self.grad_phi(v, w, p + vec * eps, p_splitter)[0]
-                - self.grad_phi(v, w, p - vec * eps, p_splitter)[0]) / (2 * eps)
+        if not self.exact_hvp_supported():
+            return (self.grad_phi(v, w, p + vec * eps, p_splitter)[0]
+                    - self.grad_phi(v, w, p - vec * eps, p_splitter)[0]) / (2 * eps)
+        _, s = self._defaults(p, p_splitter)
+        out, dout = self.dot(v, p, p_splitter, dp=vec)          # (N, B) views
+        try:
+            g, dg = torch.func.jvp(w, (out,), (dout,))
+        except Exception:
+            g = w(out)
+            dg = (w(out + dout) - w(out - dout)) / 2
+        gb = as_field(g, self.N, self.dtype, self.device)
+        dgb = as_field(dg, self.N, self.dtype, self.device)
+        return self._hvp_exact(vec, out.T, dout.T, gb, dgb, p, s)
 
     def hvp_target(self, vec, U_target, p_phase=None, p_splitter=None, eps=1e-3):
+        """H vec of L2norm(U_target) (gpu/mesh.py:253-266); exact where supported: the output
+        gradient 2 offdiag(U - 1) is affine, its tangent is 2 offdiag(dU)."""
         J = [0]
         w = self._L2norm_fn(J)
         Ut = torch.as_tensor(np.asarray(U_target) if not isinstance(U_target, torch.Tensor) else U_target)
         v = Ut.to(self.dtype).to(self.device).T.conj()
-        return self.hvp(vec, v, w, p_phase, p_splitter, eps)
+        if not self.exact_hvp_supported():
+            return self.hvp(vec, v, w, p_phase, p_splitter, eps)
+        p = self.p_phase if p_phase is None else as_params(p_phase, self.n_phase, self.device)
+        q = as_params(vec, self.n_phase, self.device)
+        _, s = self._defaults(p, p_splitter)
+        out, dout = self.dot(v, p, p_splitter, dp=q)
+        g = w(out)
+        dg = 2 * dout
+        dg = dg - torch.diag_embed(torch.diagonal(dg))
+        return self._hvp_exact(q, out.T, dout.T, as_field(g, self.N, self.dtype, self.device),
+                               as_field(dg, self.N, self.dtype, self.device), p, s)
 
     def hess(self, v, w: Callable, p_phase=None, p_splitter=None, eps=1e-3):
         # (the reference passes these arguments to hvp_target by mistake, gpu/mesh.py:276)

@@ -183,6 +183,90 @@ def crossing_dT(X, p, s=None):
     raise ValueError(X)
 
 
+def crossing_d2T(X, p, s=None):
+    """Second derivatives d2T/dp_k dp_m, shape (..., n_p, n_p, 2, 2), of the closed forms.
+
+    Derived from crossing_T / crossing_dT (the derivatives only exist here and on the GPU;
+    the reference differentiates once).  mzi and mzi_generic: every theta-dependent factor is
+    a phase or linear in (cos th/2, sin th/2) times a phase, which gives d2T/dth2 = i dT/dth,
+    d2T/dth dph = dT/dth diag(i, 0), d2T/dph2 = T diag(-1, 0).  sym: entries are linear in
+    (cos th/2, sin th/2): d2T/dth2 = -T/4; phi enters as exp(+-i ph) on the diagonal.
+    orth: -T/4.  Pinned by central finite differences of crossing_dT."""
+    p, s = _split(p, s, X)
+    T = crossing_T(X, p, s)
+    dT = crossing_dT(X, p, s)
+    n_p = N_PHASE[X]
+    out = np.zeros(p.shape[:-1] + (n_p, n_p, 2, 2), dtype=np.complex128)
+    if X in ("mzi", "mzi_generic"):
+        out[..., 0, 0, :, :] = 1j * dT[..., 0, :, :]
+        out[..., 0, 1, :, 0] = 1j * dT[..., 0, :, 0]
+        out[..., 1, 0, :, :] = out[..., 0, 1, :, :]
+        out[..., 1, 1, :, 0] = -T[..., :, 0]
+    elif X == "sym":
+        out[..., 0, 0, :, :] = -0.25 * T
+        out[..., 0, 1, 0, 0] = 1j * dT[..., 0, 0, 0]
+        out[..., 0, 1, 1, 1] = -1j * dT[..., 0, 1, 1]
+        out[..., 1, 0, :, :] = out[..., 0, 1, :, :]
+        out[..., 1, 1, 0, 0] = -T[..., 0, 0]
+        out[..., 1, 1, 1, 1] = -T[..., 1, 1]
+    elif X == "orth":
+        out[..., 0, 0, :, :] = -0.25 * T
+    return out
+
+
+def hvp_inverse(geom, X, p, s, u_in, q, target):
+    """Exact Hessian-vector product of J = sum |U(p) u_in - target|^2 along q by
+    forward-over-reverse (unitary crossings): the forward JVP gives (u_out, du_out); the
+    reverse walk carries (u, du, g, dg) with x = T^H u, dx = T^H du + dT_q^H u,
+    g <- T^H g, dg <- T^H dg + dT_q^H g (dT_q = sum_k q_k dT/dp_k), accumulates
+    dJdT = sum conj(g) x and its tangent sum conj(dg) x + conj(g) dx, and returns
+    (grad, hvp) with hvp_k = Re sum_ij [d(dJdT)_ij dT_ij/dp_k + dJdT_ij sum_m q_m d2T_ij/dp_k dp_m].
+    Inactive slots give zero.  Pinned by central differences of the FP64 gradient along q."""
+    T = slot_T(geom, X, p, s)
+    n_p = N_PHASE[X]
+    pp = np.asarray(p, dtype=np.float64).reshape(geom.L, geom.N // 2, n_p)
+    ss = None if (s is None or np.size(s) == 0) else np.asarray(s, np.float64).reshape(geom.L, geom.N // 2, N_SPLIT[X])
+    qq = np.asarray(q, dtype=np.float64).reshape(pp.shape)
+    dTk = crossing_dT(X, pp, ss)                                  # [L, N/2, n_p, 2, 2]
+    dTq = np.einsum("lmkij,lmk->lmij", dTk, qq)
+    u, du = jvp(geom, T, dTq, u_in)
+    g = 2 * (u - np.asarray(target, np.complex128))
+    dg = 2 * du
+    dJdT = np.zeros(T.shape, dtype=np.complex128)
+    ddJdT = np.zeros(T.shape, dtype=np.complex128)
+    for l in range(geom.L - 1, -1, -1):
+        m, i, j = geom.pairs(l)
+        th = np.conj(np.swapaxes(T[l, m], -1, -2))
+        dth = np.conj(np.swapaxes(dTq[l, m], -1, -2))
+
+        def ap(M, a, b):
+            return M[:, 0, 0] * a + M[:, 0, 1] * b, M[:, 1, 0] * a + M[:, 1, 1] * b
+        ui, uj, dui, duj = u[:, i].copy(), u[:, j].copy(), du[:, i].copy(), du[:, j].copy()
+        gi, gj, dgi, dgj = g[:, i].copy(), g[:, j].copy(), dg[:, i].copy(), dg[:, j].copy()
+        xi, xj = ap(th, ui, uj)
+        a1, a2 = ap(th, dui, duj)
+        b1, b2 = ap(dth, ui, uj)
+        dxi, dxj = a1 + b1, a2 + b2
+        for (r, gr, dgr) in ((0, gi, dgi), (1, gj, dgj)):
+            for (c, xc, dxc) in ((0, xi, dxi), (1, xj, dxj)):
+                dJdT[l, m, r, c] = np.sum(np.conj(gr) * xc, axis=0)
+                ddJdT[l, m, r, c] = np.sum(np.conj(dgr) * xc + np.conj(gr) * dxc, axis=0)
+        u[:, i], u[:, j] = xi, xj
+        du[:, i], du[:, j] = dxi, dxj
+        g[:, i], g[:, j] = ap(th, gi, gj)
+        c1, c2 = ap(th, dgi, dgj)
+        e1, e2 = ap(dth, gi, gj)
+        dg[:, i], dg[:, j] = c1 + e1, c2 + e2
+    d2 = crossing_d2T(X, pp, ss)                                  # [L, N/2, n_p, n_p, 2, 2]
+    Sq = np.einsum("lmkaij,lma->lmkij", d2, qq)
+    grad = np.real(np.einsum("lmkij,lmij->lmk", dTk, dJdT))
+    hv = np.real(np.einsum("lmkij,lmij->lmk", dTk, ddJdT) + np.einsum("lmkij,lmij->lmk", Sq, dJdT))
+    act = active_mask(geom)
+    grad[~act] = 0.0
+    hv[~act] = 0.0
+    return grad, hv
+
+
 def grad_from_dJdT(X, p, s, dJdT):
     """dJ/dp_k = Re sum_ij dJdT_ij dT_ij/dp_k (crossing.py:146-163, gpu/crossing.cu:150-203)."""
     dT = crossing_dT(X, p, s)
@@ -590,4 +674,6 @@ PINS = {
                       "either position + dense brute force (structured_dense) + FD of the screen phases",
     "structured_dense": "agrees with structured_run mode 0 and with the reference engine's golden vectors",
     "butterfly_perms": "permutation pairs checked against the stride-pairing definition (adjacent after gather)",
+    "crossing_d2T": "central finite differences of crossing_dT (all four types)",
+    "hvp_inverse": "central finite differences of the FP64 gradient (mesh_vjp) along q; its gradient equals mesh_vjp",
 }

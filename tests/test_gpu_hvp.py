@@ -1,0 +1,87 @@
+"""Exact Hessian-vector products (forward-over-reverse kernel) against the FP64 oracle.
+
+The oracle's hvp_inverse is pinned by central differences of its gradient; here the GPU result
+(meshprop_hvp after fwddiff along q) must match it within 1e-5 relative L2, for MZI and SYM
+crossings on NN and FFT geometries and N up to 512, and MeshNetworkGPU.hvp_target / hess_target
+must agree with the oracle and be symmetric.
+"""
+import zlib
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import meshprop_oracle as O
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+CASES = [("mzi", "nn", 16, 9, 7), ("mzi", "clements", 64, 0, 20), ("sym", "nn", 40, 11, 13),
+         ("mzi", "fft", 128, 7, 10), ("mzi", "clements", 256, 0, 6), ("mzi", "nn", 512, 12, 5),
+         ("sym", "fft", 256, 8, 9)]
+
+
+def _geom(rng, kind, N, L):
+    if kind == "clements":
+        return O.clements(N)
+    if kind == "fft":
+        return O.Geometry(N, L, strides=rng.randint(0, int(np.log2(N)), size=L))
+    return O.random_geometry_nn(rng, N, L)
+
+
+@pytest.mark.parametrize("case", CASES, ids=["-".join(map(str, c)) for c in CASES])
+def test_hvp_kernel_matches_oracle(gpu, case):
+    import meshes_amd._lib as Lb
+    from meshes_amd import MeshNetworkGPU
+    X, kind, N, L, B = case
+    rng = np.random.RandomState(zlib.crc32(str(case).encode()) % 10000)
+    g = _geom(rng, kind, N, L)
+    n_p, n_s = O.N_PHASE[X], O.N_SPLIT[X]
+    p = rng.uniform(0, 2 * np.pi, (g.L, N // 2, n_p)).astype(np.float32)
+    s = (0.05 * rng.standard_normal((g.L, N // 2, n_s))).astype(np.float32)
+    q = rng.standard_normal(p.shape).astype(np.float32)
+    u = O.unit_cn(rng, B, N).astype(np.complex64)
+    t = O.unit_cn(rng, B, N).astype(np.complex64)
+    kw = dict(strides=g.strides) if g.kind == "fft" else {}
+    mesh = MeshNetworkGPU(N, g.L, g.lens, g.shifts, p, s, X, is_phase=False, **kw)
+    out, dout = mesh.dot(torch.as_tensor(u.T), dp=q)
+    qt = torch.as_tensor(q.reshape(-1)).cuda()
+    grad = torch.zeros(mesh.n_phase, dtype=torch.float32, device="cuda")
+    hv = torch.zeros_like(grad)
+    nb = Lb.hvp_workspace_size(X, N, g.L, B, mesh.fft)
+    ws = torch.empty(nb, dtype=torch.uint8, device="cuda")
+    g0, g1 = mesh._geom()
+    Lb.hvp(X, N, g.L, B, mesh.fft, g0, g1, mesh.p_phase, qt, N // 2 * n_p, mesh.p_splitter, N // 2 * n_s, out.T,
+           dout.T, None, None, torch.as_tensor(t).cuda(), grad, hv, N, ws, nb)
+    rg, rh = O.hvp_inverse(g, X, p.astype(np.float64), s.astype(np.float64), u.astype(np.complex128),
+                           q.astype(np.float64), t.astype(np.complex128))
+    assert O.rel_l2(grad.cpu().numpy().reshape(rg.shape), rg) < TOL
+    assert O.rel_l2(hv.cpu().numpy().reshape(rh.shape), rh) < TOL
+
+
+def test_hvp_target_and_hessian(gpu):
+    """MeshNetworkGPU.hvp_target (exact path) against the oracle's forward-over-reverse of the
+    L2norm objective, and hess_target symmetric."""
+    from meshes_amd import MeshNetworkGPU
+    rng = np.random.RandomState(5)
+    N = 8
+    g = O.clements(N)
+    p = rng.uniform(0, 2 * np.pi, (N, N // 2, 2)).astype(np.float32)
+    mesh = MeshNetworkGPU(N, N, g.lens, g.shifts, p, np.zeros(0), "mzi", is_phase=False)
+    assert mesh.exact_hvp_supported()
+    U = np.linalg.qr(rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N)))[0].astype(np.complex64)
+    q = rng.standard_normal(mesh.n_phase).astype(np.float32)
+    hv = mesh.hvp_target(q, U).cpu().numpy()
+    # FP64 reference: J(p) = |offdiag(U_mesh U^H)|^2, H q by central differences of its gradient
+    v = U.astype(complex).conj()          # rows = columns of U^H
+
+    def grad(pp):
+        out = O.mesh_fwd(g, "mzi", pp, None, v)
+        V = out - np.eye(N)
+        V = V - np.diag(np.diag(V))
+        return O.mesh_vjp(g, "mzi", pp, None, out, 2 * V)[0].reshape(-1)
+    h = 1e-5
+    p64, q64 = p.astype(np.float64), q.astype(np.float64).reshape(p.shape)
+    fd = (grad(p64 + h * q64) - grad(p64 - h * q64)) / (2 * h)
+    assert O.rel_l2(hv, fd) < 1e-4
+    H = mesh.hess_target(U).cpu().numpy()
+    assert O.rel_l2(H, H.T) < 1e-4

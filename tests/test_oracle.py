@@ -289,3 +289,43 @@ def test_butterfly_perms_pair_stride_neighbours():
         s = 1 if i % 2 == 0 else ((i + 1) & -(i + 1))
         assert np.all(b - a == s) and np.all((a & s) == 0)
     assert np.array_equal(pos[perms[N - 1]], np.arange(N))
+
+
+@pytest.mark.parametrize("X", ["mzi", "sym", "orth", "mzi_generic"])
+def test_d2T_finite_difference(X):
+    rng = np.random.RandomState(41)
+    n_p = O.N_PHASE[X]
+    p = rng.uniform(0, 2 * np.pi, (5, n_p))
+    s = 0.1 * rng.standard_normal((5, O.N_SPLIT[X])) if O.N_SPLIT[X] else None
+    d2 = O.crossing_d2T(X, p, s)
+    h = 1e-6
+    for m in range(n_p):
+        e = np.zeros(n_p)
+        e[m] = h
+        fd = (O.crossing_dT(X, p + e, s) - O.crossing_dT(X, p - e, s)) / (2 * h)   # [5, k, 2, 2]
+        assert np.allclose(d2[:, :, m], fd, atol=1e-8), (X, m)
+    assert np.allclose(d2, np.swapaxes(d2, 1, 2))
+
+
+@pytest.mark.parametrize("X,kind", [("mzi", "nn"), ("sym", "nn"), ("mzi", "fft")])
+def test_hvp_finite_difference_of_gradient(X, kind):
+    """hvp_inverse against central differences (h = 1e-5) of the FP64 gradient along q."""
+    rng = np.random.RandomState(43)
+    N, L, B = 12, 9, 5
+    g = O.random_geometry_nn(rng, N, L) if kind == "nn" else O.Geometry(16, 6, strides=rng.randint(0, 4, size=6))
+    N = g.N
+    n_p = O.N_PHASE[X]
+    p = rng.uniform(0, 2 * np.pi, (g.L, N // 2, n_p))
+    s = 0.1 * rng.standard_normal((g.L, N // 2, O.N_SPLIT[X]))
+    u = O.unit_cn(rng, B, N)
+    t = O.unit_cn(rng, B, N)
+    q = rng.standard_normal(p.shape)
+
+    def grad(pp):
+        out = O.mesh_fwd(g, X, pp, s, u)
+        return O.mesh_vjp(g, X, pp, s, out, 2 * (out - t))[0]
+    gr, hv = O.hvp_inverse(g, X, p, s, u, q, t)
+    assert O.rel_l2(gr, grad(p)) < 1e-12
+    h = 1e-5
+    fd = (grad(p + h * q) - grad(p - h * q)) / (2 * h)
+    assert O.rel_l2(hv, fd) < 1e-7
