@@ -4,13 +4,16 @@
 Workload: Clements MZI mesh, N=256 modes, L=256 layers, ideal 50:50 splitters, phases
 U[0, 2pi) (float32), B=4096 complex64 unit-norm CN(0,1) input vectors.  One step is one
 fwd+grad pass of the loss J = sum |U x - y|^2 with y = U_target x (U_target a second
-random-phase mesh, precomputed): fwdprop, then backdiff with dJ/du_out = 2 (u_out - y)
-fused into its load, accumulating dJ/dp.  Inputs are resident in HBM before timing.
+random-phase mesh, precomputed), as one library call (MESHPROP_OP_FWDGRAD): one prologue
+for the forward rows, the inverse rows and dT/dp; fwdprop; the adjoint with dJ/du_out =
+2 (u_out - y) formed in registers; the fixed-order dp reduction writing dp.  --path separate
+runs fwdprop and backdiff as two calls (two prologues, dp zeroed and accumulated) instead.
+Inputs are resident in HBM before timing.
 
 Prints one JSON line.  value = MZI-vector updates/s = B * sum(lens) / step time (an update
 is one active crossing applied to one vector through the forward and the adjoint pass,
-128 FLOP).  The roofline object is for the dominant kernel (backdiff propagation, 96 FLOP
-per update), timed with HIP events on the stream it runs on.  cpu_baseline times the
+128 FLOP).  The roofline object is for the dominant kernel (the backdiff propagation,
+96 FLOP per update), timed with HIP events on the stream it runs on.  cpu_baseline times the
 compiled reference CPU engine (c/dot.cpp, built into oracle/_ref by oracle/Makefile) on a
 bounded slice of the same workload.
 """
@@ -86,9 +89,9 @@ def cpu_baseline(p, u, y, seconds_target=15.0):
             if kind == "reference" else f"NumPy FP64 oracle fwd + inverse-propagation VJP, {bs} vectors, {t:.2f} s"}
 
 
-def load_traffic():
+def load_traffic(name):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3 --pmc summary."""
-    path = os.path.join(ROOT, "profiles", "k_bwd_c2_traffic.json")
+    path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as f:
             return json.load(f).get("hbm_bytes_per_launch")
@@ -261,6 +264,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--tile-fwd", type=str, default="")
     ap.add_argument("--tile-bwd", type=str, default="")
+    ap.add_argument("--path", choices=["fwdgrad", "separate"], default="fwdgrad",
+                    help="fwd+grad as one library call (MESHPROP_OP_FWDGRAD), or fwdprop then backdiff calls")
     ap.add_argument("--workload", choices=["fwdgrad", "batched", "serving", "training"], default="fwdgrad",
                     help="fwdgrad: the single-mesh step (default); batched: the heterogeneous grouped mix; "
                          "serving: the continuous-batching request loop; training: Adam through the autograd "
@@ -316,12 +321,17 @@ def main():
     uout = torch.empty_like(xb)
     uin = torch.empty_like(xb)
     dp = torch.zeros(mesh.n_phase, dtype=torch.float32, device=dev)
-    ws_f_n = Lb.workspace_size(Lb.OP_FWDPROP, "mzi", N, L, B, False, tile_f)
-    ws_b_n = Lb.workspace_size(Lb.OP_BACKDIFF, "mzi", N, L, B, False, tile_b)
-    ws_f = torch.empty(ws_f_n, dtype=torch.uint8, device=dev)
-    ws_b = torch.empty(ws_b_n, dtype=torch.uint8, device=dev)
     geo = (mesh.lens, mesh.shifts)
     ldp = N
+    onecall = args.path == "fwdgrad"
+    if onecall:
+        ws_g_n = Lb.workspace_size(Lb.OP_FWDGRAD, "mzi", N, L, B, False, tile_b)
+        ws_g = torch.empty(ws_g_n, dtype=torch.uint8, device=dev)
+    else:
+        ws_f_n = Lb.workspace_size(Lb.OP_FWDPROP, "mzi", N, L, B, False, tile_f)
+        ws_b_n = Lb.workspace_size(Lb.OP_BACKDIFF, "mzi", N, L, B, False, tile_b)
+        ws_f = torch.empty(ws_f_n, dtype=torch.uint8, device=dev)
+        ws_b = torch.empty(ws_b_n, dtype=torch.uint8, device=dev)
 
     def fwd():
         Lb.run(Lb.OP_FWDPROP, "mzi", N, L, B, False, *geo, mesh.p_phase, None, ldp, None, 0, xb, None, uout, None,
@@ -331,9 +341,30 @@ def main():
         Lb.run(Lb.OP_BACKDIFF, "mzi", N, L, B, False, *geo, mesh.p_phase, dp, ldp, None, 0, uout, None, uin, None,
                N, target=yb, tile=tile_b, workspace=ws_b, ws_bytes=ws_b_n, flags=flags)
 
+    def fwdgrad(flags):
+        Lb.run(Lb.OP_FWDGRAD, "mzi", N, L, B, False, *geo, mesh.p_phase, dp, ldp, None, 0, xb, None, uout, None,
+               N, target=yb, tile=tile_b, workspace=ws_g, ws_bytes=ws_g_n, flags=flags)
+
     ev = [[torch.cuda.Event(enable_timing=True) for _ in range(6)] for _ in range(args.steps)]
 
     def step(e=None):
+        # e[2]..e[3] brackets the dominant kernel (the backdiff propagation).
+        if onecall:
+            # The stages of one fwd+grad call, issued one by one so that each can be timed.
+            if e:
+                e[0].record()
+            fwdgrad(Lb.ONLY_PROLOGUE)
+            fwdgrad(Lb.SKIP_PROLOGUE | Lb.ONLY_FORWARD)
+            if e:
+                e[1].record()
+                e[2].record()
+            fwdgrad(Lb.SKIP_PROLOGUE | Lb.SKIP_FORWARD | Lb.SKIP_REDUCE)
+            if e:
+                e[3].record()
+            fwdgrad(Lb.ONLY_REDUCE | Lb.DP_ASSIGN)  # dp = gradient: no separate zeroing of dp
+            if e:
+                e[4].record()
+            return
         if e:
             e[0].record()
         dp.zero_()
@@ -364,12 +395,18 @@ def main():
     ms_step = wall * 1e3 / args.steps
     step_ms = np.array([ev[k][0].elapsed_time(ev[k][4]) for k in range(args.steps)])
     kb_ms = np.array([ev[k][2].elapsed_time(ev[k][3]) for k in range(args.steps)])
-    fwd_ms = np.array([ev[k][0].elapsed_time(ev[k][1]) for k in range(args.steps)])
+    pre_ms = np.array([ev[k][0].elapsed_time(ev[k][1]) for k in range(args.steps)])
     kb_avg = float(np.mean(kb_ms))
-    flop_bwd = 96.0 * updates
-    achieved = flop_bwd / (kb_avg * 1e-3) / 1e12
+    flop_k = 96.0 * updates
+    kname = "k_bwd<complex64> (backdiff propagation)"
+    traffic = load_traffic("k_bwd_c2_traffic.json")
+    if onecall:
+        kernels_ms = {"prologue_and_fwdprop": float(np.mean(pre_ms)), "backdiff_propagation": kb_avg}
+    else:
+        kernels_ms = {"fwdprop_total": float(np.mean(pre_ms)), "backdiff_propagation": kb_avg}
+    achieved = flop_k / (kb_avg * 1e-3) / 1e12
     # correctness of the measured configuration: reconstructed input vs x, gradient finite
-    rec_err = float(torch.linalg.norm(uin - xb) / torch.linalg.norm(xb))
+    rec_err = None if onecall else float(torch.linalg.norm(uin - xb) / torch.linalg.norm(xb))
     out = {
         "metric": METRIC,
         "value": updates / (ms_step * 1e-3),
@@ -386,17 +423,17 @@ def main():
         "config": {"workload": "Clements MZI mesh fwd+grad (fwdprop + backdiff with fused L2 loss gradient), "
                                "N=256 L=256 B=4096, ideal splitters, complex64",
                    "N": N, "L": L, "B": B, "active_crossings": nact, "updates_per_step": updates,
-                   "flop_per_update": 128, "tile_fwd": tile_f, "tile_bwd": tile_b},
+                   "flop_per_update": 128, "path": args.path, "tile_fwd": tile_f, "tile_bwd": tile_b},
         "latency_ms": {"p50": float(np.percentile(step_ms, 50)), "p99": float(np.percentile(step_ms, 99)),
                        "what": "per-step fwd+grad latency (HIP events)"},
-        "kernels_ms": {"fwdprop_total": float(np.mean(fwd_ms)), "backdiff_propagation": kb_avg},
+        "kernels_ms": kernels_ms,
         "gflops_per_s_fwd_plus_grad": 128.0 * updates / (ms_step * 1e-3) / 1e9,
         "reconstruction_rel_err": rec_err,
         "roofline": {"bound": "mfma", "compute": "fp32 VALU (2x2 complex rotations: no MFMA shape)",
-                     "kernel": "k_bwd<complex64> (backdiff propagation)",
+                     "kernel": kname,
                      "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": load_traffic(),
-                     "flop_per_launch": flop_bwd, "avg_launch_ms": kb_avg},
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
+                     "flop_per_launch": flop_k, "avg_launch_ms": kb_avg},
     }
     if not args.no_cpu:
         try:

@@ -59,7 +59,7 @@ enum meshprop_op {
     MESHPROP_OP_FWDPROP = 0,
     MESHPROP_OP_FWDDIFF = 1,
     MESHPROP_OP_BACKDIFF = 2,
-    MESHPROP_OP_FWDGRAD = 3    /* group entries only: fwdprop, then backdiff of its own output */
+    MESHPROP_OP_FWDGRAD = 3    /* fwdprop, then backdiff of its own output (meshprop_run, group entries) */
 };
 
 /* Optional tile choice.  Zero fields mean "library default".  w: waveguides per lane
@@ -123,6 +123,12 @@ int meshprop_backdiff_ft(int X, int N, int L, int B, const int *strides,
  *     instead of trusting the reconstruction (checkpointed adjoint for lossy meshes).
  * fft selects the strides geometry (geom0 = strides, geom1 unused) or NN (geom0 = lens,
  * geom1 = shifts).  dp/du_in/du_out are used by fwddiff; dp/dJdu_out/dJdu_in by backdiff.
+ * op = MESHPROP_OP_FWDGRAD (unitary crossings; target required, ckpt unsupported) is the
+ * whole gradient step of J = sum |u_out - target|^2 (gpu/mesh.py:208-234 with the loss of
+ * gpu/test.py:131) in one call: one prologue builds the forward rows, the inverse rows and
+ * dT/dp together; fwdprop takes u_a = u_in to u_b = u_out; the adjoint of that output
+ * accumulates dp (or writes it, MESHPROP_DP_ASSIGN) and, if du_b is set, writes dJdu_in
+ * there.  The tile is the adjoint's; workspace from meshprop_workspace_size(MESHPROP_OP_FWDGRAD, ...).
  */
 int meshprop_run(int op, int X, int N, int L, int B, int fft, const int *geom0, const int *geom1,
                  const float *p, float *dp, int ldp, const float *s, int lds,
@@ -136,12 +142,17 @@ int meshprop_run(int op, int X, int N, int L, int B, int fft, const int *geom0, 
  * MESHPROP_ONLY_PROLOGUE runs (1) only; MESHPROP_SKIP_PROLOGUE reuses tables a previous
  * call with the same op, crossing, sizes, geometry, parameters and tile left in this
  * workspace (parameters held resident between calls); MESHPROP_SKIP_REDUCE leaves the
- * partials in the workspace and MESHPROP_ONLY_REDUCE runs (3) alone. */
+ * partials in the workspace and MESHPROP_ONLY_REDUCE runs (3) alone.  For
+ * MESHPROP_OP_FWDGRAD stage (2) is the forward, then the adjoint: MESHPROP_ONLY_FORWARD and
+ * MESHPROP_SKIP_FORWARD split it (other ops reject both flags). */
 enum meshprop_flags {
     MESHPROP_ONLY_PROLOGUE = 1,
     MESHPROP_SKIP_PROLOGUE = 2,
     MESHPROP_SKIP_REDUCE = 4,
-    MESHPROP_ONLY_REDUCE = 8
+    MESHPROP_ONLY_REDUCE = 8,
+    MESHPROP_DP_ASSIGN = 16,   /* backdiff / fwd+grad: dp = gradient instead of dp += gradient */
+    MESHPROP_ONLY_FORWARD = 32, /* fwd+grad: run the forward propagation alone */
+    MESHPROP_SKIP_FORWARD = 64  /* fwd+grad: u_b already holds the forward output; run the rest */
 };
 
 /*

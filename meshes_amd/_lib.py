@@ -157,7 +157,8 @@ def resolve_tile(op, X, N, B, tile=None):
     return (t.w, t.v, t.waves, t.chunk)
 
 
-ONLY_PROLOGUE, SKIP_PROLOGUE, SKIP_REDUCE, ONLY_REDUCE = 1, 2, 4, 8
+ONLY_PROLOGUE, SKIP_PROLOGUE, SKIP_REDUCE, ONLY_REDUCE, DP_ASSIGN = 1, 2, 4, 8, 16
+ONLY_FORWARD, SKIP_FORWARD = 32, 64
 
 
 def run(op, X, N, L, B, fft, g0, g1, p, dp, ldp, s, lds, u_a, du_a, u_b, du_b, ldu, target=None, ckpt=None,
@@ -170,7 +171,7 @@ def run(op, X, N, L, B, fft, g0, g1, p, dp, ldp, s, lds, u_a, du_a, u_b, du_b, l
                           _ptr(p), _ptr(dp), ldp, _ptr(s), lds, _ptr(u_a), _ptr(du_a), _ptr(u_b), _ptr(du_b), ldu,
                           0, _ptr(target), _ptr(ckpt), int(ckpt_every), tile_struct(tile), int(flags), _ptr(workspace),
                           ctypes.c_size_t(ws_bytes), ctypes.c_void_p(stream))
-    check(rc, ["fwdprop", "fwddiff", "backdiff"][op])
+    check(rc, ["fwdprop", "fwddiff", "backdiff", "fwd+grad"][op])
 
 
 def phase_screen(op, N, B, phi, dphi, u, du, ldu, workspace=None, ws_bytes=0, stream=None):

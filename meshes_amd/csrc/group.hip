@@ -24,6 +24,7 @@ struct GTile {
 constexpr GTile kGroupTiles[] = {{2, 1, 0}, {2, 2, 0}, {2, 4, 0}, {4, 1, 0}, {4, 2, 0}, {8, 1, 0}, {16, 1, 1}};
 constexpr int kGroupVariants = sizeof(kGroupTiles) / sizeof(kGroupTiles[0]);
 constexpr int kFamWaves[2] = {4, 2};
+static_assert(kFamWaves[0] <= kStageWaves && kFamWaves[1] <= kStageWaves, "stage conversion sums at most kStageWaves waves");
 constexpr size_t kGroupStage = 64 * 1024;  // LDS of one workgroup (backdiff stage)
 constexpr size_t kGroupLds = kGroupStage + 16;  // stage + the queue slot
 
@@ -89,10 +90,13 @@ __global__ void __launch_bounds__(256) k_group_prologue(const unsigned char* __r
         const int* g1 = M.g1;
         const float* p = M.p;
         const float* sp = M.s;
-        if (M.build & B_FWD)
+        if ((M.build & B_FWD) && (M.build & B_INV))
+            rows_at<float4>(M.X, M.N, M.NP, M.LPV, M.fft, g0, g1, p, M.ldp, sp, M.lds, nullptr,
+                            TAB_FWD | TAB_INV_B, M.kind, M.rows_fwd, M.rows_inv, l, w);
+        else if (M.build & B_FWD)
             rows_at<float4>(M.X, M.N, M.NP, M.LPV, M.fft, g0, g1, p, M.ldp, sp, M.lds, nullptr, TAB_FWD,
                             M.kind, M.rows_fwd, nullptr, l, w);
-        if (M.build & B_INV)
+        else if (M.build & B_INV)
             rows_at<float4>(M.X, M.N, M.NP, M.LPV, M.fft, g0, g1, p, M.ldp, sp, M.lds, nullptr, TAB_INV,
                             M.kind, M.rows_inv, nullptr, l, w);
         if ((M.build & B_DTDP) && w < M.NP / 2)
@@ -136,13 +140,6 @@ __global__ void __launch_bounds__(256) k_group_reduce(const unsigned char* __res
 // One work item: the vector tiles slot, slot + S, ... of one mesh, each through the whole
 // depth: forward, then (gradient items) the adjoint with inverse reconstruction; the adjoint
 // of a tile reads back the output its own threads stored.
-template <class P>
-__device__ __forceinline__ P* uniform_ptr(P* p) {
-    const unsigned long long v = reinterpret_cast<unsigned long long>(p);
-    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
-    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
-    return reinterpret_cast<P*>(((unsigned long long)hi << 32) | lo);
-}
 // The forward tile runs as a called function: inlined beside the adjoint it raises the
 // kernel's register allocation above 256 (one wave per SIMD instead of two).  Its arguments
 // arrive in vector registers and are made scalar again with readfirstlane.

@@ -90,6 +90,7 @@ struct HvpStep {
 };
 
 constexpr int kHvpWaves = 4;
+static_assert(kHvpWaves <= kStageWaves, "stage conversion sums at most kStageWaves waves");
 constexpr size_t kHvpStage = 64 * 1024;
 
 template <int W>
@@ -306,12 +307,11 @@ int meshprop_hvp(int X, int N, int L, int B, int fft, const int* geom0, const in
     unsigned char* base = (unsigned char*)workspace;
     const float* sp = ns ? s : nullptr;
     hipLaunchKernelGGL((k_rows<float4>), dim3((P.NP + 255) / 256, L), dim3(256), 0, st, X, N, L, P.NP, P.LPV, fft,
-                       geom0, geom1, p, ldp, sp, lds, q, (int)(TAB_INV | TAB_DIFF_ADJ), (int*)(base + P.off_kind),
-                       (float4*)(base + P.off_r), (float4*)(base + P.off_d));
+                       geom0, geom1, p, ldp, sp, lds, q, (int)(TAB_INV | TAB_DIFF_ADJ | TAB_DTDP),
+                       (int*)(base + P.off_kind), (float4*)(base + P.off_r), (float4*)(base + P.off_d),
+                       (float2*)(base + P.off_t1));
     if (hipGetLastError() != hipSuccess) return MESHPROP_ERR_HIP;
     const dim3 g2((P.NP / 2 + 255) / 256, L);
-    hipLaunchKernelGGL((k_dtdp<float2>), g2, dim3(256), 0, st, X, N, L, P.NP, fft, geom0, geom1, p, ldp, sp, lds,
-                       (float2*)(base + P.off_t1));
     hipLaunchKernelGGL((k_d2tdp<float2>), g2, dim3(256), 0, st, X, N, L, P.NP, fft, geom0, geom1, p, q, ldp, sp, lds,
                        (float2*)(base + P.off_t2));
     if (hipGetLastError() != hipSuccess) return MESHPROP_ERR_HIP;

@@ -13,7 +13,7 @@ namespace mp {
 // =======================================================================================
 // Prologue: row tables
 // =======================================================================================
-enum { TAB_FWD = 1, TAB_DIFF = 2, TAB_INV = 4, TAB_ADJ = 8, TAB_DTDP = 16, TAB_DIFF_ADJ = 32 };
+enum { TAB_FWD = 1, TAB_DIFF = 2, TAB_INV = 4, TAB_ADJ = 8, TAB_DTDP = 16, TAB_DIFF_ADJ = 32, TAB_INV_B = 64 };
 
 // Slot and role of waveguide w in layer l.  Returns whether an active crossing acts on it.
 // NN: gpu/gmem.cu:236-240 (slot m couples 2m + shift%2, +1; active for
@@ -92,7 +92,7 @@ __device__ void rows_at(const int X, const int N, const int NP, const int LPV, c
     if (!act) {
         if (tabs & (TAB_FWD | TAB_INV)) tab_a[o] = make_row<R>(one, zero);
         if (tabs & (TAB_DIFF | TAB_DIFF_ADJ)) tab_b[o] = make_row<R>(zero, zero);
-        if (tabs & TAB_ADJ) tab_b[o] = make_row<R>(one, zero);
+        if (tabs & (TAB_ADJ | TAB_INV_B)) tab_b[o] = make_row<R>(one, zero);
         return;
     }
     const int np = n_phase_of(X), ns = n_split_of(X);
@@ -122,6 +122,11 @@ __device__ void rows_at(const int X, const int N, const int NP, const int LPV, c
         row_of(mherm(T), upper, a, b);
         tab_b[o] = make_row<R>(a, b);
     }
+    if (tabs & TAB_INV_B) {  // the inverse rows into the second table (one evaluation for both)
+        const M2 Ti = (X == X_GENERIC) ? minv(T) : mherm(T);
+        row_of(Ti, upper, a, b);
+        tab_b[o] = make_row<R>(a, b);
+    }
     if (tabs & TAB_DIFF_ADJ) {  // rows of (sum_k dp_k dT/dp_k)^H: the tangent of T^H along dp
         const float* dpp = dp + (size_t)l * ldp + (size_t)m * np;
         M2 D = mscale(dT[0], (double)dpp[0]);
@@ -129,18 +134,6 @@ __device__ void rows_at(const int X, const int N, const int NP, const int LPV, c
         row_of(mherm(D), upper, a, b);
         tab_b[o] = make_row<R>(a, b);
     }
-}
-
-template <class R>
-__global__ void k_rows(int X, int N, int L, int NP, int LPV, int fft, const int* __restrict__ g0,
-                       const int* __restrict__ g1,
-                       const float* __restrict__ p, int ldp, const float* __restrict__ s, int lds,
-                       const float* __restrict__ dp, int tabs, int* __restrict__ kind, R* __restrict__ tab_a,
-                       R* __restrict__ tab_b) {
-    const int w = blockIdx.x * blockDim.x + threadIdx.x;
-    const int l = blockIdx.y;
-    if (l >= L || w >= NP) return;
-    rows_at<R>(X, N, NP, LPV, fft, g0, g1, p, ldp, s, lds, dp, tabs, kind, tab_a, tab_b, l, w);
 }
 
 // dT/dp_k of every slot (zero for inactive slots), for the in-kernel dJdT -> dp step.
@@ -251,9 +244,42 @@ __global__ void k_dtdp(int X, int N, int L, int NP, int fft, const int* __restri
     dtdp_at<T>(X, N, NP, fft, g0, g1, p, ldp, s, lds, dtdp, l, m);
 }
 
+// Field element type of a row type (float4 rows: complex64 fields; float2 rows: real fields).
+template <class R>
+struct FieldOfRow;
+template <>
+struct FieldOfRow<float4> {
+    typedef float2 type;
+};
+template <>
+struct FieldOfRow<float2> {
+    typedef float type;
+};
+
+// Prologue of one call: row tables (tabs) and, with TAB_DTDP, the dT/dp table of every slot,
+// all in one launch (thread w of layer l also handles slot w < NP/2).
+template <class R>
+__global__ void k_rows(int X, int N, int L, int NP, int LPV, int fft, const int* __restrict__ g0,
+                       const int* __restrict__ g1,
+                       const float* __restrict__ p, int ldp, const float* __restrict__ s, int lds,
+                       const float* __restrict__ dp, int tabs, int* __restrict__ kind, R* __restrict__ tab_a,
+                       R* __restrict__ tab_b, typename FieldOfRow<R>::type* __restrict__ dtdp) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = blockIdx.y;
+    if (l >= L || w >= NP) return;
+    rows_at<R>(X, N, NP, LPV, fft, g0, g1, p, ldp, s, lds, dp, tabs, kind, tab_a, tab_b, l, w);
+    if ((tabs & TAB_DTDP) && w < NP / 2)
+        dtdp_at<typename FieldOfRow<R>::type>(X, N, NP, fft, g0, g1, p, ldp, s, lds, dtdp, l, w);
+}
+
 // =======================================================================================
 // Propagation kernels
 // =======================================================================================
+// Layer kinds seen through the constant address space: read-only for the whole launch, so
+// their loads stay scalar (s_load through the constant cache) also where the compiler cannot
+// prove that no store of the kernel reaches them (after a call, or beside global stores).
+typedef __attribute__((address_space(4))) const int ConstInt;
+
 struct Lanes {
     int lane, wave, grp, t;
     long vbase;
@@ -493,6 +519,10 @@ __device__ __forceinline__ void bwd_layer(const int kd, const T (&u)[V][W], cons
 // wave-uniform) from the stage layer s0 and add its dp contribution to pout[m*np ...].
 __device__ __forceinline__ int s_clamp(const int last, const int s) { return s < last ? last - s : 0; }
 
+// Most waves that share one LDS stage: every kernel that converts a stage (backdiff, the
+// grouped kernel, the Hessian-vector product) runs workgroups of at most 256 threads.
+constexpr int kStageWaves = 4;
+
 template <class T, int W, int NPAR>
 __device__ __forceinline__ void convert_slot(const typename RowOf<T>::type* s0, const int wstride, const int nw,
                                              const int LPV, const int NP, float* pout, const int kd, const int m,
@@ -518,11 +548,21 @@ __device__ __forceinline__ void convert_slot(const typename RowOf<T>::type* s0, 
         return;
     }
     const int oi = (i % W) * LPV + i / W, oj = (j % W) * LPV + j / W;
-    R si = s0[oi], sj = s0[oj];
-    for (int wv = 1; wv < nw; ++wv) {
-        si = row_add(si, s0[wv * wstride + oi]);
-        sj = row_add(sj, s0[wv * wstride + oj]);
-    }
+    // All stage reads are issued before the first sum; the waves are summed in index order.
+    R ri_[kStageWaves], rj_[kStageWaves];
+#pragma unroll
+    for (int wv = 0; wv < kStageWaves; ++wv)
+        if (wv < nw) {
+            ri_[wv] = s0[wv * wstride + oi];
+            rj_[wv] = s0[wv * wstride + oj];
+        }
+    R si = ri_[0], sj = rj_[0];
+#pragma unroll
+    for (int wv = 1; wv < kStageWaves; ++wv)
+        if (wv < nw) {
+            si = row_add(si, ri_[wv]);
+            sj = row_add(sj, rj_[wv]);
+        }
     // dJdT00 = A_i, dJdT01 = B_i, dJdT11 = A_j, dJdT10 = B_j
     float val[NPAR];
 #pragma unroll
@@ -557,6 +597,7 @@ __device__ __forceinline__ void fetch_dtdp(const T* dtdp, const int NPh, const i
 // Backdiff workgroups hold at most four waves: measured on MI355X, eight-wave workgroups
 // (with the wider LDS stage) did not beat two four-wave workgroups per CU.
 __host__ __device__ constexpr int bwd_max_waves(int, int, bool) { return 4; }
+static_assert(bwd_max_waves(0, 0, true) <= kStageWaves, "stage conversion sums at most kStageWaves waves");
 
 // Number of phase parameters per crossing, fixed by the field type: real fields come only
 // from orthogonal crossings (one parameter), complex fields from the MZI family (two).
@@ -731,11 +772,19 @@ __global__ void __launch_bounds__(64 * bwd_max_waves(W, V, UNI)) k_bwd(
     float* my_partial = partial + (size_t)blockIdx.x * part_stride;
     bool first = true;
     for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        bwd_tile<T, W, V, UNI, const int*>(N, l0, nl, B, LPV, NP, C, kind, rows_inv, rows_adj, dtdp, my_partial,
-                                           u_out, g_out,
-                               target, u_in, g_in, ldu, tile, first, stage, ck);
+        bwd_tile<T, W, V, UNI, ConstInt*>(N, l0, nl, B, LPV, NP, C, (ConstInt*)kind, rows_inv, rows_adj, dtdp,
+                                          my_partial, u_out, g_out, target, u_in, g_in, ldu, tile, first, stage, ck);
         first = false;
     }
+}
+
+// A pointer known to be the same in every lane, moved to scalar registers.
+template <class P>
+__device__ __forceinline__ P* uniform_ptr(P* p) {
+    const unsigned long long v = reinterpret_cast<unsigned long long>(p);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return reinterpret_cast<P*>(((unsigned long long)hi << 32) | lo);
 }
 
 // =======================================================================================

@@ -40,9 +40,11 @@ __global__ void __launch_bounds__(256) k_dp_seg(long E, int nwg, int nseg, const
     seg[(size_t)s * E + e] = (a0 + a1) + (a2 + a3);
 }
 
-// dp reduction, stage 2: dp[l*ldp + m*np + k] += sum_s seg[s][(l*NPh + m)*np + k], m < N/2.
+// dp reduction, stage 2: dp[l*ldp + m*np + k] += sum_s seg[s][(l*NPh + m)*np + k], m < N/2
+// (or = with assign).
 __global__ void __launch_bounds__(256) k_dp_final(int L, int Nh, int NPh, int np, int nseg,
-                                                  const float* __restrict__ seg, float* __restrict__ dp, int ldp) {
+                                                  const float* __restrict__ seg, float* __restrict__ dp, int ldp,
+                                                  int assign) {
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long total = (long)L * Nh * np;
     if (idx >= total) return;
@@ -54,7 +56,8 @@ __global__ void __launch_bounds__(256) k_dp_final(int L, int Nh, int NPh, int np
     const float* src = seg + ((size_t)l * NPh + m) * np + k;
     float acc = 0.f;
     for (int s = 0; s < nseg; ++s) acc += src[(size_t)s * E];
-    dp[(size_t)l * ldp + (size_t)m * np + k] += acc;
+    float* d = dp + (size_t)l * ldp + (size_t)m * np + k;
+    *d = assign ? acc : *d + acc;
 }
 
 }  // namespace mp
@@ -124,7 +127,10 @@ void resolve_tile(int op, int N, int B, const meshprop_tile* t, int ncu, int& W,
 
 int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* tile, Plan& P) {
     if (X < 0 || X > 3) return MESHPROP_ERR_ARG;
-    if (op < 0 || op > 2) return MESHPROP_ERR_ARG;
+    if (op < 0 || op > 3) return MESHPROP_ERR_ARG;
+    // The fused forward + adjoint reconstructs with T^H: unitary crossings only.
+    if (op == MESHPROP_OP_FWDGRAD && X == X_GENERIC) return MESHPROP_ERR_UNSUPPORTED;
+    const bool adj = (op == MESHPROP_OP_BACKDIFF || op == MESHPROP_OP_FWDGRAD);
     if (N <= 0 || (N & 1) || L < 0 || B < 0) return MESHPROP_ERR_ARG;
     if (N > 1024) return MESHPROP_ERR_UNSUPPORTED;
     P.op = op;
@@ -142,17 +148,17 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
     P.NP = P.LPV * P.W;
     P.rowbytes = P.real ? sizeof(float2) : sizeof(float4);
     {
-        const int cap = (op == MESHPROP_OP_BACKDIFF) ? bwd_max_waves(P.W, P.V, P.uni) : 4;
+        const int cap = adj ? bwd_max_waves(P.W, P.V, P.uni) : 4;
         if (P.waves > cap) P.waves = cap;
     }
-    if (op == MESHPROP_OP_BACKDIFF)  // the LDS stage must hold one unrolled group of layers
+    if (adj)  // the LDS stage must hold one unrolled group of layers
         while (P.waves > 1 && (size_t)(P.W <= 4 ? 4 : 2) * P.waves * P.NP * P.rowbytes > stage_budget(P.waves)) P.waves >>= 1;
     const int G = 64 / P.LPV;
     const long per_tile = (long)P.waves * G * P.V;
     P.ntiles = ((long)B + per_tile - 1) / per_tile;
     if (P.ntiles < 1) P.ntiles = 1;
     P.grid = (int)P.ntiles;
-    if (op == MESHPROP_OP_BACKDIFF) {
+    if (adj) {
         // Cap the number of dp partials; each workgroup then walks several vector tiles.
         const long cap = 2L * ncu;
         if (P.ntiles > cap) P.grid = (int)cap;
@@ -178,11 +184,14 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
     P.off_a = off;
     off = align256(off + (size_t)L * P.NP * P.rowbytes);
     P.off_b = off;
-    const bool need_b = (op == MESHPROP_OP_FWDDIFF) || (op == MESHPROP_OP_BACKDIFF && !P.uni);
+    // Second table: fwddiff tangent rows, the adjoint rows of a lossy backdiff, or the
+    // inverse rows of the fused forward + adjoint (whose first table holds the forward rows).
+    const bool need_b = (op == MESHPROP_OP_FWDDIFF) || (op == MESHPROP_OP_BACKDIFF && !P.uni) ||
+                        (op == MESHPROP_OP_FWDGRAD);
     if (need_b) off = align256(off + (size_t)L * P.NP * P.rowbytes);
     P.off_dtdp = off;
     P.off_partial = off;
-    if (op == MESHPROP_OP_BACKDIFF) {
+    if (adj) {
         const size_t elem = P.real ? sizeof(float) : sizeof(float2);
         off = align256(off + (size_t)L * (P.NP / 2) * P.np * 4 * elem);
         P.off_partial = off;
@@ -190,7 +199,7 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
         P.off_seg = off;
         off = align256(off + (size_t)P.nseg * L * (P.NP / 2) * P.np * sizeof(float));
     }
-    P.off_seg = (op == MESHPROP_OP_BACKDIFF) ? P.off_seg : off;
+    P.off_seg = adj ? P.off_seg : off;
     P.total = off;
     return MESHPROP_OK;
 }
@@ -221,6 +230,9 @@ int run_impl(int op, int X, int N, int L, int B, int fft, const int* g0, const i
     if (op == MESHPROP_OP_FWDDIFF && L > 0 && !dp) return MESHPROP_ERR_ARG;
     if (op == MESHPROP_OP_BACKDIFF && !dp) return MESHPROP_ERR_ARG;
     if (op == MESHPROP_OP_BACKDIFF && !du_a && !target) return MESHPROP_ERR_ARG;
+    if (op == MESHPROP_OP_FWDGRAD && (!dp || !target)) return MESHPROP_ERR_ARG;
+    if (ckpt && op == MESHPROP_OP_FWDGRAD) return MESHPROP_ERR_UNSUPPORTED;
+    if ((flags & (MESHPROP_ONLY_FORWARD | MESHPROP_SKIP_FORWARD)) && op != MESHPROP_OP_FWDGRAD) return MESHPROP_ERR_ARG;
     if (ckpt && ck_every < 1) return MESHPROP_ERR_ARG;
     if (ckpt && op == MESHPROP_OP_FWDDIFF) return MESHPROP_ERR_UNSUPPORTED;
     if (ckpt && op == MESHPROP_OP_BACKDIFF && !du_b) return MESHPROP_ERR_ARG;  // carries g between segments
@@ -236,32 +248,24 @@ int run_impl(int op, int X, int N, int L, int B, int fft, const int* g0, const i
     float* partial = (float*)(base + P.off_partial);
     const float* sp = (P.ns > 0) ? s : nullptr;
 
-    const bool do_pro = !(flags & (MESHPROP_SKIP_PROLOGUE | MESHPROP_ONLY_REDUCE));
+    const bool do_pro = !(flags & (MESHPROP_SKIP_PROLOGUE | MESHPROP_ONLY_REDUCE | MESHPROP_ONLY_FORWARD));
     const bool do_main = !(flags & (MESHPROP_ONLY_PROLOGUE | MESHPROP_ONLY_REDUCE));
-    const bool do_red = !(flags & (MESHPROP_ONLY_PROLOGUE | MESHPROP_SKIP_REDUCE));
+    const bool do_red = !(flags & (MESHPROP_ONLY_PROLOGUE | MESHPROP_SKIP_REDUCE | MESHPROP_ONLY_FORWARD));
     if (L > 0 && do_pro) {
+        // One launch: row tables and, for the adjoint ops, the dT/dp table.
         int tabs = 0;
         if (op == MESHPROP_OP_FWDPROP) tabs = TAB_FWD;
         if (op == MESHPROP_OP_FWDDIFF) tabs = TAB_FWD | TAB_DIFF;
-        if (op == MESHPROP_OP_BACKDIFF) tabs = TAB_INV | (P.uni ? 0 : TAB_ADJ);
+        if (op == MESHPROP_OP_BACKDIFF) tabs = TAB_INV | (P.uni ? 0 : TAB_ADJ) | TAB_DTDP;
+        if (op == MESHPROP_OP_FWDGRAD) tabs = TAB_FWD | TAB_INV_B | TAB_DTDP;
         dim3 grid((P.NP + 255) / 256, L);
         if (P.real)
             hipLaunchKernelGGL((k_rows<float2>), grid, dim3(256), 0, st, X, N, L, P.NP, P.LPV, fft, g0, g1, p, ldp, sp, lds,
-                               (const float*)dp, tabs, kind, (float2*)tab_a, (float2*)tab_b);
+                               (const float*)dp, tabs, kind, (float2*)tab_a, (float2*)tab_b, (float*)dtdp);
         else
             hipLaunchKernelGGL((k_rows<float4>), grid, dim3(256), 0, st, X, N, L, P.NP, P.LPV, fft, g0, g1, p, ldp, sp, lds,
-                               (const float*)dp, tabs, kind, (float4*)tab_a, (float4*)tab_b);
+                               (const float*)dp, tabs, kind, (float4*)tab_a, (float4*)tab_b, (float2*)dtdp);
         if ((rc = check_launch())) return rc;
-        if (op == MESHPROP_OP_BACKDIFF) {
-            dim3 g2((P.NP / 2 + 255) / 256, L);
-            if (P.real)
-                hipLaunchKernelGGL((k_dtdp<float>), g2, dim3(256), 0, st, X, N, L, P.NP, fft, g0, g1, p, ldp, sp, lds,
-                                   (float*)dtdp);
-            else
-                hipLaunchKernelGGL((k_dtdp<float2>), g2, dim3(256), 0, st, X, N, L, P.NP, fft, g0, g1, p, ldp, sp,
-                                   lds, (float2*)dtdp);
-            if ((rc = check_launch())) return rc;
-        }
     }
 
     const void* rows_b = P.uni ? tab_a : tab_b;
@@ -276,6 +280,23 @@ int run_impl(int op, int X, int N, int L, int B, int fft, const int* g0, const i
             } else {
                 // Checkpointed forward, one launch: ckpt[c] receives the field entering layer c*K.
                 rc = launch_fwd_any(P, kind, tab_a, 0, L, u_a, u_b, ldu, st, Ckpt{ckpt, K, (size_t)B * ldu});
+            }
+        } else if (op == MESHPROP_OP_FWDGRAD) {
+            // Forward at the forward's own tile (same W, hence the same table layout), then
+            // the adjoint of its output with the inverse rows of the second table.  Two
+            // launches: on MI355X they ran faster than one kernel doing both per tile.
+            Plan PF;
+            const meshprop_tile tf = {P.W, 0, 0, 0};
+            rc = make_plan(MESHPROP_OP_FWDPROP, X, N, L, B, fft, &tf, PF);
+            if (!rc && (PF.W != P.W || PF.NP != P.NP)) rc = MESHPROP_ERR_UNSUPPORTED;
+            if (!rc && !(flags & MESHPROP_SKIP_FORWARD)) {
+                rc = launch_fwd_any(PF, kind, tab_a, 0, L, u_a, u_b, ldu, st);
+                if (!rc) rc = check_launch();
+            }
+            if (!rc && !(flags & MESHPROP_ONLY_FORWARD)) {
+                const size_t ps = (size_t)L * (P.NP / 2) * P.np;
+                rc = launch_bwd_any(P, kind, tab_b, tab_b, dtdp, partial, ps, 0, L, u_b, nullptr, target, nullptr,
+                                    du_b, ldu, st);
             }
         } else if (op == MESHPROP_OP_FWDDIFF) {
             rc = launch_fwddiff_any(P, kind, tab_a, tab_b, u_a, du_a, u_b, du_b, ldu, st);
@@ -305,7 +326,7 @@ int run_impl(int op, int X, int N, int L, int B, int fft, const int* g0, const i
         if (rc) return rc;
         if ((rc = check_launch())) return rc;
     }
-    if (op == MESHPROP_OP_BACKDIFF && L > 0 && do_red) {
+    if ((op == MESHPROP_OP_BACKDIFF || op == MESHPROP_OP_FWDGRAD) && L > 0 && do_red) {
         float* seg = (float*)(base + P.off_seg);
         const long E = (long)L * (P.NP / 2) * P.np;
         hipLaunchKernelGGL(k_dp_seg, dim3((unsigned)((E + 255) / 256), P.nseg), dim3(256), 0, st, E, P.grid, P.nseg,
@@ -313,7 +334,7 @@ int run_impl(int op, int X, int N, int L, int B, int fft, const int* g0, const i
         if ((rc = check_launch())) return rc;
         const long total = (long)L * (N / 2) * P.np;
         hipLaunchKernelGGL(k_dp_final, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, L, N / 2, P.NP / 2,
-                           P.np, P.nseg, (const float*)seg, dp, ldp);
+                           P.np, P.nseg, (const float*)seg, dp, ldp, (flags & MESHPROP_DP_ASSIGN) ? 1 : 0);
         if ((rc = check_launch())) return rc;
     }
     return MESHPROP_OK;

@@ -307,7 +307,10 @@ class MeshNetworkGPU:
 
     def grad_target_l2(self, v, y, p_phase=None, p_splitter=None, out_dp=None):
         """Fused fwd+grad of J = sum |U v - y|^2 (the loss of gpu/test.py:131): returns
-        (J, dp, u_out).  dJ/du_out = 2 (u_out - y) is formed inside the backdiff kernel."""
+        (J, dp, u_out).  dJ/du_out = 2 (u_out - y) is formed inside the adjoint kernel.  For
+        unitary crossings this is one library call (one prologue for all tables, fwdprop,
+        adjoint, dp reduction); the lossy crossing runs fwdprop, then backdiff.  dp is
+        accumulated into out_dp when it is given, else returned fresh."""
         assert not self.is_phase, "the fused target form has no phase screen"
         p, s = self._defaults(p_phase, p_splitter)
         ldp, lds = self._ld()
@@ -316,17 +319,23 @@ class MeshNetworkGPU:
         yb = as_field(y, self.N, self.dtype, self.device)
         B = vb.shape[0]
         uout = torch.empty_like(vb)
-        ws, nb = self._workspace(_lib.OP_FWDPROP, B)
-        _lib.run(_lib.OP_FWDPROP, self.X, self.N, self.L, B, self.fft, g0, g1, p, None, ldp,
-                 s if s.numel() else None, lds, vb, None, uout, None, self.N, tile=self.tile, workspace=ws,
-                 ws_bytes=nb)
+        flags = 0
         if out_dp is None:
-            out_dp = torch.zeros(self.n_phase, dtype=torch.float32, device=self.device)
-        uin = torch.empty_like(vb)
-        ws, nb = self._workspace(_lib.OP_BACKDIFF, B)
-        _lib.run(_lib.OP_BACKDIFF, self.X, self.N, self.L, B, self.fft, g0, g1, p, out_dp, ldp,
-                 s if s.numel() else None, lds, uout, None, uin, None, self.N, target=yb, tile=self.tile,
-                 workspace=ws, ws_bytes=nb)
+            out_dp = torch.empty(self.n_phase, dtype=torch.float32, device=self.device)
+            flags = _lib.DP_ASSIGN
+        sp = s if s.numel() else None
+        if self.X != "mzi_generic":
+            ws, nb = self._workspace(_lib.OP_FWDGRAD, B)
+            _lib.run(_lib.OP_FWDGRAD, self.X, self.N, self.L, B, self.fft, g0, g1, p, out_dp, ldp, sp, lds, vb,
+                     None, uout, None, self.N, target=yb, tile=self.tile, workspace=ws, ws_bytes=nb, flags=flags)
+        else:
+            ws, nb = self._workspace(_lib.OP_FWDPROP, B)
+            _lib.run(_lib.OP_FWDPROP, self.X, self.N, self.L, B, self.fft, g0, g1, p, None, ldp, sp, lds, vb, None,
+                     uout, None, self.N, tile=self.tile, workspace=ws, ws_bytes=nb)
+            uin = torch.empty_like(vb)
+            ws, nb = self._workspace(_lib.OP_BACKDIFF, B)
+            _lib.run(_lib.OP_BACKDIFF, self.X, self.N, self.L, B, self.fft, g0, g1, p, out_dp, ldp, sp, lds, uout,
+                     None, uin, None, self.N, target=yb, tile=self.tile, workspace=ws, ws_bytes=nb, flags=flags)
         J = torch.sum(torch.abs(uout - yb) ** 2)
         return J, out_dp, uout.T
 

@@ -55,6 +55,30 @@ def test_workspace_sizes(lib):
     assert L.workspace_size(L.OP_FWDPROP, "mzi", 2048, 4, 4) == 0
 
 
+def test_fwdgrad_plan_and_arguments_without_gpu(lib):
+    """The fused fwd+grad op: a third table beside the adjoint's, unitary crossings only,
+    target required, no checkpoints; all rejected on the host before any launch."""
+    import meshes_amd._lib as L
+    n_b = L.workspace_size(L.OP_BACKDIFF, "mzi", 256, 256, 4096)
+    assert L.workspace_size(L.OP_FWDGRAD, "mzi", 256, 256, 4096) >= n_b + 256 * 256 * 16
+    assert L.workspace_size(L.OP_FWDGRAD, "orth", 64, 8, 100) > 0
+    assert L.workspace_size(L.OP_FWDGRAD, "mzi_generic", 64, 8, 100) == 0
+    assert L.resolve_tile(L.OP_FWDGRAD, "mzi", 256, 4096) == L.resolve_tile(L.OP_BACKDIFF, "mzi", 256, 4096)
+    fake = ctypes.c_void_p(4096)
+    geo = (ctypes.c_int * 8)(*([4] * 8))
+
+    def run(X=L.MZI, target=fake, dp=fake, ckpt=None, every=0, op=L.OP_FWDGRAD, flags=0):
+        return lib.meshprop_run(op, X, 8, 8, 4, 0, geo, geo, fake, dp, 8, None, 0, fake, fake, fake,
+                                None, 8, 0, target, ckpt, every, None, flags, fake, 1 << 30, None)
+    assert run(X=L.MZI_GENERIC) == 3
+    assert run(target=None) == 1
+    assert run(dp=None) == 1
+    assert run(ckpt=fake, every=2) == 3
+    # the forward-stage flags belong to fwd+grad only
+    assert run(op=L.OP_BACKDIFF, flags=L.ONLY_FORWARD) == 1
+    assert run(op=L.OP_FWDPROP, target=None, dp=None, flags=L.SKIP_FORWARD) == 1
+
+
 def test_argument_validation_without_gpu(lib):
     """Bad arguments are rejected on the host before anything is launched."""
     import meshes_amd._lib as L
