@@ -510,8 +510,11 @@ __device__ __forceinline__ void bwd_layer(const int kd, const T (&u)[V][W], cons
         }
     }
     if (q.grp == 0) {
+        // This wave's rows of stage layer lc (a wave-uniform base), then k * LPV + lane.
+        typename RowOf<T>::type* ws =
+            stage + (size_t)(__builtin_amdgcn_readfirstlane(q.wave) * C + lc) * W * LPV + q.t;
 #pragma unroll
-        for (int k = 0; k < W; ++k) stage[(((size_t)q.wave * C + lc) * W + k) * LPV + q.t] = pack_row(accA[k], accB[k]);
+        for (int k = 0; k < W; ++k) ws[k * LPV] = pack_row(accA[k], accB[k]);
     }
 }
 
@@ -759,21 +762,256 @@ __device__ __forceinline__ void bwd_tile(const int N, const int l0, const int nl
     if (g_in != nullptr) store_tile(ga, g_in, q, N, B, ldu);
 }
 
-template <class T, int W, int V, bool UNI>
-__global__ void __launch_bounds__(64 * bwd_max_waves(W, V, UNI)) k_bwd(
+// ---- software-pipelined adjoint tile ----------------------------------------------------
+// Same arithmetic and the same fixed summation orders as bwd_tile, with the LDS stage
+// double-buffered (two buffers of C layers): the dJdT -> dp conversion of stage s runs while
+// stage s + 1 propagates.  Its stage reads and dT/dp loads are issued just before the first
+// layer of stage s + 1 and consumed right after it, so their latency hides under that
+// layer's arithmetic instead of stalling every wave of the workgroup between two barriers;
+// one barrier per stage remains.  Conversion items (layer of the stage, slot) are spread over
+// all threads; the host uses this form when a wave's items lie in one layer (NP/2 a multiple
+// of 64) and two buffers of at least two layers fit the LDS budget.
+// The pipelined stage serves workgroups of up to eight waves (one per CU at 512 threads);
+// NW is the most waves a kernel instance sums (its item holds one row pair per wave).
+constexpr int kPipeWaves = 8;
+
+template <class T, int NW>
+struct ConvItem {
+    typename RowOf<T>::type a[NW], b[NW];
+    T dd[4 * NPhase<T>::value];
+    float* out;
+    int state;  // 0: nothing, 1: inactive slot (zero), 2: convert
+};
+
+template <class T, int W, int NW, class KP>
+__device__ __forceinline__ void conv_issue(ConvItem<T, NW>& it, const typename RowOf<T>::type* s0, const int wstride,
+                                           const int nw, const int lpv_sh, const int NP, const int la, KP kind,
+                                           const T* __restrict__ dtdp, float* __restrict__ part, const unsigned m) {
+    constexpr int np = NPhase<T>::value;
+    constexpr int wsh = W == 2 ? 1 : W == 4 ? 2 : W == 8 ? 3 : 4;  // log2 W
+    const int NPh = NP / 2;
+    const int kd = kind[la];
+    int i, j;
+    if (kd <= 1) {
+        i = 2 * (int)m + kd;
+        j = i + 1;
+    } else if (kd != KIND_IDENTITY) {
+        const int st = kd - 2;
+        i = (((int)m >> st) << (st + 1)) + ((int)m & ((1 << st) - 1));
+        j = i + (1 << st);
+    } else {
+        i = -1;
+        j = 0;
+    }
+    it.out = part + ((size_t)la * NPh) * np + m * np;
+    if (i < 0 || j >= NP) {
+        it.state = 1;
+        return;
+    }
+    it.state = 2;
+    // Stage index of waveguide i: (i % W) * LPV + i / W, LPV = 2^lpv_sh.
+    const int oi = ((i & (W - 1)) << lpv_sh) + (i >> wsh), oj = ((j & (W - 1)) << lpv_sh) + (j >> wsh);
+#pragma unroll
+    for (int wv = 0; wv < NW; ++wv)
+        if (wv < nw) {
+            it.a[wv] = s0[wv * wstride + oi];
+            it.b[wv] = s0[wv * wstride + oj];
+        }
+    // One uniform base per table row, one 32-bit lane offset for all of them.
+    const T* src = dtdp + (size_t)la * NPh * np * 4;
+#pragma unroll
+    for (int e = 0; e < 4 * np; ++e) it.dd[e] = (src + (size_t)e * NPh)[m];
+}
+
+template <class T, int NW>
+__device__ __forceinline__ void conv_finish(const ConvItem<T, NW>& it, const int nw, const bool first_tile) {
+    typedef typename RowOf<T>::type R;
+    constexpr int np = NPhase<T>::value;
+    if (it.state == 0) return;
+    if (it.state == 1) {
+        if (first_tile)
+#pragma unroll
+            for (int k = 0; k < np; ++k) it.out[k] = 0.f;
+        return;
+    }
+    R si = it.a[0], sj = it.b[0];
+#pragma unroll
+    for (int wv = 1; wv < NW; ++wv)
+        if (wv < nw) {
+            si = row_add(si, it.a[wv]);
+            sj = row_add(sj, it.b[wv]);
+        }
+    // dJdT00 = A_i, dJdT01 = B_i, dJdT11 = A_j, dJdT10 = B_j
+    float val[np];
+#pragma unroll
+    for (int k = 0; k < np; ++k) {
+        val[k] = re_dot(row_a(si), it.dd[k * 4 + 0]);
+        val[k] += re_dot(row_b(si), it.dd[k * 4 + 1]);
+        val[k] += re_dot(row_b(sj), it.dd[k * 4 + 2]);
+        val[k] += re_dot(row_a(sj), it.dd[k * 4 + 3]);
+    }
+    if constexpr (np == 2) {
+        float2* o2 = reinterpret_cast<float2*>(it.out);
+        if (first_tile) {
+            *o2 = make_float2(val[0], val[1]);
+        } else {
+            const float2 old = *o2;
+            *o2 = make_float2(old.x + val[0], old.y + val[1]);
+        }
+    } else {
+        it.out[0] = first_tile ? val[0] : it.out[0] + val[0];
+    }
+}
+
+template <class T, int W, int V, bool UNI, int NW, class KP>
+__device__ __forceinline__ void bwd_tile_pipe(const int N, const int l0, const int nl, const int B, const int LPV,
+                                              const int NP, const int C, KP kind,
+                                              const typename RowOf<T>::type* __restrict__ rows_inv,
+                                              const typename RowOf<T>::type* __restrict__ rows_adj,
+                                              const T* __restrict__ dtdp, float* __restrict__ my_partial,
+                                              const T* u_out, const T* g_out, const T* target, T* u_in, T* g_in,
+                                              const int ldu, const long tile, const bool first,
+                                              typename RowOf<T>::type* __restrict__ stage,
+                                              const Ckpt ck = Ckpt{nullptr, 0, 0}) {
+    typedef typename RowOf<T>::type R;
+    constexpr int D = 2;  // row register sets: rows requested one layer ahead
+    const int nw = blockDim.x >> 6;
+    const int NPh = NP / 2;
+    const int wave_u = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wstride = C * W * LPV;   // one wave's rows in one buffer
+    const int bstride = nw * wstride;  // one buffer
+    const int nthr = nw * 64;
+    const KP kp = kind + l0;
+    const int last = nl - 1;
+    const Lanes q = lanes_of<V>(LPV, tile);
+    T ua[V][W], ga[V][W], ub[V][W], gb[V][W];
+    load_tile(ua, u_out, q, N, B, ldu);
+    if (target != nullptr) {
+        T y[V][W];
+        load_tile(y, target, q, N, B, ldu);
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+#pragma unroll
+            for (int k = 0; k < W; ++k) ga[v][k] = fsub2(ua[v][k], y[v][k]);
+    } else {
+        load_tile(ga, g_out, q, N, B, ldu);
+    }
+    const R* rip = rows_inv + (size_t)l0 * NP + q.t;
+    const R* rap = rows_adj + (size_t)l0 * NP + q.t;
+    R ri[D][W], ra[D][W];
+    int kk[D];
+    auto lrow = [&](const int s) { return (size_t)(s < last ? last - s : 0) * NP; };
+    if (nl > 0) {
+        kk[0] = kp[s_clamp(last, 0)];
+        load_rows(ri[0], rip + lrow(0), LPV);
+        if constexpr (!UNI) load_rows(ra[0], rap + lrow(0), LPV);
+    }
+#define MP_PIPE_LAYER(KD, U, G, UO, GO, RI, RA, LC)                                     \
+    {                                                                                   \
+        if constexpr (UNI)                                                              \
+            bwd_layer<T, W, V, UNI>(KD, U, G, UO, GO, RI, RI, buf, q, LPV, C, LC);      \
+        else                                                                            \
+            bwd_layer<T, W, V, UNI>(KD, U, G, UO, GO, RI, RA, buf, q, LPV, C, LC);      \
+    }
+    // Conversion of the previous stage: items idx = gthr + k * nthr < pcl * NPh, layer idx / NPh
+    // of that stage (the same for the whole wave), slot idx % NPh.
+    int pc0 = 0, pcl = 0;
+    R* pbuf = stage;
+    // NP is a power of two (a power-of-two lane count times W): divisions by LPV and NP/2 are shifts.
+    const int lpv_sh = __builtin_ctz(LPV), nph_sh = __builtin_ctz(NPh);
+    // Item k of this thread (none when k is past the previous stage's items).
+    ConvItem<T, NW> item;
+    item.state = 0;
+    auto issue = [&](const int k) {
+        item.state = 0;
+        const int wbase = wave_u * 64 + k * nthr;  // first item of this wave (uniform)
+        if (wbase < pcl * NPh) {
+            const int lcc = wbase >> nph_sh;
+            const int la = l0 + last - (pc0 + lcc);
+            const unsigned m = (unsigned)((wbase & (NPh - 1)) + (threadIdx.x & 63));
+            conv_issue<T, W, NW, KP>(item, pbuf + lcc * W * LPV, wstride, nw, lpv_sh, NP, la, kind, dtdp, my_partial,
+                                     m);
+        }
+    };
+    int sidx = 0;
+    for (int c0 = 0; c0 < nl; c0 += C, ++sidx) {
+        const int cl = (C < nl - c0) ? C : nl - c0;
+        R* buf = stage + (sidx & 1) * bstride;
+        if (ck.base != nullptr && c0 > 0) {
+            const int e = l0 + nl - c0;
+            if (e % ck.every == 0)
+                load_tile(ua, reinterpret_cast<const T*>(ck.base) + (size_t)(e / ck.every) * ck.stride, q, N, B, ldu);
+        }
+        const int nitem = (pcl * NPh + nthr - 1) / nthr;  // conversion items per thread (wave-uniform)
+        int lc = 0;
+        for (; lc + D <= cl; lc += D) {
+            const int s = c0 + lc;
+            if (lc < nitem) issue(lc);
+            kk[1] = kp[s_clamp(last, s + 1)];
+            load_rows(ri[1], rip + lrow(s + 1), LPV);
+            if constexpr (!UNI) load_rows(ra[1], rap + lrow(s + 1), LPV);
+            MP_PIPE_LAYER(kk[0], ua, ga, ub, gb, ri[0], ra[0], lc)
+            if (lc < nitem) conv_finish(item, nw, first);
+            if (lc + 1 < nitem) issue(lc + 1);
+            kk[0] = kp[s_clamp(last, s + 2)];
+            load_rows(ri[0], rip + lrow(s + 2), LPV);
+            if constexpr (!UNI) load_rows(ra[0], rap + lrow(s + 2), LPV);
+            MP_PIPE_LAYER(kk[1], ub, gb, ua, ga, ri[1], ra[1], lc + 1)
+            if (lc + 1 < nitem) conv_finish(item, nw, first);
+        }
+        if (lc < cl) {  // one layer left (last stage of an odd depth)
+            if (lc < nitem) issue(lc);
+            MP_PIPE_LAYER(kk[0], ua, ga, ub, gb, ri[0], ra[0], lc)
+            if (lc < nitem) conv_finish(item, nw, first);
+            copy_state(ub, ua);
+            copy_state(gb, ga);
+            ++lc;
+        }
+        for (int k = lc; k < nitem; ++k) {  // items beyond the stage's layers
+            issue(k);
+            conv_finish(item, nw, first);
+        }
+        __syncthreads();
+        pc0 = c0;
+        pcl = cl;
+        pbuf = buf;
+    }
+    // The last stage: converted after its barrier, without overlap.
+    {
+        const int nitem = (pcl * NPh + nthr - 1) / nthr;
+        for (int k = 0; k < nitem; ++k) {
+            issue(k);
+            conv_finish(item, nw, first);
+        }
+    }
+    __syncthreads();  // the next tile of this workgroup reuses the buffers
+#undef MP_PIPE_LAYER
+    if (u_in != nullptr) store_tile(ua, u_in, q, N, B, ldu);
+    if (g_in != nullptr) store_tile(ga, g_in, q, N, B, ldu);
+}
+
+// PIPE: 0 = bwd_tile; 4 or 8 = bwd_tile_pipe for workgroups of up to PIPE waves.
+template <class T, int W, int V, bool UNI, int PIPE>
+__global__ void __launch_bounds__(64 * (PIPE ? PIPE : bwd_max_waves(W, V, UNI))) k_bwd(
     int N, int l0, int nl, int B, int LPV, int NP, int C, int np_unused, const int* __restrict__ kind,
     const typename RowOf<T>::type* __restrict__ rows_inv, const typename RowOf<T>::type* __restrict__ rows_adj,
     const T* __restrict__ dtdp, float* __restrict__ partial, size_t part_stride, long ntiles, const T* u_out,
     const T* g_out, const T* target, T* u_in, T* g_in, int ldu, Ckpt ck) {
     typedef typename RowOf<T>::type R;
     extern __shared__ unsigned char smem_raw[];
-    R* stage = reinterpret_cast<R*>(smem_raw);  // [nw][C][W][LPV]
+    R* stage = reinterpret_cast<R*>(smem_raw);  // [nw][C][W][LPV], twice with PIPE
     (void)np_unused;
     float* my_partial = partial + (size_t)blockIdx.x * part_stride;
     bool first = true;
     for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        bwd_tile<T, W, V, UNI, ConstInt*>(N, l0, nl, B, LPV, NP, C, (ConstInt*)kind, rows_inv, rows_adj, dtdp,
-                                          my_partial, u_out, g_out, target, u_in, g_in, ldu, tile, first, stage, ck);
+        if constexpr (PIPE > 0)
+            bwd_tile_pipe<T, W, V, UNI, PIPE, ConstInt*>(N, l0, nl, B, LPV, NP, C, (ConstInt*)kind, rows_inv, rows_adj,
+                                                   dtdp, my_partial, u_out, g_out, target, u_in, g_in, ldu, tile,
+                                                   first, stage, ck);
+        else
+            bwd_tile<T, W, V, UNI, ConstInt*>(N, l0, nl, B, LPV, NP, C, (ConstInt*)kind, rows_inv, rows_adj, dtdp,
+                                              my_partial, u_out, g_out, target, u_in, g_in, ldu, tile, first, stage,
+                                              ck);
         first = false;
     }
 }
@@ -793,6 +1031,7 @@ __device__ __forceinline__ P* uniform_ptr(P* p) {
 struct Plan {
     int op, X, N, L, B, fft;
     int W, V, LPV, NP, waves, C;
+    int pipe;         // adjoint: double-buffered LDS stage with the conversion pipelined (bwd_tile_pipe)
     int np, ns, real;
     long ntiles;
     int grid;         // workgroups of the propagation kernel

@@ -4,28 +4,50 @@
 
 namespace mp {
 
+template <class T, int W, int V, bool UNI, int PIPE>
+static void go1(const Plan& P, const int* kind, const void* ri, const void* ra, const void* dtdp, float* partial,
+                size_t ps, int l0, int nl, const void* u_out, const void* g_out, const void* target, void* u_in,
+                void* g_in, int ldu, hipStream_t st, Ckpt ck) {
+    typedef typename RowOf<T>::type R;
+    const size_t shmem = (size_t)P.waves * P.C * P.NP * P.rowbytes * (PIPE ? 2 : 1);
+    if (shmem > 64 * 1024)  // beyond the default dynamic LDS limit of a launch
+        hipFuncSetAttribute((const void*)k_bwd<T, W, V, UNI, PIPE>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)shmem);
+    hipLaunchKernelGGL((k_bwd<T, W, V, UNI, PIPE>), dim3(P.grid), dim3(64 * P.waves), shmem, st, P.N, l0, nl, P.B,
+                       P.LPV, P.NP, P.C, P.np, kind, (const R*)ri, (const R*)ra, (const T*)dtdp, partial, ps,
+                       P.ntiles, (const T*)u_out, (const T*)g_out, (const T*)target, (T*)u_in, (T*)g_in, ldu, ck);
+}
+
+template <class T, int W, int V, int PIPE>
+static void go2(const Plan& P, const int* kind, const void* ri, const void* ra, const void* dtdp, float* partial,
+                size_t ps, int l0, int nl, const void* u_out, const void* g_out, const void* target, void* u_in,
+                void* g_in, int ldu, hipStream_t st, Ckpt ck) {
+    if (P.uni)
+        go1<T, W, V, true, PIPE>(P, kind, ri, ra, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in, g_in, ldu,
+                                 st, ck);
+    else
+        go1<T, W, V, false, PIPE>(P, kind, ri, ra, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in, g_in, ldu,
+                                  st, ck);
+}
+
 template <class T, int W, int V>
 static void go(const Plan& P, const int* kind, const void* ri, const void* ra, const void* dtdp, float* partial,
                size_t ps, int l0, int nl, const void* u_out, const void* g_out, const void* target, void* u_in,
                void* g_in, int ldu, hipStream_t st, Ckpt ck) {
-    typedef typename RowOf<T>::type R;
-    const size_t shmem = (size_t)P.waves * P.C * P.NP * P.rowbytes;
-    if (shmem > 64 * 1024) {  // beyond the default dynamic LDS limit of a launch
-        if (P.uni)
-            hipFuncSetAttribute((const void*)k_bwd<T, W, V, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)shmem);
-        else
-            hipFuncSetAttribute((const void*)k_bwd<T, W, V, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)shmem);
+    // The pipelined stage is compiled for the tiles it can serve (two buffers of two layers
+    // fit the LDS budget only for W <= 4), once for four-wave and once for eight-wave groups.
+    if constexpr (W <= 4) {
+        if (P.pipe) {
+            if (P.waves > 4)
+                go2<T, W, V, kPipeWaves>(P, kind, ri, ra, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in,
+                                         g_in, ldu, st, ck);
+            else
+                go2<T, W, V, 4>(P, kind, ri, ra, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in, g_in, ldu,
+                                st, ck);
+            return;
+        }
     }
-    if (P.uni)
-        hipLaunchKernelGGL((k_bwd<T, W, V, true>), dim3(P.grid), dim3(64 * P.waves), shmem, st, P.N, l0, nl, P.B,
-                           P.LPV, P.NP, P.C, P.np, kind, (const R*)ri, (const R*)ra, (const T*)dtdp, partial, ps,
-                           P.ntiles, (const T*)u_out, (const T*)g_out, (const T*)target, (T*)u_in, (T*)g_in, ldu, ck);
-    else
-        hipLaunchKernelGGL((k_bwd<T, W, V, false>), dim3(P.grid), dim3(64 * P.waves), shmem, st, P.N, l0, nl, P.B,
-                           P.LPV, P.NP, P.C, P.np, kind, (const R*)ri, (const R*)ra, (const T*)dtdp, partial, ps,
-                           P.ntiles, (const T*)u_out, (const T*)g_out, (const T*)target, (T*)u_in, (T*)g_in, ldu, ck);
+    go2<T, W, V, 0>(P, kind, ri, ra, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in, g_in, ldu, st, ck);
 }
 
 int launch_bwd_any(const Plan& P, const int* kind, const void* ri, const void* ra, const void* dtdp, float* partial,

@@ -148,8 +148,13 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
     P.NP = P.LPV * P.W;
     P.rowbytes = P.real ? sizeof(float2) : sizeof(float4);
     {
-        const int cap = adj ? bwd_max_waves(P.W, P.V, P.uni) : 4;
+        // Eight-wave adjoint workgroups exist only with the pipelined stage (two buffers of two
+        // layers of eight waves in the wide budget).
+        const bool wide_ok = adj && P.W <= 4 && (P.NP / 2) % 64 == 0 &&
+                             (size_t)4 * kPipeWaves * P.NP * P.rowbytes <= stage_budget(kPipeWaves);
+        const int cap = adj ? (wide_ok ? kPipeWaves : bwd_max_waves(P.W, P.V, P.uni)) : 4;
         if (P.waves > cap) P.waves = cap;
+        if (adj && P.waves > 4 && P.waves < kPipeWaves) P.waves = 4;
     }
     if (adj)  // the LDS stage must hold one unrolled group of layers
         while (P.waves > 1 && (size_t)(P.W <= 4 ? 4 : 2) * P.waves * P.NP * P.rowbytes > stage_budget(P.waves)) P.waves >>= 1;
@@ -165,17 +170,34 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
         // LDS stage depth: at least two layers; the stage fits the budget.
         const size_t per_layer = (size_t)P.waves * P.NP * P.rowbytes;
         const size_t budget = stage_budget(P.waves);
+        const int req = P.C;  // requested layers per stage (0: automatic)
         int C = (int)(budget / per_layer);
         if (C > 16) C = 16;
-        if (P.C >= 1 && (size_t)P.C * per_layer <= budget) C = P.C;
+        if (req >= 1 && (size_t)req * per_layer <= budget) C = req;
         const int depth = P.W <= 4 ? 4 : 2;  // layers per unrolled group of the backdiff loop
         C -= C % depth;
         if (C < depth) C = depth;
         if ((size_t)C * per_layer > budget) return MESHPROP_ERR_UNSUPPORTED;
+        // Pipelined stage (conversion of a stage overlapped with the next one): two buffers of
+        // an even number of layers in the same budget, and whole waves of conversion items per
+        // layer (NP/2 a multiple of 64).
+        P.pipe = 0;
+        if (P.W <= 4 && (P.NP / 2) % 64 == 0) {
+            int Cp = (int)(budget / (2 * per_layer));
+            if (Cp > 16) Cp = 16;
+            if (req >= 2 && 2 * (size_t)req * per_layer <= budget) Cp = req;
+            Cp -= Cp % 2;
+            if (Cp >= 2) {
+                P.pipe = 1;
+                C = Cp;
+            }
+        }
+        if (P.waves > 4 && !P.pipe) return MESHPROP_ERR_UNSUPPORTED;  // only the pipelined stage sums 8 waves
         P.C = C;
         P.nseg = P.grid < 16 ? P.grid : 16;
     } else {
         P.C = 0;
+        P.pipe = 0;
         P.nseg = 0;
     }
     size_t off = 0;

@@ -127,3 +127,30 @@ def test_fwdgrad_c2_full_size(gpu):
     assert torch.equal(out_f, out_s) and torch.equal(dp_f, dp_s)
     assert torch.equal(dp_m, dp_f) and torch.equal(uo, out_f.T)
     assert abs(J.item() - torch.sum(torch.abs(out_f - yb) ** 2).item()) <= 1e-6 * J.item()
+
+
+@pytest.mark.parametrize("N,L,B", [(256, 24, 300), (128, 17, 1000), (200, 9, 77)])
+def test_fwdgrad_eight_wave_workgroups(gpu, N, L, B):
+    """Adjoint workgroups of eight waves (pipelined stage, one workgroup per CU): same output,
+    dp and dJ/du_in as the four-wave default within the oracle bar (different partial sums)."""
+    import meshes_amd._lib as Lb
+    g, p, s, u, rng = _setup("mzi", "nn", N, L, B, seed=N + L)
+    mesh = _mesh(g, "mzi", p, s)
+    y = O.unit_cn(rng, B, N).astype(np.complex64)
+    ub, yb = torch.as_tensor(u).cuda().contiguous(), torch.as_tensor(y).cuda().contiguous()
+    w = Lb.resolve_tile(Lb.OP_FWDGRAD, "mzi", N, B)[0]
+    assert Lb.resolve_tile(Lb.OP_FWDGRAD, "mzi", N, B, (w, 0, 8, 0))[2] == 8
+    res = []
+    for tile in (None, (w, 0, 8, 0)):
+        out, gin, dp = torch.empty_like(ub), torch.empty_like(ub), torch.zeros_like(mesh.p_phase)
+        _run(mesh, Lb.OP_FWDGRAD, B, ub, out, gin, dp, yb, tile=tile)
+        res.append((out, gin, dp))
+    torch.cuda.synchronize()
+    assert torch.equal(res[0][0], res[1][0])
+    p64, s64 = p.astype(np.float64), _f64(s)
+    gout = 2 * (_np(res[1][0]).astype(np.complex128) - y)
+    rdp, rgin, _ = O.mesh_vjp(g, "mzi", p64, s64, _np(res[1][0]).astype(np.complex128), gout)
+    for out, gin, dp in res:
+        assert O.rel_l2(_np(dp).reshape(rdp.shape), rdp) < TOL
+        assert O.rel_l2(_np(gin), rgin) < TOL
+    assert O.rel_l2(_np(res[0][2]), _np(res[1][2])) < 1e-6
