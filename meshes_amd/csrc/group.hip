@@ -107,33 +107,49 @@ __global__ void __launch_bounds__(256) k_group_prologue(const unsigned char* __r
 // dp[l*ldp + m*np + k] += sum over slots s of partial[s*E + (l*NPh + m)*np + k], m < N/2,
 // summed in slot order (four interleaved partial sums), so the result is deterministic.
 __global__ void __launch_bounds__(256) k_group_reduce(const unsigned char* __restrict__ ws) {
+    // Unit u of a mesh = 64 consecutive dp entries; wave q of the block sums the slots
+    // [q*S/4, (q+1)*S/4) with four interleaved accumulators, and the four wave sums are added
+    // in wave order through LDS (a fixed order: bitwise reproducible).
+    __shared__ float red[4][64];
     const GHeader* H = reinterpret_cast<const GHeader*>(ws);
     KLong* begin = (KLong*)(ws + H->off_red);
     KMesh* meshes = (KMesh*)(ws + H->off_mesh);
+    const int lane = threadIdx.x & 63, wq = threadIdx.x >> 6;
     for (long u = blockIdx.x; u < H->red_total; u += gridDim.x) {
         const int e = find_mesh(begin, H->nmesh, u);
         KMesh& M = meshes[e];
-        const long idx = (u - begin[e]) * 256 + threadIdx.x;
+        const long idx = (u - begin[e]) * 64 + lane;
         const int Nh = M.N / 2;
-        if (idx >= (long)M.L * Nh * M.np) continue;
-        const int k = (int)(idx % M.np);
-        const long lm = idx / M.np;
-        const int m = (int)(lm % Nh);
-        const int l = (int)(lm / Nh);
-        const float* src = M.partial + ((size_t)l * (M.NP / 2) + m) * M.np + k;
-        const size_t E = (size_t)M.E;
-        float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
-        int s = 0;
-        for (; s + 3 < M.S; s += 4) {
-            a0 += src[(size_t)s * E];
-            a1 += src[(size_t)(s + 1) * E];
-            a2 += src[(size_t)(s + 2) * E];
-            a3 += src[(size_t)(s + 3) * E];
+        const bool ok = idx < (long)M.L * Nh * M.np;
+        int k = 0, m = 0, l = 0;
+        float acc = 0.f;
+        if (ok) {
+            k = (int)(idx % M.np);
+            const long lm = idx / M.np;
+            m = (int)(lm % Nh);
+            l = (int)(lm / Nh);
+            const float* src = M.partial + ((size_t)l * (M.NP / 2) + m) * M.np + k;
+            const size_t E = (size_t)M.E;
+            const int s0 = (int)((long)M.S * wq / 4), s1 = (int)((long)M.S * (wq + 1) / 4);
+            float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+            int s = s0;
+            for (; s + 3 < s1; s += 4) {
+                a0 += src[(size_t)s * E];
+                a1 += src[(size_t)(s + 1) * E];
+                a2 += src[(size_t)(s + 2) * E];
+                a3 += src[(size_t)(s + 3) * E];
+            }
+            for (; s < s1; ++s) a0 += src[(size_t)s * E];
+            acc = (a0 + a1) + (a2 + a3);
         }
-        for (; s < M.S; ++s) a0 += src[(size_t)s * E];
-        float* d = M.dp + (size_t)l * M.ldp + (size_t)m * M.np + k;
-        const float acc = (a0 + a1) + (a2 + a3);
-        *d = M.assign ? acc : *d + acc;
+        red[wq][lane] = acc;
+        __syncthreads();
+        if (wq == 0 && ok) {
+            const float sum = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+            float* d = M.dp + (size_t)l * M.ldp + (size_t)m * M.np + k;
+            *d = M.assign ? sum : *d + sum;
+        }
+        __syncthreads();  // red is rewritten by the next unit
     }
 }
 
@@ -406,7 +422,7 @@ int plan_group(meshprop_group& g) {
         if (grad && e.L > 0 && f.S > 0) off = a256(off + (size_t)f.S * f.E * sizeof(float));
         f.pro_units = f.build ? (long)e.L * ((f.NP + 255) / 256) : 0;
         const bool assign = (e.flags & MESHPROP_GROUP_DP_ASSIGN) != 0;
-        f.red_units = (grad && e.L > 0 && (f.S > 0 || assign)) ? ((long)e.L * (e.N / 2) * f.np + 255) / 256 : 0;
+        f.red_units = (grad && e.L > 0 && (f.S > 0 || assign)) ? ((long)e.L * (e.N / 2) * f.np + 63) / 64 : 0;
         g.pro_total += f.pro_units;
         g.red_total += f.red_units;
         // Work items, costed by layers x tile work x tiles per item.

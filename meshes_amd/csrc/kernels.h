@@ -1035,9 +1035,8 @@ struct Plan {
     int np, ns, real;
     long ntiles;
     int grid;         // workgroups of the propagation kernel
-    int nseg;         // first-stage segments of the dp reduction
     size_t rowbytes;  // sizeof(row) = 16 (complex) or 8 (real)
-    size_t off_kind, off_a, off_b, off_dtdp, off_partial, off_seg, total;  // workspace layout
+    size_t off_kind, off_a, off_b, off_dtdp, off_partial, total;  // workspace layout
     int uni;          // crossings are unitary: T^-1 = T^H
 };
 

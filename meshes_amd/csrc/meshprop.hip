@@ -18,46 +18,46 @@
 
 namespace mp {
 
-// dp reduction, stage 1: seg[s][e] = sum of partial[wg][e] over the s-th contiguous run of
-// workgroups (e indexes the partial layout [L][NP/2][np]).
-__global__ void __launch_bounds__(256) k_dp_seg(long E, int nwg, int nseg, const float* __restrict__ partial,
-                                                float* __restrict__ seg) {
-    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int s = blockIdx.y;
-    if (e >= E) return;
-    const int per = (nwg + nseg - 1) / nseg;
-    const int w0 = s * per, w1 = (w0 + per < nwg) ? w0 + per : nwg;
-    const float* src = partial + e;
-    float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
-    int w = w0;
-    for (; w + 3 < w1; w += 4) {
-        a0 += src[(size_t)w * E];
-        a1 += src[(size_t)(w + 1) * E];
-        a2 += src[(size_t)(w + 2) * E];
-        a3 += src[(size_t)(w + 3) * E];
-    }
-    for (; w < w1; ++w) a0 += src[(size_t)w * E];
-    seg[(size_t)s * E + e] = (a0 + a1) + (a2 + a3);
-}
-
-// dp reduction, stage 2: dp[l*ldp + m*np + k] += sum_s seg[s][(l*NPh + m)*np + k], m < N/2
-// (or = with assign).
-__global__ void __launch_bounds__(256) k_dp_final(int L, int Nh, int NPh, int np, int nseg,
-                                                  const float* __restrict__ seg, float* __restrict__ dp, int ldp,
-                                                  int assign) {
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+// dp reduction: dp[l*ldp + m*np + k] = (or +=) the sum over workgroups w of
+// partial[w][(l*NPh + m)*np + k], m < N/2.  A block is four waves over 64 consecutive dp
+// entries; wave q sums workgroups [q*nwg/4, (q+1)*nwg/4) with four interleaved accumulators and
+// the four wave sums are added in wave order through LDS: one launch, a fixed summation order
+// (bitwise reproducible), and four waves' worth of loads in flight per entry.
+__global__ void __launch_bounds__(256) k_dp_reduce(int L, int Nh, int NPh, int np, int nwg, size_t E,
+                                                   const float* __restrict__ partial, float* __restrict__ dp, int ldp,
+                                                   int assign) {
+    __shared__ float red[4][64];
+    const int lane = threadIdx.x & 63, wq = threadIdx.x >> 6;
+    const long idx = (long)blockIdx.x * 64 + lane;
     const long total = (long)L * Nh * np;
-    if (idx >= total) return;
-    const int k = idx % np;
-    const long lm = idx / np;
-    const int m = lm % Nh;
-    const int l = lm / Nh;
-    const size_t E = (size_t)L * NPh * np;
-    const float* src = seg + ((size_t)l * NPh + m) * np + k;
+    const bool ok = idx < total;
+    int k = 0, mm = 0, l = 0;
     float acc = 0.f;
-    for (int s = 0; s < nseg; ++s) acc += src[(size_t)s * E];
-    float* d = dp + (size_t)l * ldp + (size_t)m * np + k;
-    *d = assign ? acc : *d + acc;
+    if (ok) {
+        k = (int)(idx % np);
+        const long lm = idx / np;
+        mm = (int)(lm % Nh);
+        l = (int)(lm / Nh);
+        const float* src = partial + ((size_t)l * NPh + mm) * np + k;
+        const int w0 = (int)((long)nwg * wq / 4), w1 = (int)((long)nwg * (wq + 1) / 4);
+        float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+        int w = w0;
+        for (; w + 3 < w1; w += 4) {
+            a0 += src[(size_t)w * E];
+            a1 += src[(size_t)(w + 1) * E];
+            a2 += src[(size_t)(w + 2) * E];
+            a3 += src[(size_t)(w + 3) * E];
+        }
+        for (; w < w1; ++w) a0 += src[(size_t)w * E];
+        acc = (a0 + a1) + (a2 + a3);
+    }
+    red[wq][lane] = acc;
+    __syncthreads();
+    if (wq == 0 && ok) {
+        const float sum = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+        float* d = dp + (size_t)l * ldp + (size_t)mm * np + k;
+        *d = assign ? sum : *d + sum;
+    }
 }
 
 }  // namespace mp
@@ -194,11 +194,10 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
         }
         if (P.waves > 4 && !P.pipe) return MESHPROP_ERR_UNSUPPORTED;  // only the pipelined stage sums 8 waves
         P.C = C;
-        P.nseg = P.grid < 16 ? P.grid : 16;
+
     } else {
         P.C = 0;
         P.pipe = 0;
-        P.nseg = 0;
     }
     size_t off = 0;
     P.off_kind = off;
@@ -218,10 +217,7 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
         off = align256(off + (size_t)L * (P.NP / 2) * P.np * 4 * elem);
         P.off_partial = off;
         off = align256(off + (size_t)P.grid * L * (P.NP / 2) * P.np * sizeof(float));
-        P.off_seg = off;
-        off = align256(off + (size_t)P.nseg * L * (P.NP / 2) * P.np * sizeof(float));
     }
-    P.off_seg = adj ? P.off_seg : off;
     P.total = off;
     return MESHPROP_OK;
 }
@@ -349,14 +345,10 @@ int run_impl(int op, int X, int N, int L, int B, int fft, const int* g0, const i
         if ((rc = check_launch())) return rc;
     }
     if ((op == MESHPROP_OP_BACKDIFF || op == MESHPROP_OP_FWDGRAD) && L > 0 && do_red) {
-        float* seg = (float*)(base + P.off_seg);
-        const long E = (long)L * (P.NP / 2) * P.np;
-        hipLaunchKernelGGL(k_dp_seg, dim3((unsigned)((E + 255) / 256), P.nseg), dim3(256), 0, st, E, P.grid, P.nseg,
-                           (const float*)partial, seg);
-        if ((rc = check_launch())) return rc;
         const long total = (long)L * (N / 2) * P.np;
-        hipLaunchKernelGGL(k_dp_final, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, L, N / 2, P.NP / 2,
-                           P.np, P.nseg, (const float*)seg, dp, ldp, (flags & MESHPROP_DP_ASSIGN) ? 1 : 0);
+        const size_t E = (size_t)L * (P.NP / 2) * P.np;
+        hipLaunchKernelGGL(k_dp_reduce, dim3((unsigned)((total + 63) / 64)), dim3(256), 0, st, L, N / 2, P.NP / 2,
+                           P.np, P.grid, E, (const float*)partial, dp, ldp, (flags & MESHPROP_DP_ASSIGN) ? 1 : 0);
         if ((rc = check_launch())) return rc;
     }
     return MESHPROP_OK;
