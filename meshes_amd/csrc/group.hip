@@ -34,7 +34,7 @@ enum { B_KIND = 1, B_FWD = 2, B_INV = 4, B_DTDP = 8 };
 struct GMesh {
     int what, var, X, N, L, B, fft, LPV, NP, C, np, ns, ldp, lds, ldu, S, build, assign;
     long ntiles, E;
-    long pro_chunks;  // prologue blocks per layer
+    long pro_chunks;  // prologue blocks (of 256 slots) per layer
     const int *g0, *g1;
     const float *p, *s;
     float* dp;
@@ -84,23 +84,26 @@ __global__ void __launch_bounds__(256) k_group_prologue(const unsigned char* __r
         KMesh& M = meshes[e];
         const long local = u - begin[e];
         const int l = (int)(local / M.pro_chunks);
-        const int w = (int)(local % M.pro_chunks) * 256 + threadIdx.x;
-        if (w >= M.NP) continue;
-        const int* g0 = M.g0;
-        const int* g1 = M.g1;
-        const float* p = M.p;
-        const float* sp = M.s;
-        if ((M.build & B_FWD) && (M.build & B_INV))
-            rows_at<float4>(M.X, M.N, M.NP, M.LPV, M.fft, g0, g1, p, M.ldp, sp, M.lds, nullptr,
-                            TAB_FWD | TAB_INV_B, M.kind, M.rows_fwd, M.rows_inv, l, w);
-        else if (M.build & B_FWD)
-            rows_at<float4>(M.X, M.N, M.NP, M.LPV, M.fft, g0, g1, p, M.ldp, sp, M.lds, nullptr, TAB_FWD,
-                            M.kind, M.rows_fwd, nullptr, l, w);
-        else if (M.build & B_INV)
-            rows_at<float4>(M.X, M.N, M.NP, M.LPV, M.fft, g0, g1, p, M.ldp, sp, M.lds, nullptr, TAB_INV,
-                            M.kind, M.rows_inv, nullptr, l, w);
-        if ((M.build & B_DTDP) && w < M.NP / 2)
-            dtdp_at<float2>(M.X, M.N, M.NP, M.fft, g0, g1, p, M.ldp, sp, M.lds, M.dtdp, l, w);
+        const int m = (int)(local % M.pro_chunks) * 256 + threadIdx.x;
+        if (m >= M.NP / 2) continue;
+        // One crossing evaluation per slot for all of the entry's tables.
+        int tabs = 0;
+        float4* ta = nullptr;
+        float4* tb = nullptr;
+        if (M.build & B_FWD) {
+            tabs |= TAB_FWD;
+            ta = M.rows_fwd;
+            if (M.build & B_INV) {
+                tabs |= TAB_INV_B;
+                tb = M.rows_inv;
+            }
+        } else if (M.build & B_INV) {
+            tabs |= TAB_INV;
+            ta = M.rows_inv;
+        }
+        if (M.build & B_DTDP) tabs |= TAB_DTDP;
+        slot_tables_at<float4>(M.X, M.N, M.NP, M.LPV, M.fft, M.g0, M.g1, M.p, M.ldp, M.s, M.lds, nullptr, tabs,
+                               M.kind, ta, tb, M.dtdp, l, m);
     }
 }
 
@@ -131,16 +134,15 @@ __global__ void __launch_bounds__(256) k_group_reduce(const unsigned char* __res
             const float* src = M.partial + ((size_t)l * (M.NP / 2) + m) * M.np + k;
             const size_t E = (size_t)M.E;
             const int s0 = (int)((long)M.S * wq / 4), s1 = (int)((long)M.S * (wq + 1) / 4);
-            float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+            // Eight interleaved accumulators: eight loads in flight per lane (the sums are
+            // latency-bound otherwise: a large mesh has hundreds of slots).
+            float a[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
             int s = s0;
-            for (; s + 3 < s1; s += 4) {
-                a0 += src[(size_t)s * E];
-                a1 += src[(size_t)(s + 1) * E];
-                a2 += src[(size_t)(s + 2) * E];
-                a3 += src[(size_t)(s + 3) * E];
-            }
-            for (; s < s1; ++s) a0 += src[(size_t)s * E];
-            acc = (a0 + a1) + (a2 + a3);
+            for (; s + 7 < s1; s += 8)
+#pragma unroll
+                for (int r = 0; r < 8; ++r) a[r] += src[(size_t)(s + r) * E];
+            for (; s < s1; ++s) a[0] += src[(size_t)s * E];
+            acc = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
         }
         red[wq][lane] = acc;
         __syncthreads();
@@ -420,7 +422,7 @@ int plan_group(meshprop_group& g) {
         if (f.own_tables) off = a256(off + tt);
         f.off_partial = off;
         if (grad && e.L > 0 && f.S > 0) off = a256(off + (size_t)f.S * f.E * sizeof(float));
-        f.pro_units = f.build ? (long)e.L * ((f.NP + 255) / 256) : 0;
+        f.pro_units = f.build ? (long)e.L * ((f.NP / 2 + 255) / 256) : 0;
         const bool assign = (e.flags & MESHPROP_GROUP_DP_ASSIGN) != 0;
         f.red_units = (grad && e.L > 0 && (f.S > 0 || assign)) ? ((long)e.L * (e.N / 2) * f.np + 63) / 64 : 0;
         g.pro_total += f.pro_units;
@@ -505,7 +507,7 @@ void build_image(const meshprop_group& g, unsigned char* ws) {
         m.assign = (e.flags & MESHPROP_GROUP_DP_ASSIGN) ? 1 : 0;
         m.ntiles = f.ntiles;
         m.E = f.E;
-        m.pro_chunks = (f.NP + 255) / 256;
+        m.pro_chunks = (f.NP / 2 + 255) / 256;
         m.g0 = e.geom0;
         m.g1 = e.geom1;
         m.p = e.p;
@@ -615,7 +617,7 @@ int meshprop_group_launch(const meshprop_group* g, void* ws, size_t bytes, void*
         if ((e = hipGetLastError()) != hipSuccess) return hip_status(e);
     }
     if (g->sig & MESHPROP_GROUP_SIG_REDUCE) {
-        hipLaunchKernelGGL(k_group_reduce, dim3(4 * g->ncu), dim3(256), 0, st, (const unsigned char*)w);
+        hipLaunchKernelGGL(k_group_reduce, dim3(8 * g->ncu), dim3(256), 0, st, (const unsigned char*)w);
         if ((e = hipGetLastError()) != hipSuccess) return hip_status(e);
     }
     return MESHPROP_OK;

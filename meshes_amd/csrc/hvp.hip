@@ -306,7 +306,7 @@ int meshprop_hvp(int X, int N, int L, int B, int fft, const int* geom0, const in
     hipStream_t st = (hipStream_t)stream;
     unsigned char* base = (unsigned char*)workspace;
     const float* sp = ns ? s : nullptr;
-    hipLaunchKernelGGL((k_rows<float4>), dim3((P.NP + 255) / 256, L), dim3(256), 0, st, X, N, L, P.NP, P.LPV, fft,
+    hipLaunchKernelGGL((k_rows<float4>), dim3((P.NP / 2 + 255) / 256, L), dim3(256), 0, st, X, N, L, P.NP, P.LPV, fft,
                        geom0, geom1, p, ldp, sp, lds, q, (int)(TAB_INV | TAB_DIFF_ADJ | TAB_DTDP),
                        (int*)(base + P.off_kind), (float4*)(base + P.off_r), (float4*)(base + P.off_d),
                        (float2*)(base + P.off_t1));

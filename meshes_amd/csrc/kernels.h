@@ -72,69 +72,9 @@ __device__ __forceinline__ void row_of(const M2& M, const bool upper, dc& a, dc&
     }
 }
 
-// One thread per (layer, waveguide) of the padded width NP.
 // Row tables are stored k-major inside each layer: row of waveguide w = t*W + k (lane t of a
 // lane group, k-th waveguide of that lane) sits at [l][k][t], so the k-th row load of all
 // lanes reads one contiguous block.
-template <class R>
-__device__ void rows_at(const int X, const int N, const int NP, const int LPV, const int fft,
-                        const int* __restrict__ g0, const int* __restrict__ g1, const float* __restrict__ p,
-                        const int ldp, const float* __restrict__ s, const int lds, const float* __restrict__ dp,
-                        const int tabs, int* __restrict__ kind, R* __restrict__ tab_a, R* __restrict__ tab_b,
-                        const int l, const int w) {
-    if (w == 0 && kind) kind[l] = layer_kind(fft, g0, g1, l, NP);
-    const int Wd = NP / LPV;
-    const size_t o = (size_t)l * NP + (size_t)(w % Wd) * LPV + w / Wd;
-    int m = 0;
-    bool upper = true;
-    const bool act = wg_role(fft, g0, g1, l, w, N, m, upper);
-    const dc one = mk(1, 0), zero = mk(0, 0);
-    if (!act) {
-        if (tabs & (TAB_FWD | TAB_INV)) tab_a[o] = make_row<R>(one, zero);
-        if (tabs & (TAB_DIFF | TAB_DIFF_ADJ)) tab_b[o] = make_row<R>(zero, zero);
-        if (tabs & (TAB_ADJ | TAB_INV_B)) tab_b[o] = make_row<R>(one, zero);
-        return;
-    }
-    const int np = n_phase_of(X), ns = n_split_of(X);
-    const float* pp = p + (size_t)l * ldp + (size_t)m * np;
-    const float* sp = (s && ns) ? s + (size_t)l * lds + (size_t)m * ns : nullptr;
-    M2 T, dT[2];
-    crossing_eval(X, pp, sp, &T, (tabs & (TAB_DIFF | TAB_DIFF_ADJ)) ? dT : nullptr);
-    dc a, b;
-    if (tabs & TAB_FWD) {
-        row_of(T, upper, a, b);
-        tab_a[o] = make_row<R>(a, b);
-    }
-    if (tabs & TAB_DIFF) {
-        const float* dpp = dp + (size_t)l * ldp + (size_t)m * np;
-        M2 D = mscale(dT[0], (double)dpp[0]);
-        if (np > 1) D = madd(D, mscale(dT[1], (double)dpp[1]));
-        row_of(D, upper, a, b);
-        tab_b[o] = make_row<R>(a, b);
-    }
-    if (tabs & TAB_INV) {
-        // Unitary crossings: T^-1 = T^H.  Otherwise the adjugate inverse.
-        const M2 Ti = (X == X_GENERIC) ? minv(T) : mherm(T);
-        row_of(Ti, upper, a, b);
-        tab_a[o] = make_row<R>(a, b);
-    }
-    if (tabs & TAB_ADJ) {
-        row_of(mherm(T), upper, a, b);
-        tab_b[o] = make_row<R>(a, b);
-    }
-    if (tabs & TAB_INV_B) {  // the inverse rows into the second table (one evaluation for both)
-        const M2 Ti = (X == X_GENERIC) ? minv(T) : mherm(T);
-        row_of(Ti, upper, a, b);
-        tab_b[o] = make_row<R>(a, b);
-    }
-    if (tabs & TAB_DIFF_ADJ) {  // rows of (sum_k dp_k dT/dp_k)^H: the tangent of T^H along dp
-        const float* dpp = dp + (size_t)l * ldp + (size_t)m * np;
-        M2 D = mscale(dT[0], (double)dpp[0]);
-        if (np > 1) D = madd(D, mscale(dT[1], (double)dpp[1]));
-        row_of(mherm(D), upper, a, b);
-        tab_b[o] = make_row<R>(a, b);
-    }
-}
 
 // dT/dp_k of every slot (zero for inactive slots), for the in-kernel dJdT -> dp step.
 // Layout: dtdp[(l*np*4 + k*4 + e)*NPh + m] with e = 00, 01, 10, 11 (slot-contiguous), element
@@ -150,44 +90,17 @@ __device__ __forceinline__ float dc_to<float>(const dc z) {
     return (float)z.r;
 }
 
-template <class T>
-__device__ void dtdp_at(const int X, const int N, const int NP, const int fft, const int* __restrict__ g0,
-                        const int* __restrict__ g1, const float* __restrict__ p, const int ldp,
-                        const float* __restrict__ s, const int lds, T* __restrict__ dtdp, const int l, const int m) {
-    const int NPh = NP / 2;
-    const int np = n_phase_of(X), ns = n_split_of(X);
-    // Layout [l][e][m] (e = param*4 + entry) so that consecutive slots are contiguous.
-    T* out = dtdp + (size_t)l * NPh * np * 4 + m;
-    // Upper waveguide of slot m, then test whether the slot is active.
-    int i;
-    if (fft) {
-        const int st = g0[l];
-        bool ok = st >= 0 && st <= 20;
-        const int S = ok ? (1 << st) : 1;
-        i = 2 * S * (m / S) + (m % S);
-        ok = ok && (i + S < N);
-        if (!ok) i = -1;
-    } else {
-        i = 2 * m + (g1[l] & 1);
-    }
-    int mm = -1;
-    bool upper = false;
-    const bool act = (i >= 0) && wg_role(fft, g0, g1, l, i, N, mm, upper) && upper && mm == m;
-    if (!act) {
-        for (int k = 0; k < np * 4; ++k) out[(size_t)k * NPh] = dc_to<T>(mk(0, 0));
-        return;
-    }
-    const float* pp = p + (size_t)l * ldp + (size_t)m * np;
-    const float* sp = (s && ns) ? s + (size_t)l * lds + (size_t)m * ns : nullptr;
-    M2 Tm, dT[2];
-    crossing_eval(X, pp, sp, &Tm, dT);
-    for (int k = 0; k < np; ++k) {
-        out[(size_t)(k * 4 + 0) * NPh] = dc_to<T>(dT[k].a00);
-        out[(size_t)(k * 4 + 1) * NPh] = dc_to<T>(dT[k].a01);
-        out[(size_t)(k * 4 + 2) * NPh] = dc_to<T>(dT[k].a10);
-        out[(size_t)(k * 4 + 3) * NPh] = dc_to<T>(dT[k].a11);
-    }
-}
+// Field element type of a row type (float4 rows: complex64 fields; float2 rows: real fields).
+template <class R>
+struct FieldOfRow;
+template <>
+struct FieldOfRow<float4> {
+    typedef float2 type;
+};
+template <>
+struct FieldOfRow<float2> {
+    typedef float type;
+};
 
 // Contracted second derivatives S_k = sum_m q_m d2T/dp_k dp_m of every slot, same layout as
 // dT/dp (zero for inactive slots).
@@ -234,42 +147,115 @@ __global__ void k_d2tdp(int X, int N, int L, int NP, int fft, const int* __restr
     }
 }
 
-template <class T>
-__global__ void k_dtdp(int X, int N, int L, int NP, int fft, const int* __restrict__ g0, const int* __restrict__ g1,
-                       const float* __restrict__ p, int ldp, const float* __restrict__ s, int lds,
-                       T* __restrict__ dtdp) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    const int l = blockIdx.y;
-    if (l >= L || m >= NP / 2) return;
-    dtdp_at<T>(X, N, NP, fft, g0, g1, p, ldp, s, lds, dtdp, l, m);
+// One thread per (layer, slot m < NP/2): the crossing of the slot is evaluated once (FP64) and
+// both of its waveguides' rows are written to every requested table, with its dT/dp for
+// TAB_DTDP; waveguides that no active crossing touches get identity rows (and inactive slots
+// a zero dT/dp).  Every waveguide of the padded width is written by exactly one thread: slot m
+// holds waveguides (i, j) = (2m + shift%2, +1) in NN layers (slot 0 of an odd-shift layer also
+// writes waveguide 0) and (i, i + 2^s) in FFT layers (all waveguides identity when the
+// stride does not fit the padded width).
+template <class R>
+__device__ void slot_tables_at(const int X, const int N, const int NP, const int LPV, const int fft,
+                               const int* __restrict__ g0, const int* __restrict__ g1, const float* __restrict__ p,
+                               const int ldp, const float* __restrict__ s, const int lds,
+                               const float* __restrict__ dp, const int tabs, int* __restrict__ kind,
+                               R* __restrict__ tab_a, R* __restrict__ tab_b,
+                               typename FieldOfRow<R>::type* __restrict__ dtdp, const int l, const int m) {
+    typedef typename FieldOfRow<R>::type F;
+    const int kd = layer_kind(fft, g0, g1, l, NP);
+    if (m == 0 && kind) kind[l] = kd;
+    const int NPh = NP / 2;
+    int i, j;
+    bool act;
+    if (fft) {
+        if (kd == KIND_IDENTITY) {
+            i = 2 * m;
+            j = i + 1;
+            act = false;
+        } else {
+            const int S = 1 << (kd - 2);
+            i = 2 * S * (m / S) + (m % S);
+            j = i + S;
+            act = j < N;
+        }
+    } else {
+        const int sh = g1[l], ln = g0[l];
+        i = 2 * m + (sh & 1);
+        j = i + 1;
+        act = (m >= (sh >> 1)) && (m < (sh >> 1) + ln) && (j < N);
+    }
+    const int Wd = NP / LPV;
+    auto pos = [&](const int w) { return (size_t)l * NP + (size_t)(w % Wd) * LPV + w / Wd; };
+    const dc one = mk(1, 0), zero = mk(0, 0);
+    auto put = [&](R* tab, const int w, const dc a, const dc b) {
+        if (w < NP) tab[pos(w)] = make_row<R>(a, b);
+    };
+    const int np = n_phase_of(X), ns = n_split_of(X);
+    if (!fft && (g1[l] & 1) && m == 0) {  // waveguide 0 of an odd-shift layer: no crossing
+        if (tabs & (TAB_FWD | TAB_INV)) put(tab_a, 0, one, zero);
+        if (tabs & (TAB_DIFF | TAB_DIFF_ADJ)) put(tab_b, 0, zero, zero);
+        if (tabs & (TAB_ADJ | TAB_INV_B)) put(tab_b, 0, one, zero);
+    }
+    if (!act) {
+        for (int w = i; w <= j; w += j - i) {
+            if (tabs & (TAB_FWD | TAB_INV)) put(tab_a, w, one, zero);
+            if (tabs & (TAB_DIFF | TAB_DIFF_ADJ)) put(tab_b, w, zero, zero);
+            if (tabs & (TAB_ADJ | TAB_INV_B)) put(tab_b, w, one, zero);
+        }
+        if (tabs & TAB_DTDP) {
+            F* out = dtdp + (size_t)l * NPh * np * 4 + m;
+            for (int e = 0; e < np * 4; ++e) out[(size_t)e * NPh] = dc_to<F>(zero);
+        }
+        return;
+    }
+    const float* pp = p + (size_t)l * ldp + (size_t)m * np;
+    const float* sp = (s && ns) ? s + (size_t)l * lds + (size_t)m * ns : nullptr;
+    M2 T, dT[2];
+    const bool need_d = (tabs & (TAB_DIFF | TAB_DIFF_ADJ | TAB_DTDP)) != 0;
+    crossing_eval(X, pp, sp, &T, need_d ? dT : nullptr);
+    // Rows of matrix M: upper waveguide i (M00, M01), lower waveguide j (M11, M10).
+    auto put2 = [&](R* tab, const M2& M) {
+        put(tab, i, M.a00, M.a01);
+        put(tab, j, M.a11, M.a10);
+    };
+    if (tabs & TAB_FWD) put2(tab_a, T);
+    if (tabs & (TAB_DIFF | TAB_DIFF_ADJ)) {
+        const float* dpp = dp + (size_t)l * ldp + (size_t)m * np;
+        M2 D = mscale(dT[0], (double)dpp[0]);
+        if (np > 1) D = madd(D, mscale(dT[1], (double)dpp[1]));
+        if (tabs & TAB_DIFF) put2(tab_b, D);
+        if (tabs & TAB_DIFF_ADJ) put2(tab_b, mherm(D));
+    }
+    if (tabs & (TAB_INV | TAB_INV_B)) {
+        // Unitary crossings: T^-1 = T^H.  Otherwise the adjugate inverse.
+        const M2 Ti = (X == X_GENERIC) ? minv(T) : mherm(T);
+        if (tabs & TAB_INV) put2(tab_a, Ti);
+        if (tabs & TAB_INV_B) put2(tab_b, Ti);
+    }
+    if (tabs & TAB_ADJ) put2(tab_b, mherm(T));
+    if (tabs & TAB_DTDP) {
+        F* out = dtdp + (size_t)l * NPh * np * 4 + m;
+        for (int k2 = 0; k2 < np; ++k2) {
+            out[(size_t)(k2 * 4 + 0) * NPh] = dc_to<F>(dT[k2].a00);
+            out[(size_t)(k2 * 4 + 1) * NPh] = dc_to<F>(dT[k2].a01);
+            out[(size_t)(k2 * 4 + 2) * NPh] = dc_to<F>(dT[k2].a10);
+            out[(size_t)(k2 * 4 + 3) * NPh] = dc_to<F>(dT[k2].a11);
+        }
+    }
 }
 
-// Field element type of a row type (float4 rows: complex64 fields; float2 rows: real fields).
-template <class R>
-struct FieldOfRow;
-template <>
-struct FieldOfRow<float4> {
-    typedef float2 type;
-};
-template <>
-struct FieldOfRow<float2> {
-    typedef float type;
-};
-
 // Prologue of one call: row tables (tabs) and, with TAB_DTDP, the dT/dp table of every slot,
-// all in one launch (thread w of layer l also handles slot w < NP/2).
+// all in one launch, one thread per (layer, slot).
 template <class R>
 __global__ void k_rows(int X, int N, int L, int NP, int LPV, int fft, const int* __restrict__ g0,
                        const int* __restrict__ g1,
                        const float* __restrict__ p, int ldp, const float* __restrict__ s, int lds,
                        const float* __restrict__ dp, int tabs, int* __restrict__ kind, R* __restrict__ tab_a,
                        R* __restrict__ tab_b, typename FieldOfRow<R>::type* __restrict__ dtdp) {
-    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
     const int l = blockIdx.y;
-    if (l >= L || w >= NP) return;
-    rows_at<R>(X, N, NP, LPV, fft, g0, g1, p, ldp, s, lds, dp, tabs, kind, tab_a, tab_b, l, w);
-    if ((tabs & TAB_DTDP) && w < NP / 2)
-        dtdp_at<typename FieldOfRow<R>::type>(X, N, NP, fft, g0, g1, p, ldp, s, lds, dtdp, l, w);
+    if (l >= L || m >= NP / 2) return;
+    slot_tables_at<R>(X, N, NP, LPV, fft, g0, g1, p, ldp, s, lds, dp, tabs, kind, tab_a, tab_b, dtdp, l, m);
 }
 
 // =======================================================================================

@@ -276,7 +276,7 @@ int run_impl(int op, int X, int N, int L, int B, int fft, const int* g0, const i
         if (op == MESHPROP_OP_FWDDIFF) tabs = TAB_FWD | TAB_DIFF;
         if (op == MESHPROP_OP_BACKDIFF) tabs = TAB_INV | (P.uni ? 0 : TAB_ADJ) | TAB_DTDP;
         if (op == MESHPROP_OP_FWDGRAD) tabs = TAB_FWD | TAB_INV_B | TAB_DTDP;
-        dim3 grid((P.NP + 255) / 256, L);
+        dim3 grid((P.NP / 2 + 255) / 256, L);  // one thread per slot
         if (P.real)
             hipLaunchKernelGGL((k_rows<float2>), grid, dim3(256), 0, st, X, N, L, P.NP, P.LPV, fft, g0, g1, p, ldp, sp, lds,
                                (const float*)dp, tabs, kind, (float2*)tab_a, (float2*)tab_b, (float*)dtdp);
