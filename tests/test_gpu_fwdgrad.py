@@ -154,3 +154,22 @@ def test_fwdgrad_eight_wave_workgroups(gpu, N, L, B):
         assert O.rel_l2(_np(dp).reshape(rdp.shape), rdp) < TOL
         assert O.rel_l2(_np(gin), rgin) < TOL
     assert O.rel_l2(_np(res[0][2]), _np(res[1][2])) < 1e-6
+
+
+def test_fwdgrad_several_tiles_per_workgroup(gpu):
+    """More vector tiles than adjoint workgroups (B=9000 at N=256: 1125 tiles over at most 512
+    workgroups), so workgroups add later tiles into their dp partials; against the oracle."""
+    import meshes_amd._lib as Lb
+    N, L, B = 256, 16, 9000
+    g, p, s, u, rng = _setup("mzi", "nn", N, L, B, seed=77)
+    mesh = _mesh(g, "mzi", p, s)
+    y = O.unit_cn(rng, B, N).astype(np.complex64)
+    ub, yb = torch.as_tensor(u).cuda().contiguous(), torch.as_tensor(y).cuda().contiguous()
+    out, gin, dp = torch.empty_like(ub), torch.empty_like(ub), torch.zeros_like(mesh.p_phase)
+    _run(mesh, Lb.OP_FWDGRAD, B, ub, out, gin, dp, yb)
+    torch.cuda.synchronize()
+    p64, s64 = p.astype(np.float64), _f64(s)
+    gout = 2 * (_np(out).astype(np.complex128) - y)
+    rdp, rgin, _ = O.mesh_vjp(g, "mzi", p64, s64, _np(out).astype(np.complex128), gout)
+    assert O.rel_l2(_np(dp).reshape(rdp.shape), rdp) < TOL
+    assert O.rel_l2(_np(gin), rgin) < TOL
