@@ -627,7 +627,14 @@ __device__ __forceinline__ void bwd_tile(const int N, const int l0, const int nl
     const int wstride = C * W * LPV;
     const KP kp = kind + l0;
     const int last = nl - 1;
-    constexpr int PF = 2;  // stage-end slots per lane whose dT/dp is prefetched
+    // Lean form for the wide non-unitary one-vector tiles (two row tables of W rows per layer):
+    // no dT/dp prefetch and the adjoint rows loaded just before their layer, which keeps the
+    // kernel within the registers of two waves per SIMD (measured at N=512, L=512, B=8192:
+    // 7.2 ms against 11.6 ms for the full form at one wave per SIMD; the two-vector tile in
+    // its full form, one wave per SIMD, is faster still at 6.8 ms).
+    constexpr bool kLean = (W >= 8) && !UNI && V == 1;
+    constexpr int PF = kLean ? 0 : 2;  // stage-end slots per lane whose dT/dp is prefetched
+    constexpr int DA = kLean ? 1 : D;  // register sets of adjoint rows
     const Lanes q = lanes_of<V>(LPV, tile);
     T ua[V][W], ga[V][W], ub[V][W], gb[V][W];
     load_tile(ua, u_out, q, N, B, ldu);
@@ -644,7 +651,7 @@ __device__ __forceinline__ void bwd_tile(const int N, const int l0, const int nl
     // Local layers are visited in the order s = 0, 1, ..., as l = last - s.
     const R* rip = rows_inv + (size_t)l0 * NP + q.t;
     const R* rap = rows_adj + (size_t)l0 * NP + q.t;
-    R ri[D][W], ra[D][W];
+    R ri[D][W], ra[DA][W];
     int kk[D];
     auto lrow = [&](const int s) { return (size_t)(s < last ? last - s : 0) * NP; };
 #pragma unroll
@@ -652,7 +659,7 @@ __device__ __forceinline__ void bwd_tile(const int N, const int l0, const int nl
         if (nl > 0) {
             kk[d] = kp[s_clamp(last, d)];
             load_rows(ri[d], rip + lrow(d), LPV);
-            if constexpr (!UNI) load_rows(ra[d], rap + lrow(d), LPV);
+            if constexpr (!UNI && !kLean) load_rows(ra[d], rap + lrow(d), LPV);
         }
     }
 #define MP_BWD_LAYER(KD, U, G, UO, GO, RI, RA, LC)                                                  \
@@ -662,6 +669,9 @@ __device__ __forceinline__ void bwd_tile(const int N, const int l0, const int nl
     else                                                                                        \
         bwd_layer<T, W, V, UNI>(KD, U, G, UO, GO, RI, RA, stage, q, LPV, C, LC);                \
     }
+    // Adjoint rows of local layer SL into set AD (lean form: loaded right before the layer).
+#define MP_ADJ_ROWS(AD, SL)                                                                         \
+    if constexpr (!UNI && kLean) load_rows(ra[AD], rap + lrow(SL), LPV);
     for (int c0 = 0; c0 < nl; c0 += C) {
         const int cl = (C < nl - c0) ? C : nl - c0;
         if (ck.base != nullptr && c0 > 0) {
@@ -672,8 +682,8 @@ __device__ __forceinline__ void bwd_tile(const int N, const int l0, const int nl
                 load_tile(ua, reinterpret_cast<const T*>(ck.base) + (size_t)(e / ck.every) * ck.stride, q, N, B, ldu);
         }
         // Prefetch the dT/dp this wave converts first at the end of the stage.
-        T pre[PF][4 * np];
-        if (wave_u < cl) {
+        T pre[PF > 0 ? PF : 1][4 * np];
+        if (PF > 0 && wave_u < cl) {
             const T* src = dtdp + (size_t)(l0 + last - (c0 + wave_u)) * NPh * np * 4 + lane;
 #pragma unroll
             for (int r = 0; r < PF; ++r)
@@ -690,11 +700,12 @@ __device__ __forceinline__ void bwd_tile(const int N, const int l0, const int nl
                 const int dn = (d + PD) % D;  // register set receiving layer s + d + PD
                 kk[dn] = kp[s_clamp(last, s + d + PD)];
                 load_rows(ri[dn], rip + lrow(s + d + PD), LPV);
-                if constexpr (!UNI) load_rows(ra[dn], rap + lrow(s + d + PD), LPV);
+                if constexpr (!UNI && !kLean) load_rows(ra[dn % DA], rap + lrow(s + d + PD), LPV);
+                MP_ADJ_ROWS(0, s + d)
                 if ((d & 1) == 0)
-                    MP_BWD_LAYER(kk[d], ua, ga, ub, gb, ri[d], ra[d], lc + d)
+                    MP_BWD_LAYER(kk[d], ua, ga, ub, gb, ri[d], ra[d % DA], lc + d)
                 else
-                    MP_BWD_LAYER(kk[d], ub, gb, ua, ga, ri[d], ra[d], lc + d)
+                    MP_BWD_LAYER(kk[d], ub, gb, ua, ga, ri[d], ra[d % DA], lc + d)
             }
         }
         // Remainder of the last stage (cl - lc < D layers): register sets 0.. hold them.
@@ -706,12 +717,13 @@ __device__ __forceinline__ void bwd_tile(const int N, const int l0, const int nl
                     const int dn = (d + PD) % D;
                     kk[dn] = kp[s_clamp(last, c0 + lc + d + PD)];
                     load_rows(ri[dn], rip + lrow(c0 + lc + d + PD), LPV);
-                    if constexpr (!UNI) load_rows(ra[dn], rap + lrow(c0 + lc + d + PD), LPV);
+                    if constexpr (!UNI && !kLean) load_rows(ra[dn % DA], rap + lrow(c0 + lc + d + PD), LPV);
                 }
+                MP_ADJ_ROWS(0, c0 + lc + d)
                 if ((d & 1) == 0)
-                    MP_BWD_LAYER(kk[d], ua, ga, ub, gb, ri[d], ra[d], lc + d)
+                    MP_BWD_LAYER(kk[d], ua, ga, ub, gb, ri[d], ra[d % DA], lc + d)
                 else
-                    MP_BWD_LAYER(kk[d], ub, gb, ua, ga, ri[d], ra[d], lc + d)
+                    MP_BWD_LAYER(kk[d], ub, gb, ua, ga, ri[d], ra[d % DA], lc + d)
             }
         }
         if (left & 1) {
@@ -744,6 +756,7 @@ __device__ __forceinline__ void bwd_tile(const int N, const int l0, const int nl
         __syncthreads();
     }
 #undef MP_BWD_LAYER
+#undef MP_ADJ_ROWS
     if (u_in != nullptr) store_tile(ua, u_in, q, N, B, ldu);
     if (g_in != nullptr) store_tile(ga, g_in, q, N, B, ldu);
 }

@@ -243,11 +243,12 @@ def test_c2_full_size(gpu):
     assert abs(lhs - rhs) <= 1e-4 * max(abs(rhs), 1e-3)
 
 
-@pytest.mark.parametrize("N,K", [(64, 8), (256, 8), (256, 5)])
+@pytest.mark.parametrize("N,K", [(64, 8), (256, 8), (256, 5), (512, 8)])
 def test_lossy_checkpointed_adjoint(gpu, N, K):
     """Lossy generic MZI (non-unitary): backdiff with forward checkpoints every K layers equals
     the oracle's stored-field adjoint.  N=256 runs the pipelined adjoint stage (C=2): K=8 in one
-    launch, K=5 (not a multiple of the stage) one launch per segment."""
+    launch, K=5 (not a multiple of the stage) one launch per segment; N=512 with this batch runs
+    the one-vector W=8 tile in its lean form."""
     import meshes_amd._lib as Lb
     X, L, B = "mzi_generic", 40, 50
     g, p, s, u, rng = _setup(X, "nn", N, L, B, seed=31, lossy=True)
