@@ -13,7 +13,16 @@ namespace mp {
 // =======================================================================================
 // Prologue: row tables
 // =======================================================================================
-enum { TAB_FWD = 1, TAB_DIFF = 2, TAB_INV = 4, TAB_ADJ = 8, TAB_DTDP = 16, TAB_DIFF_ADJ = 32, TAB_INV_B = 64 };
+enum {
+    TAB_FWD = 1,
+    TAB_DIFF = 2,
+    TAB_INV = 4,
+    TAB_ADJ = 8,
+    TAB_DTDP = 16,
+    TAB_DIFF_ADJ = 32,
+    TAB_INV_B = 64,
+    TAB_KUNI = 128
+};
 
 // Slot and role of waveguide w in layer l.  Returns whether an active crossing acts on it.
 // NN: gpu/gmem.cu:236-240 (slot m couples 2m + shift%2, +1; active for
@@ -89,6 +98,33 @@ template <>
 __device__ __forceinline__ float dc_to<float>(const dc z) {
     return (float)z.r;
 }
+
+// K-form coefficients of unitary crossings (TAB_KUNI, written into the dT/dp buffer):
+// K_k = T^H dT/dp_k is anti-Hermitian, [[i a, c], [-conj(c), i d]], stored per (layer,
+// parameter, slot) as float4 (a, d, c.re, c.im); for real (orthogonal) crossings K_k =
+// [[0, c], [-c, 0]] and the float c is stored.  Layout: ktab[(l*np + k)*NPh + m].
+template <class T>
+struct KOf;
+template <>
+struct KOf<float2> {
+    typedef float4 type;
+};
+template <>
+struct KOf<float> {
+    typedef float type;
+};
+// Conversion coefficients per slot: the K-form (one element per parameter) or dT/dp (four
+// entries per parameter); both tables index [(l*NC + e)*NPh + m].
+template <class T, bool KF>
+struct CoefOf {
+    typedef T type;
+    static constexpr int per_param = 4;
+};
+template <class T>
+struct CoefOf<T, true> {
+    typedef typename KOf<T>::type type;
+    static constexpr int per_param = 1;
+};
 
 // Field element type of a row type (float4 rows: complex64 fields; float2 rows: real fields).
 template <class R>
@@ -206,12 +242,17 @@ __device__ void slot_tables_at(const int X, const int N, const int NP, const int
             F* out = dtdp + (size_t)l * NPh * np * 4 + m;
             for (int e = 0; e < np * 4; ++e) out[(size_t)e * NPh] = dc_to<F>(zero);
         }
+        if (tabs & TAB_KUNI) {
+            typedef typename KOf<F>::type K;
+            K* out = reinterpret_cast<K*>(dtdp) + (size_t)l * np * NPh + m;
+            for (int k2 = 0; k2 < np; ++k2) out[(size_t)k2 * NPh] = K{};
+        }
         return;
     }
     const float* pp = p + (size_t)l * ldp + (size_t)m * np;
     const float* sp = (s && ns) ? s + (size_t)l * lds + (size_t)m * ns : nullptr;
     M2 T, dT[2];
-    const bool need_d = (tabs & (TAB_DIFF | TAB_DIFF_ADJ | TAB_DTDP)) != 0;
+    const bool need_d = (tabs & (TAB_DIFF | TAB_DIFF_ADJ | TAB_DTDP | TAB_KUNI)) != 0;
     crossing_eval(X, pp, sp, &T, need_d ? dT : nullptr);
     // Rows of matrix M: upper waveguide i (M00, M01), lower waveguide j (M11, M10).
     auto put2 = [&](R* tab, const M2& M) {
@@ -240,6 +281,19 @@ __device__ void slot_tables_at(const int X, const int N, const int NP, const int
             out[(size_t)(k2 * 4 + 1) * NPh] = dc_to<F>(dT[k2].a01);
             out[(size_t)(k2 * 4 + 2) * NPh] = dc_to<F>(dT[k2].a10);
             out[(size_t)(k2 * 4 + 3) * NPh] = dc_to<F>(dT[k2].a11);
+        }
+    }
+    if (tabs & TAB_KUNI) {
+        // Anti-Hermitian part of T^H dT/dp (equal to the whole for a unitary T).
+        typedef typename KOf<F>::type K;
+        K* out = reinterpret_cast<K*>(dtdp) + (size_t)l * np * NPh + m;
+        for (int k2 = 0; k2 < np; ++k2) {
+            const M2 Kp = mmul(mherm(T), dT[k2]);
+            const double cr = 0.5 * (Kp.a01.r - Kp.a10.r), ci = 0.5 * (Kp.a01.i + Kp.a10.i);
+            if constexpr (sizeof(K) == sizeof(float4))
+                out[(size_t)k2 * NPh] = K{(float)Kp.a00.i, (float)Kp.a11.i, (float)cr, (float)ci};
+            else
+                out[(size_t)k2 * NPh] = K((float)cr);
         }
     }
 }
@@ -474,6 +528,12 @@ __device__ __forceinline__ float re_dot<float>(const float a, const float b) {
     return a * b;
 }
 
+// Lane-group fold of a K-form A row: only .x carries data (real fields carry none).
+__device__ __forceinline__ float2 fold_im(const float2 a, const int src) {
+    return make_float2(a.x + lane_read(a.x, src), a.y);
+}
+__device__ __forceinline__ float fold_im(const float a, const int) { return a; }
+
 template <class T, int W, int V, bool UNI>
 __device__ __forceinline__ void bwd_layer(const int kd, const T (&u)[V][W], const T (&g)[V][W], T (&uo)[V][W],
                                           T (&go)[V][W], const typename RowOf<T>::type (&ri)[W],
@@ -485,13 +545,17 @@ __device__ __forceinline__ void bwd_layer(const int kd, const T (&u)[V][W], cons
         accA[k] = fzero(T());
         accB[k] = fzero(T());
     }
-    BwdStep<T, W, V> st{u, g, uo, go, ri, ra, accA, accB, q.lane};
+    // Unitary crossings take the K-form rows (A_k.x = Im part, A_k.y = 0; no A for real fields).
+    BwdStep<T, W, V, UNI> st{u, g, uo, go, ri, ra, accA, accB, q.lane};
     dispatch_kind<W>(kd, st);
     // Fold lane groups (groups hold the same waveguides of different vectors).
     for (int off = LPV; off < 64; off <<= 1) {
 #pragma unroll
         for (int k = 0; k < W; ++k) {
-            accA[k] = fadd(accA[k], lane_read(accA[k], q.lane ^ off));
+            if constexpr (!UNI)
+                accA[k] = fadd(accA[k], lane_read(accA[k], q.lane ^ off));
+            else
+                accA[k] = fold_im(accA[k], q.lane ^ off);
             accB[k] = fadd(accB[k], lane_read(accB[k], q.lane ^ off));
         }
     }
@@ -512,10 +576,36 @@ __device__ __forceinline__ int s_clamp(const int last, const int s) { return s <
 // grouped kernel, the Hessian-vector product) runs workgroups of at most 256 threads.
 constexpr int kStageWaves = 4;
 
-template <class T, int W, int NPAR>
+// K-form dp of one parameter from the summed rows of the slot's waveguides i (upper) and j:
+// -a Im Q00 - d Im Q11 + Re(c (Q01 - conj(Q10))), with Q01 = B_i and Q10 = B_j.
+__device__ __forceinline__ float kdot(const float4 si, const float4 sj, const float4 c) {
+    const float rr = si.z - sj.z, ri = si.w + sj.w;
+    return fmaf(-c.w, ri, fmaf(c.z, rr, fmaf(-c.y, sj.x, -c.x * si.x)));
+}
+__device__ __forceinline__ float kdot(const float2 si, const float2 sj, const float c) { return c * (si.y - sj.y); }
+
+// dp of each parameter of a slot from its summed rows: K-form or dJdT . dT/dp
+// (dJdT00 = A_i, dJdT01 = B_i, dJdT11 = A_j, dJdT10 = B_j).
+template <class T, int NPAR, bool KF, class R, class CT>
+__device__ __forceinline__ void slot_dp(const R si, const R sj, const CT* dd, float (&val)[NPAR]) {
+#pragma unroll
+    for (int k = 0; k < NPAR; ++k) {
+        if constexpr (KF) {
+            val[k] = kdot(si, sj, dd[k]);
+        } else {
+            val[k] = re_dot(row_a(si), dd[k * 4 + 0]);
+            val[k] += re_dot(row_b(si), dd[k * 4 + 1]);
+            val[k] += re_dot(row_b(sj), dd[k * 4 + 2]);
+            val[k] += re_dot(row_a(sj), dd[k * 4 + 3]);
+        }
+    }
+}
+
+template <class T, int W, int NPAR, bool KF = false>
 __device__ __forceinline__ void convert_slot(const typename RowOf<T>::type* s0, const int wstride, const int nw,
                                              const int LPV, const int NP, float* pout, const int kd, const int m,
-                                             const T (&dd)[4 * NPAR], const bool first_tile) {
+                                             const typename CoefOf<T, KF>::type (&dd)[NPAR * CoefOf<T, KF>::per_param],
+                                             const bool first_tile) {
     typedef typename RowOf<T>::type R;
     int i, j;
     if (kd <= 1) {
@@ -552,15 +642,8 @@ __device__ __forceinline__ void convert_slot(const typename RowOf<T>::type* s0, 
             si = row_add(si, ri_[wv]);
             sj = row_add(sj, rj_[wv]);
         }
-    // dJdT00 = A_i, dJdT01 = B_i, dJdT11 = A_j, dJdT10 = B_j
     float val[NPAR];
-#pragma unroll
-    for (int k = 0; k < NPAR; ++k) {
-        val[k] = re_dot(row_a(si), dd[k * 4 + 0]);
-        val[k] += re_dot(row_b(si), dd[k * 4 + 1]);
-        val[k] += re_dot(row_b(sj), dd[k * 4 + 2]);
-        val[k] += re_dot(row_a(sj), dd[k * 4 + 3]);
-    }
+    slot_dp<T, NPAR, KF>(si, sj, dd, val);
     if constexpr (NPAR == 2) {
         float2* o2 = reinterpret_cast<float2*>(outp);
         if (first_tile) {
@@ -572,15 +655,6 @@ __device__ __forceinline__ void convert_slot(const typename RowOf<T>::type* s0, 
     } else {
         outp[0] = first_tile ? val[0] : outp[0] + val[0];
     }
-}
-
-// dT/dp of slot m of (absolute) layer la; table layout [layer][param*4 + entry][slot].
-template <class T>
-__device__ __forceinline__ void fetch_dtdp(const T* dtdp, const int NPh, const int np, const int la, const int m,
-                                           T (&dd)[8]) {
-    const T* src = dtdp + (size_t)la * NPh * np * 4 + m;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) dd[e] = (e < np * 4) ? src[e * NPh] : fzero(T());
 }
 
 // Backdiff workgroups hold at most four waves: measured on MI355X, eight-wave workgroups
@@ -618,6 +692,10 @@ __device__ __forceinline__ void bwd_tile(const int N, const int l0, const int nl
                                          const Ckpt ck = Ckpt{nullptr, 0, 0}) {
     typedef typename RowOf<T>::type R;
     constexpr int np = NPhase<T>::value;
+    // Conversion coefficients: K-form for unitary crossings, dT/dp otherwise.
+    typedef typename CoefOf<T, UNI>::type CT;
+    constexpr int NC = np * CoefOf<T, UNI>::per_param;
+    const CT* ctab = reinterpret_cast<const CT*>(dtdp);
     constexpr int D = BwdDepth<W>::value;  // row register sets
     constexpr int PD = D / 2;              // prefetch distance in layers
     const int nw = blockDim.x >> 6;
@@ -681,15 +759,15 @@ __device__ __forceinline__ void bwd_tile(const int N, const int l0, const int nl
             if (e % ck.every == 0)
                 load_tile(ua, reinterpret_cast<const T*>(ck.base) + (size_t)(e / ck.every) * ck.stride, q, N, B, ldu);
         }
-        // Prefetch the dT/dp this wave converts first at the end of the stage.
-        T pre[PF > 0 ? PF : 1][4 * np];
+        // Prefetch the coefficients this wave converts first at the end of the stage.
+        CT pre[PF > 0 ? PF : 1][NC];
         if (PF > 0 && wave_u < cl) {
-            const T* src = dtdp + (size_t)(l0 + last - (c0 + wave_u)) * NPh * np * 4 + lane;
+            const CT* src = ctab + (size_t)(l0 + last - (c0 + wave_u)) * NPh * NC + lane;
 #pragma unroll
             for (int r = 0; r < PF; ++r)
                 if (lane + 64 * r < NPh) {
 #pragma unroll
-                    for (int e = 0; e < 4 * np; ++e) pre[r][e] = src[e * NPh + 64 * r];
+                    for (int e = 0; e < NC; ++e) pre[r][e] = src[e * NPh + 64 * r];
                 }
         }
         int lc = 0;
@@ -737,20 +815,20 @@ __device__ __forceinline__ void bwd_tile(const int N, const int l0, const int nl
             const int kdl = kind[la];
             const R* s0 = stage + lcc * W * LPV;
             float* pout = my_partial + (size_t)la * NPh * np;
-            const T* src = dtdp + (size_t)la * NPh * np * 4;
+            const CT* src = ctab + (size_t)la * NPh * NC;
             int m0 = lane;
             if (lcc == wave_u) {
 #pragma unroll
                 for (int r = 0; r < PF; ++r)
                     if (lane + 64 * r < NPh)
-                        convert_slot<T, W, np>(s0, wstride, nw, LPV, NP, pout, kdl, lane + 64 * r, pre[r], first);
+                        convert_slot<T, W, np, UNI>(s0, wstride, nw, LPV, NP, pout, kdl, lane + 64 * r, pre[r], first);
                 m0 = lane + 64 * PF;
             }
             for (int m = m0; m < NPh; m += 64) {
-                T dd[4 * np];
+                CT dd[NC];
 #pragma unroll
-                for (int e = 0; e < 4 * np; ++e) dd[e] = src[e * NPh + m];
-                convert_slot<T, W, np>(s0, wstride, nw, LPV, NP, pout, kdl, m, dd, first);
+                for (int e = 0; e < NC; ++e) dd[e] = src[e * NPh + m];
+                convert_slot<T, W, np, UNI>(s0, wstride, nw, LPV, NP, pout, kdl, m, dd, first);
             }
         }
         __syncthreads();
@@ -774,16 +852,16 @@ __device__ __forceinline__ void bwd_tile(const int N, const int l0, const int nl
 // NW is the most waves a kernel instance sums (its item holds one row pair per wave).
 constexpr int kPipeWaves = 8;
 
-template <class T, int NW>
+template <class T, int NW, bool KF>
 struct ConvItem {
     typename RowOf<T>::type a[NW], b[NW];
-    T dd[4 * NPhase<T>::value];
+    typename CoefOf<T, KF>::type dd[NPhase<T>::value * CoefOf<T, KF>::per_param];
     float* out;
     int state;  // 0: nothing, 1: inactive slot (zero), 2: convert
 };
 
-template <class T, int W, int NW, class KP>
-__device__ __forceinline__ void conv_issue(ConvItem<T, NW>& it, const typename RowOf<T>::type* s0, const int wstride,
+template <class T, int W, int NW, bool KF, class KP>
+__device__ __forceinline__ void conv_issue(ConvItem<T, NW, KF>& it, const typename RowOf<T>::type* s0, const int wstride,
                                            const int nw, const int lpv_sh, const int NP, const int la, KP kind,
                                            const T* __restrict__ dtdp, float* __restrict__ part, const unsigned m) {
     constexpr int np = NPhase<T>::value;
@@ -817,13 +895,15 @@ __device__ __forceinline__ void conv_issue(ConvItem<T, NW>& it, const typename R
             it.b[wv] = s0[wv * wstride + oj];
         }
     // One uniform base per table row, one 32-bit lane offset for all of them.
-    const T* src = dtdp + (size_t)la * NPh * np * 4;
+    typedef typename CoefOf<T, KF>::type CT;
+    constexpr int NC = np * CoefOf<T, KF>::per_param;
+    const CT* src = reinterpret_cast<const CT*>(dtdp) + (size_t)la * NPh * NC;
 #pragma unroll
-    for (int e = 0; e < 4 * np; ++e) it.dd[e] = (src + (size_t)e * NPh)[m];
+    for (int e = 0; e < NC; ++e) it.dd[e] = (src + (size_t)e * NPh)[m];
 }
 
-template <class T, int NW>
-__device__ __forceinline__ void conv_finish(const ConvItem<T, NW>& it, const int nw, const bool first_tile) {
+template <class T, int NW, bool KF>
+__device__ __forceinline__ void conv_finish(const ConvItem<T, NW, KF>& it, const int nw, const bool first_tile) {
     typedef typename RowOf<T>::type R;
     constexpr int np = NPhase<T>::value;
     if (it.state == 0) return;
@@ -840,15 +920,8 @@ __device__ __forceinline__ void conv_finish(const ConvItem<T, NW>& it, const int
             si = row_add(si, it.a[wv]);
             sj = row_add(sj, it.b[wv]);
         }
-    // dJdT00 = A_i, dJdT01 = B_i, dJdT11 = A_j, dJdT10 = B_j
     float val[np];
-#pragma unroll
-    for (int k = 0; k < np; ++k) {
-        val[k] = re_dot(row_a(si), it.dd[k * 4 + 0]);
-        val[k] += re_dot(row_b(si), it.dd[k * 4 + 1]);
-        val[k] += re_dot(row_b(sj), it.dd[k * 4 + 2]);
-        val[k] += re_dot(row_a(sj), it.dd[k * 4 + 3]);
-    }
+    slot_dp<T, np, KF>(si, sj, it.dd, val);
     if constexpr (np == 2) {
         float2* o2 = reinterpret_cast<float2*>(it.out);
         if (first_tile) {
@@ -919,7 +992,7 @@ __device__ __forceinline__ void bwd_tile_pipe(const int N, const int l0, const i
     // NP is a power of two (a power-of-two lane count times W): divisions by LPV and NP/2 are shifts.
     const int lpv_sh = __builtin_ctz(LPV), nph_sh = __builtin_ctz(NPh);
     // Item k of this thread (none when k is past the previous stage's items).
-    ConvItem<T, NW> item;
+    ConvItem<T, NW, UNI> item;
     item.state = 0;
     auto issue = [&](const int k) {
         item.state = 0;
@@ -928,7 +1001,7 @@ __device__ __forceinline__ void bwd_tile_pipe(const int N, const int l0, const i
             const int lcc = wbase >> nph_sh;
             const int la = l0 + last - (pc0 + lcc);
             const unsigned m = (unsigned)((wbase & (NPh - 1)) + (threadIdx.x & 63));
-            conv_issue<T, W, NW, KP>(item, pbuf + lcc * W * LPV, wstride, nw, lpv_sh, NP, la, kind, dtdp, my_partial,
+            conv_issue<T, W, NW, UNI, KP>(item, pbuf + lcc * W * LPV, wstride, nw, lpv_sh, NP, la, kind, dtdp, my_partial,
                                      m);
         }
     };

@@ -270,12 +270,13 @@ int run_impl(int op, int X, int N, int L, int B, int fft, const int* g0, const i
     const bool do_main = !(flags & (MESHPROP_ONLY_PROLOGUE | MESHPROP_ONLY_REDUCE));
     const bool do_red = !(flags & (MESHPROP_ONLY_PROLOGUE | MESHPROP_SKIP_REDUCE | MESHPROP_ONLY_FORWARD));
     if (L > 0 && do_pro) {
-        // One launch: row tables and, for the adjoint ops, the dT/dp table.
+        // One launch: row tables and, for the adjoint ops, the conversion table (K-form for
+        // unitary crossings, dT/dp otherwise).
         int tabs = 0;
         if (op == MESHPROP_OP_FWDPROP) tabs = TAB_FWD;
         if (op == MESHPROP_OP_FWDDIFF) tabs = TAB_FWD | TAB_DIFF;
-        if (op == MESHPROP_OP_BACKDIFF) tabs = TAB_INV | (P.uni ? 0 : TAB_ADJ) | TAB_DTDP;
-        if (op == MESHPROP_OP_FWDGRAD) tabs = TAB_FWD | TAB_INV_B | TAB_DTDP;
+        if (op == MESHPROP_OP_BACKDIFF) tabs = TAB_INV | (P.uni ? TAB_KUNI : (TAB_ADJ | TAB_DTDP));
+        if (op == MESHPROP_OP_FWDGRAD) tabs = TAB_FWD | TAB_INV_B | TAB_KUNI;
         dim3 grid((P.NP / 2 + 255) / 256, L);  // one thread per slot
         if (P.real)
             hipLaunchKernelGGL((k_rows<float2>), grid, dim3(256), 0, st, X, N, L, P.NP, P.LPV, fft, g0, g1, p, ldp, sp, lds,

@@ -87,6 +87,12 @@ __device__ __forceinline__ float2 cjfma(const float2 g, const float2 x, float2 a
     return acc;
 }
 __device__ __forceinline__ float cjfma(const float g, const float x, float acc) { return fmaf(g, x, acc); }
+// acc.x + Im(conj(g) * x) (complex; .y is left alone); real fields have no imaginary part.
+__device__ __forceinline__ float2 cjim(const float2 g, const float2 x, float2 acc) {
+    acc.x = fmaf(-g.y, x.x, fmaf(g.x, x.y, acc.x));
+    return acc;
+}
+__device__ __forceinline__ float cjim(const float, const float, const float acc) { return acc; }
 
 __device__ __forceinline__ float4 pack_row(const float2 a, const float2 b) { return make_float4(a.x, a.y, b.x, b.y); }
 __device__ __forceinline__ float2 pack_row(const float a, const float b) { return make_float2(a, b); }
@@ -287,7 +293,13 @@ struct FwdDiffStep {
 
 // Reverse step through one layer: x = Rinv u (reconstructed layer input),
 // dJdT rows A_k += conj(g_k) x_k, B_k += conj(g_k) x_partner(k), then g <- Radj g.
-template <class T, int W, int V>
+// KF (unitary crossings): the rows are taken against the layer-input adjoint g' = T^H g
+// instead, and only what dp needs is kept: A_k.x += Im(conj(g'_k) x_k) (A_k.y stays 0; real
+// fields keep no A) and B_k += conj(g'_k) x_partner(k).  With g = T g' the gradient is
+// dJ/dp = Re sum_rc K_rc Q_rc, Q_rc = sum_b conj(g'_r) x_c, K = T^H dT/dp anti-Hermitian,
+// which needs Im Q00, Im Q11 and Q01 - conj(Q10) only: 12 FMA per crossing and vector
+// instead of 16.
+template <class T, int W, int V, bool KF = false>
 struct BwdStep {
     typedef typename RowOf<T>::type R;
     const T (&u)[V][W];
@@ -304,14 +316,24 @@ struct BwdStep {
         copy_state(u, uo);
         copy_state(g, go);
     }
+    // Accumulate waveguide k's dJdT row from its reconstructed input xk, its partner's xp and
+    // (KF) its layer-input adjoint gin, else its layer-output adjoint gout.
+    __device__ __forceinline__ void acc(const int k, const T gout, const T gin, const T xk, const T xp) {
+        if constexpr (KF) {
+            accA[k] = cjim(gin, xk, accA[k]);
+            accB[k] = cjfma(gin, xp, accB[k]);
+        } else {
+            accA[k] = cjfma(gout, xk, accA[k]);
+            accB[k] = cjfma(gout, xp, accB[k]);
+        }
+    }
     __device__ __forceinline__ void pair(const int v, const int k, const int j) {
         const T xk = rapply(ri[k], u[v][k], u[v][j]), xj = rapply(ri[j], u[v][j], u[v][k]);
-        accA[k] = cjfma(g[v][k], xk, accA[k]);
-        accB[k] = cjfma(g[v][k], xj, accB[k]);
-        accA[j] = cjfma(g[v][j], xj, accA[j]);
-        accB[j] = cjfma(g[v][j], xk, accB[j]);
-        go[v][k] = rapply(ra[k], g[v][k], g[v][j]);
-        go[v][j] = rapply(ra[j], g[v][j], g[v][k]);
+        const T gk = rapply(ra[k], g[v][k], g[v][j]), gj = rapply(ra[j], g[v][j], g[v][k]);
+        acc(k, g[v][k], gk, xk, xj);
+        acc(j, g[v][j], gj, xj, xk);
+        go[v][k] = gk;
+        go[v][j] = gj;
         uo[v][k] = xk;
         uo[v][j] = xj;
     }
@@ -338,10 +360,8 @@ struct BwdStep {
                 const T hx = lane_next(uo[v][0]), lx = lane_prev(uo[v][W - 1]);
 #pragma unroll
                 for (int k = 1; k + 1 < W; k += 2) pair(v, k, k + 1);
-                accA[W - 1] = cjfma(g[v][W - 1], uo[v][W - 1], accA[W - 1]);
-                accB[W - 1] = cjfma(g[v][W - 1], hx, accB[W - 1]);
-                accA[0] = cjfma(g[v][0], uo[v][0], accA[0]);
-                accB[0] = cjfma(g[v][0], lx, accB[0]);
+                acc(W - 1, g[v][W - 1], go[v][W - 1], uo[v][W - 1], hx);
+                acc(0, g[v][0], go[v][0], uo[v][0], lx);
             }
         } else {
             const int src = lane ^ xm;
@@ -355,10 +375,7 @@ struct BwdStep {
 #pragma unroll
             for (int v = 0; v < V; ++v)
 #pragma unroll
-                for (int k = 0; k < W; ++k) {
-                    accA[k] = cjfma(g[v][k], uo[v][k], accA[k]);
-                    accB[k] = cjfma(g[v][k], lane_read(uo[v][k], src), accB[k]);
-                }
+                for (int k = 0; k < W; ++k) acc(k, g[v][k], go[v][k], uo[v][k], lane_read(uo[v][k], src));
         }
     }
 };
