@@ -50,18 +50,29 @@ static void go(const Plan& P, const int* kind, const void* ri, const void* ra, c
     go2<T, W, V, 0>(P, kind, ri, ra, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in, g_in, ldu, st, ck);
 }
 
+#if MP_ILP_SET
+int launch_bwd_ilp(const Plan& P, const int* kind, const void* ri, const void* ra, const void* dtdp, float* partial,
+                   size_t ps, int l0, int nl, const void* u_out, const void* g_out, const void* target, void* u_in,
+                   void* g_in, int ldu, hipStream_t st, Ckpt ck) {
+#else
 int launch_bwd_any(const Plan& P, const int* kind, const void* ri, const void* ra, const void* dtdp, float* partial,
                    size_t ps, int l0, int nl, const void* u_out, const void* g_out, const void* target, void* u_in,
                    void* g_in, int ldu, hipStream_t st, Ckpt ck) {
-#define MP_CALL(W_, V_)                                                                                         \
-    if constexpr (diff_tile_ok(W_, V_)) {                                                                       \
-        if (P.real)                                                                                             \
-            go<float, W_, V_>(P, kind, ri, ra, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in, g_in, ldu, \
-                              st, ck);                                                                              \
-        else                                                                                                    \
-            go<float2, W_, V_>(P, kind, ri, ra, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in, g_in,    \
-                               ldu, st, ck);                                                                        \
-        return MESHPROP_OK;                                                                                     \
+#endif
+#define MP_CALL(W_, V_)                                                                                           \
+    if constexpr (diff_tile_ok(W_, V_)) {                                                                         \
+        if constexpr (ilp_tile(W_, V_) == (MP_ILP_SET != 0)) {                                                    \
+            if (P.real)                                                                                           \
+                go<float, W_, V_>(P, kind, ri, ra, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in, g_in,   \
+                                  ldu, st, ck);                                                                   \
+            else                                                                                                  \
+                go<float2, W_, V_>(P, kind, ri, ra, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in, g_in,  \
+                                   ldu, st, ck);                                                                  \
+            return MESHPROP_OK;                                                                                   \
+        } else if constexpr (!MP_ILP_SET) {                                                                       \
+            return launch_bwd_ilp(P, kind, ri, ra, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in, g_in,   \
+                                  ldu, st, ck);                                                                   \
+        }                                                                                                         \
     }
     MP_FOR_WV(P.W, P.V, MP_CALL)
 #undef MP_CALL

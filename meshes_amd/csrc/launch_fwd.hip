@@ -13,15 +13,24 @@ static void go(const Plan& P, const int* kind, const void* rows, int l0, int nl,
                        kind, (const R*)rows, (const T*)u_in, (T*)u_out, ldu, ck);
 }
 
+#if MP_ILP_SET
+int launch_fwd_ilp(const Plan& P, const int* kind, const void* rows, int l0, int nl, const void* u_in, void* u_out,
+                   int ldu, hipStream_t st, Ckpt ck) {
+#else
 int launch_fwd_any(const Plan& P, const int* kind, const void* rows, int l0, int nl, const void* u_in, void* u_out,
                    int ldu, hipStream_t st, Ckpt ck) {
-#define MP_CALL(W_, V_)                                                        \
-    if constexpr (fwd_tile_ok(W_, V_)) {                                       \
-        if (P.real)                                                            \
-            go<float, W_, V_>(P, kind, rows, l0, nl, u_in, u_out, ldu, st, ck);  \
-        else                                                                   \
-            go<float2, W_, V_>(P, kind, rows, l0, nl, u_in, u_out, ldu, st, ck);  \
-        return MESHPROP_OK;                                                    \
+#endif
+#define MP_CALL(W_, V_)                                                          \
+    if constexpr (fwd_tile_ok(W_, V_)) {                                         \
+        if constexpr (ilp_tile(W_, V_) == (MP_ILP_SET != 0)) {                   \
+            if (P.real)                                                          \
+                go<float, W_, V_>(P, kind, rows, l0, nl, u_in, u_out, ldu, st, ck);  \
+            else                                                                 \
+                go<float2, W_, V_>(P, kind, rows, l0, nl, u_in, u_out, ldu, st, ck); \
+            return MESHPROP_OK;                                                  \
+        } else if constexpr (!MP_ILP_SET) {                                      \
+            return launch_fwd_ilp(P, kind, rows, l0, nl, u_in, u_out, ldu, st, ck); \
+        }                                                                        \
     }
     MP_FOR_WV(P.W, P.V, MP_CALL)
 #undef MP_CALL

@@ -1,0 +1,4 @@
+// launch_fwd_ilp.hip -- the fwdprop tiles of ilp_tile() (tiles.h), built with the max-ilp
+// machine scheduler (Makefile).
+#define MP_ILP_SET 1
+#include "launch_fwd.hip"

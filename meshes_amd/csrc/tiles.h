@@ -21,3 +21,16 @@
         case 16 * 16 + 8: { CALL(16, 8) } break; \
         default: break;                          \
     }
+
+// Tiles whose fwdprop and backdiff kernels are compiled with LLVM's max-ilp machine scheduler
+// (launch_fwd_ilp.hip, launch_bwd_ilp.hip; the flag is per translation unit).  Measured at
+// N=256, L=256, B=4096 (two waves per SIMD): 85.6 / 320 us for fwdprop / backdiff against
+// 89.2 / 329 us with the default scheduler.  Applied to every tile it raised register use
+// elsewhere: spills in the wide lossy adjoint and lower occupancy for the small tiles.
+constexpr bool ilp_tile(int W, int V) { return W == 4 && V == 2; }
+
+// Each launch_*.hip file is compiled twice: MP_ILP_SET = 0 holds the other tiles and forwards
+// the ILP set to the second object, MP_ILP_SET = 1 (the *_ilp.hip wrapper) holds the ILP set.
+#ifndef MP_ILP_SET
+#define MP_ILP_SET 0
+#endif
