@@ -392,7 +392,7 @@ int plan_group(meshprop_group& g) {
         f.ntiles = ((long)e.B + (long)nw * G * f.V - 1) / ((long)nw * G * f.V);
         // Layers per LDS stage: at least one unrolled group of the adjoint loop.
         const int depth = f.W <= 4 ? 4 : 2;
-        const size_t per_layer = (size_t)nw * f.NP * sizeof(float4);
+        const size_t per_layer = (size_t)nw * (f.NP / 2) * sizeof(float4);  // K-form: one entry per slot
         int C = (int)(kGroupStage / per_layer);
         if (C > 16) C = 16;
         if (e.tile.chunk >= 1 && (size_t)e.tile.chunk * per_layer <= kGroupStage) C = e.tile.chunk;

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "crossing.h"
 #include "meshprop.h"
 #include "propagate.h"
@@ -528,6 +530,82 @@ __device__ __forceinline__ float re_dot<float>(const float a, const float b) {
     return a * b;
 }
 
+// K-form stage entry of one crossing, i its upper and j its lower waveguide, from their rows
+// (A = Im part in .x, B = conj(g') x_partner): (D_i, D_j, B_i - conj(B_j)) for complex fields,
+// (B_i - B_j, 0) for real fields.
+__device__ __forceinline__ float4 kquad(const float2 ai, const float2 aj, const float2 bi, const float2 bj) {
+    return make_float4(ai.x, aj.x, bi.x - bj.x, bi.y + bj.y);
+}
+__device__ __forceinline__ float2 kquad(const float, const float, const float bi, const float bj) {
+    return make_float2(bi - bj, 0.f);
+}
+// Cross-lane reads of a K-form A row (.x only) and of a B row.
+__device__ __forceinline__ float2 a_next(const float2 a) { return make_float2(lane_next(a.x), 0.f); }
+__device__ __forceinline__ float a_next(const float) { return 0.f; }
+__device__ __forceinline__ float2 a_read(const float2 a, const int src) { return make_float2(lane_read(a.x, src), 0.f); }
+__device__ __forceinline__ float a_read(const float, const int) { return 0.f; }
+
+// Writes the K-form stage entries of one layer, slot-major: entry m is the crossing of slot m
+// (slot numbering of the prologue), ws the wave's stage layer.  In-lane pairings (NN layers,
+// FFT strides below W, identity layers) give lane t the slots t*W/2 .. t*W/2 + W/2 - 1; odd NN
+// layers take the last pair's lower waveguide from lane t+1; FFT strides of W and more pair
+// lane t with t ^ xm, and each of the two lanes writes half of the upper lane's W slots.
+// Every slot below NP/2 is written, so the conversion never reads stale LDS.  The lane
+// exchanges run with every lane active; only the first lane group writes.
+template <class T, int W>
+__device__ __forceinline__ void kstage_write(const int kd, const T (&a)[W], const T (&b)[W],
+                                             typename RowOf<T>::type* ws, const Lanes& q) {
+    constexpr int H = W / 2;
+    constexpr int LW = Log2<W>::value;
+    const bool wr = q.grp == 0;
+    typename RowOf<T>::type* wl = ws + q.t * H;
+    if (kd == 1) {
+        const T na = a_next(a[0]), nb = lane_next(b[0]);
+        if (wr) {
+#pragma unroll
+            for (int p = 0; p + 1 < H; ++p) wl[p] = kquad(a[2 * p + 1], a[2 * p + 2], b[2 * p + 1], b[2 * p + 2]);
+            wl[H - 1] = kquad(a[W - 1], na, b[W - 1], nb);
+        }
+        return;
+    }
+    const int st = (kd == 0 || kd == KIND_IDENTITY) ? 0 : kd - 2;
+    if (st < LW) {
+        if (!wr) return;
+        auto inlane = [&](auto sc) {
+            constexpr int S = decltype(sc)::value;
+#pragma unroll
+            for (int k = 0; k < W; ++k)
+                if ((k & S) == 0) wl[(k / (2 * S)) * S + (k % S)] = kquad(a[k], a[k + S], b[k], b[k + S]);
+        };
+        if (st == 0) inlane(std::integral_constant<int, 1>());
+        if constexpr (W > 2) {
+            if (st == 1) inlane(std::integral_constant<int, 2>());
+        }
+        if constexpr (W > 4) {
+            if (st == 2) inlane(std::integral_constant<int, 4>());
+        }
+        if constexpr (W > 8) {
+            if (st == 3) inlane(std::integral_constant<int, 8>());
+        }
+        return;
+    }
+    const int xm = 1 << (st - LW);
+    const bool top = (q.t & xm) == 0;
+    const int src = q.lane ^ xm;
+    // Upper lane tt of the pair: its W slots start at ((tt / 2xm) * xm + tt % xm) * W.
+    const int tt = top ? q.t : (q.t ^ xm);
+    typename RowOf<T>::type* wt = ws + ((tt / (2 * xm)) * xm + (tt % xm)) * W + (top ? 0 : H);
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+        // The upper lane keeps waveguides k < H and needs the lower lane's; the lower lane
+        // keeps k >= H and needs the upper lane's.
+        const T ra = a_read(top ? a[k + H] : a[k], src);
+        const T rb = lane_read(top ? b[k + H] : b[k], src);
+        const auto e = top ? kquad(a[k], ra, b[k], rb) : kquad(ra, a[k + H], rb, b[k + H]);
+        if (wr) wt[k] = e;
+    }
+}
+
 // Lane-group fold of a K-form A row: only .x carries data (real fields carry none).
 __device__ __forceinline__ float2 fold_im(const float2 a, const int src) {
     return make_float2(a.x + lane_read(a.x, src), a.y);
@@ -559,7 +637,11 @@ __device__ __forceinline__ void bwd_layer(const int kd, const T (&u)[V][W], cons
             accB[k] = fadd(accB[k], lane_read(accB[k], q.lane ^ off));
         }
     }
-    if (q.grp == 0) {
+    if constexpr (UNI) {
+        // K-form: one entry per crossing slot, NP/2 per wave and layer.
+        kstage_write<T, W>(kd, accA, accB,
+                           stage + (size_t)(__builtin_amdgcn_readfirstlane(q.wave) * C + lc) * (W / 2) * LPV, q);
+    } else if (q.grp == 0) {
         // This wave's rows of stage layer lc (a wave-uniform base), then k * LPV + lane.
         typename RowOf<T>::type* ws =
             stage + (size_t)(__builtin_amdgcn_readfirstlane(q.wave) * C + lc) * W * LPV + q.t;
@@ -576,36 +658,67 @@ __device__ __forceinline__ int s_clamp(const int last, const int s) { return s <
 // grouped kernel, the Hessian-vector product) runs workgroups of at most 256 threads.
 constexpr int kStageWaves = 4;
 
-// K-form dp of one parameter from the summed rows of the slot's waveguides i (upper) and j:
-// -a Im Q00 - d Im Q11 + Re(c (Q01 - conj(Q10))), with Q01 = B_i and Q10 = B_j.
-__device__ __forceinline__ float kdot(const float4 si, const float4 sj, const float4 c) {
-    const float rr = si.z - sj.z, ri = si.w + sj.w;
-    return fmaf(-c.w, ri, fmaf(c.z, rr, fmaf(-c.y, sj.x, -c.x * si.x)));
+// K-form dp of one parameter from the summed stage entry e = (Im Q00, Im Q11, Q01 - conj(Q10))
+// of a slot: -a Im Q00 - d Im Q11 + Re(c (Q01 - conj(Q10))); real fields: c (Q01 - Q10).
+__device__ __forceinline__ float kdot(const float4 e, const float4 c) {
+    return fmaf(-c.w, e.w, fmaf(c.z, e.z, fmaf(-c.y, e.y, -c.x * e.x)));
 }
-__device__ __forceinline__ float kdot(const float2 si, const float2 sj, const float c) { return c * (si.y - sj.y); }
+__device__ __forceinline__ float kdot(const float2 e, const float c) { return c * e.x; }
 
-// dp of each parameter of a slot from its summed rows: K-form or dJdT . dT/dp
+// dp of each parameter of a slot from its summed rows i, j: dJdT . dT/dp
 // (dJdT00 = A_i, dJdT01 = B_i, dJdT11 = A_j, dJdT10 = B_j).
-template <class T, int NPAR, bool KF, class R, class CT>
+template <class T, int NPAR, class R, class CT>
 __device__ __forceinline__ void slot_dp(const R si, const R sj, const CT* dd, float (&val)[NPAR]) {
 #pragma unroll
     for (int k = 0; k < NPAR; ++k) {
-        if constexpr (KF) {
-            val[k] = kdot(si, sj, dd[k]);
-        } else {
-            val[k] = re_dot(row_a(si), dd[k * 4 + 0]);
-            val[k] += re_dot(row_b(si), dd[k * 4 + 1]);
-            val[k] += re_dot(row_b(sj), dd[k * 4 + 2]);
-            val[k] += re_dot(row_a(sj), dd[k * 4 + 3]);
-        }
+        val[k] = re_dot(row_a(si), dd[k * 4 + 0]);
+        val[k] += re_dot(row_b(si), dd[k * 4 + 1]);
+        val[k] += re_dot(row_b(sj), dd[k * 4 + 2]);
+        val[k] += re_dot(row_a(sj), dd[k * 4 + 3]);
     }
 }
 
-template <class T, int W, int NPAR, bool KF = false>
+// Stores (first vector tile) or adds a slot's dp values.
+template <int NPAR>
+__device__ __forceinline__ void put_dp(float* outp, const float (&val)[NPAR], const bool first_tile) {
+    if constexpr (NPAR == 2) {
+        float2* o2 = reinterpret_cast<float2*>(outp);
+        if (first_tile) {
+            *o2 = make_float2(val[0], val[1]);
+        } else {
+            const float2 old = *o2;
+            *o2 = make_float2(old.x + val[0], old.y + val[1]);
+        }
+    } else {
+        outp[0] = first_tile ? val[0] : outp[0] + val[0];
+    }
+}
+
+// K-form conversion of slot m: sum the waves' stage entries (slot-major layers, wave stride
+// wstride) in wave order and write the slot's dp.  Inactive slots have zero coefficients.
+template <class T, int NPAR>
+__device__ __forceinline__ void convert_slot_k(const typename RowOf<T>::type* s0, const int wstride, const int nw,
+                                               float* pout, const int m, const typename KOf<T>::type (&kc)[NPAR],
+                                               const bool first_tile) {
+    typedef typename RowOf<T>::type R;
+    R e_[kStageWaves];
+#pragma unroll
+    for (int wv = 0; wv < kStageWaves; ++wv)
+        if (wv < nw) e_[wv] = s0[wv * wstride + m];
+    R e = e_[0];
+#pragma unroll
+    for (int wv = 1; wv < kStageWaves; ++wv)
+        if (wv < nw) e = row_add(e, e_[wv]);
+    float val[NPAR];
+#pragma unroll
+    for (int k = 0; k < NPAR; ++k) val[k] = kdot(e, kc[k]);
+    put_dp<NPAR>(pout + m * NPAR, val, first_tile);
+}
+
+template <class T, int W, int NPAR>
 __device__ __forceinline__ void convert_slot(const typename RowOf<T>::type* s0, const int wstride, const int nw,
                                              const int LPV, const int NP, float* pout, const int kd, const int m,
-                                             const typename CoefOf<T, KF>::type (&dd)[NPAR * CoefOf<T, KF>::per_param],
-                                             const bool first_tile) {
+                                             const T (&dd)[4 * NPAR], const bool first_tile) {
     typedef typename RowOf<T>::type R;
     int i, j;
     if (kd <= 1) {
@@ -643,18 +756,8 @@ __device__ __forceinline__ void convert_slot(const typename RowOf<T>::type* s0, 
             sj = row_add(sj, rj_[wv]);
         }
     float val[NPAR];
-    slot_dp<T, NPAR, KF>(si, sj, dd, val);
-    if constexpr (NPAR == 2) {
-        float2* o2 = reinterpret_cast<float2*>(outp);
-        if (first_tile) {
-            *o2 = make_float2(val[0], val[1]);
-        } else {
-            const float2 old = *o2;
-            *o2 = make_float2(old.x + val[0], old.y + val[1]);
-        }
-    } else {
-        outp[0] = first_tile ? val[0] : outp[0] + val[0];
-    }
+    slot_dp<T, NPAR>(si, sj, dd, val);
+    put_dp<NPAR>(outp, val, first_tile);
 }
 
 // Backdiff workgroups hold at most four waves: measured on MI355X, eight-wave workgroups
@@ -702,7 +805,8 @@ __device__ __forceinline__ void bwd_tile(const int N, const int l0, const int nl
     const int NPh = NP / 2;
     const int lane = threadIdx.x & 63;
     const int wave_u = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave index as a scalar
-    const int wstride = C * W * LPV;
+    const int lrows = UNI ? NPh : NP;  // stage entries per wave and layer (K-form: one per slot)
+    const int wstride = C * lrows;
     const KP kp = kind + l0;
     const int last = nl - 1;
     // Lean form for the wide non-unitary one-vector tiles (two row tables of W rows per layer):
@@ -812,23 +916,27 @@ __device__ __forceinline__ void bwd_tile(const int N, const int l0, const int nl
         // dJdT -> dp: wave w converts stage layers w, w + nw, ... (the first one prefetched).
         for (int lcc = wave_u; lcc < cl; lcc += nw) {
             const int la = l0 + last - (c0 + lcc);
-            const int kdl = kind[la];
-            const R* s0 = stage + lcc * W * LPV;
+            const R* s0 = stage + lcc * lrows;
             float* pout = my_partial + (size_t)la * NPh * np;
             const CT* src = ctab + (size_t)la * NPh * NC;
+            auto conv = [&](const int m, const CT (&dd)[NC]) {
+                if constexpr (UNI)
+                    convert_slot_k<T, np>(s0, wstride, nw, pout, m, dd, first);
+                else
+                    convert_slot<T, W, np>(s0, wstride, nw, LPV, NP, pout, kind[la], m, dd, first);
+            };
             int m0 = lane;
             if (lcc == wave_u) {
 #pragma unroll
                 for (int r = 0; r < PF; ++r)
-                    if (lane + 64 * r < NPh)
-                        convert_slot<T, W, np, UNI>(s0, wstride, nw, LPV, NP, pout, kdl, lane + 64 * r, pre[r], first);
+                    if (lane + 64 * r < NPh) conv(lane + 64 * r, pre[r]);
                 m0 = lane + 64 * PF;
             }
             for (int m = m0; m < NPh; m += 64) {
                 CT dd[NC];
 #pragma unroll
                 for (int e = 0; e < NC; ++e) dd[e] = src[e * NPh + m];
-                convert_slot<T, W, np, UNI>(s0, wstride, nw, LPV, NP, pout, kdl, m, dd, first);
+                conv(m, dd);
             }
         }
         __syncthreads();
@@ -854,7 +962,7 @@ constexpr int kPipeWaves = 8;
 
 template <class T, int NW, bool KF>
 struct ConvItem {
-    typename RowOf<T>::type a[NW], b[NW];
+    typename RowOf<T>::type a[NW], b[KF ? 1 : NW];  // K-form: one slot-major entry per wave
     typename CoefOf<T, KF>::type dd[NPhase<T>::value * CoefOf<T, KF>::per_param];
     float* out;
     int state;  // 0: nothing, 1: inactive slot (zero), 2: convert
@@ -867,6 +975,20 @@ __device__ __forceinline__ void conv_issue(ConvItem<T, NW, KF>& it, const typena
     constexpr int np = NPhase<T>::value;
     constexpr int wsh = W == 2 ? 1 : W == 4 ? 2 : W == 8 ? 3 : 4;  // log2 W
     const int NPh = NP / 2;
+    typedef typename CoefOf<T, KF>::type CT;
+    constexpr int NC = np * CoefOf<T, KF>::per_param;
+    const CT* src = reinterpret_cast<const CT*>(dtdp) + (size_t)la * NPh * NC;
+    it.out = part + ((size_t)la * NPh) * np + m * np;
+    if constexpr (KF) {
+        // Slot-major entries; inactive slots have zero coefficients.
+        it.state = 2;
+#pragma unroll
+        for (int wv = 0; wv < NW; ++wv)
+            if (wv < nw) it.a[wv] = s0[wv * wstride + m];
+#pragma unroll
+        for (int e = 0; e < NC; ++e) it.dd[e] = (src + (size_t)e * NPh)[m];
+        return;
+    }
     const int kd = kind[la];
     int i, j;
     if (kd <= 1) {
@@ -880,7 +1002,6 @@ __device__ __forceinline__ void conv_issue(ConvItem<T, NW, KF>& it, const typena
         i = -1;
         j = 0;
     }
-    it.out = part + ((size_t)la * NPh) * np + m * np;
     if (i < 0 || j >= NP) {
         it.state = 1;
         return;
@@ -895,9 +1016,6 @@ __device__ __forceinline__ void conv_issue(ConvItem<T, NW, KF>& it, const typena
             it.b[wv] = s0[wv * wstride + oj];
         }
     // One uniform base per table row, one 32-bit lane offset for all of them.
-    typedef typename CoefOf<T, KF>::type CT;
-    constexpr int NC = np * CoefOf<T, KF>::per_param;
-    const CT* src = reinterpret_cast<const CT*>(dtdp) + (size_t)la * NPh * NC;
 #pragma unroll
     for (int e = 0; e < NC; ++e) it.dd[e] = (src + (size_t)e * NPh)[m];
 }
@@ -913,26 +1031,25 @@ __device__ __forceinline__ void conv_finish(const ConvItem<T, NW, KF>& it, const
             for (int k = 0; k < np; ++k) it.out[k] = 0.f;
         return;
     }
-    R si = it.a[0], sj = it.b[0];
-#pragma unroll
-    for (int wv = 1; wv < NW; ++wv)
-        if (wv < nw) {
-            si = row_add(si, it.a[wv]);
-            sj = row_add(sj, it.b[wv]);
-        }
     float val[np];
-    slot_dp<T, np, KF>(si, sj, it.dd, val);
-    if constexpr (np == 2) {
-        float2* o2 = reinterpret_cast<float2*>(it.out);
-        if (first_tile) {
-            *o2 = make_float2(val[0], val[1]);
-        } else {
-            const float2 old = *o2;
-            *o2 = make_float2(old.x + val[0], old.y + val[1]);
-        }
+    if constexpr (KF) {
+        R e = it.a[0];
+#pragma unroll
+        for (int wv = 1; wv < NW; ++wv)
+            if (wv < nw) e = row_add(e, it.a[wv]);
+#pragma unroll
+        for (int k = 0; k < np; ++k) val[k] = kdot(e, it.dd[k]);
     } else {
-        it.out[0] = first_tile ? val[0] : it.out[0] + val[0];
+        R si = it.a[0], sj = it.b[0];
+#pragma unroll
+        for (int wv = 1; wv < NW; ++wv)
+            if (wv < nw) {
+                si = row_add(si, it.a[wv]);
+                sj = row_add(sj, it.b[wv]);
+            }
+        slot_dp<T, np>(si, sj, it.dd, val);
     }
+    put_dp<np>(it.out, val, first_tile);
 }
 
 template <class T, int W, int V, bool UNI, int NW, class KP>
@@ -950,7 +1067,8 @@ __device__ __forceinline__ void bwd_tile_pipe(const int N, const int l0, const i
     const int nw = blockDim.x >> 6;
     const int NPh = NP / 2;
     const int wave_u = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int wstride = C * W * LPV;   // one wave's rows in one buffer
+    const int lrows = UNI ? NPh : NP;  // stage entries per wave and layer (K-form: one per slot)
+    const int wstride = C * lrows;     // one wave's rows in one buffer
     const int bstride = nw * wstride;  // one buffer
     const int nthr = nw * 64;
     const KP kp = kind + l0;
@@ -1001,7 +1119,7 @@ __device__ __forceinline__ void bwd_tile_pipe(const int N, const int l0, const i
             const int lcc = wbase >> nph_sh;
             const int la = l0 + last - (pc0 + lcc);
             const unsigned m = (unsigned)((wbase & (NPh - 1)) + (threadIdx.x & 63));
-            conv_issue<T, W, NW, UNI, KP>(item, pbuf + lcc * W * LPV, wstride, nw, lpv_sh, NP, la, kind, dtdp, my_partial,
+            conv_issue<T, W, NW, UNI, KP>(item, pbuf + lcc * lrows, wstride, nw, lpv_sh, NP, la, kind, dtdp, my_partial,
                                      m);
         }
     };
@@ -1071,7 +1189,8 @@ __global__ void __launch_bounds__(64 * (PIPE ? PIPE : bwd_max_waves(W, V, UNI)))
     const T* g_out, const T* target, T* u_in, T* g_in, int ldu, Ckpt ck) {
     typedef typename RowOf<T>::type R;
     extern __shared__ unsigned char smem_raw[];
-    R* stage = reinterpret_cast<R*>(smem_raw);  // [nw][C][W][LPV], twice with PIPE
+    // [nw][C][W][LPV] (K-form: [nw][C][NP/2]), twice with PIPE
+    R* stage = reinterpret_cast<R*>(smem_raw);
     (void)np_unused;
     float* my_partial = partial + (size_t)blockIdx.x * part_stride;
     bool first = true;
@@ -1111,6 +1230,10 @@ struct Plan {
     size_t off_kind, off_a, off_b, off_dtdp, off_partial, total;  // workspace layout
     int uni;          // crossings are unitary: T^-1 = T^H
 };
+
+// LDS of one wave's stage layer in the adjoint: one entry per waveguide, or one per crossing
+// slot for the K-form of unitary crossings.
+inline size_t stage_layer_bytes(const Plan& P) { return (size_t)(P.uni ? P.NP / 2 : P.NP) * P.rowbytes; }
 
 // Tiles compiled into the library.  fwdprop keeps u[V][W]; fwddiff and backdiff keep two
 // field tiles plus row tables, so their W*V is capped lower.

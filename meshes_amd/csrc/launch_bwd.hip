@@ -9,7 +9,7 @@ static void go1(const Plan& P, const int* kind, const void* ri, const void* ra, 
                 size_t ps, int l0, int nl, const void* u_out, const void* g_out, const void* target, void* u_in,
                 void* g_in, int ldu, hipStream_t st, Ckpt ck) {
     typedef typename RowOf<T>::type R;
-    const size_t shmem = (size_t)P.waves * P.C * P.NP * P.rowbytes * (PIPE ? 2 : 1);
+    const size_t shmem = (size_t)P.waves * P.C * stage_layer_bytes(P) * (PIPE ? 2 : 1);
     if (shmem > 64 * 1024)  // beyond the default dynamic LDS limit of a launch
         hipFuncSetAttribute((const void*)k_bwd<T, W, V, UNI, PIPE>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)shmem);

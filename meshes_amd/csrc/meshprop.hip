@@ -151,13 +151,14 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
         // Eight-wave adjoint workgroups exist only with the pipelined stage (two buffers of two
         // layers of eight waves in the wide budget).
         const bool wide_ok = adj && P.W <= 4 && (P.NP / 2) % 64 == 0 &&
-                             (size_t)4 * kPipeWaves * P.NP * P.rowbytes <= stage_budget(kPipeWaves);
+                             (size_t)4 * kPipeWaves * stage_layer_bytes(P) <= stage_budget(kPipeWaves);
         const int cap = adj ? (wide_ok ? kPipeWaves : bwd_max_waves(P.W, P.V, P.uni)) : 4;
         if (P.waves > cap) P.waves = cap;
         if (adj && P.waves > 4 && P.waves < kPipeWaves) P.waves = 4;
     }
     if (adj)  // the LDS stage must hold one unrolled group of layers
-        while (P.waves > 1 && (size_t)(P.W <= 4 ? 4 : 2) * P.waves * P.NP * P.rowbytes > stage_budget(P.waves)) P.waves >>= 1;
+        while (P.waves > 1 && (size_t)(P.W <= 4 ? 4 : 2) * P.waves * stage_layer_bytes(P) > stage_budget(P.waves))
+            P.waves >>= 1;
     const int G = 64 / P.LPV;
     const long per_tile = (long)P.waves * G * P.V;
     P.ntiles = ((long)B + per_tile - 1) / per_tile;
@@ -168,7 +169,7 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
         const long cap = 2L * ncu;
         if (P.ntiles > cap) P.grid = (int)cap;
         // LDS stage depth: at least two layers; the stage fits the budget.
-        const size_t per_layer = (size_t)P.waves * P.NP * P.rowbytes;
+        const size_t per_layer = (size_t)P.waves * stage_layer_bytes(P);
         const size_t budget = stage_budget(P.waves);
         const int req = P.C;  // requested layers per stage (0: automatic)
         int C = (int)(budget / per_layer);
