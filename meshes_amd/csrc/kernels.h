@@ -539,6 +539,14 @@ __device__ __forceinline__ float4 kquad(const float2 ai, const float2 aj, const 
 __device__ __forceinline__ float2 kquad(const float, const float, const float bi, const float bj) {
     return make_float2(bi - bj, 0.f);
 }
+// Per-lane choice between two entries, component by component (a select of the vector
+// structs is lowered through a stack temporary).
+__device__ __forceinline__ float4 esel(const bool c, const float4 x, const float4 y) {
+    return make_float4(c ? x.x : y.x, c ? x.y : y.y, c ? x.z : y.z, c ? x.w : y.w);
+}
+__device__ __forceinline__ float2 esel(const bool c, const float2 x, const float2 y) {
+    return make_float2(c ? x.x : y.x, c ? x.y : y.y);
+}
 // Cross-lane reads of a K-form A row (.x only) and of a B row.
 __device__ __forceinline__ float2 a_next(const float2 a) { return make_float2(lane_next(a.x), 0.f); }
 __device__ __forceinline__ float a_next(const float) { return 0.f; }
@@ -595,14 +603,25 @@ __device__ __forceinline__ void kstage_write(const int kd, const T (&a)[W], cons
     // Upper lane tt of the pair: its W slots start at ((tt / 2xm) * xm + tt % xm) * W.
     const int tt = top ? q.t : (q.t ^ xm);
     typename RowOf<T>::type* wt = ws + ((tt / (2 * xm)) * xm + (tt % xm)) * W + (top ? 0 : H);
+    if constexpr (H == 1) {
+        // W = 2: the upper lane writes crossing 0 (its waveguide 0, the lower lane's 0), the
+        // lower lane crossing 1 (the upper lane's waveguide 1, its own 1).  Both waveguides
+        // are exchanged: a lane-dependent pick between the two array elements would be
+        // lowered to a stack access.
+        const T ra0 = a_read(a[0], src), rb0 = lane_read(b[0], src);
+        const T ra1 = a_read(a[1], src), rb1 = lane_read(b[1], src);
+        const auto e_up = kquad(a[0], ra0, b[0], rb0), e_lo = kquad(ra1, a[1], rb1, b[1]);
+        if (wr) wt[0] = esel(top, e_up, e_lo);
+    } else {
 #pragma unroll
-    for (int k = 0; k < H; ++k) {
-        // The upper lane keeps waveguides k < H and needs the lower lane's; the lower lane
-        // keeps k >= H and needs the upper lane's.
-        const T ra = a_read(top ? a[k + H] : a[k], src);
-        const T rb = lane_read(top ? b[k + H] : b[k], src);
-        const auto e = top ? kquad(a[k], ra, b[k], rb) : kquad(ra, a[k + H], rb, b[k + H]);
-        if (wr) wt[k] = e;
+        for (int k = 0; k < H; ++k) {
+            // The upper lane keeps waveguides k < H and needs the lower lane's; the lower lane
+            // keeps k >= H and needs the upper lane's.
+            const T ra = a_read(top ? a[k + H] : a[k], src);
+            const T rb = lane_read(top ? b[k + H] : b[k], src);
+            const auto e = esel(top, kquad(a[k], ra, b[k], rb), kquad(ra, a[k + H], rb, b[k + H]));
+            if (wr) wt[k] = e;
+        }
     }
 }
 

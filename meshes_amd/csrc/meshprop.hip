@@ -153,6 +153,15 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
         const bool wide_ok = adj && P.W <= 4 && (P.NP / 2) % 64 == 0 &&
                              (size_t)4 * kPipeWaves * stage_layer_bytes(P) <= stage_budget(kPipeWaves);
         const int cap = adj ? (wide_ok ? kPipeWaves : bwd_max_waves(P.W, P.V, P.uni)) : 4;
+        // Automatic choice: eight-wave adjoint workgroups for unitary crossings when the batch
+        // still fills every CU with them (half the dp partials of four-wave groups; measured at
+        // N=256, L=256, B=4096: 0.406-0.413 ms per fwd+grad step against 0.419 ms).  The
+        // general (lossy) adjoint keeps four waves: its eight-wave kernel spills registers.
+        const bool auto_waves = !(tile && tile->waves >= 1 && tile->waves <= 8);
+        if (auto_waves && wide_ok && P.uni) {
+            const long per_wg = (long)kPipeWaves * (64 / P.LPV) * P.V;
+            if (((long)B + per_wg - 1) / per_wg >= ncu) P.waves = kPipeWaves;
+        }
         if (P.waves > cap) P.waves = cap;
         if (adj && P.waves > 4 && P.waves < kPipeWaves) P.waves = 4;
     }
