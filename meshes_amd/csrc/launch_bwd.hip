@@ -25,7 +25,7 @@ static void go2(const Plan& P, const int* kind, const void* ri, const void* ra, 
     if (P.uni)
         go1<T, W, V, true, PIPE>(P, kind, ri, ra, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in, g_in, ldu,
                                  st, ck);
-    else
+    else if constexpr (PIPE == 0 || W <= 4)  // the general stage fits two buffers only for W <= 4
         go1<T, W, V, false, PIPE>(P, kind, ri, ra, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in, g_in, ldu,
                                   st, ck);
 }
@@ -35,8 +35,9 @@ static void go(const Plan& P, const int* kind, const void* ri, const void* ra, c
                size_t ps, int l0, int nl, const void* u_out, const void* g_out, const void* target, void* u_in,
                void* g_in, int ldu, hipStream_t st, Ckpt ck) {
     // The pipelined stage is compiled for the tiles it can serve (two buffers of two layers
-    // fit the LDS budget only for W <= 4), once for four-wave and once for eight-wave groups.
-    if constexpr (W <= 4) {
+    // fit the LDS budget for W <= 4, and for W = 8 with the per-crossing K-form stage), once
+    // for four-wave and once for eight-wave groups.
+    if constexpr (W <= 8) {
         if (P.pipe) {
             if (P.waves > 4)
                 go2<T, W, V, kPipeWaves>(P, kind, ri, ra, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in,

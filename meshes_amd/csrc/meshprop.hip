@@ -145,12 +145,17 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
     P.uni = (X != X_GENERIC);
     const int ncu = num_cus();
     resolve_tile(op, N, B, tile, ncu, P.W, P.V, P.LPV, P.waves, P.C);
+    // Unitary adjoints at W = 8 hold one vector per lane unless asked otherwise: the two-vector
+    // tile needs one wave per SIMD (pipelined, eight waves: spills).  Measured at N=512, L=512,
+    // B=8192 with checkpoints every 32 layers: 3.52 ms (8,1) pipelined against 4.10 ms (8,2).
+    if (adj && P.uni && P.W == 8 && !(tile && tile->v >= 1)) P.V = 1;
     P.NP = P.LPV * P.W;
     P.rowbytes = P.real ? sizeof(float2) : sizeof(float4);
     {
         // Eight-wave adjoint workgroups exist only with the pipelined stage (two buffers of two
         // layers of eight waves in the wide budget).
-        const bool wide_ok = adj && P.W <= 4 && (P.NP / 2) % 64 == 0 &&
+        const bool pipe_w = P.W <= 4 || (P.W == 8 && P.uni);  // widths with a pipelined stage
+        const bool wide_ok = adj && pipe_w && (P.NP / 2) % 64 == 0 &&
                              (size_t)4 * kPipeWaves * stage_layer_bytes(P) <= stage_budget(kPipeWaves);
         const int cap = adj ? (wide_ok ? kPipeWaves : bwd_max_waves(P.W, P.V, P.uni)) : 4;
         // Automatic choice: eight-wave adjoint workgroups for unitary crossings when the batch
@@ -192,7 +197,7 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
         // an even number of layers in the same budget, and whole waves of conversion items per
         // layer (NP/2 a multiple of 64).
         P.pipe = 0;
-        if (P.W <= 4 && (P.NP / 2) % 64 == 0) {
+        if ((P.W <= 4 || (P.W == 8 && P.uni)) && (P.NP / 2) % 64 == 0) {
             int Cp = (int)(budget / (2 * per_layer));
             if (Cp > 16) Cp = 16;
             if (req >= 2 && 2 * (size_t)req * per_layer <= budget) Cp = req;

@@ -173,3 +173,26 @@ def test_fwdgrad_several_tiles_per_workgroup(gpu):
     rdp, rgin, _ = O.mesh_vjp(g, "mzi", p64, s64, _np(out).astype(np.complex128), gout)
     assert O.rel_l2(_np(dp).reshape(rdp.shape), rdp) < TOL
     assert O.rel_l2(_np(gin), rgin) < TOL
+
+
+@pytest.mark.parametrize("tile", [None, (8, 1, 4, 0), (8, 1, 8, 0)], ids=["auto", "w8-4waves", "w8-8waves"])
+def test_fwdgrad_w8_pipelined(gpu, tile):
+    """N=512 (W=8) unitary adjoint on the pipelined per-crossing stage, four- and eight-wave
+    workgroups (B=2600: the automatic choice is eight waves), against the FP64 oracle."""
+    import meshes_amd._lib as Lb
+    N, L, B = 512, 20, 2600
+    g, p, s, u, rng = _setup("mzi", "nn", N, L, B, seed=512)
+    mesh = _mesh(g, "mzi", p, s)
+    y = O.unit_cn(rng, B, N).astype(np.complex64)
+    ub, yb = torch.as_tensor(u).cuda().contiguous(), torch.as_tensor(y).cuda().contiguous()
+    t = Lb.resolve_tile(Lb.OP_FWDGRAD, "mzi", N, B, tile)
+    assert t[0] == 8 and t[1] == 1 and t[2] == (4 if tile and tile[2] == 4 else 8)
+    out, gin, dp = torch.empty_like(ub), torch.empty_like(ub), torch.zeros_like(mesh.p_phase)
+    _run(mesh, Lb.OP_FWDGRAD, B, ub, out, gin, dp, yb, tile=tile)
+    torch.cuda.synchronize()
+    p64, s64 = p.astype(np.float64), _f64(s)
+    assert O.rel_l2(_np(out), O.mesh_fwd(g, "mzi", p64, s64, u.astype(np.complex128))) < TOL
+    gout = 2 * (_np(out).astype(np.complex128) - y)
+    rdp, rgin, _ = O.mesh_vjp(g, "mzi", p64, s64, _np(out).astype(np.complex128), gout)
+    assert O.rel_l2(_np(dp).reshape(rdp.shape), rdp) < TOL
+    assert O.rel_l2(_np(gin), rgin) < TOL

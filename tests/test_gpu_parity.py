@@ -243,6 +243,35 @@ def test_c2_full_size(gpu):
     assert abs(lhs - rhs) <= 1e-4 * max(abs(rhs), 1e-3)
 
 
+def test_unitary_checkpointed_adjoint_w8(gpu):
+    """Unitary MZI at N=512 (W=8 pipelined K-form stage, C=2): backdiff with checkpoints every 8
+    layers in one launch and every 5 layers (one launch per segment) equals the oracle."""
+    import meshes_amd._lib as Lb
+    X, N, L, B = "mzi", 512, 30, 300
+    g, p, s, u, rng = _setup(X, "nn", N, L, B, seed=77)
+    mesh = _mesh(g, X, p, s)
+    ldp, lds = mesh._ld()
+    ub = torch.as_tensor(u).cuda().contiguous()
+    y = torch.as_tensor(O.unit_cn(rng, B, N).astype(np.complex64)).cuda()
+    for K in (8, 5):
+        out = torch.empty_like(ub)
+        ck = torch.empty(((L + K - 1) // K, B, N), dtype=ub.dtype, device=ub.device)
+        ws, nb = mesh._workspace(Lb.OP_FWDPROP, B)
+        Lb.run(Lb.OP_FWDPROP, X, N, L, B, False, mesh.lens, mesh.shifts, mesh.p_phase, None, ldp, mesh.p_splitter,
+               lds, ub, None, out, None, N, ckpt=ck, ckpt_every=K, workspace=ws, ws_bytes=nb)
+        gb = (2 * (out - y)).contiguous()
+        dp, uin, gin = torch.zeros_like(mesh.p_phase), torch.empty_like(ub), torch.empty_like(ub)
+        ws, nb = mesh._workspace(Lb.OP_BACKDIFF, B)
+        Lb.run(Lb.OP_BACKDIFF, X, N, L, B, False, mesh.lens, mesh.shifts, mesh.p_phase, dp, ldp, mesh.p_splitter,
+               lds, out, gb, uin, gin, N, ckpt=ck, ckpt_every=K, workspace=ws, ws_bytes=nb)
+        torch.cuda.synchronize()
+        rdp, rgin, _ = O.mesh_vjp(g, X, p.astype(np.float64), _f64(s), _np(out).astype(np.complex128),
+                                  _np(gb).astype(np.complex128), u_in=u.astype(np.complex128))
+        assert O.rel_l2(_np(dp).reshape(rdp.shape), rdp) < TOL
+        assert O.rel_l2(_np(gin), rgin) < TOL
+        assert O.rel_l2(_np(uin), u) < TOL
+
+
 @pytest.mark.parametrize("N,K", [(64, 8), (256, 8), (256, 5), (512, 8)])
 def test_lossy_checkpointed_adjoint(gpu, N, K):
     """Lossy generic MZI (non-unitary): backdiff with forward checkpoints every K layers equals
