@@ -152,7 +152,13 @@ enum meshprop_flags {
     MESHPROP_ONLY_REDUCE = 8,
     MESHPROP_DP_ASSIGN = 16,   /* backdiff / fwd+grad: dp = gradient instead of dp += gradient */
     MESHPROP_ONLY_FORWARD = 32, /* fwd+grad: run the forward propagation alone */
-    MESHPROP_SKIP_FORWARD = 64  /* fwd+grad: u_b already holds the forward output; run the rest */
+    MESHPROP_SKIP_FORWARD = 64, /* fwd+grad: u_b already holds the forward output; run the rest */
+    /* MESHPROP_MZI_GENERIC backdiff / fwd+grad: the caller asserts that every crossing is a
+     * scaled unitary, T = c U (balanced arm gains: g11 = g21 and g12 = g22 in the splitter
+     * row).  The adjoint then runs as for unitary crossings on one table of inverse rows, with
+     * T^H = |det T| T^-1, and fwd+grad accepts the generic crossing.  The prologue checks the
+     * assertion in FP64 per crossing; where it fails, dp and dJ/du_in come out NaN. */
+    MESHPROP_SCALED_UNITARY = 128
 };
 
 /*

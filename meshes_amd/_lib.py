@@ -159,6 +159,7 @@ def resolve_tile(op, X, N, B, tile=None):
 
 ONLY_PROLOGUE, SKIP_PROLOGUE, SKIP_REDUCE, ONLY_REDUCE, DP_ASSIGN = 1, 2, 4, 8, 16
 ONLY_FORWARD, SKIP_FORWARD = 32, 64
+SCALED_UNITARY = 128
 
 
 def run(op, X, N, L, B, fft, g0, g1, p, dp, ldp, s, lds, u_a, du_a, u_b, du_b, ldu, target=None, ckpt=None,

@@ -10,7 +10,8 @@ Adjoint.  For unitary crossings (mzi, sym, orth) backdiff reconstructs the layer
 inverse propagation, so the forward saves only its output.  For lossy crossings
 (mzi_generic with gains) the forward stores the field entering every ``ckpt_every``-th
 layer and the backward restarts each segment from the stored field (checkpointed adjoint),
-with T^-1 for the reconstruction and T^H for the adjoint.
+with T^-1 for the reconstruction and T^H for the adjoint; with balanced arm gains
+(T = c U) T^H = |det T| T^-1 and the unitary adjoint runs on the inverse rows alone.
 """
 from __future__ import annotations
 
@@ -25,8 +26,10 @@ def _launch(mesh, op, B, p, dp, u_a, du_a, u_b, du_b, target=None, ckpt=None, ck
     g0, g1 = mesh._geom()
     s = mesh.p_splitter if mesh.p_splitter.numel() else None
     ws, nb = mesh._workspace(op, B)
+    flags = mesh._adj_flags(mesh.p_splitter) if op == _lib.OP_BACKDIFF else 0
     _lib.run(op, mesh.X, mesh.N, mesh.L, B, mesh.fft, g0, g1, p, dp, ldp, s, lds, u_a, du_a, u_b, du_b, mesh.N,
-             target=target, ckpt=ckpt, ckpt_every=ckpt_every, tile=mesh.tile, workspace=ws, ws_bytes=nb)
+             target=target, ckpt=ckpt, ckpt_every=ckpt_every, tile=mesh.tile, workspace=ws, ws_bytes=nb,
+             flags=flags)
 
 
 class MeshFunction(torch.autograd.Function):

@@ -31,6 +31,7 @@ __device__ __forceinline__ dc cexpi(double t) {
     return dc{c, s};
 }
 __device__ __forceinline__ dc cmuli(dc a) { return dc{-a.i, a.r}; }  // i*a
+__device__ __forceinline__ double cabs(dc a) { return sqrt(a.r * a.r + a.i * a.i); }
 
 struct M2 {
     dc a00, a01, a10, a11;
@@ -54,6 +55,7 @@ __device__ __forceinline__ M2 mscalec(const M2& A, dc s) {
 __device__ __forceinline__ M2 mherm(const M2& A) {
     return M2{cconj(A.a00), cconj(A.a10), cconj(A.a01), cconj(A.a11)};
 }
+__device__ __forceinline__ dc mdet(const M2& A) { return csub(cmul(A.a00, A.a11), cmul(A.a01, A.a10)); }
 // Inverse via the adjugate.
 __device__ __forceinline__ M2 minv(const M2& A) {
     dc det = csub(cmul(A.a00, A.a11), cmul(A.a01, A.a10));

@@ -103,7 +103,7 @@ __global__ void __launch_bounds__(256) k_group_prologue(const unsigned char* __r
         }
         if (M.build & B_DTDP) tabs |= TAB_KUNI;  // grouped meshes are unitary: K-form
         slot_tables_at<float4>(M.X, M.N, M.NP, M.LPV, M.fft, M.g0, M.g1, M.p, M.ldp, M.s, M.lds, nullptr, tabs,
-                               M.kind, ta, tb, M.dtdp, l, m);
+                               M.kind, ta, tb, M.dtdp, nullptr, l, m);
     }
 }
 

@@ -309,7 +309,7 @@ int meshprop_hvp(int X, int N, int L, int B, int fft, const int* geom0, const in
     hipLaunchKernelGGL((k_rows<float4>), dim3((P.NP / 2 + 255) / 256, L), dim3(256), 0, st, X, N, L, P.NP, P.LPV, fft,
                        geom0, geom1, p, ldp, sp, lds, q, (int)(TAB_INV | TAB_DIFF_ADJ | TAB_DTDP),
                        (int*)(base + P.off_kind), (float4*)(base + P.off_r), (float4*)(base + P.off_d),
-                       (float2*)(base + P.off_t1));
+                       (float2*)(base + P.off_t1), nullptr);
     if (hipGetLastError() != hipSuccess) return MESHPROP_ERR_HIP;
     const dim3 g2((P.NP / 2 + 255) / 256, L);
     hipLaunchKernelGGL((k_d2tdp<float2>), g2, dim3(256), 0, st, X, N, L, P.NP, fft, geom0, geom1, p, q, ldp, sp, lds,

@@ -23,7 +23,8 @@ enum {
     TAB_DTDP = 16,
     TAB_DIFF_ADJ = 32,
     TAB_INV_B = 64,
-    TAB_KUNI = 128
+    TAB_KUNI = 128,
+    TAB_SCALE = 256  // per-waveguide |det T| of scaled-unitary crossings (1 on identity rows)
 };
 
 // Slot and role of waveguide w in layer l.  Returns whether an active crossing acts on it.
@@ -198,7 +199,8 @@ __device__ void slot_tables_at(const int X, const int N, const int NP, const int
                                const int ldp, const float* __restrict__ s, const int lds,
                                const float* __restrict__ dp, const int tabs, int* __restrict__ kind,
                                R* __restrict__ tab_a, R* __restrict__ tab_b,
-                               typename FieldOfRow<R>::type* __restrict__ dtdp, const int l, const int m) {
+                               typename FieldOfRow<R>::type* __restrict__ dtdp, float* __restrict__ scl,
+                               const int l, const int m) {
     typedef typename FieldOfRow<R>::type F;
     const int kd = layer_kind(fft, g0, g1, l, NP);
     if (m == 0 && kind) kind[l] = kd;
@@ -228,17 +230,22 @@ __device__ void slot_tables_at(const int X, const int N, const int NP, const int
     auto put = [&](R* tab, const int w, const dc a, const dc b) {
         if (w < NP) tab[pos(w)] = make_row<R>(a, b);
     };
+    auto put_scale = [&](const int w, const float v) {
+        if (w < NP) scl[pos(w)] = v;
+    };
     const int np = n_phase_of(X), ns = n_split_of(X);
     if (!fft && (g1[l] & 1) && m == 0) {  // waveguide 0 of an odd-shift layer: no crossing
         if (tabs & (TAB_FWD | TAB_INV)) put(tab_a, 0, one, zero);
         if (tabs & (TAB_DIFF | TAB_DIFF_ADJ)) put(tab_b, 0, zero, zero);
         if (tabs & (TAB_ADJ | TAB_INV_B)) put(tab_b, 0, one, zero);
+        if (tabs & TAB_SCALE) put_scale(0, 1.f);
     }
     if (!act) {
         for (int w = i; w <= j; w += j - i) {
             if (tabs & (TAB_FWD | TAB_INV)) put(tab_a, w, one, zero);
             if (tabs & (TAB_DIFF | TAB_DIFF_ADJ)) put(tab_b, w, zero, zero);
             if (tabs & (TAB_ADJ | TAB_INV_B)) put(tab_b, w, one, zero);
+            if (tabs & TAB_SCALE) put_scale(w, 1.f);
         }
         if (tabs & TAB_DTDP) {
             F* out = dtdp + (size_t)l * NPh * np * 4 + m;
@@ -285,12 +292,25 @@ __device__ void slot_tables_at(const int X, const int N, const int NP, const int
             out[(size_t)(k2 * 4 + 3) * NPh] = dc_to<F>(dT[k2].a11);
         }
     }
+    if (tabs & TAB_SCALE) {
+        // Scaled unitary T = c U: T^H T = |det T| I, and the adjoint rows are |det T| times
+        // the inverse rows.  A crossing that is not (beyond FP64 rounding) gets NaN.
+        const M2 E = mmul(mherm(T), T);
+        const double sc = cabs(mdet(T));
+        const double err = fabs(E.a00.r - sc) + fabs(E.a11.r - sc) + fabs(E.a00.i) + fabs(E.a11.i) +
+                           cabs(E.a01) + cabs(E.a10);
+        const float v = (err <= 1e-9 * sc) ? (float)sc : __int_as_float(0x7fc00000);
+        put_scale(i, v);
+        put_scale(j, v);
+    }
     if (tabs & TAB_KUNI) {
-        // Anti-Hermitian part of T^H dT/dp (equal to the whole for a unitary T).
+        // Anti-Hermitian part of K = T^-1 dT/dp (T^-1 = T^H for a unitary T; for a scaled
+        // unitary T = c U it equals U^H dU/dp).
         typedef typename KOf<F>::type K;
         K* out = reinterpret_cast<K*>(dtdp) + (size_t)l * np * NPh + m;
+        const M2 Ti = (X == X_GENERIC) ? minv(T) : mherm(T);
         for (int k2 = 0; k2 < np; ++k2) {
-            const M2 Kp = mmul(mherm(T), dT[k2]);
+            const M2 Kp = mmul(Ti, dT[k2]);
             const double cr = 0.5 * (Kp.a01.r - Kp.a10.r), ci = 0.5 * (Kp.a01.i + Kp.a10.i);
             if constexpr (sizeof(K) == sizeof(float4))
                 out[(size_t)k2 * NPh] = K{(float)Kp.a00.i, (float)Kp.a11.i, (float)cr, (float)ci};
@@ -307,11 +327,12 @@ __global__ void k_rows(int X, int N, int L, int NP, int LPV, int fft, const int*
                        const int* __restrict__ g1,
                        const float* __restrict__ p, int ldp, const float* __restrict__ s, int lds,
                        const float* __restrict__ dp, int tabs, int* __restrict__ kind, R* __restrict__ tab_a,
-                       R* __restrict__ tab_b, typename FieldOfRow<R>::type* __restrict__ dtdp) {
+                       R* __restrict__ tab_b, typename FieldOfRow<R>::type* __restrict__ dtdp,
+                       float* __restrict__ scl) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     const int l = blockIdx.y;
     if (l >= L || m >= NP / 2) return;
-    slot_tables_at<R>(X, N, NP, LPV, fft, g0, g1, p, ldp, s, lds, dp, tabs, kind, tab_a, tab_b, dtdp, l, m);
+    slot_tables_at<R>(X, N, NP, LPV, fft, g0, g1, p, ldp, s, lds, dp, tabs, kind, tab_a, tab_b, dtdp, scl, l, m);
 }
 
 // =======================================================================================
@@ -631,11 +652,12 @@ __device__ __forceinline__ float2 fold_im(const float2 a, const int src) {
 }
 __device__ __forceinline__ float fold_im(const float a, const int) { return a; }
 
-template <class T, int W, int V, bool UNI>
+template <class T, int W, int V, bool UNI, bool SC = false>
 __device__ __forceinline__ void bwd_layer(const int kd, const T (&u)[V][W], const T (&g)[V][W], T (&uo)[V][W],
                                           T (&go)[V][W], const typename RowOf<T>::type (&ri)[W],
                                           const typename RowOf<T>::type (&ra)[W], typename RowOf<T>::type* stage,
-                                          const Lanes& q, const int LPV, const int C, const int lc) {
+                                          const Lanes& q, const int LPV, const int C, const int lc,
+                                          const float* scl = nullptr) {
     T accA[W], accB[W];
 #pragma unroll
     for (int k = 0; k < W; ++k) {
@@ -643,7 +665,7 @@ __device__ __forceinline__ void bwd_layer(const int kd, const T (&u)[V][W], cons
         accB[k] = fzero(T());
     }
     // Unitary crossings take the K-form rows (A_k.x = Im part, A_k.y = 0; no A for real fields).
-    BwdStep<T, W, V, UNI> st{u, g, uo, go, ri, ra, accA, accB, q.lane};
+    BwdStep<T, W, V, UNI, SC> st{u, g, uo, go, ri, ra, accA, accB, q.lane, scl};
     dispatch_kind<W>(kd, st);
     // Fold lane groups (groups hold the same waveguides of different vectors).
     for (int off = LPV; off < 64; off <<= 1) {
@@ -802,7 +824,14 @@ struct BwdDepth {
     static constexpr int value = W <= 4 ? 4 : 2;
 };
 
-template <class T, int W, int V, bool UNI, class KP>
+// Per-waveguide factors of one layer (same k-major layout as the row tables).
+template <int W>
+__device__ __forceinline__ void load_scl(float (&s)[W], const float* p, const int LPV) {
+#pragma unroll
+    for (int k = 0; k < W; ++k) s[k] = p[(size_t)k * LPV];
+}
+
+template <class T, int W, int V, bool UNI, class KP, bool SC = false>
 __device__ __forceinline__ void bwd_tile(const int N, const int l0, const int nl, const int B, const int LPV,
                                          const int NP, const int C, KP kind,
                                          const typename RowOf<T>::type* __restrict__ rows_inv,
@@ -811,7 +840,7 @@ __device__ __forceinline__ void bwd_tile(const int N, const int l0, const int nl
                                          const T* g_out, const T* target, T* u_in, T* g_in, const int ldu,
                                          const long tile, const bool first,
                                          typename RowOf<T>::type* __restrict__ stage,
-                                         const Ckpt ck = Ckpt{nullptr, 0, 0}) {
+                                         const Ckpt ck = Ckpt{nullptr, 0, 0}, const float* __restrict__ scales = nullptr) {
     typedef typename RowOf<T>::type R;
     constexpr int np = NPhase<T>::value;
     // Conversion coefficients: K-form for unitary crossings, dT/dp otherwise.
@@ -853,6 +882,9 @@ __device__ __forceinline__ void bwd_tile(const int N, const int l0, const int nl
     const R* rip = rows_inv + (size_t)l0 * NP + q.t;
     const R* rap = rows_adj + (size_t)l0 * NP + q.t;
     R ri[D][W], ra[DA][W];
+    constexpr int SD = SC ? D : 1;  // register sets of per-waveguide adjoint factors
+    float sc[SD][SC ? W : 1];
+    const float* scp = SC ? scales + (size_t)l0 * NP + q.t : nullptr;
     int kk[D];
     auto lrow = [&](const int s) { return (size_t)(s < last ? last - s : 0) * NP; };
 #pragma unroll
@@ -860,13 +892,14 @@ __device__ __forceinline__ void bwd_tile(const int N, const int l0, const int nl
         if (nl > 0) {
             kk[d] = kp[s_clamp(last, d)];
             load_rows(ri[d], rip + lrow(d), LPV);
+            if constexpr (SC) load_scl(sc[d], scp + lrow(d), LPV);
             if constexpr (!UNI && !kLean) load_rows(ra[d], rap + lrow(d), LPV);
         }
     }
-#define MP_BWD_LAYER(KD, U, G, UO, GO, RI, RA, LC)                                                  \
+#define MP_BWD_LAYER(KD, U, G, UO, GO, RI, RA, LC, SCL)                                             \
     {                                                                                               \
     if constexpr (UNI)                                                                          \
-        bwd_layer<T, W, V, UNI>(KD, U, G, UO, GO, RI, RI, stage, q, LPV, C, LC);                \
+        bwd_layer<T, W, V, UNI, SC>(KD, U, G, UO, GO, RI, RI, stage, q, LPV, C, LC, SCL);       \
     else                                                                                        \
         bwd_layer<T, W, V, UNI>(KD, U, G, UO, GO, RI, RA, stage, q, LPV, C, LC);                \
     }
@@ -901,12 +934,13 @@ __device__ __forceinline__ void bwd_tile(const int N, const int l0, const int nl
                 const int dn = (d + PD) % D;  // register set receiving layer s + d + PD
                 kk[dn] = kp[s_clamp(last, s + d + PD)];
                 load_rows(ri[dn], rip + lrow(s + d + PD), LPV);
+                if constexpr (SC) load_scl(sc[dn % SD], scp + lrow(s + d + PD), LPV);
                 if constexpr (!UNI && !kLean) load_rows(ra[dn % DA], rap + lrow(s + d + PD), LPV);
                 MP_ADJ_ROWS(0, s + d)
                 if ((d & 1) == 0)
-                    MP_BWD_LAYER(kk[d], ua, ga, ub, gb, ri[d], ra[d % DA], lc + d)
+                    MP_BWD_LAYER(kk[d], ua, ga, ub, gb, ri[d], ra[d % DA], lc + d, sc[d % SD])
                 else
-                    MP_BWD_LAYER(kk[d], ub, gb, ua, ga, ri[d], ra[d % DA], lc + d)
+                    MP_BWD_LAYER(kk[d], ub, gb, ua, ga, ri[d], ra[d % DA], lc + d, sc[d % SD])
             }
         }
         // Remainder of the last stage (cl - lc < D layers): register sets 0.. hold them.
@@ -918,13 +952,14 @@ __device__ __forceinline__ void bwd_tile(const int N, const int l0, const int nl
                     const int dn = (d + PD) % D;
                     kk[dn] = kp[s_clamp(last, c0 + lc + d + PD)];
                     load_rows(ri[dn], rip + lrow(c0 + lc + d + PD), LPV);
+                    if constexpr (SC) load_scl(sc[dn % SD], scp + lrow(c0 + lc + d + PD), LPV);
                     if constexpr (!UNI && !kLean) load_rows(ra[dn % DA], rap + lrow(c0 + lc + d + PD), LPV);
                 }
                 MP_ADJ_ROWS(0, c0 + lc + d)
                 if ((d & 1) == 0)
-                    MP_BWD_LAYER(kk[d], ua, ga, ub, gb, ri[d], ra[d % DA], lc + d)
+                    MP_BWD_LAYER(kk[d], ua, ga, ub, gb, ri[d], ra[d % DA], lc + d, sc[d % SD])
                 else
-                    MP_BWD_LAYER(kk[d], ub, gb, ua, ga, ri[d], ra[d % DA], lc + d)
+                    MP_BWD_LAYER(kk[d], ub, gb, ua, ga, ri[d], ra[d % DA], lc + d, sc[d % SD])
             }
         }
         if (left & 1) {
@@ -1071,7 +1106,7 @@ __device__ __forceinline__ void conv_finish(const ConvItem<T, NW, KF>& it, const
     put_dp<np>(it.out, val, first_tile);
 }
 
-template <class T, int W, int V, bool UNI, int NW, class KP>
+template <class T, int W, int V, bool UNI, int NW, class KP, bool SC = false>
 __device__ __forceinline__ void bwd_tile_pipe(const int N, const int l0, const int nl, const int B, const int LPV,
                                               const int NP, const int C, KP kind,
                                               const typename RowOf<T>::type* __restrict__ rows_inv,
@@ -1080,7 +1115,8 @@ __device__ __forceinline__ void bwd_tile_pipe(const int N, const int l0, const i
                                               const T* u_out, const T* g_out, const T* target, T* u_in, T* g_in,
                                               const int ldu, const long tile, const bool first,
                                               typename RowOf<T>::type* __restrict__ stage,
-                                              const Ckpt ck = Ckpt{nullptr, 0, 0}) {
+                                              const Ckpt ck = Ckpt{nullptr, 0, 0},
+                                              const float* __restrict__ scales = nullptr) {
     typedef typename RowOf<T>::type R;
     constexpr int D = 2;  // row register sets: rows requested one layer ahead
     const int nw = blockDim.x >> 6;
@@ -1108,17 +1144,20 @@ __device__ __forceinline__ void bwd_tile_pipe(const int N, const int l0, const i
     const R* rip = rows_inv + (size_t)l0 * NP + q.t;
     const R* rap = rows_adj + (size_t)l0 * NP + q.t;
     R ri[D][W], ra[D][W];
+    float sc[SC ? D : 1][SC ? W : 1];  // per-waveguide adjoint factors (scaled-unitary crossings)
+    const float* scp = SC ? scales + (size_t)l0 * NP + q.t : nullptr;
     int kk[D];
     auto lrow = [&](const int s) { return (size_t)(s < last ? last - s : 0) * NP; };
     if (nl > 0) {
         kk[0] = kp[s_clamp(last, 0)];
         load_rows(ri[0], rip + lrow(0), LPV);
+        if constexpr (SC) load_scl(sc[0], scp + lrow(0), LPV);
         if constexpr (!UNI) load_rows(ra[0], rap + lrow(0), LPV);
     }
-#define MP_PIPE_LAYER(KD, U, G, UO, GO, RI, RA, LC)                                     \
+#define MP_PIPE_LAYER(KD, U, G, UO, GO, RI, RA, LC, SCL)                                \
     {                                                                                   \
         if constexpr (UNI)                                                              \
-            bwd_layer<T, W, V, UNI>(KD, U, G, UO, GO, RI, RI, buf, q, LPV, C, LC);      \
+            bwd_layer<T, W, V, UNI, SC>(KD, U, G, UO, GO, RI, RI, buf, q, LPV, C, LC, SCL); \
         else                                                                            \
             bwd_layer<T, W, V, UNI>(KD, U, G, UO, GO, RI, RA, buf, q, LPV, C, LC);      \
     }
@@ -1158,19 +1197,21 @@ __device__ __forceinline__ void bwd_tile_pipe(const int N, const int l0, const i
             if (lc < nitem) issue(lc);
             kk[1] = kp[s_clamp(last, s + 1)];
             load_rows(ri[1], rip + lrow(s + 1), LPV);
+            if constexpr (SC) load_scl(sc[1 % (SC ? D : 1)], scp + lrow(s + 1), LPV);
             if constexpr (!UNI) load_rows(ra[1], rap + lrow(s + 1), LPV);
-            MP_PIPE_LAYER(kk[0], ua, ga, ub, gb, ri[0], ra[0], lc)
+            MP_PIPE_LAYER(kk[0], ua, ga, ub, gb, ri[0], ra[0], lc, sc[0])
             if (lc < nitem) conv_finish(item, nw, first);
             if (lc + 1 < nitem) issue(lc + 1);
             kk[0] = kp[s_clamp(last, s + 2)];
             load_rows(ri[0], rip + lrow(s + 2), LPV);
+            if constexpr (SC) load_scl(sc[0], scp + lrow(s + 2), LPV);
             if constexpr (!UNI) load_rows(ra[0], rap + lrow(s + 2), LPV);
-            MP_PIPE_LAYER(kk[1], ub, gb, ua, ga, ri[1], ra[1], lc + 1)
+            MP_PIPE_LAYER(kk[1], ub, gb, ua, ga, ri[1], ra[1], lc + 1, sc[1 % (SC ? D : 1)])
             if (lc + 1 < nitem) conv_finish(item, nw, first);
         }
         if (lc < cl) {  // one layer left (last stage of an odd depth)
             if (lc < nitem) issue(lc);
-            MP_PIPE_LAYER(kk[0], ua, ga, ub, gb, ri[0], ra[0], lc)
+            MP_PIPE_LAYER(kk[0], ua, ga, ub, gb, ri[0], ra[0], lc, sc[0])
             if (lc < nitem) conv_finish(item, nw, first);
             copy_state(ub, ua);
             copy_state(gb, ga);
@@ -1200,12 +1241,13 @@ __device__ __forceinline__ void bwd_tile_pipe(const int N, const int l0, const i
 }
 
 // PIPE: 0 = bwd_tile; 4 or 8 = bwd_tile_pipe for workgroups of up to PIPE waves.
-template <class T, int W, int V, bool UNI, int PIPE>
+// SC: scaled-unitary crossings, adjoint rows = scales (per waveguide) times rows_inv.
+template <class T, int W, int V, bool UNI, int PIPE, bool SC = false>
 __global__ void __launch_bounds__(64 * (PIPE ? PIPE : bwd_max_waves(W, V, UNI))) k_bwd(
     int N, int l0, int nl, int B, int LPV, int NP, int C, int np_unused, const int* __restrict__ kind,
     const typename RowOf<T>::type* __restrict__ rows_inv, const typename RowOf<T>::type* __restrict__ rows_adj,
     const T* __restrict__ dtdp, float* __restrict__ partial, size_t part_stride, long ntiles, const T* u_out,
-    const T* g_out, const T* target, T* u_in, T* g_in, int ldu, Ckpt ck) {
+    const T* g_out, const T* target, T* u_in, T* g_in, int ldu, Ckpt ck, const float* __restrict__ scales) {
     typedef typename RowOf<T>::type R;
     extern __shared__ unsigned char smem_raw[];
     // [nw][C][W][LPV] (K-form: [nw][C][NP/2]), twice with PIPE
@@ -1215,13 +1257,13 @@ __global__ void __launch_bounds__(64 * (PIPE ? PIPE : bwd_max_waves(W, V, UNI)))
     bool first = true;
     for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         if constexpr (PIPE > 0)
-            bwd_tile_pipe<T, W, V, UNI, PIPE, ConstInt*>(N, l0, nl, B, LPV, NP, C, (ConstInt*)kind, rows_inv, rows_adj,
-                                                   dtdp, my_partial, u_out, g_out, target, u_in, g_in, ldu, tile,
-                                                   first, stage, ck);
+            bwd_tile_pipe<T, W, V, UNI, PIPE, ConstInt*, SC>(N, l0, nl, B, LPV, NP, C, (ConstInt*)kind, rows_inv,
+                                                             rows_adj, dtdp, my_partial, u_out, g_out, target, u_in,
+                                                             g_in, ldu, tile, first, stage, ck, scales);
         else
-            bwd_tile<T, W, V, UNI, ConstInt*>(N, l0, nl, B, LPV, NP, C, (ConstInt*)kind, rows_inv, rows_adj, dtdp,
-                                              my_partial, u_out, g_out, target, u_in, g_in, ldu, tile, first, stage,
-                                              ck);
+            bwd_tile<T, W, V, UNI, ConstInt*, SC>(N, l0, nl, B, LPV, NP, C, (ConstInt*)kind, rows_inv, rows_adj,
+                                                  dtdp, my_partial, u_out, g_out, target, u_in, g_in, ldu, tile, first,
+                                                  stage, ck, scales);
         first = false;
     }
 }
@@ -1247,7 +1289,9 @@ struct Plan {
     int grid;         // workgroups of the propagation kernel
     size_t rowbytes;  // sizeof(row) = 16 (complex) or 8 (real)
     size_t off_kind, off_a, off_b, off_dtdp, off_partial, total;  // workspace layout
-    int uni;          // crossings are unitary: T^-1 = T^H
+    int uni;          // crossings are unitary (T^-1 = T^H) or, with scaled, scaled unitaries
+    int scaled;       // generic crossings asserted to be c U: adjoint rows = |det T| x inverse rows
+    const float* scales = nullptr;  // scaled: per-waveguide |det T| table (set by the caller of a launch)
 };
 
 // LDS of one wave's stage layer in the adjoint: one entry per waveguide, or one per crossing
@@ -1269,6 +1313,10 @@ int launch_bwd_any(const Plan& P, const int* kind, const void* ri, const void* r
 // The tiles compiled with the max-ilp scheduler (one object per direction).
 int launch_fwd_ilp(const Plan& P, const int* kind, const void* rows, int l0, int nl, const void* u_in, void* u_out,
                    int ldu, hipStream_t st, Ckpt ck);
+// The scaled-unitary adjoints (their own object: only used for balanced lossy meshes).
+int launch_bwd_sc(const Plan& P, const int* kind, const void* ri, const void* ra, const void* dtdp, float* partial,
+                  size_t part_stride, int l0, int nl, const void* u_out, const void* g_out, const void* target,
+                  void* u_in, void* g_in, int ldu, hipStream_t st, Ckpt ck);
 int launch_bwd_ilp(const Plan& P, const int* kind, const void* ri, const void* ra, const void* dtdp, float* partial,
                    size_t part_stride, int l0, int nl, const void* u_out, const void* g_out, const void* target,
                    void* u_in, void* g_in, int ldu, hipStream_t st, Ckpt ck);

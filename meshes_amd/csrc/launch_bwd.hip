@@ -1,33 +1,48 @@
 // launch_bwd.hip -- instantiates and launches the backdiff kernels (unitary and general).
+// Compiled three times: the default object (MP_ILP_SET = MP_SC_SET = 0) holds every tile
+// except ilp_tile() and forwards those to launch_bwd_ilp.hip (MP_ILP_SET = 1, max-ilp
+// scheduler); launch_bwd_sc.hip (MP_SC_SET = 1) holds the scaled-unitary adjoints of every
+// complex tile.
 #include "kernels.h"
 #include "tiles.h"
 
+#ifndef MP_SC_SET
+#define MP_SC_SET 0
+#endif
+
 namespace mp {
 
-template <class T, int W, int V, bool UNI, int PIPE>
+template <class T, int W, int V, bool UNI, int PIPE, bool SC>
 static void go1(const Plan& P, const int* kind, const void* ri, const void* ra, const void* dtdp, float* partial,
                 size_t ps, int l0, int nl, const void* u_out, const void* g_out, const void* target, void* u_in,
                 void* g_in, int ldu, hipStream_t st, Ckpt ck) {
     typedef typename RowOf<T>::type R;
     const size_t shmem = (size_t)P.waves * P.C * stage_layer_bytes(P) * (PIPE ? 2 : 1);
     if (shmem > 64 * 1024)  // beyond the default dynamic LDS limit of a launch
-        hipFuncSetAttribute((const void*)k_bwd<T, W, V, UNI, PIPE>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        hipFuncSetAttribute((const void*)k_bwd<T, W, V, UNI, PIPE, SC>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)shmem);
-    hipLaunchKernelGGL((k_bwd<T, W, V, UNI, PIPE>), dim3(P.grid), dim3(64 * P.waves), shmem, st, P.N, l0, nl, P.B,
-                       P.LPV, P.NP, P.C, P.np, kind, (const R*)ri, (const R*)ra, (const T*)dtdp, partial, ps,
-                       P.ntiles, (const T*)u_out, (const T*)g_out, (const T*)target, (T*)u_in, (T*)g_in, ldu, ck);
+    hipLaunchKernelGGL((k_bwd<T, W, V, UNI, PIPE, SC>), dim3(P.grid), dim3(64 * P.waves), shmem, st, P.N, l0, nl,
+                       P.B, P.LPV, P.NP, P.C, P.np, kind, (const R*)ri, (const R*)ra, (const T*)dtdp, partial, ps,
+                       P.ntiles, (const T*)u_out, (const T*)g_out, (const T*)target, (T*)u_in, (T*)g_in, ldu, ck,
+                       P.scales);
 }
 
 template <class T, int W, int V, int PIPE>
 static void go2(const Plan& P, const int* kind, const void* ri, const void* ra, const void* dtdp, float* partial,
                 size_t ps, int l0, int nl, const void* u_out, const void* g_out, const void* target, void* u_in,
                 void* g_in, int ldu, hipStream_t st, Ckpt ck) {
-    if (P.uni)
-        go1<T, W, V, true, PIPE>(P, kind, ri, ra, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in, g_in, ldu,
-                                 st, ck);
-    else if constexpr (PIPE == 0 || W <= 4)  // the general stage fits two buffers only for W <= 4
-        go1<T, W, V, false, PIPE>(P, kind, ri, ra, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in, g_in, ldu,
-                                  st, ck);
+    if constexpr (MP_SC_SET) {
+        // Scaled unitaries are complex (generic MZI) and run the unitary adjoint.
+        if constexpr (std::is_same<T, float2>::value)
+            go1<T, W, V, true, PIPE, true>(P, kind, ri, ra, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in,
+                                           g_in, ldu, st, ck);
+    } else if (P.uni) {
+        go1<T, W, V, true, PIPE, false>(P, kind, ri, ra, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in, g_in,
+                                        ldu, st, ck);
+    } else if constexpr (PIPE == 0 || W <= 4) {  // the general stage fits two buffers only for W <= 4
+        go1<T, W, V, false, PIPE, false>(P, kind, ri, ra, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in,
+                                         g_in, ldu, st, ck);
+    }
 }
 
 template <class T, int W, int V>
@@ -51,7 +66,14 @@ static void go(const Plan& P, const int* kind, const void* ri, const void* ra, c
     go2<T, W, V, 0>(P, kind, ri, ra, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in, g_in, ldu, st, ck);
 }
 
-#if MP_ILP_SET
+// This object's tiles: every tile for the scaled set, else the ILP split.
+constexpr bool own_tile(int W, int V) { return MP_SC_SET || ilp_tile(W, V) == (MP_ILP_SET != 0); }
+
+#if MP_SC_SET
+int launch_bwd_sc(const Plan& P, const int* kind, const void* ri, const void* ra, const void* dtdp, float* partial,
+                  size_t ps, int l0, int nl, const void* u_out, const void* g_out, const void* target, void* u_in,
+                  void* g_in, int ldu, hipStream_t st, Ckpt ck) {
+#elif MP_ILP_SET
 int launch_bwd_ilp(const Plan& P, const int* kind, const void* ri, const void* ra, const void* dtdp, float* partial,
                    size_t ps, int l0, int nl, const void* u_out, const void* g_out, const void* target, void* u_in,
                    void* g_in, int ldu, hipStream_t st, Ckpt ck) {
@@ -59,10 +81,14 @@ int launch_bwd_ilp(const Plan& P, const int* kind, const void* ri, const void* r
 int launch_bwd_any(const Plan& P, const int* kind, const void* ri, const void* ra, const void* dtdp, float* partial,
                    size_t ps, int l0, int nl, const void* u_out, const void* g_out, const void* target, void* u_in,
                    void* g_in, int ldu, hipStream_t st, Ckpt ck) {
+    if (P.scaled)
+        return P.real ? MESHPROP_ERR_UNSUPPORTED
+                      : launch_bwd_sc(P, kind, ri, ra, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in, g_in,
+                                      ldu, st, ck);
 #endif
 #define MP_CALL(W_, V_)                                                                                           \
     if constexpr (diff_tile_ok(W_, V_)) {                                                                         \
-        if constexpr (ilp_tile(W_, V_) == (MP_ILP_SET != 0)) {                                                    \
+        if constexpr (own_tile(W_, V_)) {                                                                         \
             if (P.real)                                                                                           \
                 go<float, W_, V_>(P, kind, ri, ra, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in, g_in,   \
                                   ldu, st, ck);                                                                   \
@@ -70,7 +96,7 @@ int launch_bwd_any(const Plan& P, const int* kind, const void* ri, const void* r
                 go<float2, W_, V_>(P, kind, ri, ra, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in, g_in,  \
                                    ldu, st, ck);                                                                  \
             return MESHPROP_OK;                                                                                   \
-        } else if constexpr (!MP_ILP_SET) {                                                                       \
+        } else if constexpr (!MP_ILP_SET && !MP_SC_SET) {                                                         \
             return launch_bwd_ilp(P, kind, ri, ra, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in, g_in,   \
                                   ldu, st, ck);                                                                   \
         }                                                                                                         \

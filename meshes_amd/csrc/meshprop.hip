@@ -10,6 +10,8 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <algorithm>
+
 #include "crossing.h"
 #include "meshprop.h"
 #include "propagate.h"
@@ -125,12 +127,15 @@ void resolve_tile(int op, int N, int B, const meshprop_tile* t, int ncu, int& W,
     C = (t && t->chunk >= 1) ? t->chunk : 0;
 }
 
-int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* tile, Plan& P) {
+int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* tile, Plan& P, int flags = 0) {
     if (X < 0 || X > 3) return MESHPROP_ERR_ARG;
     if (op < 0 || op > 3) return MESHPROP_ERR_ARG;
-    // The fused forward + adjoint reconstructs with T^H: unitary crossings only.
-    if (op == MESHPROP_OP_FWDGRAD && X == X_GENERIC) return MESHPROP_ERR_UNSUPPORTED;
     const bool adj = (op == MESHPROP_OP_BACKDIFF || op == MESHPROP_OP_FWDGRAD);
+    // Generic crossings asserted to be scaled unitaries run the unitary adjoint.
+    P.scaled = adj && X == X_GENERIC && (flags & MESHPROP_SCALED_UNITARY) != 0;
+    P.scales = nullptr;
+    // The fused forward + adjoint runs the unitary adjoint: unitary or scaled-unitary crossings.
+    if (op == MESHPROP_OP_FWDGRAD && X == X_GENERIC && !P.scaled) return MESHPROP_ERR_UNSUPPORTED;
     if (N <= 0 || (N & 1) || L < 0 || B < 0) return MESHPROP_ERR_ARG;
     if (N > 1024) return MESHPROP_ERR_UNSUPPORTED;
     P.op = op;
@@ -142,7 +147,7 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
     P.np = n_phase_of(X);
     P.ns = n_split_of(X);
     P.real = (X == X_ORTH);
-    P.uni = (X != X_GENERIC);
+    P.uni = (X != X_GENERIC) || P.scaled;
     const int ncu = num_cus();
     resolve_tile(op, N, B, tile, ncu, P.W, P.V, P.LPV, P.waves, P.C);
     // Unitary adjoints at W = 8 hold one vector per lane unless asked otherwise: the two-vector
@@ -228,6 +233,8 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
     P.off_dtdp = off;
     P.off_partial = off;
     if (adj) {
+        // The conversion table (dT/dp size; the K-form table takes half of it and a
+        // scaled-unitary plan keeps its per-waveguide |det T| table [L][NP] behind it).
         const size_t elem = P.real ? sizeof(float) : sizeof(float2);
         off = align256(off + (size_t)L * (P.NP / 2) * P.np * 4 * elem);
         P.off_partial = off;
@@ -246,13 +253,16 @@ int check_launch() {
     return MESHPROP_OK;
 }
 
+// Offset of the |det T| table of a scaled-unitary plan inside the conversion table region.
+size_t scale_offset(const Plan& P) { return align256((size_t)P.L * P.np * (P.NP / 2) * sizeof(float4)); }
+
 int run_impl(int op, int X, int N, int L, int B, int fft, const int* g0, const int* g1, const float* p, float* dp,
              int ldp, const float* s, int lds, const void* u_a, const void* du_a, void* u_b, void* du_b, int ldu,
              int mode, const void* target, void* ckpt, int ck_every, const meshprop_tile* tile, int flags, void* ws,
              size_t ws_bytes, hipStream_t st) {
     if (mode != 0) return MESHPROP_ERR_UNSUPPORTED;  // gpu/crossing.cu:122-125: only mode 0 exists
     Plan P;
-    int rc = make_plan(op, X, N, L, B, fft, tile, P);
+    int rc = make_plan(op, X, N, L, B, fft, tile, P, flags);
     if (rc) return rc;
     if (B == 0) return MESHPROP_OK;
     if (ldu < N) return MESHPROP_ERR_ARG;
@@ -279,6 +289,8 @@ int run_impl(int op, int X, int N, int L, int B, int fft, const int* g0, const i
     void* tab_b = base + P.off_b;
     void* dtdp = base + P.off_dtdp;
     float* partial = (float*)(base + P.off_partial);
+    float* scl = P.scaled ? (float*)(base + P.off_dtdp + scale_offset(P)) : nullptr;
+    P.scales = scl;
     const float* sp = (P.ns > 0) ? s : nullptr;
 
     const bool do_pro = !(flags & (MESHPROP_SKIP_PROLOGUE | MESHPROP_ONLY_REDUCE | MESHPROP_ONLY_FORWARD));
@@ -292,13 +304,14 @@ int run_impl(int op, int X, int N, int L, int B, int fft, const int* g0, const i
         if (op == MESHPROP_OP_FWDDIFF) tabs = TAB_FWD | TAB_DIFF;
         if (op == MESHPROP_OP_BACKDIFF) tabs = TAB_INV | (P.uni ? TAB_KUNI : (TAB_ADJ | TAB_DTDP));
         if (op == MESHPROP_OP_FWDGRAD) tabs = TAB_FWD | TAB_INV_B | TAB_KUNI;
+        if (P.scaled) tabs |= TAB_SCALE;
         dim3 grid((P.NP / 2 + 255) / 256, L);  // one thread per slot
         if (P.real)
             hipLaunchKernelGGL((k_rows<float2>), grid, dim3(256), 0, st, X, N, L, P.NP, P.LPV, fft, g0, g1, p, ldp, sp, lds,
-                               (const float*)dp, tabs, kind, (float2*)tab_a, (float2*)tab_b, (float*)dtdp);
+                               (const float*)dp, tabs, kind, (float2*)tab_a, (float2*)tab_b, (float*)dtdp, scl);
         else
             hipLaunchKernelGGL((k_rows<float4>), grid, dim3(256), 0, st, X, N, L, P.NP, P.LPV, fft, g0, g1, p, ldp, sp, lds,
-                               (const float*)dp, tabs, kind, (float4*)tab_a, (float4*)tab_b, (float2*)dtdp);
+                               (const float*)dp, tabs, kind, (float4*)tab_a, (float4*)tab_b, (float2*)dtdp, scl);
         if ((rc = check_launch())) return rc;
     }
 
@@ -376,8 +389,12 @@ extern "C" {
 
 size_t meshprop_workspace_size(int op, int X, int N, int L, int B, int fft, const meshprop_tile* tile) {
     Plan P;
-    if (make_plan(op, X, N, L, B, fft, tile, P) != MESHPROP_OK) return 0;
-    return P.total;
+    size_t total = 0;
+    if (make_plan(op, X, N, L, B, fft, tile, P) == MESHPROP_OK) total = P.total;
+    // Generic crossings: enough for the scaled-unitary plan as well (MESHPROP_SCALED_UNITARY).
+    if (X == X_GENERIC && make_plan(op, X, N, L, B, fft, tile, P, MESHPROP_SCALED_UNITARY) == MESHPROP_OK)
+        total = std::max(total, P.total);
+    return total;
 }
 
 int meshprop_resolve_tile(int op, int X, int N, int B, meshprop_tile* out) {

@@ -93,6 +93,8 @@ __device__ __forceinline__ float2 cjim(const float2 g, const float2 x, float2 ac
     return acc;
 }
 __device__ __forceinline__ float cjim(const float, const float, const float acc) { return acc; }
+__device__ __forceinline__ float2 fscl(const float2 a, const float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ float fscl(const float a, const float s) { return a * s; }
 
 __device__ __forceinline__ float4 pack_row(const float2 a, const float2 b) { return make_float4(a.x, a.y, b.x, b.y); }
 __device__ __forceinline__ float2 pack_row(const float a, const float b) { return make_float2(a, b); }
@@ -298,8 +300,9 @@ struct FwdDiffStep {
 // fields keep no A) and B_k += conj(g'_k) x_partner(k).  With g = T g' the gradient is
 // dJ/dp = Re sum_rc K_rc Q_rc, Q_rc = sum_b conj(g'_r) x_c, K = T^H dT/dp anti-Hermitian,
 // which needs Im Q00, Im Q11 and Q01 - conj(Q10) only: 12 FMA per crossing and vector
-// instead of 16.
-template <class T, int W, int V, bool KF = false>
+// instead of 16.  SC (scaled-unitary crossings T = c U): the adjoint rows are |c|^2 times the
+// inverse rows, g' = sc_k (Rinv g), with the per-waveguide factor sc of the layer.
+template <class T, int W, int V, bool KF = false, bool SC = false>
 struct BwdStep {
     typedef typename RowOf<T>::type R;
     const T (&u)[V][W];
@@ -311,6 +314,14 @@ struct BwdStep {
     T (&accA)[W];
     T (&accB)[W];
     int lane;
+    const float* sc;  // SC: this layer's W factors (registers after unrolling)
+    __device__ __forceinline__ T adj(const int k, const T gk, const T gp) const {
+        const T r = rapply(ra[k], gk, gp);
+        if constexpr (SC)
+            return fscl(r, sc[k]);
+        else
+            return r;
+    }
     __device__ __forceinline__ void identity() {
         // Identity layer: fields pass through; its rows meet a zero dT/dp.
         copy_state(u, uo);
@@ -329,7 +340,7 @@ struct BwdStep {
     }
     __device__ __forceinline__ void pair(const int v, const int k, const int j) {
         const T xk = rapply(ri[k], u[v][k], u[v][j]), xj = rapply(ri[j], u[v][j], u[v][k]);
-        const T gk = rapply(ra[k], g[v][k], g[v][j]), gj = rapply(ra[j], g[v][j], g[v][k]);
+        const T gk = adj(k, g[v][k], g[v][j]), gj = adj(j, g[v][j], g[v][k]);
         acc(k, g[v][k], gk, xk, xj);
         acc(j, g[v][j], gj, xj, xk);
         go[v][k] = gk;
@@ -352,8 +363,8 @@ struct BwdStep {
                 const T lu = lane_prev(u[v][W - 1]), lg = lane_prev(g[v][W - 1]);
                 uo[v][W - 1] = rapply(ri[W - 1], u[v][W - 1], hu);
                 uo[v][0] = rapply(ri[0], u[v][0], lu);
-                go[v][W - 1] = rapply(ra[W - 1], g[v][W - 1], hg);
-                go[v][0] = rapply(ra[0], g[v][0], lg);
+                go[v][W - 1] = adj(W - 1, g[v][W - 1], hg);
+                go[v][0] = adj(0, g[v][0], lg);
             }
 #pragma unroll
             for (int v = 0; v < V; ++v) {
@@ -370,7 +381,7 @@ struct BwdStep {
 #pragma unroll
                 for (int k = 0; k < W; ++k) {
                     uo[v][k] = rapply(ri[k], u[v][k], lane_read(u[v][k], src));
-                    go[v][k] = rapply(ra[k], g[v][k], lane_read(g[v][k], src));
+                    go[v][k] = adj(k, g[v][k], lane_read(g[v][k], src));
                 }
 #pragma unroll
             for (int v = 0; v < V; ++v)

@@ -110,6 +110,7 @@ class MeshNetworkGPU:
             self.shifts = torch.as_tensor(sh.astype(np.int32), device=self.device)
             self.strides = None
         self._ws = {}
+        self._balanced = {}  # balanced-gain check of the splitter tensor (generic MZI)
 
     # ---- properties (gpu/mesh.py:95-137) ----------------------------------------------
     @property
@@ -176,6 +177,24 @@ class MeshNetworkGPU:
 
     def _ld(self):
         return self.N // 2 * self.X_np, self.N // 2 * self.X_ns
+
+    def _adj_flags(self, s) -> int:
+        """MESHPROP_SCALED_UNITARY for a generic MZI whose arm gains are balanced in every
+        crossing (g11 = g21, g12 = g22: T = c U with U unitary, since the generic splitters and
+        phase errors are lossless), 0 otherwise.  Such a mesh runs the unitary adjoint on one
+        inverse-row table with T^H = |det T| T^-1.  Checked on the device once per splitter
+        tensor version (one synchronisation)."""
+        if self.X != "mzi_generic":
+            return 0
+        if s.numel() == 0:
+            return _lib.SCALED_UNITARY
+        key = (s.data_ptr(), s._version, s.numel())
+        hit = self._balanced.get(key)
+        if hit is None:
+            q = s.view(-1, 14)
+            hit = bool(torch.logical_and(q[:, 10] == q[:, 11], q[:, 12] == q[:, 13]).all().item())
+            self._balanced = {key: hit}
+        return _lib.SCALED_UNITARY if hit else 0
 
     # ---- propagation (gpu/mesh.py:144-174) --------------------------------------------
     def matrix(self, p_phase=None, p_splitter=None) -> torch.Tensor:
@@ -300,7 +319,8 @@ class MeshNetworkGPU:
             self._screen(_lib.OP_BACKDIFF, B, p[n_cr_p:], out_dp[n_cr_p:], vb, gb)
         ws, nb = self._workspace(_lib.OP_BACKDIFF, B)
         _lib.run(_lib.OP_BACKDIFF, self.X, self.N, self.L, B, self.fft, g0, g1, p, out_dp, ldp,
-                 s if s.numel() else None, lds, vb, gb, uin, gin, self.N, tile=self.tile, workspace=ws, ws_bytes=nb)
+                 s if s.numel() else None, lds, vb, gb, uin, gin, self.N, tile=self.tile, workspace=ws, ws_bytes=nb,
+                 flags=self._adj_flags(s))
         if self.is_phase and self.phi_pos == "in":
             self._screen(_lib.OP_BACKDIFF, B, p[n_cr_p:], out_dp[n_cr_p:], uin, gin)
         return out_dp, self._result(gin, out_dJdv)
@@ -308,9 +328,10 @@ class MeshNetworkGPU:
     def grad_target_l2(self, v, y, p_phase=None, p_splitter=None, out_dp=None):
         """Fused fwd+grad of J = sum |U v - y|^2 (the loss of gpu/test.py:131): returns
         (J, dp, u_out).  dJ/du_out = 2 (u_out - y) is formed inside the adjoint kernel.  For
-        unitary crossings this is one library call (one prologue for all tables, fwdprop,
-        adjoint, dp reduction); the lossy crossing runs fwdprop, then backdiff.  dp is
-        accumulated into out_dp when it is given, else returned fresh."""
+        unitary crossings (and generic ones with balanced arm gains) this is one library call (one
+        prologue for all tables, fwdprop, adjoint, dp reduction); other lossy crossings run
+        fwdprop, then backdiff.  dp is accumulated into out_dp when it is given, else returned
+        fresh."""
         assert not self.is_phase, "the fused target form has no phase screen"
         p, s = self._defaults(p_phase, p_splitter)
         ldp, lds = self._ld()
@@ -324,10 +345,11 @@ class MeshNetworkGPU:
             out_dp = torch.empty(self.n_phase, dtype=torch.float32, device=self.device)
             flags = _lib.DP_ASSIGN
         sp = s if s.numel() else None
-        if self.X != "mzi_generic":
+        sc = self._adj_flags(s)
+        if self.X != "mzi_generic" or sc:
             ws, nb = self._workspace(_lib.OP_FWDGRAD, B)
             _lib.run(_lib.OP_FWDGRAD, self.X, self.N, self.L, B, self.fft, g0, g1, p, out_dp, ldp, sp, lds, vb,
-                     None, uout, None, self.N, target=yb, tile=self.tile, workspace=ws, ws_bytes=nb, flags=flags)
+                     None, uout, None, self.N, target=yb, tile=self.tile, workspace=ws, ws_bytes=nb, flags=flags | sc)
         else:
             ws, nb = self._workspace(_lib.OP_FWDPROP, B)
             _lib.run(_lib.OP_FWDPROP, self.X, self.N, self.L, B, self.fft, g0, g1, p, None, ldp, sp, lds, vb, None,

@@ -56,13 +56,15 @@ def test_workspace_sizes(lib):
 
 
 def test_fwdgrad_plan_and_arguments_without_gpu(lib):
-    """The fused fwd+grad op: a third table beside the adjoint's, unitary crossings only,
-    target required, no checkpoints; all rejected on the host before any launch."""
+    """The fused fwd+grad op: a third table beside the adjoint's, unitary (or asserted
+    scaled-unitary) crossings only, target required, no checkpoints; all rejected on the host
+    before any launch."""
     import meshes_amd._lib as L
     n_b = L.workspace_size(L.OP_BACKDIFF, "mzi", 256, 256, 4096)
     assert L.workspace_size(L.OP_FWDGRAD, "mzi", 256, 256, 4096) >= n_b + 256 * 256 * 16
     assert L.workspace_size(L.OP_FWDGRAD, "orth", 64, 8, 100) > 0
-    assert L.workspace_size(L.OP_FWDGRAD, "mzi_generic", 64, 8, 100) == 0
+    # generic crossings: sized for the scaled-unitary plan, which the call must assert
+    assert L.workspace_size(L.OP_FWDGRAD, "mzi_generic", 64, 8, 100) > 0
     assert L.resolve_tile(L.OP_FWDGRAD, "mzi", 256, 4096) == L.resolve_tile(L.OP_BACKDIFF, "mzi", 256, 4096)
     fake = ctypes.c_void_p(4096)
     geo = (ctypes.c_int * 8)(*([4] * 8))
@@ -71,6 +73,7 @@ def test_fwdgrad_plan_and_arguments_without_gpu(lib):
         return lib.meshprop_run(op, X, 8, 8, 4, 0, geo, geo, fake, dp, 8, None, 0, fake, fake, fake,
                                 None, 8, 0, target, ckpt, every, None, flags, fake, 1 << 30, None)
     assert run(X=L.MZI_GENERIC) == 3
+    assert run(X=L.MZI_GENERIC, flags=L.SCALED_UNITARY, target=None) == 1  # accepted up to its arguments
     assert run(target=None) == 1
     assert run(dp=None) == 1
     assert run(ckpt=fake, every=2) == 3
