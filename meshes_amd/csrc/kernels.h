@@ -618,12 +618,13 @@ __device__ __forceinline__ void kstage_write(const int kd, const T (&a)[W], cons
         }
         return;
     }
-    const int xm = 1 << (st - LW);
+    const int lx = st - LW, xm = 1 << lx;
     const bool top = (q.t & xm) == 0;
     const int src = q.lane ^ xm;
-    // Upper lane tt of the pair: its W slots start at ((tt / 2xm) * xm + tt % xm) * W.
+    // Upper lane tt of the pair: its W slots start at ((tt / 2xm) * xm + tt % xm) * W (xm is a
+    // power of two: shifts and masks).
     const int tt = top ? q.t : (q.t ^ xm);
-    typename RowOf<T>::type* wt = ws + ((tt / (2 * xm)) * xm + (tt % xm)) * W + (top ? 0 : H);
+    typename RowOf<T>::type* wt = ws + ((((tt >> (lx + 1)) << lx) | (tt & (xm - 1))) << LW) + (top ? 0 : H);
     if constexpr (H == 1) {
         // W = 2: the upper lane writes crossing 0 (its waveguide 0, the lower lane's 0), the
         // lower lane crossing 1 (the upper lane's waveguide 1, its own 1).  Both waveguides
