@@ -103,7 +103,8 @@ __device__ __forceinline__ float dc_to<float>(const dc z) {
 }
 
 // K-form coefficients of unitary crossings (TAB_KUNI, written into the dT/dp buffer):
-// K_k = T^H dT/dp_k is anti-Hermitian, [[i a, c], [-conj(c), i d]], stored per (layer,
+// K_k = T^-1 dT/dp_k (= T^H dT/dp_k for unitary T; the same for a scaled unitary T = c U) is
+// anti-Hermitian, [[i a, c], [-conj(c), i d]], stored per (layer,
 // parameter, slot) as float4 (a, d, c.re, c.im); for real (orthogonal) crossings K_k =
 // [[0, c], [-c, 0]] and the float c is stored.  Layout: ktab[(l*np + k)*NPh + m].
 template <class T>
@@ -533,9 +534,10 @@ __global__ void __launch_bounds__(256) k_fwddiff(int N, int L, int B, int LPV, i
 // ---- backdiff -------------------------------------------------------------------------
 // Workgroup = nw waves.  Per layer every lane accumulates its dJdT rows over its V
 // vectors in registers; lane groups are folded with cross-lane adds; the wave writes the
-// rows to an LDS stage of C layers (C even).  At the end of each stage the workgroup sums
-// the waves in a fixed order, converts dJdT to dp with the dT/dp table and writes (or, from
-// its second vector tile on, adds) a per-workgroup partial.  The dp reduction kernels then
+// rows (unitary crossings: one K-form entry per crossing slot) to an LDS stage of C layers.
+// At the end of each stage the workgroup sums the waves in a fixed order, converts to dp
+// with the K-form or dT/dp table and writes (or, from its second vector tile on, adds) a
+// per-workgroup partial.  The dp reduction kernels then
 // sum the partials in a fixed order: the gradient is bitwise reproducible (the reference
 // uses shared and global float atomics, gpu/crossing.cu:179-180, 295-296).
 // Layers [l0, l0 + nl) are walked from the last to the first; checkpointed adjoints run
@@ -802,8 +804,8 @@ __device__ __forceinline__ void convert_slot(const typename RowOf<T>::type* s0, 
     put_dp<NPAR>(outp, val, first_tile);
 }
 
-// Backdiff workgroups hold at most four waves: measured on MI355X, eight-wave workgroups
-// (with the wider LDS stage) did not beat two four-wave workgroups per CU.
+// Workgroups of the unpipelined adjoint stage hold at most four waves (eight-wave workgroups
+// exist only with the pipelined stage, kPipeWaves).
 __host__ __device__ constexpr int bwd_max_waves(int, int, bool) { return 4; }
 static_assert(bwd_max_waves(0, 0, true) <= kStageWaves, "stage conversion sums at most kStageWaves waves");
 
@@ -1005,7 +1007,7 @@ __device__ __forceinline__ void bwd_tile(const int N, const int l0, const int nl
 // ---- software-pipelined adjoint tile ----------------------------------------------------
 // Same arithmetic and the same fixed summation orders as bwd_tile, with the LDS stage
 // double-buffered (two buffers of C layers): the dJdT -> dp conversion of stage s runs while
-// stage s + 1 propagates.  Its stage reads and dT/dp loads are issued just before the first
+// stage s + 1 propagates.  Its stage reads and table loads are issued just before the first
 // layer of stage s + 1 and consumed right after it, so their latency hides under that
 // layer's arithmetic instead of stalling every wave of the workgroup between two barriers;
 // one barrier per stage remains.  Conversion items (layer of the stage, slot) are spread over
