@@ -89,6 +89,69 @@ struct HvpStep {
     }
 };
 
+// Both conversions of one slot in one pass over the stage: rows (A, B) of dJdT in s0 and of
+// its tangent in s1, summed over the waves in wave order; grad_k = Re dJdT . dT/dp_k (e1),
+// hvp_k = Re [d(dJdT) . dT/dp_k + dJdT . S_k] (e1, e2).
+template <int W>
+__device__ __forceinline__ void convert_slot_hvp(const float4* s0, const float4* s1, const int wstride, const int nw,
+                                                 const int LPV, const int NP, float* og, float* oh, const int kd,
+                                                 const int m, const float2 (&e1)[8], const float2 (&e2)[8],
+                                                 const bool first) {
+    int i, j;
+    if (kd <= 1) {
+        i = 2 * m + kd;
+        j = i + 1;
+    } else if (kd != KIND_IDENTITY) {
+        const int st = kd - 2;
+        i = ((m >> st) << (st + 1)) + (m & ((1 << st) - 1));
+        j = i + (1 << st);
+    } else {
+        i = -1;
+        j = 0;
+    }
+    float2* g2 = reinterpret_cast<float2*>(og + m * 2);
+    float2* h2 = reinterpret_cast<float2*>(oh + m * 2);
+    if (i < 0 || j >= NP) {
+        if (first) {
+            *g2 = make_float2(0.f, 0.f);
+            *h2 = make_float2(0.f, 0.f);
+        }
+        return;
+    }
+    const int oi = (i % W) * LPV + i / W, oj = (j % W) * LPV + j / W;
+    float4 a_[kStageWaves], b_[kStageWaves], c_[kStageWaves], d_[kStageWaves];
+#pragma unroll
+    for (int wv = 0; wv < kStageWaves; ++wv)
+        if (wv < nw) {
+            a_[wv] = s0[wv * wstride + oi];
+            b_[wv] = s0[wv * wstride + oj];
+            c_[wv] = s1[wv * wstride + oi];
+            d_[wv] = s1[wv * wstride + oj];
+        }
+    float4 si = a_[0], sj = b_[0], ti = c_[0], tj = d_[0];
+#pragma unroll
+    for (int wv = 1; wv < kStageWaves; ++wv)
+        if (wv < nw) {
+            si = row_add(si, a_[wv]);
+            sj = row_add(sj, b_[wv]);
+            ti = row_add(ti, c_[wv]);
+            tj = row_add(tj, d_[wv]);
+        }
+    float vg[2], vh[2], vs[2];
+    slot_dp<float2, 2>(si, sj, e1, vg);
+    slot_dp<float2, 2>(ti, tj, e1, vh);
+    slot_dp<float2, 2>(si, sj, e2, vs);
+    const float2 gn = make_float2(vg[0], vg[1]), hn = make_float2(vh[0] + vs[0], vh[1] + vs[1]);
+    if (first) {
+        *g2 = gn;
+        *h2 = hn;
+    } else {
+        const float2 go = *g2, ho = *h2;
+        *g2 = make_float2(go.x + gn.x, go.y + gn.y);
+        *h2 = make_float2(ho.x + hn.x, ho.y + hn.y);
+    }
+}
+
 constexpr int kHvpWaves = 4;
 static_assert(kHvpWaves <= kStageWaves, "stage conversion sums at most kStageWaves waves");
 constexpr size_t kHvpStage = 64 * 1024;
@@ -126,45 +189,56 @@ __global__ void __launch_bounds__(64 * kHvpWaves) k_hvp(
             load_tile(g, g_out, q, N, B, ldu);
             load_tile(dg, dg_out, q, N, B, ldu);
         }
-        for (int c0 = 0; c0 < L; c0 += C) {
-            const int cl = (C < L - c0) ? C : L - c0;
-            for (int lc = 0; lc < cl; ++lc) {
-                const int la = L - 1 - (c0 + lc);
-                const int kd = kind[la];
-                float4 r[W], d[W];
-                load_rows(r, rows_r + (size_t)la * NP + q.t, LPV);
-                load_rows(d, rows_d + (size_t)la * NP + q.t, LPV);
-                float2 uo[W], duo[W], go[W], dgo[W], A[W], Bq[W], dA[W], dB[W];
+        // Layers are walked as s = 0, 1, ... (layer L-1-s) in pairs with two row buffers, so the
+        // rows of layer s+1 are in flight while layer s is applied; every C layers the stage is
+        // converted.
+        // W = 8 keeps one row buffer (loaded right before its layer): two do not fit the
+        // registers of the wide tile.
+        constexpr bool kPrefetch = W <= 4;
+        float4 rA[W], dA_[W], rB[kPrefetch ? W : 1], dB_[kPrefetch ? W : 1];
+        auto rows_of = [&](const int s, float4 (&r)[W], float4 (&d)[W]) {
+            if (s < L) {
+                load_rows(r, rows_r + (size_t)(L - 1 - s) * NP + q.t, LPV);
+                load_rows(d, rows_d + (size_t)(L - 1 - s) * NP + q.t, LPV);
+            }
+        };
+        auto step = [&](const int s, const float4 (&r)[W], const float4 (&d)[W]) {
+            const int kd = kind[L - 1 - s];
+            const int lc = s % C;
+            float2 uo[W], duo[W], go[W], dgo[W], A[W], Bq[W], dA[W], dB[W];
 #pragma unroll
-                for (int k = 0; k < W; ++k) A[k] = Bq[k] = dA[k] = dB[k] = make_float2(0.f, 0.f);
-                HvpStep<W> st{u[0], du[0], g[0], dg[0], uo, duo, go, dgo, r, d, A, Bq, dA, dB, lane};
-                dispatch_kind<W>(kd, st);
-                // Fold lane groups (they hold the same waveguides of different vectors).
-                for (int off = LPV; off < 64; off <<= 1) {
-#pragma unroll
-                    for (int k = 0; k < W; ++k) {
-                        A[k] = fadd(A[k], lane_read(A[k], lane ^ off));
-                        Bq[k] = fadd(Bq[k], lane_read(Bq[k], lane ^ off));
-                        dA[k] = fadd(dA[k], lane_read(dA[k], lane ^ off));
-                        dB[k] = fadd(dB[k], lane_read(dB[k], lane ^ off));
-                    }
-                }
-                if (q.grp == 0) {
-                    float4* s0 = stage + (((size_t)q.wave * C + lc) * 2) * W * LPV;
-#pragma unroll
-                    for (int k = 0; k < W; ++k) {
-                        s0[k * LPV + q.t] = pack_row(A[k], Bq[k]);
-                        s0[(W + k) * LPV + q.t] = pack_row(dA[k], dB[k]);
-                    }
-                }
+            for (int k = 0; k < W; ++k) A[k] = Bq[k] = dA[k] = dB[k] = make_float2(0.f, 0.f);
+            HvpStep<W> st{u[0], du[0], g[0], dg[0], uo, duo, go, dgo, r, d, A, Bq, dA, dB, lane};
+            dispatch_kind<W>(kd, st);
+            // Fold lane groups (they hold the same waveguides of different vectors).
+            for (int off = LPV; off < 64; off <<= 1) {
 #pragma unroll
                 for (int k = 0; k < W; ++k) {
-                    u[0][k] = uo[k];
-                    du[0][k] = duo[k];
-                    g[0][k] = go[k];
-                    dg[0][k] = dgo[k];
+                    A[k] = fadd(A[k], lane_read(A[k], lane ^ off));
+                    Bq[k] = fadd(Bq[k], lane_read(Bq[k], lane ^ off));
+                    dA[k] = fadd(dA[k], lane_read(dA[k], lane ^ off));
+                    dB[k] = fadd(dB[k], lane_read(dB[k], lane ^ off));
                 }
             }
+            if (q.grp == 0) {
+                float4* s0 = stage + (((size_t)q.wave * C + lc) * 2) * W * LPV;
+#pragma unroll
+                for (int k = 0; k < W; ++k) {
+                    s0[k * LPV + q.t] = pack_row(A[k], Bq[k]);
+                    s0[(W + k) * LPV + q.t] = pack_row(dA[k], dB[k]);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < W; ++k) {
+                u[0][k] = uo[k];
+                du[0][k] = duo[k];
+                g[0][k] = go[k];
+                dg[0][k] = dgo[k];
+            }
+        };
+        // Stage end after layer s: convert its cl layers (gradient and HVP of every slot).
+        auto convert = [&](const int s) {
+            const int c0 = s - s % C, cl = s % C + 1;
             __syncthreads();
             for (int lcc = wave_u; lcc < cl; lcc += nw) {
                 const int la = L - 1 - (c0 + lcc);
@@ -182,12 +256,29 @@ __global__ void __launch_bounds__(64 * kHvpWaves) k_hvp(
                         e1[e] = t1[e * NPh + m];
                         e2[e] = t2[e * NPh + m];
                     }
-                    convert_slot<float2, W, 2>(s0, wstride, nw, LPV, NP, og, kdl, m, e1, first);
-                    convert_slot<float2, W, 2>(s1, wstride, nw, LPV, NP, oh, kdl, m, e1, first);
-                    convert_slot<float2, W, 2>(s0, wstride, nw, LPV, NP, oh, kdl, m, e2, false);
+                    convert_slot_hvp<W>(s0, s1, wstride, nw, LPV, NP, og, oh, kdl, m, e1, e2, first);
                 }
             }
             __syncthreads();
+        };
+        if constexpr (kPrefetch) {
+            rows_of(0, rA, dA_);
+            for (int s = 0; s < L; s += 2) {
+                rows_of(s + 1, rB, dB_);
+                step(s, rA, dA_);
+                if (s % C == C - 1 || s == L - 1) convert(s);
+                if (s + 1 < L) {
+                    rows_of(s + 2, rA, dA_);
+                    step(s + 1, rB, dB_);
+                    if ((s + 1) % C == C - 1 || s + 1 == L - 1) convert(s + 1);
+                }
+            }
+        } else {
+            for (int s = 0; s < L; ++s) {
+                rows_of(s, rA, dA_);
+                step(s, rA, dA_);
+                if (s % C == C - 1 || s == L - 1) convert(s);
+            }
         }
         first = false;
     }
