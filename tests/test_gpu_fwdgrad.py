@@ -129,17 +129,19 @@ def test_fwdgrad_c2_full_size(gpu):
     assert abs(J.item() - torch.sum(torch.abs(out_f - yb) ** 2).item()) <= 1e-6 * J.item()
 
 
-@pytest.mark.parametrize("N,L,B", [(256, 24, 300), (128, 17, 1000), (200, 9, 77)])
-def test_fwdgrad_eight_wave_workgroups(gpu, N, L, B):
+@pytest.mark.parametrize("X,N,L,B", [("mzi", 256, 24, 300), ("mzi", 128, 17, 1000), ("mzi", 200, 9, 77),
+                                     ("orth", 256, 12, 300), ("sym", 256, 10, 200)])
+def test_fwdgrad_eight_wave_workgroups(gpu, X, N, L, B):
     """Adjoint workgroups of eight waves (pipelined stage, one workgroup per CU): same output,
-    dp and dJ/du_in as the four-wave default within the oracle bar (different partial sums)."""
+    dp and dJ/du_in as the four-wave default within the oracle bar (different partial sums);
+    complex and real (orth) fields."""
     import meshes_amd._lib as Lb
-    g, p, s, u, rng = _setup("mzi", "nn", N, L, B, seed=N + L)
-    mesh = _mesh(g, "mzi", p, s)
-    y = O.unit_cn(rng, B, N).astype(np.complex64)
+    g, p, s, u, rng = _setup(X, "nn", N, L, B, seed=N + L)
+    mesh = _mesh(g, X, p, s)
+    y = O.unit_cn(rng, B, N, real=O.REAL_FIELD[X]).astype(u.dtype)
     ub, yb = torch.as_tensor(u).cuda().contiguous(), torch.as_tensor(y).cuda().contiguous()
-    w = Lb.resolve_tile(Lb.OP_FWDGRAD, "mzi", N, B)[0]
-    assert Lb.resolve_tile(Lb.OP_FWDGRAD, "mzi", N, B, (w, 0, 8, 0))[2] == 8
+    w = Lb.resolve_tile(Lb.OP_FWDGRAD, X, N, B)[0]
+    assert Lb.resolve_tile(Lb.OP_FWDGRAD, X, N, B, (w, 0, 8, 0))[2] == 8
     res = []
     for tile in (None, (w, 0, 8, 0)):
         out, gin, dp = torch.empty_like(ub), torch.empty_like(ub), torch.zeros_like(mesh.p_phase)
@@ -148,8 +150,9 @@ def test_fwdgrad_eight_wave_workgroups(gpu, N, L, B):
     torch.cuda.synchronize()
     assert torch.equal(res[0][0], res[1][0])
     p64, s64 = p.astype(np.float64), _f64(s)
-    gout = 2 * (_np(res[1][0]).astype(np.complex128) - y)
-    rdp, rgin, _ = O.mesh_vjp(g, "mzi", p64, s64, _np(res[1][0]).astype(np.complex128), gout)
+    ft = np.float64 if O.REAL_FIELD[X] else np.complex128
+    gout = 2 * (_np(res[1][0]).astype(ft) - y)
+    rdp, rgin, _ = O.mesh_vjp(g, X, p64, s64, _np(res[1][0]).astype(ft), gout)
     for out, gin, dp in res:
         assert O.rel_l2(_np(dp).reshape(rdp.shape), rdp) < TOL
         assert O.rel_l2(_np(gin), rgin) < TOL

@@ -73,10 +73,11 @@ def test_scaled_checkpointed_adjoint(gpu, N, K):
     assert O.rel_l2(_np(dp), _np(dpg)) < TOL and O.rel_l2(_np(gin), _np(ging)) < TOL
 
 
-@pytest.mark.parametrize("N,L,B", [(64, 30, 40), (256, 40, 300), (512, 24, 2600)])
+@pytest.mark.parametrize("N,L,B", [(64, 30, 40), (256, 40, 300), (512, 24, 2600), (700, 8, 30)])
 def test_scaled_fwdgrad_one_call(gpu, N, L, B):
     """fwd+grad in one call for balanced generic crossings (reconstruction through T^-1): u_out,
-    dp and dJ/du_in against the oracle; N=512 B=2600 runs eight-wave W=8 workgroups."""
+    dp and dJ/du_in against the oracle; N=512 B=2600 runs eight-wave W=8 workgroups, N=700 the
+    W=16 tile."""
     import meshes_amd._lib as Lb
     g, p, s, u, rng = _balanced(N, L, B, seed=N + L, clem=False)
     mesh = _mesh(g, X, p, s)
