@@ -6,10 +6,11 @@
 // dg = 2 du), one reverse walk over the layers carries (u, du, g, dg):
 //   x  = T^H u,   dx = T^H du + dT_q^H u        (reconstructed input and its tangent)
 //   g' = T^H g,   dg' = T^H dg + dT_q^H g      (adjoint and its tangent)
-//   dJdT += conj(g) x,   d(dJdT) += conj(dg) x + conj(g) dx
-// with dT_q = sum_k q_k dT/dp_k.  At every LDS stage end the workgroup converts, per slot,
-//   grad_k += Re sum dJdT_ij dT_ij/dp_k
-//   hvp_k  += Re sum [d(dJdT)_ij dT_ij/dp_k + dJdT_ij S_k,ij],  S_k = sum_m q_m d2T/dp_k dp_m,
+// with dT_q = sum_k q_k dT/dp_k, and keeps the K-form quantities of Q_rc = sum_b conj(g'_r) x_c
+// (Im Q00, Im Q11, Q01 - conj(Q10)) and of its tangent dQ = sum_b conj(dg') x + conj(g') dx.
+// With K_k = T^H dT/dp_k and its tangent dK_k along q (both anti-Hermitian), every LDS stage
+// end converts, per slot,
+//   grad_k += Re sum K_k Q,   hvp_k += Re sum (K_k dQ + dK_k Q)
 // into per-workgroup partials, reduced afterwards in fixed order.  Unitary complex crossings
 // (MZI, SYM), one vector per lane group, N up to 512.
 #include <algorithm>
@@ -17,6 +18,55 @@
 #include "kernels.h"
 
 namespace mp {
+
+// Tangent of the K-form coefficients along the parameter direction q, for the exact
+// Hessian-vector product of unitary complex crossings: K_k = T^H dT/dp_k is anti-Hermitian for
+// every p, hence so is dK_k = (sum_m q_m dT/dp_m)^H dT/dp_k + T^H S_k with the contracted second
+// derivatives S_k = sum_m q_m d2T/dp_k dp_m; stored like K (float4 (a, d, c) per (layer,
+// parameter, slot), zero for inactive slots).
+__global__ void k_dkform(int X, int N, int L, int NP, int fft, const int* __restrict__ g0, const int* __restrict__ g1,
+                         const float* __restrict__ p, const float* __restrict__ q, int ldp, const float* __restrict__ s,
+                         int lds, float4* __restrict__ dk) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = blockIdx.y;
+    const int NPh = NP / 2;
+    if (l >= L || m >= NPh) return;
+    const int np = n_phase_of(X), ns = n_split_of(X);
+    float4* out = dk + (size_t)l * np * NPh + m;
+    int i;
+    if (fft) {
+        const int st = g0[l];
+        bool ok = st >= 0 && st <= 20;
+        const int S = ok ? (1 << st) : 1;
+        i = 2 * S * (m / S) + (m % S);
+        ok = ok && (i + S < N);
+        if (!ok) i = -1;
+    } else {
+        i = 2 * m + (g1[l] & 1);
+    }
+    int mm = -1;
+    bool upper = false;
+    const bool act = (i >= 0) && wg_role(fft, g0, g1, l, i, N, mm, upper) && upper && mm == m;
+    if (!act) {
+        for (int k = 0; k < np; ++k) out[(size_t)k * NPh] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
+    const float* pp = p + (size_t)l * ldp + (size_t)m * np;
+    const float* qp = q + (size_t)l * ldp + (size_t)m * np;
+    const float* sp = (s && ns) ? s + (size_t)l * lds + (size_t)m * ns : nullptr;
+    M2 Tm, dT[2], S2[2];
+    crossing_eval(X, pp, sp, &Tm, dT);
+    const double qq[2] = {qp[0], np > 1 ? (double)qp[1] : 0.0};
+    crossing_d2q(X, Tm, dT, qq, S2);
+    M2 Dq = mscale(dT[0], qq[0]);
+    if (np > 1) Dq = madd(Dq, mscale(dT[1], qq[1]));
+    for (int k = 0; k < np; ++k) {
+        const M2 dK = madd(mmul(mherm(Dq), dT[k]), mmul(mherm(Tm), S2[k]));
+        const double cr = 0.5 * (dK.a01.r - dK.a10.r), ci = 0.5 * (dK.a01.i + dK.a10.i);
+        out[(size_t)k * NPh] = make_float4((float)dK.a00.i, (float)dK.a11.i, (float)cr, (float)ci);
+    }
+}
+
 
 template <int W>
 struct HvpStep {
@@ -79,78 +129,16 @@ struct HvpStep {
         float2 px[W], pdx[W];
         partners<KC, S>(uo, px, xm);
         partners<KC, S>(duo, pdx, xm);
+        // K-form rows: A = Im(conj(g') x) (.x), B = conj(g') x_partner, and their tangents.
 #pragma unroll
         for (int k = 0; k < W; ++k) {
-            A[k] = cjfma(g[k], uo[k], A[k]);
-            Bq[k] = cjfma(g[k], px[k], Bq[k]);
-            dA[k] = cjfma(g[k], duo[k], cjfma(dg[k], uo[k], dA[k]));
-            dB[k] = cjfma(g[k], pdx[k], cjfma(dg[k], px[k], dB[k]));
+            A[k] = cjim(go[k], uo[k], A[k]);
+            Bq[k] = cjfma(go[k], px[k], Bq[k]);
+            dA[k] = cjim(go[k], duo[k], cjim(dgo[k], uo[k], dA[k]));
+            dB[k] = cjfma(go[k], pdx[k], cjfma(dgo[k], px[k], dB[k]));
         }
     }
 };
-
-// Both conversions of one slot in one pass over the stage: rows (A, B) of dJdT in s0 and of
-// its tangent in s1, summed over the waves in wave order; grad_k = Re dJdT . dT/dp_k (e1),
-// hvp_k = Re [d(dJdT) . dT/dp_k + dJdT . S_k] (e1, e2).
-template <int W>
-__device__ __forceinline__ void convert_slot_hvp(const float4* s0, const float4* s1, const int wstride, const int nw,
-                                                 const int LPV, const int NP, float* og, float* oh, const int kd,
-                                                 const int m, const float2 (&e1)[8], const float2 (&e2)[8],
-                                                 const bool first) {
-    int i, j;
-    if (kd <= 1) {
-        i = 2 * m + kd;
-        j = i + 1;
-    } else if (kd != KIND_IDENTITY) {
-        const int st = kd - 2;
-        i = ((m >> st) << (st + 1)) + (m & ((1 << st) - 1));
-        j = i + (1 << st);
-    } else {
-        i = -1;
-        j = 0;
-    }
-    float2* g2 = reinterpret_cast<float2*>(og + m * 2);
-    float2* h2 = reinterpret_cast<float2*>(oh + m * 2);
-    if (i < 0 || j >= NP) {
-        if (first) {
-            *g2 = make_float2(0.f, 0.f);
-            *h2 = make_float2(0.f, 0.f);
-        }
-        return;
-    }
-    const int oi = (i % W) * LPV + i / W, oj = (j % W) * LPV + j / W;
-    float4 a_[kStageWaves], b_[kStageWaves], c_[kStageWaves], d_[kStageWaves];
-#pragma unroll
-    for (int wv = 0; wv < kStageWaves; ++wv)
-        if (wv < nw) {
-            a_[wv] = s0[wv * wstride + oi];
-            b_[wv] = s0[wv * wstride + oj];
-            c_[wv] = s1[wv * wstride + oi];
-            d_[wv] = s1[wv * wstride + oj];
-        }
-    float4 si = a_[0], sj = b_[0], ti = c_[0], tj = d_[0];
-#pragma unroll
-    for (int wv = 1; wv < kStageWaves; ++wv)
-        if (wv < nw) {
-            si = row_add(si, a_[wv]);
-            sj = row_add(sj, b_[wv]);
-            ti = row_add(ti, c_[wv]);
-            tj = row_add(tj, d_[wv]);
-        }
-    float vg[2], vh[2], vs[2];
-    slot_dp<float2, 2>(si, sj, e1, vg);
-    slot_dp<float2, 2>(ti, tj, e1, vh);
-    slot_dp<float2, 2>(si, sj, e2, vs);
-    const float2 gn = make_float2(vg[0], vg[1]), hn = make_float2(vh[0] + vs[0], vh[1] + vs[1]);
-    if (first) {
-        *g2 = gn;
-        *h2 = hn;
-    } else {
-        const float2 go = *g2, ho = *h2;
-        *g2 = make_float2(go.x + gn.x, go.y + gn.y);
-        *h2 = make_float2(ho.x + hn.x, ho.y + hn.y);
-    }
-}
 
 constexpr int kHvpWaves = 4;
 static_assert(kHvpWaves <= kStageWaves, "stage conversion sums at most kStageWaves waves");
@@ -159,16 +147,16 @@ constexpr size_t kHvpStage = 64 * 1024;
 template <int W>
 __global__ void __launch_bounds__(64 * kHvpWaves) k_hvp(
     int N, int L, int B, int LPV, int NP, int C, const int* __restrict__ kind, const float4* __restrict__ rows_r,
-    const float4* __restrict__ rows_d, const float2* __restrict__ dtdp, const float2* __restrict__ d2,
+    const float4* __restrict__ rows_d, const float4* __restrict__ ktab, const float4* __restrict__ dktab,
     float* __restrict__ part_g, float* __restrict__ part_h, size_t part_stride, long ntiles, const float2* u_out,
     const float2* du_out, const float2* g_out, const float2* dg_out, const float2* target, int ldu) {
     extern __shared__ unsigned char smem_raw[];
-    float4* stage = reinterpret_cast<float4*>(smem_raw);  // [nw][C][2][W][LPV]: (A,B) then (dA,dB)
+    float4* stage = reinterpret_cast<float4*>(smem_raw);  // [nw][C][2][NP/2]: Q entries, then dQ
     const int nw = blockDim.x >> 6;
     const int NPh = NP / 2;
     const int lane = threadIdx.x & 63;
     const int wave_u = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int wstride = C * 2 * W * LPV;
+    const int wstride = C * 2 * NPh;
     float* pg = part_g + (size_t)blockIdx.x * part_stride;
     float* ph = part_h + (size_t)blockIdx.x * part_stride;
     bool first = true;
@@ -214,20 +202,16 @@ __global__ void __launch_bounds__(64 * kHvpWaves) k_hvp(
             for (int off = LPV; off < 64; off <<= 1) {
 #pragma unroll
                 for (int k = 0; k < W; ++k) {
-                    A[k] = fadd(A[k], lane_read(A[k], lane ^ off));
+                    A[k] = fold_im(A[k], lane ^ off);
                     Bq[k] = fadd(Bq[k], lane_read(Bq[k], lane ^ off));
-                    dA[k] = fadd(dA[k], lane_read(dA[k], lane ^ off));
+                    dA[k] = fold_im(dA[k], lane ^ off);
                     dB[k] = fadd(dB[k], lane_read(dB[k], lane ^ off));
                 }
             }
-            if (q.grp == 0) {
-                float4* s0 = stage + (((size_t)q.wave * C + lc) * 2) * W * LPV;
-#pragma unroll
-                for (int k = 0; k < W; ++k) {
-                    s0[k * LPV + q.t] = pack_row(A[k], Bq[k]);
-                    s0[(W + k) * LPV + q.t] = pack_row(dA[k], dB[k]);
-                }
-            }
+            // One entry per crossing slot for Q and for dQ (slot-major, every slot written).
+            float4* sq = stage + ((size_t)__builtin_amdgcn_readfirstlane(q.wave) * C + lc) * 2 * NPh;
+            kstage_write<float2, W>(kd, A, Bq, sq, q);
+            kstage_write<float2, W>(kd, dA, dB, sq + NPh, q);
 #pragma unroll
             for (int k = 0; k < W; ++k) {
                 u[0][k] = uo[k];
@@ -242,21 +226,33 @@ __global__ void __launch_bounds__(64 * kHvpWaves) k_hvp(
             __syncthreads();
             for (int lcc = wave_u; lcc < cl; lcc += nw) {
                 const int la = L - 1 - (c0 + lcc);
-                const int kdl = kind[la];
-                const float4* s0 = stage + (size_t)lcc * 2 * W * LPV;
-                const float4* s1 = s0 + W * LPV;
-                const float2* t1 = dtdp + (size_t)la * NPh * 8;
-                const float2* t2 = d2 + (size_t)la * NPh * 8;
+                const float4* s0 = stage + (size_t)lcc * 2 * NPh;
+                const float4* s1 = s0 + NPh;
+                const float4* k1 = ktab + (size_t)la * 2 * NPh;
+                const float4* k2 = dktab + (size_t)la * 2 * NPh;
                 float* og = pg + (size_t)la * NPh * 2;
                 float* oh = ph + (size_t)la * NPh * 2;
                 for (int m = lane; m < NPh; m += 64) {
-                    float2 e1[8], e2[8];
+                    // Inactive slots have zero coefficients.
+                    const float4 K0 = k1[m], K1 = k1[NPh + m], D0 = k2[m], D1 = k2[NPh + m];
+                    float4 a_[kStageWaves], b_[kStageWaves];
 #pragma unroll
-                    for (int e = 0; e < 8; ++e) {
-                        e1[e] = t1[e * NPh + m];
-                        e2[e] = t2[e * NPh + m];
-                    }
-                    convert_slot_hvp<W>(s0, s1, wstride, nw, LPV, NP, og, oh, kdl, m, e1, e2, first);
+                    for (int wv = 0; wv < kStageWaves; ++wv)
+                        if (wv < nw) {
+                            a_[wv] = s0[wv * wstride + m];
+                            b_[wv] = s1[wv * wstride + m];
+                        }
+                    float4 Q = a_[0], dQ = b_[0];
+#pragma unroll
+                    for (int wv = 1; wv < kStageWaves; ++wv)
+                        if (wv < nw) {
+                            Q = row_add(Q, a_[wv]);
+                            dQ = row_add(dQ, b_[wv]);
+                        }
+                    const float gv[2] = {kdot(Q, K0), kdot(Q, K1)};
+                    const float hv[2] = {kdot(dQ, K0) + kdot(Q, D0), kdot(dQ, K1) + kdot(Q, D1)};
+                    put_dp<2>(og + m * 2, gv, first);
+                    put_dp<2>(oh + m * 2, hv, first);
                 }
             }
             __syncthreads();
@@ -334,7 +330,7 @@ int hvp_plan(int X, int N, int L, int B, HvpPlan& P) {
     if (hipGetDevice(&dev) == hipSuccess)
         if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0) ncu = 256;
     P.grid = (int)std::min<long>(P.ntiles, 2L * ncu);
-    const size_t per_layer = (size_t)kHvpWaves * 2 * P.NP * sizeof(float4);
+    const size_t per_layer = (size_t)kHvpWaves * 2 * (P.NP / 2) * sizeof(float4);  // Q and dQ per slot
     P.C = (int)std::max<size_t>(1, std::min<size_t>(8, kHvpStage / per_layer));
     if ((size_t)P.C * per_layer > kHvpStage) return MESHPROP_ERR_UNSUPPORTED;
     const size_t NPh = P.NP / 2;
@@ -361,11 +357,11 @@ int hvp_plan(int X, int N, int L, int B, HvpPlan& P) {
 template <int W>
 void launch_hvp(const HvpPlan& P, int N, int L, int B, unsigned char* base, const void* u_out, const void* du_out,
                 const void* g, const void* dg, const void* target, int ldu, hipStream_t st) {
-    const size_t shmem = (size_t)kHvpWaves * P.C * 2 * P.NP * sizeof(float4);
+    const size_t shmem = (size_t)kHvpWaves * P.C * 2 * (P.NP / 2) * sizeof(float4);
     hipLaunchKernelGGL((k_hvp<W>), dim3(P.grid), dim3(64 * kHvpWaves), shmem, st, N, L, B, P.LPV, P.NP, P.C,
                        (const int*)(base + P.off_kind), (const float4*)(base + P.off_r),
-                       (const float4*)(base + P.off_d), (const float2*)(base + P.off_t1),
-                       (const float2*)(base + P.off_t2), (float*)(base + P.off_pg), (float*)(base + P.off_ph), P.E,
+                       (const float4*)(base + P.off_d), (const float4*)(base + P.off_t1),
+                       (const float4*)(base + P.off_t2), (float*)(base + P.off_pg), (float*)(base + P.off_ph), P.E,
                        P.ntiles, (const float2*)u_out, (const float2*)du_out, (const float2*)g, (const float2*)dg,
                        (const float2*)target, ldu);
 }
@@ -398,13 +394,13 @@ int meshprop_hvp(int X, int N, int L, int B, int fft, const int* geom0, const in
     unsigned char* base = (unsigned char*)workspace;
     const float* sp = ns ? s : nullptr;
     hipLaunchKernelGGL((k_rows<float4>), dim3((P.NP / 2 + 255) / 256, L), dim3(256), 0, st, X, N, L, P.NP, P.LPV, fft,
-                       geom0, geom1, p, ldp, sp, lds, q, (int)(TAB_INV | TAB_DIFF_ADJ | TAB_DTDP),
+                       geom0, geom1, p, ldp, sp, lds, q, (int)(TAB_INV | TAB_DIFF_ADJ | TAB_KUNI),
                        (int*)(base + P.off_kind), (float4*)(base + P.off_r), (float4*)(base + P.off_d),
                        (float2*)(base + P.off_t1), nullptr);
     if (hipGetLastError() != hipSuccess) return MESHPROP_ERR_HIP;
     const dim3 g2((P.NP / 2 + 255) / 256, L);
-    hipLaunchKernelGGL((k_d2tdp<float2>), g2, dim3(256), 0, st, X, N, L, P.NP, fft, geom0, geom1, p, q, ldp, sp, lds,
-                       (float2*)(base + P.off_t2));
+    hipLaunchKernelGGL(k_dkform, g2, dim3(256), 0, st, X, N, L, P.NP, fft, geom0, geom1, p, q, ldp, sp, lds,
+                       (float4*)(base + P.off_t2));
     if (hipGetLastError() != hipSuccess) return MESHPROP_ERR_HIP;
     switch (P.W) {
         case 2: launch_hvp<2>(P, N, L, B, base, u_out, du_out, dJdu_out, d_dJdu_out, target, ldu, st); break;

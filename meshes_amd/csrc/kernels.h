@@ -142,51 +142,6 @@ struct FieldOfRow<float2> {
     typedef float type;
 };
 
-// Contracted second derivatives S_k = sum_m q_m d2T/dp_k dp_m of every slot, same layout as
-// dT/dp (zero for inactive slots).
-template <class T>
-__global__ void k_d2tdp(int X, int N, int L, int NP, int fft, const int* __restrict__ g0, const int* __restrict__ g1,
-                        const float* __restrict__ p, const float* __restrict__ q, int ldp, const float* __restrict__ s,
-                        int lds, T* __restrict__ d2) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    const int l = blockIdx.y;
-    const int NPh = NP / 2;
-    if (l >= L || m >= NPh) return;
-    const int np = n_phase_of(X), ns = n_split_of(X);
-    T* out = d2 + (size_t)l * NPh * np * 4 + m;
-    int i;
-    if (fft) {
-        const int st = g0[l];
-        bool ok = st >= 0 && st <= 20;
-        const int S = ok ? (1 << st) : 1;
-        i = 2 * S * (m / S) + (m % S);
-        ok = ok && (i + S < N);
-        if (!ok) i = -1;
-    } else {
-        i = 2 * m + (g1[l] & 1);
-    }
-    int mm = -1;
-    bool upper = false;
-    const bool act = (i >= 0) && wg_role(fft, g0, g1, l, i, N, mm, upper) && upper && mm == m;
-    if (!act) {
-        for (int k = 0; k < np * 4; ++k) out[(size_t)k * NPh] = dc_to<T>(mk(0, 0));
-        return;
-    }
-    const float* pp = p + (size_t)l * ldp + (size_t)m * np;
-    const float* qp = q + (size_t)l * ldp + (size_t)m * np;
-    const float* sp = (s && ns) ? s + (size_t)l * lds + (size_t)m * ns : nullptr;
-    M2 Tm, dT[2], S2[2];
-    crossing_eval(X, pp, sp, &Tm, dT);
-    const double qq[2] = {qp[0], np > 1 ? (double)qp[1] : 0.0};
-    crossing_d2q(X, Tm, dT, qq, S2);
-    for (int k = 0; k < np; ++k) {
-        out[(size_t)(k * 4 + 0) * NPh] = dc_to<T>(S2[k].a00);
-        out[(size_t)(k * 4 + 1) * NPh] = dc_to<T>(S2[k].a01);
-        out[(size_t)(k * 4 + 2) * NPh] = dc_to<T>(S2[k].a10);
-        out[(size_t)(k * 4 + 3) * NPh] = dc_to<T>(S2[k].a11);
-    }
-}
-
 // One thread per (layer, slot m < NP/2): the crossing of the slot is evaluated once (FP64) and
 // both of its waveguides' rows are written to every requested table, with its dT/dp for
 // TAB_DTDP; waveguides that no active crossing touches get identity rows (and inactive slots
