@@ -821,7 +821,10 @@ __device__ __forceinline__ void bwd_tile(const int N, const int l0, const int nl
     // 7.2 ms against 11.6 ms for the full form at one wave per SIMD; the two-vector tile in
     // its full form, one wave per SIMD, is faster still at 6.8 ms).
     constexpr bool kLean = (W >= 8) && !UNI && V == 1;
-    constexpr int PF = kLean ? 0 : 2;  // stage-end slots per lane whose dT/dp is prefetched
+    // Lean unitary form for the widest tile (W = 16): its rows are loaded right before their
+    // layer instead of one layer ahead, so one row set is live instead of two.
+    constexpr bool kLeanU = UNI && W >= 16;
+    constexpr int PF = (kLean || kLeanU) ? 0 : 2;  // stage-end slots per lane whose dT/dp is prefetched
     constexpr int DA = kLean ? 1 : D;  // register sets of adjoint rows
     const Lanes q = lanes_of<V>(LPV, tile);
     T ua[V][W], ga[V][W], ub[V][W], gb[V][W];
@@ -849,8 +852,8 @@ __device__ __forceinline__ void bwd_tile(const int N, const int l0, const int nl
     for (int d = 0; d < PD; ++d) {
         if (nl > 0) {
             kk[d] = kp[s_clamp(last, d)];
-            load_rows(ri[d], rip + lrow(d), LPV);
-            if constexpr (SC) load_scl(sc[d], scp + lrow(d), LPV);
+            if constexpr (!kLeanU) load_rows(ri[d], rip + lrow(d), LPV);
+            if constexpr (SC && !kLeanU) load_scl(sc[d], scp + lrow(d), LPV);
             if constexpr (!UNI && !kLean) load_rows(ra[d], rap + lrow(d), LPV);
         }
     }
@@ -864,6 +867,12 @@ __device__ __forceinline__ void bwd_tile(const int N, const int l0, const int nl
     // Adjoint rows of local layer SL into set AD (lean form: loaded right before the layer).
 #define MP_ADJ_ROWS(AD, SL)                                                                         \
     if constexpr (!UNI && kLean) load_rows(ra[AD], rap + lrow(SL), LPV);
+    // Rows (and scaled-unitary factors) of local layer SL into set RD (lean unitary form).
+#define MP_LEAN_U_ROWS(RD, SL)                                                                      \
+    if constexpr (kLeanU) {                                                                         \
+        load_rows(ri[RD], rip + lrow(SL), LPV);                                                     \
+        if constexpr (SC) load_scl(sc[(RD) % SD], scp + lrow(SL), LPV);                             \
+    }
     for (int c0 = 0; c0 < nl; c0 += C) {
         const int cl = (C < nl - c0) ? C : nl - c0;
         if (ck.base != nullptr && c0 > 0) {
@@ -891,10 +900,11 @@ __device__ __forceinline__ void bwd_tile(const int N, const int l0, const int nl
             for (int d = 0; d < D; ++d) {
                 const int dn = (d + PD) % D;  // register set receiving layer s + d + PD
                 kk[dn] = kp[s_clamp(last, s + d + PD)];
-                load_rows(ri[dn], rip + lrow(s + d + PD), LPV);
-                if constexpr (SC) load_scl(sc[dn % SD], scp + lrow(s + d + PD), LPV);
+                if constexpr (!kLeanU) load_rows(ri[dn], rip + lrow(s + d + PD), LPV);
+                if constexpr (SC && !kLeanU) load_scl(sc[dn % SD], scp + lrow(s + d + PD), LPV);
                 if constexpr (!UNI && !kLean) load_rows(ra[dn % DA], rap + lrow(s + d + PD), LPV);
                 MP_ADJ_ROWS(0, s + d)
+                MP_LEAN_U_ROWS(d, s + d)
                 if ((d & 1) == 0)
                     MP_BWD_LAYER(kk[d], ua, ga, ub, gb, ri[d], ra[d % DA], lc + d, sc[d % SD])
                 else
@@ -909,11 +919,12 @@ __device__ __forceinline__ void bwd_tile(const int N, const int l0, const int nl
                 if (d + PD < D) {
                     const int dn = (d + PD) % D;
                     kk[dn] = kp[s_clamp(last, c0 + lc + d + PD)];
-                    load_rows(ri[dn], rip + lrow(c0 + lc + d + PD), LPV);
-                    if constexpr (SC) load_scl(sc[dn % SD], scp + lrow(c0 + lc + d + PD), LPV);
+                    if constexpr (!kLeanU) load_rows(ri[dn], rip + lrow(c0 + lc + d + PD), LPV);
+                    if constexpr (SC && !kLeanU) load_scl(sc[dn % SD], scp + lrow(c0 + lc + d + PD), LPV);
                     if constexpr (!UNI && !kLean) load_rows(ra[dn % DA], rap + lrow(c0 + lc + d + PD), LPV);
                 }
                 MP_ADJ_ROWS(0, c0 + lc + d)
+                MP_LEAN_U_ROWS(d, c0 + lc + d)
                 if ((d & 1) == 0)
                     MP_BWD_LAYER(kk[d], ua, ga, ub, gb, ri[d], ra[d % DA], lc + d, sc[d % SD])
                 else
@@ -955,6 +966,7 @@ __device__ __forceinline__ void bwd_tile(const int N, const int l0, const int nl
     }
 #undef MP_BWD_LAYER
 #undef MP_ADJ_ROWS
+#undef MP_LEAN_U_ROWS
     if (u_in != nullptr) store_tile(ua, u_in, q, N, B, ldu);
     if (g_in != nullptr) store_tile(ga, g_in, q, N, B, ldu);
 }
@@ -1202,6 +1214,35 @@ __device__ __forceinline__ void bwd_tile_pipe(const int N, const int l0, const i
 // SC: scaled-unitary crossings, adjoint rows = scales (per waveguide) times rows_inv.
 template <class T, int W, int V, bool UNI, int PIPE, bool SC = false>
 __global__ void __launch_bounds__(64 * (PIPE ? PIPE : bwd_max_waves(W, V, UNI))) k_bwd(
+    int N, int l0, int nl, int B, int LPV, int NP, int C, int np_unused, const int* __restrict__ kind,
+    const typename RowOf<T>::type* __restrict__ rows_inv, const typename RowOf<T>::type* __restrict__ rows_adj,
+    const T* __restrict__ dtdp, float* __restrict__ partial, size_t part_stride, long ntiles, const T* u_out,
+    const T* g_out, const T* target, T* u_in, T* g_in, int ldu, Ckpt ck, const float* __restrict__ scales) {
+    typedef typename RowOf<T>::type R;
+    extern __shared__ unsigned char smem_raw[];
+    // [nw][C][W][LPV] (K-form: [nw][C][NP/2]), twice with PIPE
+    R* stage = reinterpret_cast<R*>(smem_raw);
+    (void)np_unused;
+    float* my_partial = partial + (size_t)blockIdx.x * part_stride;
+    bool first = true;
+    for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        if constexpr (PIPE > 0)
+            bwd_tile_pipe<T, W, V, UNI, PIPE, ConstInt*, SC>(N, l0, nl, B, LPV, NP, C, (ConstInt*)kind, rows_inv,
+                                                             rows_adj, dtdp, my_partial, u_out, g_out, target, u_in,
+                                                             g_in, ldu, tile, first, stage, ck, scales);
+        else
+            bwd_tile<T, W, V, UNI, ConstInt*, SC>(N, l0, nl, B, LPV, NP, C, (ConstInt*)kind, rows_inv, rows_adj,
+                                                  dtdp, my_partial, u_out, g_out, target, u_in, g_in, ldu, tile, first,
+                                                  stage, ck, scales);
+        first = false;
+    }
+}
+
+// The same kernel for the widest tile (W = 16), compiled for two waves per SIMD: with the lean
+// unitary form its registers fit 256, which the default target (one wave) does not enforce.
+template <class T, int W, int V, bool UNI, int PIPE, bool SC = false>
+__global__ void __launch_bounds__(64 * (PIPE ? PIPE : bwd_max_waves(W, V, UNI)))
+__attribute__((amdgpu_waves_per_eu(2))) k_bwd_w16(
     int N, int l0, int nl, int B, int LPV, int NP, int C, int np_unused, const int* __restrict__ kind,
     const typename RowOf<T>::type* __restrict__ rows_inv, const typename RowOf<T>::type* __restrict__ rows_adj,
     const T* __restrict__ dtdp, float* __restrict__ partial, size_t part_stride, long ntiles, const T* u_out,

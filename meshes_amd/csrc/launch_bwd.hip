@@ -18,13 +18,25 @@ static void go1(const Plan& P, const int* kind, const void* ri, const void* ra, 
                 void* g_in, int ldu, hipStream_t st, Ckpt ck) {
     typedef typename RowOf<T>::type R;
     const size_t shmem = (size_t)P.waves * P.C * stage_layer_bytes(P) * (PIPE ? 2 : 1);
+    // W = 16 unitary adjoints: the entry compiled for two waves per SIMD.
+    constexpr bool w16 = W >= 16 && UNI;
+    const void* fn;
+    if constexpr (w16)
+        fn = (const void*)k_bwd_w16<T, W, V, UNI, PIPE, SC>;
+    else
+        fn = (const void*)k_bwd<T, W, V, UNI, PIPE, SC>;
     if (shmem > 64 * 1024)  // beyond the default dynamic LDS limit of a launch
-        hipFuncSetAttribute((const void*)k_bwd<T, W, V, UNI, PIPE, SC>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)shmem);
-    hipLaunchKernelGGL((k_bwd<T, W, V, UNI, PIPE, SC>), dim3(P.grid), dim3(64 * P.waves), shmem, st, P.N, l0, nl,
-                       P.B, P.LPV, P.NP, P.C, P.np, kind, (const R*)ri, (const R*)ra, (const T*)dtdp, partial, ps,
-                       P.ntiles, (const T*)u_out, (const T*)g_out, (const T*)target, (T*)u_in, (T*)g_in, ldu, ck,
-                       P.scales);
+        hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+    if constexpr (w16)
+        hipLaunchKernelGGL((k_bwd_w16<T, W, V, UNI, PIPE, SC>), dim3(P.grid), dim3(64 * P.waves), shmem, st, P.N, l0,
+                           nl, P.B, P.LPV, P.NP, P.C, P.np, kind, (const R*)ri, (const R*)ra, (const T*)dtdp, partial,
+                           ps, P.ntiles, (const T*)u_out, (const T*)g_out, (const T*)target, (T*)u_in, (T*)g_in, ldu,
+                           ck, P.scales);
+    else
+        hipLaunchKernelGGL((k_bwd<T, W, V, UNI, PIPE, SC>), dim3(P.grid), dim3(64 * P.waves), shmem, st, P.N, l0, nl,
+                           P.B, P.LPV, P.NP, P.C, P.np, kind, (const R*)ri, (const R*)ra, (const T*)dtdp, partial, ps,
+                           P.ntiles, (const T*)u_out, (const T*)g_out, (const T*)target, (T*)u_in, (T*)g_in, ldu, ck,
+                           P.scales);
 }
 
 template <class T, int W, int V, int PIPE>
