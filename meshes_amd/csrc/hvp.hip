@@ -186,8 +186,8 @@ __global__ void __launch_bounds__(64 * kHvpWaves) k_hvp(
         float4 rA[W], dA_[W], rB[kPrefetch ? W : 1], dB_[kPrefetch ? W : 1];
         auto rows_of = [&](const int s, float4 (&r)[W], float4 (&d)[W]) {
             if (s < L) {
-                load_rows(r, rows_r + (size_t)(L - 1 - s) * NP + q.t, LPV);
-                load_rows(d, rows_d + (size_t)(L - 1 - s) * NP + q.t, LPV);
+                load_rows(r, lane_ptr(rows_r + (size_t)(L - 1 - s) * NP, q.t), LPV);
+                load_rows(d, lane_ptr(rows_d + (size_t)(L - 1 - s) * NP, q.t), LPV);
             }
         };
         auto step = [&](const int s, const float4 (&r)[W], const float4 (&d)[W]) {
