@@ -144,8 +144,10 @@ constexpr int kHvpWaves = 4;
 static_assert(kHvpWaves <= kStageWaves, "stage conversion sums at most kStageWaves waves");
 constexpr size_t kHvpStage = 64 * 1024;
 
+// Compiled for at least two waves per SIMD: the W = 8 tile otherwise exceeds 256 registers
+// by a few and runs one wave per SIMD (the smaller tiles already fit more).
 template <int W>
-__global__ void __launch_bounds__(64 * kHvpWaves) k_hvp(
+__global__ void __launch_bounds__(64 * kHvpWaves) __attribute__((amdgpu_waves_per_eu(2))) k_hvp(
     int N, int L, int B, int LPV, int NP, int C, const int* __restrict__ kind, const float4* __restrict__ rows_r,
     const float4* __restrict__ rows_d, const float4* __restrict__ ktab, const float4* __restrict__ dktab,
     float* __restrict__ part_g, float* __restrict__ part_h, size_t part_stride, long ntiles, const float2* u_out,
