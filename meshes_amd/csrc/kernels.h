@@ -356,9 +356,9 @@ __device__ __forceinline__ void load_rows(R (&r)[W], const R* base, const int LP
 // (scalar instructions), the k offset in the scalar offset and the lane's offset in the vector
 // offset, so that no per-layer 64-bit address arithmetic runs on the vector ALU (with plain
 // pointers the compiler kept a 64-bit per-lane address and added to it before every load).
-// Used by fwddiff (two tables per layer: N=1024 L=1024 B=2048 2.26 ms against 2.63 ms) and the
-// HVP walk; the forward and adjoint tiles keep plain pointers, which measured as fast or faster
-// there (N=1024 adjoint 5.40 ms against 5.61 ms, grouped launch 1.59 ms against 1.64 ms).
+// Used by the forward (N=1024 L=1024 B=2048: 1.14 ms against 1.17 ms), fwddiff (two tables per
+// layer: 2.30 ms against 2.63 ms) and the HVP walk; the adjoint tiles keep plain pointers, which
+// measured faster there (N=1024 adjoint 5.40 ms against 5.61 ms with buffer loads).
 template <class R>
 struct LanePtr {
     const R* base;
@@ -420,7 +420,7 @@ __device__ __forceinline__ void fwd_tile(const int N, const int l0, const int nl
     load_tile(ua, u_in, q, N, B, ldu);
     if (nl > 0) {
         const KP kp = kind + l0;
-        const R* rp = rows + (size_t)l0 * NP + q.t;
+        const LanePtr<R> rp = lane_ptr(rows + (size_t)l0 * NP, q.t);
         R ra[W], rb[W];
         int ka = kp[0];
         load_rows(ra, rp, LPV);
