@@ -290,8 +290,8 @@ def main():
         print(json.dumps({"probe_copy_MiB": args.probe_copy, "copies": 5}))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: the workload runs on a single MI355X (batch sharding and the dp "
-              "all-reduce across GPUs are described in DESIGN.md, not run).", file=sys.stderr)
+        print("bench.py measures one GPU: the workload runs on a single MI355X; sharding the batch across "
+              "GPUs (with one all-reduce of dp per step) is not run by this benchmark.", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "batched":

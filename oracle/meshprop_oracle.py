@@ -515,7 +515,7 @@ def random_geometry_nn(rng, N, L):
 
 
 def unit_cn(rng, B, N, real=False):
-    """B i.i.d. CN(0,1) vectors of length N, each normalised to unit norm (BASELINE configs)."""
+    """B i.i.d. CN(0,1) vectors of length N, each normalised to unit norm."""
     if real:
         x = rng.standard_normal((B, N))
     else:
