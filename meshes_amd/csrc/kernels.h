@@ -1354,7 +1354,7 @@ int launch_fwddiff_any(const Plan& P, const int* kind, const void* rows, const v
 int launch_bwd_any(const Plan& P, const int* kind, const void* ri, const void* ra, const void* dtdp, float* partial,
                    size_t part_stride, int l0, int nl, const void* u_out, const void* g_out, const void* target,
                    void* u_in, void* g_in, int ldu, hipStream_t st, Ckpt ck = Ckpt{nullptr, 0, 0});
-// The tiles compiled with the max-ilp scheduler (one object per direction).
+// The tiles compiled with an ILP-oriented scheduler (one object per direction).
 int launch_fwd_ilp(const Plan& P, const int* kind, const void* rows, int l0, int nl, const void* u_in, void* u_out,
                    int ldu, hipStream_t st, Ckpt ck);
 // The scaled-unitary adjoints (their own object: only used for balanced lossy meshes).

@@ -1,6 +1,6 @@
 // launch_bwd.hip -- instantiates and launches the backdiff kernels (unitary and general).
 // Compiled three times: the default object (MP_ILP_SET = MP_SC_SET = 0) holds every tile
-// except ilp_tile() and forwards those to launch_bwd_ilp.hip (MP_ILP_SET = 1, max-ilp
+// except ilp_tile() and forwards those to launch_bwd_ilp.hip (MP_ILP_SET = 1, iterative-ilp
 // scheduler); launch_bwd_sc.hip (MP_SC_SET = 1) holds the scaled-unitary adjoints of every
 // complex tile.
 #include "kernels.h"

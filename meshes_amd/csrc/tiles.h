@@ -22,10 +22,11 @@
         default: break;                          \
     }
 
-// Tiles whose fwdprop and backdiff kernels are compiled with LLVM's max-ilp machine scheduler
-// (launch_fwd_ilp.hip, launch_bwd_ilp.hip; the flag is per translation unit).  Measured at
-// N=256, L=256, B=4096 (two waves per SIMD): 85.6 / 320 us for fwdprop / backdiff against
-// 89.2 / 329 us with the default scheduler.  Applied to every tile it raised register use
+// Tiles whose fwdprop and backdiff kernels are compiled with LLVM's ILP-oriented machine
+// schedulers (launch_fwd_ilp.hip: max-ilp, launch_bwd_ilp.hip: iterative-ilp; the flag is per
+// translation unit).  Measured at N=256, L=256, B=4096 (two waves per SIMD): 85.6 / 320 us for
+// fwdprop / backdiff against 89.2 / 329 us with the default scheduler; with the later adjoint,
+// iterative-ilp 272 us against 276 us for max-ilp.  Applied to every tile it raised register use
 // elsewhere: spills in the wide lossy adjoint and lower occupancy for the small tiles.
 constexpr bool ilp_tile(int W, int V) { return W == 4 && V == 2; }
 
