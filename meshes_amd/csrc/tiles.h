@@ -23,7 +23,7 @@
     }
 
 // Tiles whose fwdprop and backdiff kernels are compiled with LLVM's ILP-oriented machine
-// schedulers (launch_fwd_ilp.hip: max-ilp, launch_bwd_ilp.hip: iterative-ilp; the flag is per
+// schedulers (launch_fwd_ilp.hip and launch_bwd_ilp.hip: iterative-ilp; the flag is per
 // translation unit).  Measured at N=256, L=256, B=4096 (two waves per SIMD): 85.6 / 320 us for
 // fwdprop / backdiff against 89.2 / 329 us with the default scheduler; with the later adjoint,
 // iterative-ilp 272 us against 276 us for max-ilp.  Applied to every tile it raised register use
