@@ -22,7 +22,7 @@ int launch_fwd_any(const Plan& P, const int* kind, const void* rows, int l0, int
 #endif
 #define MP_CALL(W_, V_)                                                          \
     if constexpr (fwd_tile_ok(W_, V_)) {                                         \
-        if constexpr (ilp_tile(W_, V_) == (MP_ILP_SET != 0)) {                   \
+        if constexpr (ilp_fwd_tile(W_, V_) == (MP_ILP_SET != 0)) {               \
             if (P.real)                                                          \
                 go<float, W_, V_>(P, kind, rows, l0, nl, u_in, u_out, ldu, st, ck);  \
             else                                                                 \

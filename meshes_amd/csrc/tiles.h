@@ -29,6 +29,9 @@
 // iterative-ilp 272 us against 276 us for max-ilp.  Applied to every tile it raised register use
 // elsewhere: spills in the wide lossy adjoint and lower occupancy for the small tiles.
 constexpr bool ilp_tile(int W, int V) { return W == 4 && V == 2; }
+// The forward's set adds the W = 16 tiles: N=1024 L=1024 B=2048 fwdprop 1.07 ms against 1.14 ms
+// with the default scheduler (the W = 16 adjoint is slower under it and stays out).
+constexpr bool ilp_fwd_tile(int W, int V) { return ilp_tile(W, V) || W == 16; }
 
 // Each launch_*.hip file is compiled twice: MP_ILP_SET = 0 holds the other tiles and forwards
 // the ILP set to the second object, MP_ILP_SET = 1 (the *_ilp.hip wrapper) holds the ILP set.
