@@ -52,7 +52,8 @@ enum meshprop_status {
     MESHPROP_ERR_ARG = 1,          /* bad size, stride or null pointer */
     MESHPROP_ERR_WORKSPACE = 2,    /* workspace missing or smaller than meshprop_workspace_size */
     MESHPROP_ERR_UNSUPPORTED = 3,  /* mode != 0, N beyond 1024, FFT stride that does not tile N */
-    MESHPROP_ERR_HIP = 4           /* a HIP launch failed; see meshprop_last_hip_error() */
+    MESHPROP_ERR_HIP = 4,          /* a HIP launch failed; see meshprop_last_hip_error() */
+    MESHPROP_ERR_SCRATCH = 5       /* meshprop_run_noalloc: the call needs more than the static scratch */
 };
 
 enum meshprop_op {
@@ -135,6 +136,29 @@ int meshprop_run(int op, int X, int N, int L, int B, int fft, const int *geom0, 
                  const void *u_a, const void *du_a, void *u_b, void *du_b, int ldu, int mode,
                  const void *target, void *ckpt, int ckpt_every, const meshprop_tile *tile, int flags,
                  void *workspace, size_t workspace_bytes, void *stream);
+
+/*
+ * meshprop_run without a workspace argument.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, semantics and results (within
+ * FP32 rounding; bitwise equal where the partials fit, see below) as meshprop_run, with the
+ * workspace replaced by a file-scope static device array of MESHPROP_NOALLOC_SCRATCH_BYTES
+ * (96 MiB) that every device the library runs on holds until the process ends.  The call
+ * allocates nothing (no hipMalloc, hipFree or allocating library routine).  If the row tables
+ * alone exceed the scratch it returns MESHPROP_ERR_SCRATCH; if the adjoint's per-workgroup dp
+ * partials do not fit, the adjoint runs on fewer workgroups (each walking more vector tiles)
+ * so that they do: the summation order then differs from meshprop_run's, not its accuracy.
+ * Every call on a device shares that device's scratch and leaves its contents there (the
+ * stage flags MESHPROP_SKIP_PROLOGUE / MESHPROP_ONLY_REDUCE rely on that): calls that overlap
+ * on one device (different streams or host threads) race and give wrong results.  The first
+ * call on a device looks up the scratch address there (hipGetSymbolAddress, no allocation).
+ */
+#define MESHPROP_NOALLOC_SCRATCH_BYTES ((size_t)96 << 20)
+int meshprop_run_noalloc(int op, int X, int N, int L, int B, int fft, const int *geom0, const int *geom1,
+                         const float *p, float *dp, int ldp, const float *s, int lds,
+                         const void *u_a, const void *du_a, void *u_b, void *du_b, int ldu, int mode,
+                         const void *target, void *ckpt, int ckpt_every, const meshprop_tile *tile, int flags,
+                         void *stream);
 
 /* flags of meshprop_run.  A call is three stages: (1) the prologue that builds the row
  * tables from p, s (and the tangent dp for fwddiff) into the workspace, (2) the

@@ -22,7 +22,8 @@ N_PHASE = {"mzi": 2, "sym": 2, "orth": 1, "mzi_generic": 2}
 N_SPLIT = {"mzi": 2, "sym": 1, "orth": 0, "mzi_generic": 14}
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "workspace missing or too small", 3: "unsupported configuration",
-          4: "HIP launch error"}
+          4: "HIP launch error", 5: "needs more than the static scratch of meshprop_run_noalloc"}
+ERR_SCRATCH = 5
 
 # Every symbol include/meshprop.h declares.
 EXPORTS = (
@@ -32,8 +33,11 @@ EXPORTS = (
     "meshprop_group_create", "meshprop_group_destroy", "meshprop_group_workspace_size",
     "meshprop_group_tables_size", "meshprop_group_upload", "meshprop_group_launch", "meshprop_group_signature",
     "meshprop_group_entry_info", "meshprop_phase_screen", "meshprop_phase_screen_workspace_size",
-    "meshprop_hvp", "meshprop_hvp_workspace_size",
+    "meshprop_hvp", "meshprop_hvp_workspace_size", "meshprop_run_noalloc",
 )
+
+# Bytes of the static device scratch behind meshprop_run_noalloc (MESHPROP_NOALLOC_SCRATCH_BYTES).
+NOALLOC_SCRATCH_BYTES = 96 << 20
 
 
 class MeshpropUnavailable(RuntimeError):
@@ -79,6 +83,8 @@ def load():
                                  + [c_void_p] * 4 + [c_int, c_int] + [c_void_p, c_void_p, c_int]
                                  + [ctypes.POINTER(Tile), c_int, c_void_p, c_size_t, c_void_p])
     lib.meshprop_run.restype = c_int
+    lib.meshprop_run_noalloc.argtypes = lib.meshprop_run.argtypes[:24] + [c_void_p]
+    lib.meshprop_run_noalloc.restype = c_int
     lib.meshprop_resolve_tile.argtypes = [c_int, c_int, c_int, c_int, ctypes.POINTER(Tile)]
     lib.meshprop_resolve_tile.restype = c_int
     lib.meshprop_last_hip_error.restype = c_int
@@ -173,6 +179,20 @@ def run(op, X, N, L, B, fft, g0, g1, p, dp, ldp, s, lds, u_a, du_a, u_b, du_b, l
                           0, _ptr(target), _ptr(ckpt), int(ckpt_every), tile_struct(tile), int(flags), _ptr(workspace),
                           ctypes.c_size_t(ws_bytes), ctypes.c_void_p(stream))
     check(rc, ["fwdprop", "fwddiff", "backdiff", "fwd+grad"][op])
+
+
+def run_noalloc(op, X, N, L, B, fft, g0, g1, p, dp, ldp, s, lds, u_a, du_a, u_b, du_b, ldu, target=None,
+                ckpt=None, ckpt_every=0, tile=None, stream=None, flags=0):
+    """meshprop_run_noalloc: meshprop_run with the library's static device scratch in place of a
+    workspace (nothing is allocated; calls that overlap on one device race)."""
+    lib = load()
+    if stream is None:
+        stream = torch.cuda.current_stream().cuda_stream
+    rc = lib.meshprop_run_noalloc(op, X_CODE[X] if isinstance(X, str) else X, N, L, B, int(fft), _ptr(g0), _ptr(g1),
+                                  _ptr(p), _ptr(dp), ldp, _ptr(s), lds, _ptr(u_a), _ptr(du_a), _ptr(u_b), _ptr(du_b),
+                                  ldu, 0, _ptr(target), _ptr(ckpt), int(ckpt_every), tile_struct(tile), int(flags),
+                                  ctypes.c_void_p(stream))
+    check(rc, ["fwdprop", "fwddiff", "backdiff", "fwd+grad"][op] + " (no-alloc)")
 
 
 def phase_screen(op, N, B, phi, dphi, u, du, ldu, workspace=None, ws_bytes=0, stream=None):

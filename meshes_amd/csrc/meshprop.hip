@@ -5,12 +5,14 @@
 // geometry into per-waveguide row tables (FP64 maths, FP32 storage) plus one kind code
 // per layer, (2) one propagation kernel that keeps each field tile in VGPRs across all L
 // layers, and (3) for backdiff, a fixed-order reduction of per-workgroup dp partials.
-// Kernels never allocate: everything lives in the caller's workspace.
+// Kernels never allocate: everything lives in the caller's workspace (or, for
+// meshprop_run_noalloc, in the library's static device scratch).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "crossing.h"
 #include "meshprop.h"
@@ -127,7 +129,10 @@ void resolve_tile(int op, int N, int B, const meshprop_tile* t, int ncu, int& W,
     C = (t && t->chunk >= 1) ? t->chunk : 0;
 }
 
-int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* tile, Plan& P, int flags = 0) {
+// grid_cap > 0 limits the adjoint's workgroups (and with them its dp partials) further; the
+// workgroups then walk more vector tiles each.
+int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* tile, Plan& P, int flags = 0,
+              int grid_cap = 0) {
     if (X < 0 || X > 3) return MESHPROP_ERR_ARG;
     if (op < 0 || op > 3) return MESHPROP_ERR_ARG;
     const bool adj = (op == MESHPROP_OP_BACKDIFF || op == MESHPROP_OP_FWDGRAD);
@@ -187,6 +192,7 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
         // Cap the number of dp partials; each workgroup then walks several vector tiles.
         const long cap = 2L * ncu;
         if (P.ntiles > cap) P.grid = (int)cap;
+        if (grid_cap > 0 && P.grid > grid_cap) P.grid = grid_cap;
         // LDS stage depth: at least two layers; the stage fits the budget.
         const size_t per_layer = (size_t)P.waves * stage_layer_bytes(P);
         const size_t budget = stage_budget(P.waves);
@@ -259,10 +265,10 @@ size_t scale_offset(const Plan& P) { return align256((size_t)P.L * P.np * (P.NP 
 int run_impl(int op, int X, int N, int L, int B, int fft, const int* g0, const int* g1, const float* p, float* dp,
              int ldp, const float* s, int lds, const void* u_a, const void* du_a, void* u_b, void* du_b, int ldu,
              int mode, const void* target, void* ckpt, int ck_every, const meshprop_tile* tile, int flags, void* ws,
-             size_t ws_bytes, hipStream_t st) {
+             size_t ws_bytes, hipStream_t st, int grid_cap = 0) {
     if (mode != 0) return MESHPROP_ERR_UNSUPPORTED;  // gpu/crossing.cu:122-125: only mode 0 exists
     Plan P;
-    int rc = make_plan(op, X, N, L, B, fft, tile, P, flags);
+    int rc = make_plan(op, X, N, L, B, fft, tile, P, flags, grid_cap);
     if (rc) return rc;
     if (B == 0) return MESHPROP_OK;
     if (ldu < N) return MESHPROP_ERR_ARG;
@@ -383,9 +389,78 @@ int run_impl(int op, int X, int N, int L, int B, int fft, const int* g0, const i
     return MESHPROP_OK;
 }
 
+// ---- static scratch of meshprop_run_noalloc --------------------------------------------
+// Its device address differs per device; host code never uses the array's name as an
+// address (it is a host-side stand-in), only the pointer hipGetSymbolAddress returns.
+constexpr size_t kScratchBytes = MESHPROP_NOALLOC_SCRATCH_BYTES;
+static_assert(kScratchBytes % 256 == 0, "scratch size keeps the workspace alignment");
+constexpr int kMaxDevices = 64;
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+// The scratch's address on the current device, looked up once per device (with that device
+// current) on its first call.
+int scratch_on_current_device(void** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        g_last_hip_error = (int)e;
+        return MESHPROP_ERR_HIP;
+    }
+    if (dev < 0 || dev >= kMaxDevices) return MESHPROP_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || ptr == nullptr) {
+            g_last_hip_error = (int)e;
+            return MESHPROP_ERR_HIP;
+        }
+        if (((uintptr_t)ptr) & 255) return MESHPROP_ERR_WORKSPACE;
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return MESHPROP_OK;
+}
+
+// Workgroup cap under which a call's workspace fits the scratch: 0 when it fits as planned,
+// -1 when it cannot fit (row tables alone too large, or no partial slab fits).
+int scratch_grid_cap(int op, int X, int N, int L, int B, int fft, const meshprop_tile* tile, int flags, int& rc) {
+    Plan P;
+    rc = make_plan(op, X, N, L, B, fft, tile, P, flags);
+    if (rc) return -1;
+    if (P.total <= kScratchBytes) return 0;
+    const bool adj = (op == MESHPROP_OP_BACKDIFF || op == MESHPROP_OP_FWDGRAD);
+    if (!adj || P.off_partial >= kScratchBytes) return -1;
+    const size_t slab = (size_t)L * (P.NP / 2) * P.np * sizeof(float);  // one workgroup's partial
+    const size_t cap = (kScratchBytes - P.off_partial) / slab;  // a multiple of 256 bytes stays in range
+    return cap >= 1 ? (int)std::min<size_t>(cap, (size_t)P.grid) : -1;
+}
+
 }  // namespace
 
 extern "C" {
+
+int meshprop_run_noalloc(int op, int X, int N, int L, int B, int fft, const int* geom0, const int* geom1,
+                         const float* p, float* dp, int ldp, const float* s, int lds, const void* u_a,
+                         const void* du_a, void* u_b, void* du_b, int ldu, int mode, const void* target, void* ckpt,
+                         int ckpt_every, const meshprop_tile* tile, int flags, void* stream) {
+    int rc = MESHPROP_OK;
+    const int cap = scratch_grid_cap(op, X, N, L, B, fft, tile, flags, rc);
+    if (rc) return rc;
+    if (cap < 0) return MESHPROP_ERR_SCRATCH;
+    void* ws = nullptr;
+    if ((rc = scratch_on_current_device(&ws))) return rc;
+    return run_impl(op, X, N, L, B, fft, geom0, geom1, p, dp, ldp, s, lds, u_a, du_a, u_b, du_b, ldu, mode, target,
+                    ckpt, ckpt_every, tile, flags, ws, kScratchBytes, (hipStream_t)stream, cap);
+}
 
 size_t meshprop_workspace_size(int op, int X, int N, int L, int B, int fft, const meshprop_tile* tile) {
     Plan P;
@@ -472,6 +547,7 @@ const char* meshprop_status_string(int status) {
         case MESHPROP_ERR_WORKSPACE: return "workspace missing, too small or not 256-byte aligned";
         case MESHPROP_ERR_UNSUPPORTED: return "unsupported configuration";
         case MESHPROP_ERR_HIP: return "HIP launch error";
+        case MESHPROP_ERR_SCRATCH: return "call needs more device memory than the static scratch of meshprop_run_noalloc";
         default: return "unknown status";
     }
 }

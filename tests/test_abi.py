@@ -34,8 +34,9 @@ def test_exports_every_declared_symbol(lib):
 
 
 def test_status_strings(lib):
-    for rc in range(5):
+    for rc in range(6):
         assert lib.meshprop_status_string(rc)
+    assert b"scratch" in lib.meshprop_status_string(5)
     assert b"gfx950" in lib.meshprop_version()
 
 
@@ -103,6 +104,27 @@ def test_argument_validation_without_gpu(lib):
     assert fn(L.MZI, 2048, 2, 1, fake, fake, fake, 2048, None, 0, fake, fake, 2048, 0, fake, 1 << 20, None) == 3
     # B = 0 is a no-op
     assert fn(L.MZI, 8, 2, 0, fake, fake, fake, 8, None, 0, fake, fake, 8, 0, fake, 1 << 20, None) == 0
+
+
+def test_noalloc_rejects_beyond_scratch_without_gpu(lib):
+    """meshprop_run_noalloc checks on the host, before touching a device, that the call fits
+    its static scratch: row tables beyond MESHPROP_NOALLOC_SCRATCH_BYTES are refused with
+    MESHPROP_ERR_SCRATCH (an adjoint whose partials alone do not fit runs on fewer workgroups
+    instead); argument errors come first, as from meshprop_run."""
+    import meshes_amd._lib as L
+    assert b"96" in open(HEADER, "rb").read().split(b"MESHPROP_NOALLOC_SCRATCH_BYTES")[1][:40]
+    fake = ctypes.c_void_p(4096)
+    geo = (ctypes.c_int * 8192)(*([4] * 8192))
+
+    def run(op, N, Lyr, B, dp=None, target=None):
+        return lib.meshprop_run_noalloc(op, L.MZI, N, Lyr, B, 0, geo, geo, fake, dp, N, None, 0, fake, None, fake,
+                                        None, N, 0, target, None, 0, None, 0, None)
+    # 8192 layers of N=1024 rows: 128 MiB of forward rows alone
+    assert L.workspace_size(L.OP_FWDPROP, "mzi", 1024, 8192, 64) > L.NOALLOC_SCRATCH_BYTES
+    assert run(L.OP_FWDPROP, 1024, 8192, 64) == L.ERR_SCRATCH
+    assert run(L.OP_FWDGRAD, 1024, 8192, 64, dp=fake, target=fake) == L.ERR_SCRATCH
+    assert run(L.OP_FWDPROP, 1023, 8, 64) == 1  # odd N
+    assert run(L.OP_FWDPROP, 2048, 8, 64) == 3  # N beyond 1024
 
 
 def _entry(Lb, **kw):
