@@ -37,8 +37,16 @@ class MeshFunction(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, p, x, mesh: MeshNetworkGPU, ckpt_every: int):
-        p = p.detach().to(torch.float32).contiguous()
-        xb = x.detach().to(mesh.dtype).contiguous()
+        # The kernels read raw pointers: shapes, devices and lazy views are settled here.
+        if x.ndim != 2 or x.shape[1] != mesh.N:
+            raise ValueError(f"x must be (B, N={mesh.N}), got {tuple(x.shape)}")
+        if p.numel() != mesh.n_phase:
+            raise ValueError(f"p must hold n_phase={mesh.n_phase} values, got {p.numel()}")
+        if x.device != mesh.device or p.device != mesh.device:
+            raise ValueError(f"x ({x.device}) and p ({p.device}) must be on the mesh's device {mesh.device}")
+        ctx.p_shape = p.shape
+        p = p.detach().resolve_neg().to(torch.float32).reshape(-1).contiguous()
+        xb = x.detach().resolve_conj().resolve_neg().to(mesh.dtype).contiguous()
         B = xb.shape[0]
         y = torch.empty_like(xb)
         ckpt = None
@@ -56,13 +64,14 @@ class MeshFunction(torch.autograd.Function):
         p, y, ckpt = ctx.saved_tensors
         mesh = ctx.mesh
         B = y.shape[0]
-        g = gy.to(mesh.dtype).contiguous()
+        # A loss built from y.conj() hands in a lazy conjugate view: resolve it to plain data.
+        g = gy.resolve_conj().resolve_neg().to(mesh.dtype).contiguous()
         dp = torch.zeros_like(p)
         gx = torch.empty_like(y)
         xr = torch.empty_like(y)
         _launch(mesh, _lib.OP_BACKDIFF, B, p, dp, y, g, xr, gx, ckpt=ckpt if ckpt.numel() else None,
                 ckpt_every=ctx.ckpt_every)
-        return dp, gx, None, None
+        return dp.view(ctx.p_shape), gx, None, None
 
 
 def mesh_apply(mesh: MeshNetworkGPU, p: torch.Tensor, x: torch.Tensor, ckpt_every: int = 0) -> torch.Tensor:

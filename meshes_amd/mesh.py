@@ -252,6 +252,7 @@ class MeshNetworkGPU:
         if out is None:
             return torch.empty(B, self.N, dtype=self.dtype, device=self.device)
         assert isinstance(out, torch.Tensor) and out.shape == (self.N, B) and out.dtype == self.dtype
+        assert out.device == self.device, f"out is on {out.device}, the mesh on {self.device}"
         t = out.T
         assert t.is_contiguous(), "out must be an (N, B) Fortran-ordered tensor (transpose of a contiguous (B, N))"
         return t
@@ -310,6 +311,7 @@ class MeshNetworkGPU:
             out_dp = torch.zeros(self.n_phase, dtype=torch.float32, device=self.device)
         assert isinstance(out_dp, torch.Tensor) and out_dp.dtype == torch.float32 and out_dp.is_contiguous()
         assert out_dp.numel() == self.n_phase
+        assert out_dp.device == self.device, f"out_dp is on {out_dp.device}, the mesh on {self.device}"
         gin = self._out_buffer(out_dJdv, B)
         uin = torch.empty_like(vb)
         n_cr_p = self.n_cr * self.X_np
@@ -377,26 +379,29 @@ class MeshNetworkGPU:
                  g, dg, None, None, hv, self.N, ws, nb)
         return hv
 
+    def _hvp_difference(self, vec, v, w, p, p_splitter, eps):
+        """Central difference of grad_phi along vec with step eps (gpu/mesh.py:236-251)."""
+        return (self.grad_phi(v, w, p + vec * eps, p_splitter)[0]
+                - self.grad_phi(v, w, p - vec * eps, p_splitter)[0]) / (2 * eps)
+
     def hvp(self, vec, v, w: Callable, p_phase=None, p_splitter=None, eps=1e-3):
         """H vec for the objective whose output gradient is w(v_out).
 
         Where exact_hvp_supported(): forward-over-reverse on the GPU (fwddiff along vec, then the
-        HVP adjoint kernel); the tangent of w along the output tangent is torch.func.jvp of w, or
-        a unit-step central difference when w cannot be traced (exact for the affine gradients of
-        quadratic losses).  Otherwise central differences of grad_phi with step eps, as the
-        reference does (gpu/mesh.py:236-251)."""
+        HVP adjoint kernel), with the tangent of w along the output tangent from torch.func.jvp.
+        Otherwise -- and when w cannot be traced by torch.func.jvp (it leaves torch, e.g. through
+        NumPy or .item(), so its tangent is unknown) -- central differences of grad_phi with step
+        eps, as the reference does (gpu/mesh.py:236-251); that is accurate for any w."""
         p = self.p_phase if p_phase is None else as_params(p_phase, self.n_phase, self.device)
         vec = as_params(vec, self.n_phase, self.device)
         if not self.exact_hvp_supported():
-            return (self.grad_phi(v, w, p + vec * eps, p_splitter)[0]
-                    - self.grad_phi(v, w, p - vec * eps, p_splitter)[0]) / (2 * eps)
+            return self._hvp_difference(vec, v, w, p, p_splitter, eps)
         _, s = self._defaults(p, p_splitter)
         out, dout = self.dot(v, p, p_splitter, dp=vec)          # (N, B) views
         try:
             g, dg = torch.func.jvp(w, (out,), (dout,))
-        except Exception:
-            g = w(out)
-            dg = (w(out + dout) - w(out - dout)) / 2
+        except (RuntimeError, TypeError, ValueError, NotImplementedError):
+            return self._hvp_difference(vec, v, w, p, p_splitter, eps)
         gb = as_field(g, self.N, self.dtype, self.device)
         dgb = as_field(dg, self.N, self.dtype, self.device)
         return self._hvp_exact(vec, out.T, dout.T, gb, dgb, p, s)

@@ -251,18 +251,19 @@ class MeshServer:
                 pass
             m = self.meshes[r.mesh_id]
             g0, g1 = m._geom()
-            ldp = m.N // 2 * m.X_np
+            ldp, lds = m.N // 2 * m.X_np, m.N // 2 * m.X_ns
+            s = m.p_splitter if m.p_splitter.numel() else None  # as the served entries carry them
             ops = [_lib.OP_FWDPROP] + ([_lib.OP_BACKDIFF] if r.op == "grad" else [])
             need = max(_lib.workspace_size(op, m.X, m.N, m.L, r.B, m.fft) for op in ops)
             if ws.numel() < need:
                 ws = torch.empty(need, dtype=torch.uint8, device=self.device)
             r.y = torch.empty((r.B, m.N), dtype=torch.complex64, device=self.device)
-            _lib.run(_lib.OP_FWDPROP, m.X, m.N, m.L, r.B, m.fft, g0, g1, m.p_phase, None, ldp, None, 0, r.x, None,
+            _lib.run(_lib.OP_FWDPROP, m.X, m.N, m.L, r.B, m.fft, g0, g1, m.p_phase, None, ldp, s, lds, r.x, None,
                      r.y, None, m.N, workspace=ws, ws_bytes=need)
             if r.op == "grad":
                 r.dp = torch.zeros(m.n_phase, dtype=torch.float32, device=self.device)
                 rec = torch.empty_like(r.y)
-                _lib.run(_lib.OP_BACKDIFF, m.X, m.N, m.L, r.B, m.fft, g0, g1, m.p_phase, r.dp, ldp, None, 0, r.y,
+                _lib.run(_lib.OP_BACKDIFF, m.X, m.N, m.L, r.B, m.fft, g0, g1, m.p_phase, r.dp, ldp, s, lds, r.y,
                          None, rec, None, m.N, target=r.target, workspace=ws, ws_bytes=need)
             torch.cuda.current_stream().synchronize()
             r.t_done = clock() - t0

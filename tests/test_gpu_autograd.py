@@ -147,3 +147,45 @@ def test_c3_full_size_gradient(gpu, K):
             vals.append(np.sum(np.abs(O.fwd(g, Tp, xn) - tn) ** 2))
         fd = (vals[0] - vals[1]) / (2 * h)
         assert abs(gp[l, m, k] - fd) <= 1e-4 * max(1.0, abs(fd)), (l, m, k, gp[l, m, k], fd)
+
+
+def test_autograd_loss_through_conjugate_view(gpu):
+    """A fidelity loss built from y.conj(), J = sum_b |<y_b, t_b>|^2, hands the backward a lazy
+    conjugate view; its gradients match the FP64 oracle (meshes convention g = 2 dJ/dy* =
+    2 conj(c_b) t_b with c_b = sum_n conj(y_bn) t_bn)."""
+    from meshes_amd.autograd import mesh_apply
+    rng = np.random.RandomState(47)
+    N, B = 32, 40
+    g = O.clements(N)
+    p = rng.uniform(0, 2 * np.pi, (N, N // 2, 2)).astype(np.float32)
+    mesh = _mesh(g, "mzi", p, np.zeros(0))
+    x = torch.as_tensor(O.unit_cn(rng, B, N).astype(np.complex64)).cuda().requires_grad_(True)
+    t = torch.as_tensor(O.unit_cn(rng, B, N).astype(np.complex64)).cuda()
+    pt = mesh.p_phase.clone().requires_grad_(True)
+    y = mesh_apply(mesh, pt, x)
+    loss = torch.sum(torch.abs(torch.sum(y.conj() * t, dim=1)) ** 2)
+    loss.backward()
+    yo = O.mesh_fwd(g, "mzi", p.astype(np.float64), None, x.detach().cpu().numpy().astype(np.complex128))
+    tn = t.cpu().numpy().astype(np.complex128)
+    c = np.sum(np.conj(yo) * tn, axis=1)
+    rdp, rgx, _ = O.mesh_vjp(g, "mzi", p.astype(np.float64), None, yo, 2 * np.conj(c)[:, None] * tn)
+    assert O.rel_l2(pt.grad.cpu().numpy().reshape(rdp.shape), rdp) < TOL
+    assert O.rel_l2(x.grad.cpu().numpy(), rgx) < TOL
+
+
+def test_autograd_rejects_bad_inputs(gpu):
+    """Shapes, sizes and devices are checked before any pointer reaches a kernel."""
+    from meshes_amd.autograd import mesh_apply
+    rng = np.random.RandomState(48)
+    g = O.clements(16)
+    mesh = _mesh(g, "mzi", rng.uniform(0, 6, (16, 8, 2)).astype(np.float32), np.zeros(0))
+    x = torch.as_tensor(O.unit_cn(rng, 5, 16).astype(np.complex64))
+    p = mesh.p_phase.clone()
+    with pytest.raises(ValueError):
+        mesh_apply(mesh, p, x)  # host tensor
+    with pytest.raises(ValueError):
+        mesh_apply(mesh, p, x.cuda().T.contiguous())  # (N, B) instead of (B, N)
+    with pytest.raises(ValueError):
+        mesh_apply(mesh, p[:-1], x.cuda())  # wrong number of phases
+    y = mesh_apply(mesh, p.reshape(16, 8, 2).requires_grad_(True), x.cuda())  # any shape of n_phase values
+    assert y.shape == (5, 16)

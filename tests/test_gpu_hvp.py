@@ -85,3 +85,48 @@ def test_hvp_target_and_hessian(gpu):
     assert O.rel_l2(hv, fd) < 1e-4
     H = mesh.hess_target(U).cpu().numpy()
     assert O.rel_l2(H, H.T) < 1e-4
+
+
+def _quartic_case():
+    from meshes_amd import MeshNetworkGPU
+    rng = np.random.RandomState(9)
+    N, B = 16, 12
+    g = O.clements(N)
+    p = rng.uniform(0, 2 * np.pi, (N, N // 2, 2)).astype(np.float32)
+    mesh = MeshNetworkGPU(N, N, g.lens, g.shifts, p, np.zeros(0), "mzi", is_phase=False)
+    v = O.unit_cn(rng, B, N).astype(np.complex64)
+    q = rng.standard_normal(mesh.n_phase).astype(np.float32)
+
+    def grad(pp):  # FP64 gradient of J = sum |y|^4 (output gradient 4 |y|^2 y)
+        out = O.mesh_fwd(g, "mzi", pp, None, v.astype(np.complex128))
+        return O.mesh_vjp(g, "mzi", pp, None, out, 4 * np.abs(out) ** 2 * out)[0].reshape(-1)
+    h = 1e-5
+    p64, q64 = p.astype(np.float64), q.astype(np.float64).reshape(p.shape)
+    fd = (grad(p64 + h * q64) - grad(p64 - h * q64)) / (2 * h)
+    return mesh, v, q, fd
+
+
+def test_hvp_non_affine_output_gradient(gpu):
+    """hvp with a nonlinear (but traceable) w: w(y) = 4 |y|^2 y, the output gradient of
+    J = sum |y|^4.  The exact path takes w's tangent from torch.func.jvp; checked against an FP64
+    central difference of the oracle's gradient."""
+    mesh, v, q, fd = _quartic_case()
+    assert mesh.exact_hvp_supported()
+
+    def w(y):
+        return 4 * (y * y.conj()).real * y
+    hv = mesh.hvp(q, torch.as_tensor(v.T).cuda(), w).cpu().numpy()
+    assert O.rel_l2(hv, fd) < TOL
+
+
+def test_hvp_untraceable_w_falls_back_to_gradient_difference(gpu):
+    """A w that leaves torch (through NumPy) cannot be traced by torch.func.jvp: hvp then takes
+    the reference's central difference of grad_phi (gpu/mesh.py:236-251), accurate for any w
+    (FP32 gradients, eps = 1e-3: the reference's own 1e-2 gradient bar, gpu/test.py:154)."""
+    mesh, v, q, fd = _quartic_case()
+
+    def w(y):
+        a = y.detach().cpu().numpy()
+        return torch.as_tensor(4 * np.abs(a) ** 2 * a, device=y.device)
+    hv = mesh.hvp(q, torch.as_tensor(v.T).cuda(), w).cpu().numpy()
+    assert O.rel_l2(hv, fd) < 1e-2

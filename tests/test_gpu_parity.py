@@ -204,9 +204,9 @@ def test_edge_sizes(gpu):
 
 def test_c2_full_size(gpu):
     """Config C2 (Clements MZI, N=256, L=256, B=4096, ideal splitters): oracle parity on a
-    256-vector slice and size-independent checks on the whole batch: unitarity, exact
-    reconstruction of the input, the fused L2 gradient equal to the explicit one, and the
-    adjoint identity <g, J dp> = <dp_grad, dp> between the fwddiff and backdiff kernels."""
+    256-vector slice (forward, JVP and gradient) and size-independent checks on the whole
+    batch: unitarity, exact reconstruction of the input, the fused L2 gradient equal to the
+    explicit one, and the adjoint identity <g, J dp> = <dp_grad, dp> between the fwddiff and backdiff kernels."""
     N = 256
     rng = np.random.RandomState(2)
     g = O.clements(N)
@@ -238,6 +238,9 @@ def test_c2_full_size(gpu):
     d[~O.active_mask(g)] = 0
     d = (d / np.linalg.norm(d)).astype(np.float32)
     _, dout = mesh.dot(ut, dp=d)
+    # the JVP (fwddiff) itself against the FP64 oracle on the slice
+    _, dref = O.mesh_jvp(g, "mzi", p.astype(np.float64), None, u[sl].astype(np.complex128), d.astype(np.float64))
+    assert O.rel_l2(_np(dout).T[sl], dref) < TOL
     lhs = torch.sum(torch.real(torch.conj(2 * (out - y)) * dout)).item()
     rhs = float(np.sum(_np(dp_e).astype(np.float64) * d.reshape(-1)))
     assert abs(lhs - rhs) <= 1e-4 * max(abs(rhs), 1e-3)

@@ -92,3 +92,32 @@ def test_oversized_request_is_rejected(gpu):
     reqs = poisson_requests(meshes, rate=1000.0, n=3, seed=7, b_median=512, b_sigma=0.0)
     with pytest.raises(ValueError):
         server.serve(reqs)
+
+
+def test_served_equals_sequential_with_splitter_errors(gpu):
+    """Meshes with splitter imperfections: the sequential baseline carries p_splitter (and its
+    leading dimension) exactly as the served entries do, so both compute the same mesh; both
+    agree with the FP64 oracle."""
+    from meshes_amd import MeshNetworkGPU, geometry
+    from meshes_amd.serving import MeshServer, poisson_requests
+    rng = np.random.RandomState(17)
+    meshes = []
+    for N in (16, 48, 96):
+        lens, shifts = geometry.clements(N)
+        s = (0.05 * rng.standard_normal((N, N // 2, 2))).astype(np.float32)
+        meshes.append(MeshNetworkGPU(N, N, lens, shifts, geometry.uniform_phases(rng, N, N), s, "mzi",
+                                     is_phase=False))
+    server = MeshServer(meshes, hbm_budget=1 << 30, slots=2)
+    reqs = poisson_requests(meshes, rate=20000.0, n=24, seed=6, b_max=40)
+    done, _ = server.serve(reqs)
+    seq, _ = server.serve_sequential(_clone(reqs))
+    for r, q in zip(done, seq):
+        assert O.rel_l2(q.y.cpu().numpy(), r.y.cpu().numpy()) < TOL, (r.rid, r.op)
+        if r.op == "grad":
+            assert O.rel_l2(q.dp.cpu().numpy(), r.dp.cpu().numpy()) < TOL, r.rid
+        m = meshes[r.mesh_id]
+        g = O.clements(m.N)
+        p = m.p_phase.cpu().numpy().astype(np.float64).reshape(m.N, m.N // 2, 2)
+        s = m.p_splitter.cpu().numpy().astype(np.float64).reshape(m.N, m.N // 2, 2)
+        y = O.mesh_fwd(g, "mzi", p, s, r.x.cpu().numpy().astype(np.complex128))
+        assert O.rel_l2(q.y.cpu().numpy(), y) < TOL, (r.rid, m.N)

@@ -329,3 +329,77 @@ def test_hvp_finite_difference_of_gradient(X, kind):
     h = 1e-5
     fd = (grad(p + h * q) - grad(p - h * q)) / (2 * h)
     assert O.rel_l2(hv, fd) < 1e-7
+
+
+def _orth_reference_walk(g, p, u_in, du_in, dp, g_out):
+    """The reference GPU kernel's own orth formulas, restated on one NN mesh:
+    forward matmult_orth u1' = T0 u1 + T32 u2, u2' = -T32 u1 + T0 u2 with T0 = S, T32 = -C
+    (gpu/crossing.cu:132-139, 230-236); tangent matmult_d_orth with dth = dp/2
+    (gpu/crossing.cu:142-147, 264-272); adjoint matmult_bk_orth: reconstruct
+    x1 = T0 u1 - T32 u2, x2 = T32 u1 + T0 u2, then dth += T0 (x2 g1 - x1 g2) - T32 (x1 g1 + x2 g2)
+    with g the layer-output adjoint, g <- (T0 g1 - T32 g2, T32 g1 + T0 g2) (gpu/crossing.cu:320-333),
+    and dp = dth / 2 (dp_mzi_orth, gpu/crossing.cu:200-203).  Uses no function of the oracle."""
+    u, du = u_in.copy(), du_in.copy()
+    for l in range(g.L):
+        m, i, j = g.pairs(l)
+        th = p[l, m, 0]
+        T0, T32, dth = np.sin(th / 2), -np.cos(th / 2), 0.5 * dp[l, m, 0]
+        u1, u2, d1, d2 = u[:, i].copy(), u[:, j].copy(), du[:, i].copy(), du[:, j].copy()
+        du[:, i] = T0 * d1 + T32 * d2 + dth * (-T32 * u1 + T0 * u2)
+        du[:, j] = -T32 * d1 + T0 * d2 + dth * (-T0 * u1 - T32 * u2)
+        u[:, i], u[:, j] = T0 * u1 + T32 * u2, -T32 * u1 + T0 * u2
+    out, dout = u.copy(), du.copy()
+    grad = np.zeros_like(p)
+    gg = g_out.copy()
+    for l in range(g.L - 1, -1, -1):
+        m, i, j = g.pairs(l)
+        th = p[l, m, 0]
+        T0, T32 = np.sin(th / 2), -np.cos(th / 2)
+        u1, u2, g1, g2 = u[:, i].copy(), u[:, j].copy(), gg[:, i].copy(), gg[:, j].copy()
+        x1, x2 = T0 * u1 - T32 * u2, T32 * u1 + T0 * u2
+        d_th = np.sum(T0 * (x2 * g1 - x1 * g2) - T32 * (x1 * g1 + x2 * g2), axis=0)
+        grad[l, m, 0] = 0.5 * d_th
+        u[:, i], u[:, j] = x1, x2
+        gg[:, i], gg[:, j] = T0 * g1 - T32 * g2, T32 * g1 + T0 * g2
+    return out, dout, grad, gg
+
+
+def _orth_case():
+    rng = np.random.RandomState(41)
+    g = O.random_geometry_nn(rng, 14, 9)
+    p = rng.uniform(0, 2 * np.pi, (g.L, 7, 1))
+    u = rng.standard_normal((5, 14))
+    du = rng.standard_normal((5, 14))
+    dp = rng.standard_normal(p.shape)
+    gout = rng.standard_normal((5, 14))
+    return g, p, u, du, dp, gout
+
+
+def test_orth_matches_reference_kernel_formulas():
+    """The oracle's orth forward, JVP and gradient equal the reference kernel's own orth
+    formulas (an independent statement of the orth convention: T, dth = dp/2, and the adjoint
+    accumulation of gpu/crossing.cu:320-333)."""
+    g, p, u, du, dp, gout = _orth_case()
+    out, dout, grad, gin = _orth_reference_walk(g, p, u, du, dp, gout)
+    o_out, o_dout = O.mesh_jvp(g, "orth", p, None, u, dp, du)
+    assert np.max(np.abs(o_out - out)) < 1e-12 and np.max(np.abs(o_dout - dout)) < 1e-12
+    o_dp, o_gin, o_rec = O.mesh_vjp(g, "orth", p, None, out, gout)
+    assert np.max(np.abs(o_dp - grad)) < 1e-12
+    assert np.max(np.abs(o_gin - gin)) < 1e-12 and np.max(np.abs(o_rec - u)) < 1e-12
+
+
+def test_orth_planted_transpose_is_caught(monkeypatch):
+    """A transposed orth T ([[S, C], [-C, S]], still orthogonal, so unitarity and the oracle's
+    own self-consistent FD checks pass) fails the comparison with the reference formulas."""
+    g, p, u, du, dp, gout = _orth_case()
+    out, dout, grad, gin = _orth_reference_walk(g, p, u, du, dp, gout)
+    good = O.crossing_T
+
+    def transposed(X, pp, s=None):
+        T = good(X, pp, s)
+        return np.swapaxes(T, -1, -2).copy() if X == "orth" else T
+    monkeypatch.setattr(O, "crossing_T", transposed)
+    o_dp, _, _ = O.mesh_vjp(g, "orth", p, None, out, gout)
+    assert np.max(np.abs(o_dp - grad)) > 1e-3
+    o_out = O.mesh_fwd(g, "orth", p, None, u)
+    assert np.max(np.abs(o_out - out)) > 1e-3
