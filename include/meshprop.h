@@ -292,7 +292,9 @@ int meshprop_hvp(int X, int N, int L, int B, int fft, const int *geom0, const in
 /* The tile the library would use for a call (after defaults and clamping). */
 int meshprop_resolve_tile(int op, int X, int N, int B, meshprop_tile *out);
 
-/* Last HIP error code seen by a failing call on this thread (hipError_t as int). */
+/* Last HIP error code seen by a failing call on this thread (hipError_t as int).  A HIP error
+ * left pending on the thread by earlier, unrelated work is cleared when a call starts launching,
+ * so MESHPROP_ERR_HIP reports a failure of that call's own launches. */
 int meshprop_last_hip_error(void);
 
 /* Human-readable status. */

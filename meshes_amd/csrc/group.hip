@@ -588,6 +588,7 @@ int meshprop_group_launch(const meshprop_group* g, void* ws, size_t bytes, void*
     if (!ws || bytes < g->total || (((uintptr_t)ws) & 255)) return MESHPROP_ERR_WORKSPACE;
     DeviceGroupConfig* dc = device_config(g->dev);
     if (!dc) return MESHPROP_ERR_ARG;
+    clear_pending_hip_error();
     hipStream_t st = (hipStream_t)stream;
     unsigned char* w = (unsigned char*)ws;
     hipError_t e;

@@ -392,6 +392,7 @@ int meshprop_hvp(int X, int N, int L, int B, int fft, const int* geom0, const in
     const int ns = n_split_of(X);
     if (s && ns && lds < (N / 2) * ns) return MESHPROP_ERR_ARG;
     if (!workspace || workspace_bytes < P.total || (((uintptr_t)workspace) & 255)) return MESHPROP_ERR_WORKSPACE;
+    clear_pending_hip_error();
     hipStream_t st = (hipStream_t)stream;
     unsigned char* base = (unsigned char*)workspace;
     const float* sp = ns ? s : nullptr;

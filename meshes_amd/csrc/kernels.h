@@ -1340,6 +1340,10 @@ struct Plan {
 
 // LDS of one wave's stage layer in the adjoint: one entry per waveguide, or one per crossing
 // slot for the K-form of unitary crossings.
+// A HIP error left pending on the calling thread by earlier, unrelated work is cleared before
+// an entry point's first launch, so the launch checks that follow report this call's own.
+inline void clear_pending_hip_error() { (void)hipGetLastError(); }
+
 inline size_t stage_layer_bytes(const Plan& P) { return (size_t)(P.uni ? P.NP / 2 : P.NP) * P.rowbytes; }
 
 // Tiles compiled into the library.  fwdprop keeps u[V][W]; fwddiff and backdiff keep two

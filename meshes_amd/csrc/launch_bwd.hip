@@ -3,6 +3,8 @@
 // except ilp_tile() and forwards those to launch_bwd_ilp.hip (MP_ILP_SET = 1, iterative-ilp
 // scheduler); launch_bwd_sc.hip (MP_SC_SET = 1) holds the scaled-unitary adjoints of every
 // complex tile.
+#include <atomic>
+
 #include "kernels.h"
 #include "tiles.h"
 
@@ -12,21 +14,35 @@
 
 namespace mp {
 
+// Raises a kernel's dynamic LDS limit above the default 64 KiB once per device (the attribute
+// persists), and reports a failure instead of launching into it.
+static int lds_limit_once(const void* fn, std::atomic<unsigned long long>& done, size_t shmem) {
+    if (shmem <= 64 * 1024) return MESHPROP_OK;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return MESHPROP_ERR_HIP;
+    const unsigned long long bit = 1ull << dev;
+    if (done.load(std::memory_order_acquire) & bit) return MESHPROP_OK;
+    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem) != hipSuccess)
+        return MESHPROP_ERR_HIP;
+    done.fetch_or(bit, std::memory_order_acq_rel);
+    return MESHPROP_OK;
+}
+
 template <class T, int W, int V, bool UNI, int PIPE, bool SC>
-static void go1(const Plan& P, const int* kind, const void* ri, const void* ra, const void* dtdp, float* partial,
-                size_t ps, int l0, int nl, const void* u_out, const void* g_out, const void* target, void* u_in,
-                void* g_in, int ldu, hipStream_t st, Ckpt ck) {
+static int go1(const Plan& P, const int* kind, const void* ri, const void* ra, const void* dtdp, float* partial,
+               size_t ps, int l0, int nl, const void* u_out, const void* g_out, const void* target, void* u_in,
+               void* g_in, int ldu, hipStream_t st, Ckpt ck) {
     typedef typename RowOf<T>::type R;
     const size_t shmem = (size_t)P.waves * P.C * stage_layer_bytes(P) * (PIPE ? 2 : 1);
     // W = 16 unitary adjoints: the entry compiled for two waves per SIMD.
     constexpr bool w16 = W >= 16 && UNI;
+    static std::atomic<unsigned long long> lds_set{0};  // devices whose LDS limit this kernel has raised
     const void* fn;
     if constexpr (w16)
         fn = (const void*)k_bwd_w16<T, W, V, UNI, PIPE, SC>;
     else
         fn = (const void*)k_bwd<T, W, V, UNI, PIPE, SC>;
-    if (shmem > 64 * 1024)  // beyond the default dynamic LDS limit of a launch
-        hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+    if (int rc = lds_limit_once(fn, lds_set, shmem)) return rc;
     if constexpr (w16)
         hipLaunchKernelGGL((k_bwd_w16<T, W, V, UNI, PIPE, SC>), dim3(P.grid), dim3(64 * P.waves), shmem, st, P.N, l0,
                            nl, P.B, P.LPV, P.NP, P.C, P.np, kind, (const R*)ri, (const R*)ra, (const T*)dtdp, partial,
@@ -37,45 +53,39 @@ static void go1(const Plan& P, const int* kind, const void* ri, const void* ra, 
                            P.B, P.LPV, P.NP, P.C, P.np, kind, (const R*)ri, (const R*)ra, (const T*)dtdp, partial, ps,
                            P.ntiles, (const T*)u_out, (const T*)g_out, (const T*)target, (T*)u_in, (T*)g_in, ldu, ck,
                            P.scales);
+    return MESHPROP_OK;
 }
 
+#define MP_GO_ARGS P, kind, ri, ra, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in, g_in, ldu, st, ck
 template <class T, int W, int V, int PIPE>
-static void go2(const Plan& P, const int* kind, const void* ri, const void* ra, const void* dtdp, float* partial,
-                size_t ps, int l0, int nl, const void* u_out, const void* g_out, const void* target, void* u_in,
-                void* g_in, int ldu, hipStream_t st, Ckpt ck) {
+static int go2(const Plan& P, const int* kind, const void* ri, const void* ra, const void* dtdp, float* partial,
+               size_t ps, int l0, int nl, const void* u_out, const void* g_out, const void* target, void* u_in,
+               void* g_in, int ldu, hipStream_t st, Ckpt ck) {
     if constexpr (MP_SC_SET) {
         // Scaled unitaries are complex (generic MZI) and run the unitary adjoint.
-        if constexpr (std::is_same<T, float2>::value)
-            go1<T, W, V, true, PIPE, true>(P, kind, ri, ra, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in,
-                                           g_in, ldu, st, ck);
+        if constexpr (std::is_same<T, float2>::value) return go1<T, W, V, true, PIPE, true>(MP_GO_ARGS);
     } else if (P.uni) {
-        go1<T, W, V, true, PIPE, false>(P, kind, ri, ra, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in, g_in,
-                                        ldu, st, ck);
+        return go1<T, W, V, true, PIPE, false>(MP_GO_ARGS);
     } else if constexpr (PIPE == 0 || W <= 4) {  // the general stage fits two buffers only for W <= 4
-        go1<T, W, V, false, PIPE, false>(P, kind, ri, ra, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in,
-                                         g_in, ldu, st, ck);
+        return go1<T, W, V, false, PIPE, false>(MP_GO_ARGS);
     }
+    return MESHPROP_ERR_UNSUPPORTED;
 }
 
 template <class T, int W, int V>
-static void go(const Plan& P, const int* kind, const void* ri, const void* ra, const void* dtdp, float* partial,
-               size_t ps, int l0, int nl, const void* u_out, const void* g_out, const void* target, void* u_in,
-               void* g_in, int ldu, hipStream_t st, Ckpt ck) {
+static int go(const Plan& P, const int* kind, const void* ri, const void* ra, const void* dtdp, float* partial,
+              size_t ps, int l0, int nl, const void* u_out, const void* g_out, const void* target, void* u_in,
+              void* g_in, int ldu, hipStream_t st, Ckpt ck) {
     // The pipelined stage is compiled for the tiles it can serve (two buffers of two layers
     // fit the LDS budget for W <= 4, and for W = 8 with the per-crossing K-form stage), once
     // for four-wave and once for eight-wave groups.
     if constexpr (W <= 8) {
         if (P.pipe) {
-            if (P.waves > 4)
-                go2<T, W, V, kPipeWaves>(P, kind, ri, ra, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in,
-                                         g_in, ldu, st, ck);
-            else
-                go2<T, W, V, 4>(P, kind, ri, ra, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in, g_in, ldu,
-                                st, ck);
-            return;
+            if (P.waves > 4) return go2<T, W, V, kPipeWaves>(MP_GO_ARGS);
+            return go2<T, W, V, 4>(MP_GO_ARGS);
         }
     }
-    go2<T, W, V, 0>(P, kind, ri, ra, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in, g_in, ldu, st, ck);
+    return go2<T, W, V, 0>(MP_GO_ARGS);
 }
 
 // This object's tiles: every tile for the scaled set, else the ILP split.
@@ -101,13 +111,8 @@ int launch_bwd_any(const Plan& P, const int* kind, const void* ri, const void* r
 #define MP_CALL(W_, V_)                                                                                           \
     if constexpr (diff_tile_ok(W_, V_)) {                                                                         \
         if constexpr (own_tile(W_, V_)) {                                                                         \
-            if (P.real)                                                                                           \
-                go<float, W_, V_>(P, kind, ri, ra, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in, g_in,   \
-                                  ldu, st, ck);                                                                   \
-            else                                                                                                  \
-                go<float2, W_, V_>(P, kind, ri, ra, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in, g_in,  \
-                                   ldu, st, ck);                                                                  \
-            return MESHPROP_OK;                                                                                   \
+            if (P.real) return go<float, W_, V_>(MP_GO_ARGS);                                                    \
+            return go<float2, W_, V_>(MP_GO_ARGS);                                                                \
         } else if constexpr (!MP_ILP_SET && !MP_SC_SET) {                                                         \
             return launch_bwd_ilp(P, kind, ri, ra, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in, g_in,   \
                                   ldu, st, ck);                                                                   \
@@ -115,6 +120,7 @@ int launch_bwd_any(const Plan& P, const int* kind, const void* ri, const void* r
     }
     MP_FOR_WV(P.W, P.V, MP_CALL)
 #undef MP_CALL
+#undef MP_GO_ARGS
     return MESHPROP_ERR_UNSUPPORTED;
 }
 

@@ -288,6 +288,7 @@ int run_impl(int op, int X, int N, int L, int B, int fft, const int* g0, const i
     if (!ckpt || ck_every > L) ck_every = L > 0 ? L : 1;
     if (P.total > 0 && (!ws || ws_bytes < P.total)) return MESHPROP_ERR_WORKSPACE;
     if (((uintptr_t)ws) & 255) return MESHPROP_ERR_WORKSPACE;
+    clear_pending_hip_error();
 
     unsigned char* base = (unsigned char*)ws;
     int* kind = (int*)(base + P.off_kind);

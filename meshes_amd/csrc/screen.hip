@@ -82,6 +82,7 @@ int meshprop_phase_screen(int op, int N, int B, const float* phi, float* dphi, v
         if (!du || !dphi) return MESHPROP_ERR_ARG;
         if (!workspace || workspace_bytes < meshprop_phase_screen_workspace_size(N, B)) return MESHPROP_ERR_WORKSPACE;
     }
+    (void)hipGetLastError();  // clear an error left by earlier work: the checks below report this call's
     hipStream_t st = (hipStream_t)stream;
     const dim3 grid((N + 255) / 256, nblocks(B));
     hipLaunchKernelGGL(k_screen, grid, dim3(256), 0, st, op, N, B, phi, dphi, (float2*)u, (float2*)du, ldu,
