@@ -103,6 +103,27 @@ bool tile_ok(int op, int W, int V) {
     return op == MESHPROP_OP_FWDPROP ? fwd_tile_ok(W, V) : diff_tile_ok(W, V);
 }
 
+// Measured tile table of the plain path (gpu/__init__.py:26-35 keeps the reference's measured
+// warps per kernel, crossing type, square/fat batch and N bucket): the fastest tile of whole
+// meshprop_run calls on square N x N x N and fat N x N x 4096 Clements meshes, written by
+// meshes_amd/tune_tiles.py.  A call is fat when B > 2N (gpu/mesh.py:161) and takes the entry of
+// the next tabulated N at or above its own (gpu/mesh.py:84).
+struct TileEntry {
+    int op, X, fat, n0, w, v, waves;
+};
+const TileEntry kTileTable[] = {
+#include "tile_table.inc"
+    {-1, -1, 0, 0, 0, 0, 0},  // end (matches no call)
+};
+
+const TileEntry* table_tile(int op, int X, int N, int B) {
+    const int fat = (long)B > 2L * N ? 1 : 0;
+    const TileEntry* best = nullptr;
+    for (const TileEntry& e : kTileTable)
+        if (e.op == op && e.X == X && e.fat == fat && e.n0 >= N && (!best || e.n0 < best->n0)) best = &e;
+    return best;
+}
+
 // Default tile: narrowest W that keeps a vector inside one wave; the largest compiled V
 // that still gives two waves per SIMD (8 per CU).
 void resolve_tile(int op, int N, int B, const meshprop_tile* t, int ncu, int& W, int& V, int& LPV, int& waves,
@@ -154,6 +175,18 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
     P.real = (X == X_ORTH);
     P.uni = (X != X_GENERIC) || P.scaled;
     const int ncu = num_cus();
+    // Without an explicit tile, neighbour meshes take the measured table's (a requested stage
+    // depth is kept); butterfly meshes and untabulated cases take the heuristic.
+    meshprop_tile tabled;
+    const bool explicit_tile = tile && (tile->w > 0 || tile->v > 0 || tile->waves > 0);
+    if (!explicit_tile && !fft) {
+        if (const TileEntry* e = table_tile(op, X, N, B)) {
+            if ((long)e->w * 64 >= N && tile_ok(op, e->w, e->v)) {
+                tabled = meshprop_tile{e->w, e->v, e->waves, tile ? tile->chunk : 0};
+                tile = &tabled;
+            }
+        }
+    }
     resolve_tile(op, N, B, tile, ncu, P.W, P.V, P.LPV, P.waves, P.C);
     // Unitary adjoints at W = 8 hold one vector per lane unless asked otherwise: the two-vector
     // tile needs one wave per SIMD (pipelined, eight waves: spills).  Measured at N=512, L=512,

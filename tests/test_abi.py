@@ -171,3 +171,32 @@ def test_group_plan_without_gpu(lib):
     lib.meshprop_group_destroy(plan)
     assert lib.meshprop_group_tables_size(L.MZI, 256, 256, 0, None) >= 256 * 256 * 16 * 2 + 256 * 128 * 64
     assert lib.meshprop_group_tables_size(L.ORTH, 256, 256, 0, None) == 0
+
+
+def test_tile_table_consulted_without_gpu(lib):
+    """A call without a tile takes the measured table's entry for its op, crossing type,
+    square/fat batch (fat: B > 2N, gpu/mesh.py:161) and next tabulated N at or above its own
+    (gpu/mesh.py:84); an explicit tile overrides it.  Host-side planning only."""
+    import re
+    import meshes_amd._lib as L
+    inc = os.path.join(os.path.dirname(__file__), "..", "meshes_amd", "csrc", "tile_table.inc")
+    rows = [tuple(int(x) for x in m.groups()) for m in
+            (re.match(r"\{(-?\d+), (-?\d+), (\d+), (\d+), (\d+), (\d+), (\d+)\},", ln.strip()) for ln in open(inc))
+            if m]
+    assert len(rows) > 100
+    names = {0: "mzi", 1: "sym", 2: "orth", 3: "mzi_generic"}
+    checked = 0
+    for op, X, fat, n0, W, V, waves in rows:
+        if op == L.OP_FWDGRAD and X == 3:
+            continue  # needs the scaled-unitary flag, which resolve_tile does not take
+        for N in {n0, max(2, n0 // 2 + 2)}:
+            if N > n0 or (n0 > 8 and N <= n0 // 2):
+                continue
+            B = 2 * N + 1 if fat else 2 * N
+            got = L.resolve_tile(op, names[X], N, B)
+            cap = 4 if op == L.OP_FWDPROP or op == L.OP_FWDDIFF else 8  # the plan clamps waves
+            assert got[:2] == (W, V) and got[2] == min(waves, cap), (op, X, fat, n0, N, got)
+            checked += 1
+    assert checked > 100
+    # an explicit tile wins over the table
+    assert L.resolve_tile(L.OP_FWDPROP, "orth", 512, 4096, (8, 2, 4, 0))[:3] == (8, 2, 4)
