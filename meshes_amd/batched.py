@@ -13,6 +13,13 @@ input (gx); ``op='backdiff'`` takes y and dJ/dy (or target) as the reference's b
 (gpu/mesh.py:196-231).  Fields are (B, N) tensors, one vector per row (the (N, B) Fortran arrays
 of the reference, transposed).  Gradient jobs need unitary crossings (mzi, sym).
 
+Meshes with a phase screen (``is_phase``: exp(i phi) at the input or output, mesh.py:38-41,
+c/dot.cpp:97-143) run their crossings in the group and the screen as launches of the screen
+kernel around it (``ScreenPlan``): the screened input or the target seen through the screen
+before the group launch, the screened output and the screen's adjoint (dJ/dphi into the last N
+entries of dp) after it.  These launches use buffers allocated with the job, so a captured graph
+records them too.
+
 The per-mesh reference path, one plain launch sequence per mesh (gpu/mesh.py:158-231, as the
 reference runs each mesh), is ``run_plain``.
 """
@@ -50,7 +57,6 @@ class GroupJob:
         assert self.op in OPS, self.op
         m = self.mesh
         assert m.X != "orth", "real (orth) meshes are not grouped"
-        assert not m.is_phase, "grouped jobs propagate through the crossings only (no phase screen)"
         if self.op != "fwd":
             assert m.X in ("mzi", "sym"), "gradient jobs need unitary crossings"
         ref = self.x if self.op != "backdiff" else self.y
@@ -69,6 +75,100 @@ class GroupJob:
             assert self.target is not None or self.dy is not None, "gradient jobs need target or dy"
             if self.dp is None:
                 self.dp = torch.zeros(m.n_phase, dtype=torch.float32, device=dev)
+        self.screen = ScreenPlan(m, self.op, self.B, x=self.x, y=self.y, target=self.target, dy=self.dy,
+                                 gx=self.gx, dp=self.dp, accumulate=self.accumulate) if m.is_phase else None
+
+
+class ScreenPlan:
+    """The phase screen of one grouped job: entry buffers that replace the job's (``sub``) and the
+    screen launches before (``pre``) and after (``post``) the group launch.  With S = diag(exp(i
+    phi)) and the mesh's crossings M:
+
+    * fwd: in -- the entry reads S x; out -- y <- S y after the launch.
+    * grad (J = |y - target|^2 or a given dJ/dy): in -- the entry runs on S x and returns its input
+      gradient, whose pass back through the screen's adjoint gives dJ/dphi (and dJ/dx); out -- the
+      entry runs against S^H target (or S^H dJ/dy), then y <- S y and the screen's adjoint on
+      (y, dJ/dy) gives dJ/dphi.
+    * backdiff (y, dJ/dy given): out -- the screen's adjoint maps (y, dJ/dy) to the mesh's output
+      and its gradient before the launch; in -- the entry's reconstructed input and input gradient
+      go back through the screen's adjoint after it.
+    """
+
+    def __init__(self, mesh, op, B, x=None, y=None, target=None, dy=None, gx=None, dp=None, accumulate=False):
+        self.mesh, self.op, self.B, self.pos = mesh, op, B, mesh.phi_pos
+        self.x, self.y, self.target, self.dy, self.gx, self.dp = x, y, target, dy, gx, dp
+        self.accumulate = accumulate
+        N, dev = mesh.N, mesh.device
+        self.n_m = mesh.n_cr * mesh.X_np
+        self.phi = mesh.p_phase[self.n_m:]
+        self.nphi = torch.empty_like(self.phi)
+        self.ws_n = _lib.phase_screen_workspace_size(N, B)
+        self.ws = torch.empty(max(self.ws_n, 1), dtype=torch.uint8, device=dev)
+        f = lambda: torch.empty((B, N), dtype=torch.complex64, device=dev)  # noqa: E731
+        self.sub = {}
+        if op == "fwd" and self.pos == "in":
+            self.xs = f()
+            self.sub["u_in"] = self.xs
+        elif op == "grad" and self.pos == "in":
+            self.xs, self.gxs = f(), f()
+            self.sub.update(u_in=self.xs, dJdu_in=self.gxs)
+        elif op == "grad":
+            self.ts, self.gbuf, self.ys = f(), f(), f()
+            self.sub["target" if target is not None else "dJdu_out"] = self.ts
+        elif op == "backdiff" and self.pos == "out":
+            self.ys, self.gbuf = f(), f()
+            self.sub.update(u_out=self.ys, dJdu_out=self.gbuf, target=None)
+        elif op == "backdiff":
+            self.xs, self.gxs = f(), f()
+            self.sub.update(u_in=self.xs, dJdu_in=self.gxs)
+
+    def _scr(self, op, phi, u, du=None):
+        dphi = self.dp[self.n_m:] if op == _lib.OP_BACKDIFF else None
+        _lib.phase_screen(op, self.mesh.N, self.B, phi, dphi, u, du, self.mesh.N, workspace=self.ws,
+                          ws_bytes=self.ws_n)
+
+    def _grad_out(self):
+        """dJ/dy into gbuf: 2 (y - target) or the given dJ/dy."""
+        if self.target is not None:
+            torch.sub(self.y, self.target, out=self.gbuf)
+            self.gbuf.mul_(2)
+        else:
+            self.gbuf.copy_(self.dy)
+
+    def pre(self):
+        if self.B == 0:
+            return
+        if self.op != "fwd" and not self.accumulate:
+            self.dp[self.n_m:].zero_()  # the screen's adjoint adds into its part
+        if self.op in ("fwd", "grad") and self.pos == "in":
+            self.xs.copy_(self.x)
+            self._scr(_lib.OP_FWDPROP, self.phi, self.xs)
+        elif self.op == "grad":
+            torch.neg(self.phi, out=self.nphi)
+            self.ts.copy_(self.target if self.target is not None else self.dy)
+            self._scr(_lib.OP_FWDPROP, self.nphi, self.ts)  # S^H target (or S^H dJ/dy)
+        elif self.op == "backdiff" and self.pos == "out":
+            self.ys.copy_(self.y)
+            self._grad_out()
+            self._scr(_lib.OP_BACKDIFF, self.phi, self.ys, self.gbuf)
+
+    def post(self):
+        if self.B == 0:
+            return
+        if self.op == "fwd":
+            if self.pos == "out":
+                self._scr(_lib.OP_FWDPROP, self.phi, self.y)
+        elif self.pos == "in":  # grad and backdiff: back through the input screen
+            self._scr(_lib.OP_BACKDIFF, self.phi, self.xs, self.gxs)
+            if self.op == "backdiff" and self.x is not None:
+                self.x.copy_(self.xs)
+            if self.gx is not None:
+                self.gx.copy_(self.gxs)
+        elif self.op == "grad":
+            self._scr(_lib.OP_FWDPROP, self.phi, self.y)  # y = S (M x)
+            self._grad_out()
+            self.ys.copy_(self.y)
+            self._scr(_lib.OP_BACKDIFF, self.phi, self.ys, self.gbuf)
 
 
 def tables_for(mesh: MeshNetworkGPU, tile=None) -> torch.Tensor:
@@ -114,6 +214,9 @@ def _entry(job: GroupJob) -> _lib.GroupEntry:
     e.tables = _aligned_ptr(job.tables) if job.tables is not None else None
     e.build_tables = int(bool(job.build_tables))
     e.flags = 0 if job.accumulate else _lib.GROUP_DP_ASSIGN
+    if job.screen is not None:
+        for name, t in job.screen.sub.items():
+            setattr(e, name, t.data_ptr() if t is not None else None)
     return e
 
 
@@ -176,8 +279,13 @@ class MeshGroup:
     # -- launches --------------------------------------------------------------------------
     def launch(self):
         s = torch.cuda.current_stream().cuda_stream
+        screens = [j.screen for j in self.jobs if j.screen is not None]
+        for sp in screens:
+            sp.pre()
         _lib.check(self._lib.meshprop_group_launch(self._plan, _aligned_ptr(self._ws), self.ws_bytes,
                                                    ctypes.c_void_p(s)), "group_launch")
+        for sp in screens:
+            sp.post()
 
     def capture(self, warm: bool = True):
         """Record prologue, persistent kernel and reduction as one replayable graph (kept per
@@ -246,14 +354,27 @@ class PlainRunner:
             s = m.p_splitter if m.p_splitter.numel() else None
             if j.B == 0:
                 continue
-            if j.op in ("fwd", "grad"):
-                _lib.run(_lib.OP_FWDPROP, m.X, m.N, m.L, j.B, m.fft, g0, g1, m.p_phase, None, ldp, s, lds, j.x, None,
-                         j.y, None, m.N, tile=tile, workspace=ws, ws_bytes=nb)
-            if j.op in ("grad", "backdiff"):
-                if not j.accumulate:
+            # the job's buffers, or the screen plan's in their place
+            f = dict(u_in=j.x, u_out=j.y, target=j.target, dJdu_out=j.dy, dJdu_in=j.gx)
+            if j.screen is not None:
+                f.update(j.screen.sub)
+                if not j.accumulate and j.op != "fwd":
                     j.dp.zero_()
-                _lib.run(_lib.OP_BACKDIFF, m.X, m.N, m.L, j.B, m.fft, g0, g1, m.p_phase, j.dp, ldp, s, lds, j.y,
-                         j.dy, rec, j.gx, m.N, target=j.target, tile=tile, workspace=ws, ws_bytes=nb)
+                j.screen.pre()
+            if j.op in ("fwd", "grad"):
+                _lib.run(_lib.OP_FWDPROP, m.X, m.N, m.L, j.B, m.fft, g0, g1, m.p_phase, None, ldp, s, lds,
+                         f["u_in"], None, j.y, None, m.N, tile=tile, workspace=ws, ws_bytes=nb)
+            if j.op in ("grad", "backdiff"):
+                if not j.accumulate and j.screen is None:
+                    j.dp.zero_()
+                u_rec = f["u_in"] if (j.op == "backdiff" and f["u_in"] is not None) else rec
+                if j.op == "grad":
+                    u_rec = rec
+                _lib.run(_lib.OP_BACKDIFF, m.X, m.N, m.L, j.B, m.fft, g0, g1, m.p_phase, j.dp, ldp, s, lds,
+                         f["u_out"], f["dJdu_out"], u_rec, f["dJdu_in"], m.N, target=f["target"], tile=tile,
+                         workspace=ws, ws_bytes=nb)
+            if j.screen is not None:
+                j.screen.post()
 
 
 def run_plain(jobs: List[GroupJob]):

@@ -333,8 +333,12 @@ class MeshNetworkGPU:
         unitary crossings (and generic ones with balanced arm gains) this is one library call (one
         prologue for all tables, fwdprop, adjoint, dp reduction); other lossy crossings run
         fwdprop, then backdiff.  dp is accumulated into out_dp when it is given, else returned
-        fresh."""
-        assert not self.is_phase, "the fused target form has no phase screen"
+        fresh.
+
+        With a phase screen S = diag(exp(i phi)) (is_phase): at the output, J = |S M v - y|^2 =
+        |M v - S^H y|^2, so the mesh part runs against the target S^H y and dJ/dphi comes from the
+        screen's adjoint on (u_out, 2 (u_out - y)); at the input, the mesh part runs on S v, returns
+        its input gradient, and the screen's adjoint on (S v, that gradient) gives dJ/dphi."""
         p, s = self._defaults(p_phase, p_splitter)
         ldp, lds = self._ld()
         g0, g1 = self._geom()
@@ -346,20 +350,38 @@ class MeshNetworkGPU:
         if out_dp is None:
             out_dp = torch.empty(self.n_phase, dtype=torch.float32, device=self.device)
             flags = _lib.DP_ASSIGN
+        n_m = self.n_cr * self.X_np
+        if self.is_phase and flags:
+            out_dp[n_m:].zero_()  # the screen's adjoint adds into its part
+        src, tgt, gin = vb, yb, None
+        if self.is_phase and self.phi_pos == "in":
+            src = vb.clone()
+            self._screen(_lib.OP_FWDPROP, B, p[n_m:], None, src, None)
+            gin = torch.empty_like(vb)
+        elif self.is_phase:
+            tgt = yb.clone()
+            self._screen(_lib.OP_FWDPROP, B, -p[n_m:], None, tgt, None)  # S^H y
         sp = s if s.numel() else None
         sc = self._adj_flags(s)
         if self.X != "mzi_generic" or sc:
             ws, nb = self._workspace(_lib.OP_FWDGRAD, B)
-            _lib.run(_lib.OP_FWDGRAD, self.X, self.N, self.L, B, self.fft, g0, g1, p, out_dp, ldp, sp, lds, vb,
-                     None, uout, None, self.N, target=yb, tile=self.tile, workspace=ws, ws_bytes=nb, flags=flags | sc)
+            _lib.run(_lib.OP_FWDGRAD, self.X, self.N, self.L, B, self.fft, g0, g1, p, out_dp, ldp, sp, lds, src,
+                     None, uout, gin, self.N, target=tgt, tile=self.tile, workspace=ws, ws_bytes=nb,
+                     flags=flags | sc)
         else:
             ws, nb = self._workspace(_lib.OP_FWDPROP, B)
-            _lib.run(_lib.OP_FWDPROP, self.X, self.N, self.L, B, self.fft, g0, g1, p, None, ldp, sp, lds, vb, None,
+            _lib.run(_lib.OP_FWDPROP, self.X, self.N, self.L, B, self.fft, g0, g1, p, None, ldp, sp, lds, src, None,
                      uout, None, self.N, tile=self.tile, workspace=ws, ws_bytes=nb)
             uin = torch.empty_like(vb)
             ws, nb = self._workspace(_lib.OP_BACKDIFF, B)
             _lib.run(_lib.OP_BACKDIFF, self.X, self.N, self.L, B, self.fft, g0, g1, p, out_dp, ldp, sp, lds, uout,
-                     None, uin, None, self.N, target=yb, tile=self.tile, workspace=ws, ws_bytes=nb, flags=flags)
+                     None, uin, gin, self.N, target=tgt, tile=self.tile, workspace=ws, ws_bytes=nb, flags=flags)
+        if self.is_phase and self.phi_pos == "in":
+            self._screen(_lib.OP_BACKDIFF, B, p[n_m:], out_dp[n_m:], src, gin)
+        elif self.is_phase:
+            self._screen(_lib.OP_FWDPROP, B, p[n_m:], None, uout, None)  # u_out = S (M v)
+            g = 2 * (uout - yb)
+            self._screen(_lib.OP_BACKDIFF, B, p[n_m:], out_dp[n_m:], uout.clone(), g)
         J = torch.sum(torch.abs(uout - yb) ** 2)
         return J, out_dp, uout.T
 

@@ -28,7 +28,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .batched import GroupJob, MeshGroup, _entry, tables_for
+from .batched import GroupJob, MeshGroup, ScreenPlan, _entry, tables_for
 from .mesh import MeshNetworkGPU
 
 
@@ -44,6 +44,7 @@ class Request:
     dp: Optional[torch.Tensor] = None
     t_done: float = -1.0
     footprint: int = 0
+    screen: Optional[ScreenPlan] = None  # launches of the mesh's phase screen, if it has one
 
     @property
     def B(self):
@@ -174,6 +175,11 @@ class MeshServer:
                     e.target = r.target.data_ptr()
                 else:
                     e.op = _lib.OP_FWDPROP
+                # a phase screen runs as screen launches around the grouped launch
+                r.screen = ScreenPlan(m, r.op, r.B, x=r.x, y=r.y, target=r.target, dp=r.dp) if m.is_phase else None
+                if r.screen is not None:
+                    for name, t in r.screen.sub.items():
+                        setattr(e, name, t.data_ptr() if t is not None else None)
             if slot.group is None:
                 slot.group = MeshGroup([], workspace=slot.workspace)
             slot.group.replan_entries(ents, n, self.device)
@@ -183,8 +189,13 @@ class MeshServer:
 
     def _launch(self, slot: _Slot, batch: List[Request]):
         self._plan(slot, batch)
+        screens = [r.screen for r in batch if r.screen is not None]
         with torch.cuda.stream(slot.stream):
+            for sp in screens:
+                sp.pre()
             slot.group.run()
+            for sp in screens:
+                sp.post()
             slot.event.record(slot.stream)
 
     def serve(self, requests: List[Request], clock=time.perf_counter):
@@ -250,6 +261,15 @@ class MeshServer:
             while clock() - t0 < r.t_arrive:
                 pass
             m = self.meshes[r.mesh_id]
+            if m.is_phase:  # the mesh's own methods put the screen around its launches
+                if r.op == "grad":
+                    _, r.dp, uo = m.grad_target_l2(r.x.T, r.target.T)
+                    r.y = uo.T
+                else:
+                    r.y = m.dot(r.x.T).T
+                torch.cuda.current_stream().synchronize()
+                r.t_done = clock() - t0
+                continue
             g0, g1 = m._geom()
             ldp, lds = m.N // 2 * m.X_np, m.N // 2 * m.X_ns
             s = m.p_splitter if m.p_splitter.numel() else None  # as the served entries carry them
@@ -282,11 +302,13 @@ def request_footprint(m: MeshNetworkGPU, op: str, B: int, ncu: int) -> int:
     per_tile = waves * (64 // lpv)
     tiles = (B + per_tile - 1) // per_tile
     plan = 2048
+    # a phase screen's buffers: up to three fields, its dphi partials and -phi
+    screen = (3 * field_bytes + ((B + 63) // 64) * m.N * 4 + m.N * 4 + 256) if m.is_phase else 0
     if op == "fwd":
-        return 2 * field_bytes + plan + 8 * tiles
+        return 2 * field_bytes + plan + 8 * tiles + screen
     slabs = min(tiles, 2 * ncu)
     partial = slabs * m.L * (lpv * W // 2) * 2 * 4
-    return 3 * field_bytes + m.n_phase * 4 + partial + plan + 8 * slabs
+    return 3 * field_bytes + m.n_phase * 4 + partial + plan + 8 * slabs + screen
 
 
 def _stats(reqs, meshes, **extra):
