@@ -1,0 +1,57 @@
+"""Times one library call shape: a Clements mesh of N modes and L layers, B vectors, one op.
+
+    python -m meshes_amd.profile_case --op fwdgrad --X mzi --N 1024 --L 1024 --B 2048 --steps 5
+
+Runs ``--warmup`` untimed calls, then ``--steps`` calls bracketed by HIP events on the current
+stream, and prints one JSON line (ms per call and the stage split of a fwd+grad call).  Meant as
+the program under ``rocprofv3`` for kernel traces and counter passes of a given size; the tile is
+the library's choice unless ``--tile w,v,waves,chunk`` is given.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+
+import torch
+
+from . import _lib as Lb
+from .tune_tiles import OPS, Case
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--op", choices=list(OPS), default="fwdgrad")
+    ap.add_argument("--X", choices=["mzi", "sym", "orth", "mzi_generic"], default="mzi")
+    ap.add_argument("--N", type=int, default=256)
+    ap.add_argument("--L", type=int, default=0, help="layers (default N)")
+    ap.add_argument("--B", type=int, default=4096)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--tile", type=str, default="")
+    args = ap.parse_args(argv)
+    L = args.L or args.N
+    tile = tuple(int(t) for t in args.tile.split(",")) if args.tile else None
+    op = OPS[args.op]
+    case = Case(op, args.X, args.N, L, args.B)
+    nb = Lb.workspace_size(op, args.X, args.N, L, args.B, False, tile)
+    ws = torch.empty(max(nb, 256), dtype=torch.uint8, device="cuda")
+    for _ in range(args.warmup):
+        case.run(tile, ws, nb)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(args.steps):
+        case.run(tile, ws, nb)
+    b.record()
+    b.synchronize()
+    ms = a.elapsed_time(b) / max(1, args.steps)
+    out = {"op": args.op, "X": args.X, "N": args.N, "L": L, "B": args.B,
+           "tile": list(Lb.resolve_tile(op, args.X, args.N, args.B, tile)) if not (args.X == "mzi_generic" and
+                                                                                  op == Lb.OP_FWDGRAD) else tile,
+           "ms_per_call": ms, "updates_per_s": args.B * case.n_active / (ms * 1e-3),
+           "device": torch.cuda.get_device_name(0)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -69,6 +69,10 @@ class Case:
         rng = np.random.RandomState(seed)
         self.op, self.X, self.N, self.L, self.B = op, X, N, L, B
         lens, shifts = geometry.clements(N)
+        # depth L: the Clements layer pair repeated (the first L layers when L <= N)
+        lens = [int(lens[l % 2]) for l in range(L)]
+        shifts = [int(shifts[l % 2]) for l in range(L)]
+        self.n_active = int(sum(lens))
         dev = torch.device("cuda", 0)
         self.lens = torch.as_tensor(np.asarray(lens, np.int32)).to(dev)
         self.shifts = torch.as_tensor(np.asarray(shifts, np.int32)).to(dev)
