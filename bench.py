@@ -317,7 +317,7 @@ def main():
                          "serving: the continuous-batching request loop; training: Adam through the autograd "
                          "adjoint on the lossy N=512 mesh")
     ap.add_argument("--train-steps", type=int, default=200)
-    ap.add_argument("--lr", type=float, default=0.05)
+    ap.add_argument("--lr", type=float, default=0.003)
     ap.add_argument("--train-loss", choices=["fidelity", "l2"], default="fidelity",
                     help="training: 1 - F^2 with the scale-invariant batch fidelity F, or sum |U x - t|^2 / B")
     ap.add_argument("--rates", type=str, default="100,1000,5000,20000")

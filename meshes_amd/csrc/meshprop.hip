@@ -180,7 +180,8 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
     meshprop_tile tabled;
     const bool explicit_tile = tile && (tile->w > 0 || tile->v > 0 || tile->waves > 0);
     if (!explicit_tile && !fft) {
-        if (const TileEntry* e = table_tile(op, X, N, B)) {
+        // Scaled-unitary crossings run the unitary adjoint: they take the MZI entries.
+        if (const TileEntry* e = table_tile(op, P.scaled ? X_MZI : X, N, B)) {
             if ((long)e->w * 64 >= N && tile_ok(op, e->w, e->v)) {
                 tabled = meshprop_tile{e->w, e->v, e->waves, tile ? tile->chunk : 0};
                 tile = &tabled;
