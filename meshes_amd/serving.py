@@ -91,7 +91,7 @@ class MeshServer:
     """
 
     def __init__(self, meshes: List[MeshNetworkGPU], hbm_budget: int = 8 << 30, slots: int = 3,
-                 max_batch: int = 256, graphs: bool = True):
+                 max_batch: int = 256, graphs: bool = True, direct: bool = True):
         _lib.load()
         self.meshes = list(meshes)
         self.device = self.meshes[0].device
@@ -102,6 +102,8 @@ class MeshServer:
         self.slots = [_Slot(self.device, self.slot_capacity) for _ in range(slots)]
         self.inflight_bytes = 0
         self.graphs = graphs
+        self.direct = direct
+        self.direct_launches = 0
         # Row tables of every mesh, built once and kept resident.
         self.tables = [tables_for(m) for m in self.meshes]
         build = [GroupJob(m, "fwd", x=torch.empty((0, m.N), dtype=torch.complex64, device=self.device),
@@ -187,6 +189,33 @@ class MeshServer:
                 raise RuntimeError("a batch outgrew its slot workspace")
         slot.reqs = batch
 
+    def _launch_direct(self, slot: _Slot, r: Request) -> bool:
+        """A lone request (the other slots idle) as one plain meshprop_run call on the slot's
+        stream, with the slot's workspace: nothing to batch with, so no plan upload and no
+        grouped launch.  False when its workspace would not fit the slot's."""
+        m = self.meshes[r.mesh_id]
+        op = _lib.OP_FWDGRAD if r.op == "grad" else _lib.OP_FWDPROP
+        need = _lib.workspace_size(op, m.X, m.N, m.L, r.B, m.fft)
+        if m.is_phase or need > slot.workspace.numel():
+            return False
+        g0, g1 = m._geom()
+        ldp, lds = m.N // 2 * m.X_np, m.N // 2 * m.X_ns
+        sp = m.p_splitter if m.p_splitter.numel() else None
+        with torch.cuda.stream(slot.stream):
+            r.y = torch.empty((r.B, m.N), dtype=torch.complex64, device=self.device)
+            r.screen = None
+            if r.op == "grad":
+                r.dp = torch.empty(m.n_phase, dtype=torch.float32, device=self.device)
+                _lib.run(op, m.X, m.N, m.L, r.B, m.fft, g0, g1, m.p_phase, r.dp, ldp, sp, lds, r.x, None, r.y, None,
+                         m.N, target=r.target, workspace=slot.workspace, ws_bytes=need, flags=_lib.DP_ASSIGN)
+            else:
+                _lib.run(op, m.X, m.N, m.L, r.B, m.fft, g0, g1, m.p_phase, None, ldp, sp, lds, r.x, None, r.y, None,
+                         m.N, workspace=slot.workspace, ws_bytes=need)
+            slot.event.record(slot.stream)
+        slot.reqs = [r]
+        self.direct_launches += 1
+        return True
+
     def _launch(self, slot: _Slot, batch: List[Request]):
         self._plan(slot, batch)
         screens = [r.screen for r in batch if r.screen is not None]
@@ -210,6 +239,7 @@ class MeshServer:
             if r.footprint > self.slot_capacity:
                 raise ValueError(f"request {r.rid} needs {r.footprint} B, above one slot's {self.slot_capacity} B")
         torch.cuda.synchronize()
+        self.direct_launches = 0
         pending = collections.deque()
         n, i, done, batches, peak = len(reqs), 0, 0, 0, 0
         sizes = []
@@ -242,11 +272,14 @@ class MeshServer:
                 s.bytes = nbytes
                 self.inflight_bytes += nbytes
                 peak = max(peak, self.inflight_bytes)
-                self._launch(s, batch)
+                lone = len(batch) == 1 and not any(o.busy for o in self.slots if o is not s)
+                if not (self.direct and lone and self._launch_direct(s, batch[0])):
+                    self._launch(s, batch)
                 batches += 1
                 sizes.append(len(batch))
         torch.cuda.synchronize()
-        return reqs, _stats(reqs, self.meshes, batches=batches, mean_batch=float(np.mean(sizes)) if sizes else 0.0,
+        return reqs, _stats(reqs, self.meshes, batches=batches, direct_launches=self.direct_launches,
+                            mean_batch=float(np.mean(sizes)) if sizes else 0.0,
                             peak_inflight_bytes=peak)
 
     # -- baseline ---------------------------------------------------------------------------

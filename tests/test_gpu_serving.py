@@ -35,7 +35,7 @@ def test_served_equals_alone_plain_and_oracle(gpu):
     from meshes_amd.batched import GroupJob, MeshGroup
     from meshes_amd.serving import MeshServer, poisson_requests
     meshes = _meshes()
-    server = MeshServer(meshes, hbm_budget=3 << 30, slots=3)
+    server = MeshServer(meshes, hbm_budget=3 << 30, slots=3, direct=False)  # every batch grouped: bitwise claim
     reqs = poisson_requests(meshes, rate=50000.0, n=80, seed=5, b_max=300)
     done, st = server.serve(reqs)
     assert st["requests"] == 80 and st["mean_batch"] > 1.0
@@ -121,3 +121,31 @@ def test_served_equals_sequential_with_splitter_errors(gpu):
         s = m.p_splitter.cpu().numpy().astype(np.float64).reshape(m.N, m.N // 2, 2)
         y = O.mesh_fwd(g, "mzi", p, s, r.x.cpu().numpy().astype(np.complex128))
         assert O.rel_l2(q.y.cpu().numpy(), y) < TOL, (r.rid, m.N)
+
+
+def test_lone_requests_run_direct(gpu):
+    """At low load a request that finds the other slots idle runs as one plain meshprop_run call
+    (no plan upload, no grouped launch); its results equal the grouped launch of the same
+    request and the FP64 oracle within 1e-5."""
+    from meshes_amd.batched import GroupJob, MeshGroup
+    from meshes_amd.serving import MeshServer, poisson_requests
+    meshes = _meshes(seed=8)[:6]
+    server = MeshServer(meshes, hbm_budget=1 << 30, slots=2)
+    reqs = poisson_requests(meshes, rate=300.0, n=16, seed=9, b_max=64)
+    done, st = server.serve(reqs)
+    assert st["direct_launches"] >= 8, st
+    for r in done:
+        m = meshes[r.mesh_id]
+        if r.op == "grad":
+            job = GroupJob(m, "grad", x=r.x, target=r.target, tables=server.tables[r.mesh_id], build_tables=False)
+        else:
+            job = GroupJob(m, "fwd", x=r.x, tables=server.tables[r.mesh_id], build_tables=False)
+        MeshGroup([job]).run()
+        torch.cuda.synchronize()
+        assert O.rel_l2(job.y.cpu().numpy(), r.y.cpu().numpy()) < TOL, (r.rid, m.N, r.op)
+        if r.op == "grad":
+            assert O.rel_l2(job.dp.cpu().numpy(), r.dp.cpu().numpy()) < TOL, (r.rid, m.N)
+        g = O.clements(m.N)
+        p = m.p_phase.cpu().numpy().astype(np.float64).reshape(m.N, m.N // 2, 2)
+        y = O.mesh_fwd(g, "mzi", p, None, r.x.cpu().numpy().astype(np.complex128))
+        assert O.rel_l2(r.y.cpu().numpy(), y) < TOL, (r.rid, m.N)
