@@ -182,7 +182,11 @@ enum meshprop_flags {
      * row).  The adjoint then runs as for unitary crossings on one table of inverse rows, with
      * T^H = |det T| T^-1, and fwd+grad accepts the generic crossing.  The prologue checks the
      * assertion in FP64 per crossing; where it fails, dp and dJ/du_in come out NaN. */
-    MESHPROP_SCALED_UNITARY = 128
+    MESHPROP_SCALED_UNITARY = 128,
+    /* backdiff / fwd+grad of unitary crossings on neighbour (NN) geometries: run the general
+     * adjoint kernel instead of the one specialised for such meshes (their results are
+     * bitwise equal; for comparison and timing). */
+    MESHPROP_GENERIC_ADJOINT = 256
 };
 
 /*
@@ -291,6 +295,12 @@ int meshprop_hvp(int X, int N, int L, int B, int fft, const int *geom0, const in
 
 /* The tile the library would use for a call (after defaults and clamping). */
 int meshprop_resolve_tile(int op, int X, int N, int B, meshprop_tile *out);
+
+/* Which adjoint kernel a backdiff / fwd+grad call with these arguments runs (no reference
+ * counterpart: the reference has one backdiff kernel per size): 1 = the one specialised for
+ * unitary crossings on NN geometries, 0 = the general one (also for ops without an adjoint),
+ * -status for arguments meshprop_run would reject.  fft as in meshprop_run, flags its flags. */
+int meshprop_adjoint_kernel(int op, int X, int N, int L, int B, int fft, const meshprop_tile *tile, int flags);
 
 /* Last HIP error code seen by a failing call on this thread (hipError_t as int).  A HIP error
  * left pending on the thread by earlier, unrelated work is cleared when a call starts launching,

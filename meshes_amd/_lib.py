@@ -33,7 +33,7 @@ EXPORTS = (
     "meshprop_group_create", "meshprop_group_destroy", "meshprop_group_workspace_size",
     "meshprop_group_tables_size", "meshprop_group_upload", "meshprop_group_launch", "meshprop_group_signature",
     "meshprop_group_entry_info", "meshprop_phase_screen", "meshprop_phase_screen_workspace_size",
-    "meshprop_hvp", "meshprop_hvp_workspace_size", "meshprop_run_noalloc",
+    "meshprop_hvp", "meshprop_hvp_workspace_size", "meshprop_run_noalloc", "meshprop_adjoint_kernel",
 )
 
 # Bytes of the static device scratch behind meshprop_run_noalloc (MESHPROP_NOALLOC_SCRATCH_BYTES).
@@ -87,6 +87,8 @@ def load():
     lib.meshprop_run_noalloc.restype = c_int
     lib.meshprop_resolve_tile.argtypes = [c_int, c_int, c_int, c_int, ctypes.POINTER(Tile)]
     lib.meshprop_resolve_tile.restype = c_int
+    lib.meshprop_adjoint_kernel.argtypes = [c_int, c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(Tile), c_int]
+    lib.meshprop_adjoint_kernel.restype = c_int
     lib.meshprop_last_hip_error.restype = c_int
     lib.meshprop_status_string.restype = ctypes.c_char_p
     lib.meshprop_status_string.argtypes = [c_int]
@@ -163,9 +165,21 @@ def resolve_tile(op, X, N, B, tile=None):
     return (t.w, t.v, t.waves, t.chunk)
 
 
+def adjoint_kernel(op, X, N, L, B, fft=False, tile=None, flags=0) -> str:
+    """Which adjoint kernel a backdiff / fwd+grad call runs: "nn" (specialised for unitary
+    crossings on NN geometries) or "general"."""
+    lib = load()
+    rc = lib.meshprop_adjoint_kernel(op, X_CODE[X] if isinstance(X, str) else X, N, L, B, int(fft),
+                                     tile_struct(tile), int(flags))
+    if rc < 0:
+        check(-rc, "adjoint_kernel")
+    return "nn" if rc == 1 else "general"
+
+
 ONLY_PROLOGUE, SKIP_PROLOGUE, SKIP_REDUCE, ONLY_REDUCE, DP_ASSIGN = 1, 2, 4, 8, 16
 ONLY_FORWARD, SKIP_FORWARD = 32, 64
 SCALED_UNITARY = 128
+GENERIC_ADJOINT = 256
 
 
 def run(op, X, N, L, B, fft, g0, g1, p, dp, ldp, s, lds, u_a, du_a, u_b, du_b, ldu, target=None, ckpt=None,

@@ -1255,6 +1255,219 @@ __device__ __forceinline__ void bwd_tile_pipe(const int N, const int l0, const i
     if (g_in != nullptr) store_tile(ga, g_in, q, N, B, ldu);
 }
 
+// ---- neighbour-mesh adjoint with a compile-time stage ------------------------------------
+// The pipelined unitary adjoint for NN geometries (layer kinds 0 and 1 only) whose vectors
+// fill one wave (LPV = 64, so NP = 64 W), compiled with the workgroup's wave count NW and the
+// stage depth C as constants.  Same arithmetic and the same fixed summation orders as
+// bwd_tile_pipe (its results are bitwise equal), with the run-time structure of the general
+// tile removed from the layer loop: no lane-group folds, no FFT stage-write variants, stage
+// offsets and conversion items known at compile time (the C layers of a stage are unrolled),
+// the C layer kinds of a stage read once at its start, and the rows (and scaled-unitary
+// factors) read with raw buffer loads (scalar layer offset, fixed lane offset).
+//
+// K-form stage entries of one NN layer (lane t owns slots t*W/2 .. t*W/2 + W/2 - 1; an odd
+// layer pairs the lane's last waveguide with the next lane's first).
+template <class T, int W>
+__device__ __forceinline__ void kstage_write_nn(const bool odd, const T (&a)[W], const T (&b)[W],
+                                                typename RowOf<T>::type* wl) {
+    constexpr int H = W / 2;
+    if (odd) {
+        const T na = a_next(a[0]), nb = lane_next(b[0]);
+#pragma unroll
+        for (int p = 0; p + 1 < H; ++p) wl[p] = kquad(a[2 * p + 1], a[2 * p + 2], b[2 * p + 1], b[2 * p + 2]);
+        wl[H - 1] = kquad(a[W - 1], na, b[W - 1], nb);
+    } else {
+#pragma unroll
+        for (int p = 0; p < H; ++p) wl[p] = kquad(a[2 * p], a[2 * p + 1], b[2 * p], b[2 * p + 1]);
+    }
+}
+
+// One NN layer of the unitary adjoint (bwd_layer without folds or kind dispatch).
+template <class T, int W, int V, bool SC>
+__device__ __forceinline__ void bwd_layer_nn(const bool odd, const T (&u)[V][W], const T (&g)[V][W], T (&uo)[V][W],
+                                             T (&go)[V][W], const typename RowOf<T>::type (&ri)[W],
+                                             typename RowOf<T>::type* wl, const int lane, const float* scl) {
+    T accA[W], accB[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        accA[k] = fzero(T());
+        accB[k] = fzero(T());
+    }
+    BwdStep<T, W, V, true, SC> st{u, g, uo, go, ri, ri, accA, accB, lane, scl};
+    if (odd)
+        st.template run<1, 1>(0);
+    else
+        st.template run<0, 1>(0);
+    kstage_write_nn<T, W>(odd, accA, accB, wl);
+}
+
+template <class T, int W, int V, int NW, int C, bool SC>
+__device__ __forceinline__ void bwd_tile_nn(const int N, const int l0, const int nl, const int B, ConstInt* kind,
+                                            const typename RowOf<T>::type* __restrict__ rows_inv,
+                                            const T* __restrict__ dtdp, float* __restrict__ my_partial,
+                                            const T* u_out, const T* g_out, const T* target, T* u_in, T* g_in,
+                                            const int ldu, const long tile, const bool first,
+                                            typename RowOf<T>::type* __restrict__ stage, const Ckpt ck,
+                                            const float* __restrict__ scales) {
+    typedef typename RowOf<T>::type R;
+    typedef typename KOf<T>::type KT;
+    constexpr int LPV = 64, NP = 64 * W, NPh = NP / 2;
+    constexpr int np = NPhase<T>::value;
+    constexpr int NTHR = 64 * NW;
+    constexpr int ITEMS = C * NPh;                     // conversion items of a whole stage
+    constexpr int NI = (ITEMS + NTHR - 1) / NTHR;      // items per thread and stage
+    constexpr int WSTRIDE = C * NPh;                   // one wave's entries in one buffer
+    constexpr int BSTRIDE = NW * WSTRIDE;              // one buffer
+    static_assert(C % 2 == 0, "the field and row register sets alternate per layer");
+    const int lane = threadIdx.x & 63;
+    const int wave_u = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int last = nl - 1;
+    const Lanes q = lanes_of<V>(LPV, tile);
+    T ua[V][W], ga[V][W], ub[V][W], gb[V][W];
+    load_tile(ua, u_out, q, N, B, ldu);
+    if (target != nullptr) {
+        T y[V][W];
+        load_tile(y, target, q, N, B, ldu);
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+#pragma unroll
+            for (int k = 0; k < W; ++k) ga[v][k] = fsub2(ua[v][k], y[v][k]);
+    } else {
+        load_tile(ga, g_out, q, N, B, ldu);
+    }
+    // Rows of local layer s (global layer l0 + last - s; past the end: layer l0) by buffer loads.
+    const __amdgpu_buffer_rsrc_t rrs = table_rsrc(rows_inv + (size_t)l0 * NP);
+    const __amdgpu_buffer_rsrc_t srs = table_rsrc(SC ? scales + (size_t)l0 * NP : nullptr);
+    const unsigned roff = (unsigned)lane * (unsigned)sizeof(R), soff = (unsigned)lane * (unsigned)sizeof(float);
+    R ri[2][W];
+    float sc[2][SC ? W : 1];
+    auto rows_at = [&](const int s, R (&r)[W], float (&f)[SC ? W : 1]) {
+        const int lr = s < last ? last - s : 0;
+#pragma unroll
+        for (int k = 0; k < W; ++k) r[k] = buf_load<R>(rrs, roff, (lr * NP + k * LPV) * (int)sizeof(R));
+        if constexpr (SC) {
+#pragma unroll
+            for (int k = 0; k < W; ++k) f[k] = buf_load<float>(srs, soff, (lr * NP + k * LPV) * (int)sizeof(float));
+        }
+    };
+    // Conversion item of the previous stage (k-th of this thread): the NW waves' entries of one
+    // slot, its K-form coefficients and the partial it lands in.
+    struct Item {
+        R a[NW];
+        KT dd[np];
+        float* out;
+        bool on;
+    } it;
+    int pc0 = 0;
+    R* pbuf = stage;
+    auto issue = [&](const int k, const int pitems) {
+        const int wbase = wave_u * 64 + k * NTHR;  // first item of this wave (uniform)
+        it.on = wbase < pitems;
+        if (!it.on) return;
+        const int lcc = wbase / NPh;  // stage layer of the wave's items
+        const int m = (wbase % NPh) + lane;
+        const int la = l0 + last - (pc0 + lcc);
+        const R* s0 = pbuf + lcc * NPh + m;
+#pragma unroll
+        for (int wv = 0; wv < NW; ++wv) it.a[wv] = s0[wv * WSTRIDE];
+        const KT* src = reinterpret_cast<const KT*>(dtdp) + (size_t)la * np * NPh + m;
+#pragma unroll
+        for (int e = 0; e < np; ++e) it.dd[e] = src[e * NPh];
+        it.out = my_partial + ((size_t)la * NPh + m) * np;
+    };
+    auto finish = [&]() {
+        if (!it.on) return;
+        R e = it.a[0];
+#pragma unroll
+        for (int wv = 1; wv < NW; ++wv) e = row_add(e, it.a[wv]);
+        float val[np];
+#pragma unroll
+        for (int k = 0; k < np; ++k) val[k] = kdot(e, it.dd[k]);
+        put_dp<np>(it.out, val, first);
+    };
+    if (nl > 0) rows_at(0, ri[0], sc[0]);
+    bool have_prev = false;
+    // One stage of cl layers starting at local layer c0 (GUARD: cl < C, the last stage).
+    auto run_stage = [&](auto guard, const int c0, const int cl, R* buf) {
+        constexpr bool GUARD = decltype(guard)::value;
+        if (ck.base != nullptr && c0 > 0) {
+            const int e = l0 + nl - c0;
+            if (e % ck.every == 0)
+                load_tile(ua, reinterpret_cast<const T*>(ck.base) + (size_t)(e / ck.every) * ck.stride, q, N, B, ldu);
+        }
+        int kd[C];
+#pragma unroll
+        for (int lc = 0; lc < C; ++lc) kd[lc] = kind[l0 + (c0 + lc < last ? last - (c0 + lc) : 0)];
+        R* wl = buf + wave_u * WSTRIDE + lane * (W / 2);
+#pragma unroll
+        for (int lc = 0; lc < C; ++lc) {
+            if (!GUARD || lc < cl) {
+                if (lc < NI && have_prev) issue(lc, ITEMS);
+                rows_at(c0 + lc + 1, ri[(lc + 1) & 1], sc[(lc + 1) & 1]);
+                if ((lc & 1) == 0)
+                    bwd_layer_nn<T, W, V, SC>(kd[lc] & 1, ua, ga, ub, gb, ri[0], wl + lc * NPh, lane, sc[0]);
+                else
+                    bwd_layer_nn<T, W, V, SC>(kd[lc] & 1, ub, gb, ua, ga, ri[1], wl + lc * NPh, lane, sc[1]);
+                if (lc < NI && have_prev) finish();
+            }
+        }
+        if (have_prev) {
+            // Items of the previous stage that found no layer to hide under.
+            const int done = GUARD ? (cl < NI ? cl : NI) : (C < NI ? C : NI);
+            for (int k = done; k < NI; ++k) {
+                issue(k, ITEMS);
+                finish();
+            }
+        }
+        if (GUARD && (cl & 1)) {
+            copy_state(ub, ua);
+            copy_state(gb, ga);
+            // The next layer's rows were loaded into set 1 - (cl & 1); the walk ends here.
+        }
+        __syncthreads();
+        pc0 = c0;
+        pbuf = buf;
+        have_prev = true;
+    };
+    int c0 = 0, sidx = 0;
+    for (; c0 + C <= nl; c0 += C, ++sidx) run_stage(std::false_type(), c0, C, stage + (sidx & 1) * BSTRIDE);
+    int pitems = ITEMS;
+    if (c0 < nl) {
+        run_stage(std::true_type(), c0, nl - c0, stage + (sidx & 1) * BSTRIDE);
+        pitems = (nl - c0) * NPh;
+    }
+    // The last stage: converted after its barrier, without overlap.
+    if (have_prev) {
+        const int ni = (pitems + NTHR - 1) / NTHR;
+        for (int k = 0; k < ni; ++k) {
+            issue(k, pitems);
+            finish();
+        }
+    }
+    __syncthreads();  // the next tile of this workgroup reuses the buffers
+    if (u_in != nullptr) store_tile(ua, u_in, q, N, B, ldu);
+    if (g_in != nullptr) store_tile(ga, g_in, q, N, B, ldu);
+}
+
+template <class T, int W, int V, int NW, int C, bool SC>
+__global__ void __launch_bounds__(64 * NW) k_bwd_nn(int N, int l0, int nl, int B, const int* __restrict__ kind,
+                                                    const typename RowOf<T>::type* __restrict__ rows_inv,
+                                                    const T* __restrict__ dtdp, float* __restrict__ partial,
+                                                    size_t part_stride, long ntiles, const T* u_out, const T* g_out,
+                                                    const T* target, T* u_in, T* g_in, int ldu, Ckpt ck,
+                                                    const float* __restrict__ scales) {
+    typedef typename RowOf<T>::type R;
+    extern __shared__ unsigned char smem_raw[];
+    R* stage = reinterpret_cast<R*>(smem_raw);  // [2][NW][C][NP/2]
+    float* my_partial = partial + (size_t)blockIdx.x * part_stride;
+    bool first = true;
+    for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        bwd_tile_nn<T, W, V, NW, C, SC>(N, l0, nl, B, (ConstInt*)kind, rows_inv, dtdp, my_partial, u_out, g_out,
+                                        target, u_in, g_in, ldu, tile, first, stage, ck, scales);
+        first = false;
+    }
+}
+
 // PIPE: 0 = bwd_tile; 4 or 8 = bwd_tile_pipe for workgroups of up to PIPE waves.
 // SC: scaled-unitary crossings, adjoint rows = scales (per waveguide) times rows_inv.
 template <class T, int W, int V, bool UNI, int PIPE, bool SC = false>
@@ -1336,7 +1549,20 @@ struct Plan {
     int uni;          // crossings are unitary (T^-1 = T^H) or, with scaled, scaled unitaries
     int scaled;       // generic crossings asserted to be c U: adjoint rows = |det T| x inverse rows
     const float* scales = nullptr;  // scaled: per-waveguide |det T| table (set by the caller of a launch)
+    int nn = 1;       // the NN-specialised adjoint may serve this plan (off: MESHPROP_GENERIC_ADJOINT)
 };
+
+// Stage depth the NN-specialised adjoint is compiled for: two buffers of C layers of the K-form
+// stage fill the 64 KiB (four waves) or 128 KiB (eight waves) budget, C = 16 / W.
+__host__ __device__ constexpr int nn_stage_depth(int W) { return 16 / W; }
+
+// Whether the NN-specialised adjoint (k_bwd_nn) serves a plan: pipelined unitary (or
+// scaled-unitary) adjoint of complex fields on an NN geometry, one lane group per wave, four- or
+// eight-wave workgroups and the stage depth it is compiled for.
+inline bool nn_adjoint(const Plan& P) {
+    return P.nn && P.pipe && !P.fft && P.uni && !P.real && P.LPV == 64 && P.W >= 2 && P.W <= 8 &&
+           (P.waves == 4 || P.waves == 8) && P.C == nn_stage_depth(P.W);
+}
 
 // LDS of one wave's stage layer in the adjoint: one entry per waveguide, or one per crossing
 // slot for the K-form of unitary crossings.

@@ -160,6 +160,7 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
     // Generic crossings asserted to be scaled unitaries run the unitary adjoint.
     P.scaled = adj && X == X_GENERIC && (flags & MESHPROP_SCALED_UNITARY) != 0;
     P.scales = nullptr;
+    P.nn = (flags & MESHPROP_GENERIC_ADJOINT) ? 0 : 1;
     // The fused forward + adjoint runs the unitary adjoint: unitary or scaled-unitary crossings.
     if (op == MESHPROP_OP_FWDGRAD && X == X_GENERIC && !P.scaled) return MESHPROP_ERR_UNSUPPORTED;
     if (N <= 0 || (N & 1) || L < 0 || B < 0) return MESHPROP_ERR_ARG;
@@ -516,6 +517,14 @@ int meshprop_resolve_tile(int op, int X, int N, int B, meshprop_tile* out) {
     out->waves = P.waves;
     out->chunk = P.C;
     return MESHPROP_OK;
+}
+
+int meshprop_adjoint_kernel(int op, int X, int N, int L, int B, int fft, const meshprop_tile* tile, int flags) {
+    Plan P;
+    const int rc = make_plan(op, X, N, L, B, fft, tile, P, flags);
+    if (rc) return -rc;
+    if (op != MESHPROP_OP_BACKDIFF && op != MESHPROP_OP_FWDGRAD) return 0;
+    return nn_adjoint(P) ? 1 : 0;
 }
 
 int meshprop_run(int op, int X, int N, int L, int B, int fft, const int* geom0, const int* geom1, const float* p,
