@@ -454,7 +454,7 @@ def main():
     ms_step = wall * 1e3 / args.steps
     kb_avg = float(np.mean(kb_ms))
     flop_k = 96.0 * updates
-    kname = "k_bwd<complex64> (backdiff propagation)"
+    kname = "k_bwd_nn<complex64, 4, 2, 8, 4> (backdiff propagation, NN-specialised adjoint)"
     traffic = load_traffic("k_bwd_c2_traffic.json")
     if onecall:
         kernels_ms = {"prologue_and_fwdprop": float(np.mean(pre_ms)), "backdiff_propagation": kb_avg}
