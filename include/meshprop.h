@@ -64,13 +64,14 @@ enum meshprop_op {
 };
 
 /* Optional tile choice.  Zero fields mean "library default".  w: waveguides per lane
- * (2, 4, 8, 16); v: vectors per lane (1, 2, 4, 8); waves: wave64s per workgroup (1..4);
- * chunk: layers per LDS reduction stage of backdiff. */
+ * (2, 4, 8, 16); v: vectors per lane (1, 2, 4, 8); waves: wave64s per workgroup (1..4, or 8
+ * for the pipelined unitary adjoints); chunk: layers per LDS reduction stage of backdiff. */
 typedef struct meshprop_tile {
     int w, v, waves, chunk;
 } meshprop_tile;
 
-/* Bytes of device workspace the given call needs (0 is possible). */
+/* Bytes of device workspace the given call needs (0 is possible), with any flags of
+ * meshprop_run. */
 size_t meshprop_workspace_size(int op, int X, int N, int L, int B, int fft, const meshprop_tile *tile);
 
 /* Replaces fwdprop_N{64K}_{mzi,sym,orth} (gpu/fwdprop.cu:25-27; launched at gpu/mesh.py:158-163). */

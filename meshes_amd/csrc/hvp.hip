@@ -399,7 +399,7 @@ int meshprop_hvp(int X, int N, int L, int B, int fft, const int* geom0, const in
     hipLaunchKernelGGL((k_rows<float4>), dim3((P.NP / 2 + 255) / 256, L), dim3(256), 0, st, X, N, L, P.NP, P.LPV, fft,
                        geom0, geom1, p, ldp, sp, lds, q, (int)(TAB_INV | TAB_DIFF_ADJ | TAB_KUNI),
                        (int*)(base + P.off_kind), (float4*)(base + P.off_r), (float4*)(base + P.off_d),
-                       (float2*)(base + P.off_t1), nullptr);
+                       (float2*)(base + P.off_t1), nullptr, 0);
     if (hipGetLastError() != hipSuccess) return MESHPROP_ERR_HIP;
     const dim3 g2((P.NP / 2 + 255) / 256, L);
     hipLaunchKernelGGL(k_dkform, g2, dim3(256), 0, st, X, N, L, P.NP, fft, geom0, geom1, p, q, ldp, sp, lds,

@@ -156,11 +156,15 @@ __device__ void slot_tables_at(const int X, const int N, const int NP, const int
                                const float* __restrict__ dp, const int tabs, int* __restrict__ kind,
                                R* __restrict__ tab_a, R* __restrict__ tab_b,
                                typename FieldOfRow<R>::type* __restrict__ dtdp, float* __restrict__ scl,
-                               const int l, const int m) {
+                               const int l, const int m, const int kso = 0) {
     typedef typename FieldOfRow<R>::type F;
     const int kd = layer_kind(fft, g0, g1, l, NP);
     if (m == 0 && kind) kind[l] = kd;
     const int NPh = NP / 2;
+    // Index of slot m in the K-form table: slot order, or (kso > 0, one 64-lane vector per wave)
+    // the adjoint's stage order, in which lane t's kso slots t*kso .. t*kso + kso - 1 sit 64
+    // entries apart (position p*64 + t for slot t*kso + p).
+    const int km = kso > 0 ? (m % kso) * 64 + m / kso : m;
     int i, j;
     bool act;
     if (fft) {
@@ -209,7 +213,7 @@ __device__ void slot_tables_at(const int X, const int N, const int NP, const int
         }
         if (tabs & TAB_KUNI) {
             typedef typename KOf<F>::type K;
-            K* out = reinterpret_cast<K*>(dtdp) + (size_t)l * np * NPh + m;
+            K* out = reinterpret_cast<K*>(dtdp) + (size_t)l * np * NPh + km;
             for (int k2 = 0; k2 < np; ++k2) out[(size_t)k2 * NPh] = K{};
         }
         return;
@@ -263,7 +267,7 @@ __device__ void slot_tables_at(const int X, const int N, const int NP, const int
         // Anti-Hermitian part of K = T^-1 dT/dp (T^-1 = T^H for a unitary T; for a scaled
         // unitary T = c U it equals U^H dU/dp).
         typedef typename KOf<F>::type K;
-        K* out = reinterpret_cast<K*>(dtdp) + (size_t)l * np * NPh + m;
+        K* out = reinterpret_cast<K*>(dtdp) + (size_t)l * np * NPh + km;
         const M2 Ti = (X == X_GENERIC) ? minv(T) : mherm(T);
         for (int k2 = 0; k2 < np; ++k2) {
             const M2 Kp = mmul(Ti, dT[k2]);
@@ -277,18 +281,19 @@ __device__ void slot_tables_at(const int X, const int N, const int NP, const int
 }
 
 // Prologue of one call: row tables (tabs) and, with TAB_DTDP, the dT/dp table of every slot,
-// all in one launch, one thread per (layer, slot).
+// all in one launch, one thread per (layer, slot).  kso > 0: K-form table in stage order.
 template <class R>
 __global__ void k_rows(int X, int N, int L, int NP, int LPV, int fft, const int* __restrict__ g0,
                        const int* __restrict__ g1,
                        const float* __restrict__ p, int ldp, const float* __restrict__ s, int lds,
                        const float* __restrict__ dp, int tabs, int* __restrict__ kind, R* __restrict__ tab_a,
                        R* __restrict__ tab_b, typename FieldOfRow<R>::type* __restrict__ dtdp,
-                       float* __restrict__ scl) {
+                       float* __restrict__ scl, int kso) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     const int l = blockIdx.y;
     if (l >= L || m >= NP / 2) return;
-    slot_tables_at<R>(X, N, NP, LPV, fft, g0, g1, p, ldp, s, lds, dp, tabs, kind, tab_a, tab_b, dtdp, scl, l, m);
+    slot_tables_at<R>(X, N, NP, LPV, fft, g0, g1, p, ldp, s, lds, dp, tabs, kind, tab_a, tab_b, dtdp, scl, l, m,
+                      kso);
 }
 
 // =======================================================================================
@@ -1265,20 +1270,29 @@ __device__ __forceinline__ void bwd_tile_pipe(const int N, const int l0, const i
 // the C layer kinds of a stage read once at its start, and the rows (and scaled-unitary
 // factors) read with raw buffer loads (scalar layer offset, fixed lane offset).
 //
+// Stage order of the NN-specialised adjoint of width W.  Slot order puts lane t's W/2 entries
+// side by side (lane stride W/2 entries of 16 B: bank conflicts on the stage writes grow with
+// W); stage order puts entry p of lane t at p*64 + t, so each write instruction of a wave is
+// 1 KiB of consecutive LDS.  The conversion, the K-form table (written by the prologue in the
+// same order) and the dp partials are then indexed by stage position, and the dp reduction
+// maps positions back to slots.
+__host__ __device__ constexpr bool nn_stage_order(int W) { return W >= 16; }
+
 // K-form stage entries of one NN layer (lane t owns slots t*W/2 .. t*W/2 + W/2 - 1; an odd
-// layer pairs the lane's last waveguide with the next lane's first).
-template <class T, int W>
+// layer pairs the lane's last waveguide with the next lane's first); entry p at wl[p * ES].
+template <class T, int W, int ES>
 __device__ __forceinline__ void kstage_write_nn(const bool odd, const T (&a)[W], const T (&b)[W],
                                                 typename RowOf<T>::type* wl) {
     constexpr int H = W / 2;
     if (odd) {
         const T na = a_next(a[0]), nb = lane_next(b[0]);
 #pragma unroll
-        for (int p = 0; p + 1 < H; ++p) wl[p] = kquad(a[2 * p + 1], a[2 * p + 2], b[2 * p + 1], b[2 * p + 2]);
-        wl[H - 1] = kquad(a[W - 1], na, b[W - 1], nb);
+        for (int p = 0; p + 1 < H; ++p)
+            wl[p * ES] = kquad(a[2 * p + 1], a[2 * p + 2], b[2 * p + 1], b[2 * p + 2]);
+        wl[(H - 1) * ES] = kquad(a[W - 1], na, b[W - 1], nb);
     } else {
 #pragma unroll
-        for (int p = 0; p < H; ++p) wl[p] = kquad(a[2 * p], a[2 * p + 1], b[2 * p], b[2 * p + 1]);
+        for (int p = 0; p < H; ++p) wl[p * ES] = kquad(a[2 * p], a[2 * p + 1], b[2 * p], b[2 * p + 1]);
     }
 }
 
@@ -1298,7 +1312,7 @@ __device__ __forceinline__ void bwd_layer_nn(const bool odd, const T (&u)[V][W],
         st.template run<1, 1>(0);
     else
         st.template run<0, 1>(0);
-    kstage_write_nn<T, W>(odd, accA, accB, wl);
+    kstage_write_nn<T, W, nn_stage_order(W) ? 64 : 1>(odd, accA, accB, wl);
 }
 
 template <class T, int W, int V, int NW, int C, bool SC>
@@ -1318,7 +1332,12 @@ __device__ __forceinline__ void bwd_tile_nn(const int N, const int l0, const int
     constexpr int NI = (ITEMS + NTHR - 1) / NTHR;      // items per thread and stage
     constexpr int WSTRIDE = C * NPh;                   // one wave's entries in one buffer
     constexpr int BSTRIDE = NW * WSTRIDE;              // one buffer
-    static_assert(C % 2 == 0, "the field and row register sets alternate per layer");
+    // The field and row register sets alternate per layer (set of local layer s: s & 1); a
+    // stage of odd depth starts on either set, so such stages run in pairs.
+    static_assert(C % 2 == 0 || C == 1, "stage depth: even, or one layer");
+    static_assert(C > 1 || nn_stage_order(W), "one-layer stages are compiled for the stage order only");
+    // Widest tile: the rows are loaded right before their layer (one row set live).
+    constexpr bool LEAN = W >= 16;
     const int lane = threadIdx.x & 63;
     const int wave_u = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int last = nl - 1;
@@ -1385,29 +1404,46 @@ __device__ __forceinline__ void bwd_tile_nn(const int N, const int l0, const int
         for (int k = 0; k < np; ++k) val[k] = kdot(e, it.dd[k]);
         put_dp<np>(it.out, val, first);
     };
-    if (nl > 0) rows_at(0, ri[0], sc[0]);
+    if (!LEAN && nl > 0) rows_at(0, ri[0], sc[0]);
     bool have_prev = false;
-    // One stage of cl layers starting at local layer c0 (GUARD: cl < C, the last stage).
-    auto run_stage = [&](auto guard, const int c0, const int cl, R* buf) {
+    // One stage of cl layers starting at local layer c0 (GUARD: cl < C, the last stage); PAR is
+    // the parity of c0, which picks the register sets of its layers; ODD is the shift parity of
+    // a one-layer stage when known at compile time (-1: read per layer).
+    auto run_stage = [&](auto guard, auto par, auto odd_c, const int c0, const int cl, R* buf) {
         constexpr bool GUARD = decltype(guard)::value;
+        constexpr int PAR = decltype(par)::value;
+        constexpr int ODD = decltype(odd_c)::value;
         if (ck.base != nullptr && c0 > 0) {
             const int e = l0 + nl - c0;
-            if (e % ck.every == 0)
-                load_tile(ua, reinterpret_cast<const T*>(ck.base) + (size_t)(e / ck.every) * ck.stride, q, N, B, ldu);
+            if (e % ck.every == 0) {
+                const T* cp = reinterpret_cast<const T*>(ck.base) + (size_t)(e / ck.every) * ck.stride;
+                if constexpr (PAR == 0)
+                    load_tile(ua, cp, q, N, B, ldu);
+                else
+                    load_tile(ub, cp, q, N, B, ldu);
+            }
         }
         int kd[C];
 #pragma unroll
         for (int lc = 0; lc < C; ++lc) kd[lc] = kind[l0 + (c0 + lc < last ? last - (c0 + lc) : 0)];
-        R* wl = buf + wave_u * WSTRIDE + lane * (W / 2);
+        R* wl = buf + wave_u * WSTRIDE + (nn_stage_order(W) ? lane : lane * (W / 2));
 #pragma unroll
         for (int lc = 0; lc < C; ++lc) {
             if (!GUARD || lc < cl) {
+                const int cur = (lc + PAR) & 1;  // register sets of this layer (a constant)
+                const int rs = LEAN ? 0 : cur;
                 if (lc < NI && have_prev) issue(lc, ITEMS);
-                rows_at(c0 + lc + 1, ri[(lc + 1) & 1], sc[(lc + 1) & 1]);
-                if ((lc & 1) == 0)
-                    bwd_layer_nn<T, W, V, SC>(kd[lc] & 1, ua, ga, ub, gb, ri[0], wl + lc * NPh, lane, sc[0]);
+                if constexpr (LEAN)
+                    rows_at(c0 + lc, ri[0], sc[0]);
                 else
-                    bwd_layer_nn<T, W, V, SC>(kd[lc] & 1, ub, gb, ua, ga, ri[1], wl + lc * NPh, lane, sc[1]);
+                    rows_at(c0 + lc + 1, ri[cur ^ 1], sc[cur ^ 1]);
+                const bool odd = ODD >= 0 ? ODD != 0 : (kd[lc] & 1) != 0;
+                if constexpr (C == 1)  // one-layer stages update the state in place
+                    bwd_layer_nn<T, W, V, SC>(odd, ua, ga, ua, ga, ri[rs], wl, lane, sc[rs]);
+                else if (cur == 0)
+                    bwd_layer_nn<T, W, V, SC>(odd, ua, ga, ub, gb, ri[rs], wl + lc * NPh, lane, sc[rs]);
+                else
+                    bwd_layer_nn<T, W, V, SC>(odd, ub, gb, ua, ga, ri[rs], wl + lc * NPh, lane, sc[rs]);
                 if (lc < NI && have_prev) finish();
             }
         }
@@ -1419,22 +1455,39 @@ __device__ __forceinline__ void bwd_tile_nn(const int N, const int l0, const int
                 finish();
             }
         }
-        if (GUARD && (cl & 1)) {
-            copy_state(ub, ua);
-            copy_state(gb, ga);
-            // The next layer's rows were loaded into set 1 - (cl & 1); the walk ends here.
-        }
         __syncthreads();
         pc0 = c0;
         pbuf = buf;
         have_prev = true;
     };
+    typedef std::integral_constant<int, 0> I0;
+    typedef std::integral_constant<int, 1> I1;
+    typedef std::integral_constant<int, -1> Irt;
     int c0 = 0, sidx = 0;
-    for (; c0 + C <= nl; c0 += C, ++sidx) run_stage(std::false_type(), c0, C, stage + (sidx & 1) * BSTRIDE);
+    if constexpr (C % 2 == 0) {
+        for (; c0 + C <= nl; c0 += C, ++sidx)
+            run_stage(std::false_type(), I0(), Irt(), c0, C, stage + (sidx & 1) * BSTRIDE);
+    } else {
+        // One-layer stages, the state updated in place, one body per shift parity.
+        for (; c0 < nl; ++c0, ++sidx) {
+            R* buf = stage + (sidx & 1) * BSTRIDE;
+            if (kind[l0 + last - c0] & 1)
+                run_stage(std::false_type(), I0(), I1(), c0, 1, buf);
+            else
+                run_stage(std::false_type(), I0(), I0(), c0, 1, buf);
+        }
+    }
     int pitems = ITEMS;
-    if (c0 < nl) {
-        run_stage(std::true_type(), c0, nl - c0, stage + (sidx & 1) * BSTRIDE);
-        pitems = (nl - c0) * NPh;
+    if constexpr (C > 1) {
+        if (c0 < nl) {
+            run_stage(std::true_type(), I0(), Irt(), c0, nl - c0, stage + (sidx & 1) * BSTRIDE);
+            pitems = (nl - c0) * NPh;
+        }
+    }
+    // An odd number of layers leaves the state in the second register set.
+    if (C > 1 && (nl & 1)) {
+        copy_state(ub, ua);
+        copy_state(gb, ga);
     }
     // The last stage: converted after its barrier, without overlap.
     if (have_prev) {
@@ -1449,13 +1502,12 @@ __device__ __forceinline__ void bwd_tile_nn(const int N, const int l0, const int
     if (g_in != nullptr) store_tile(ga, g_in, q, N, B, ldu);
 }
 
+#define MP_BWD_NN_PARAMS                                                                                  \
+    int N, int l0, int nl, int B, const int *__restrict__ kind, const typename RowOf<T>::type *__restrict__ rows_inv, \
+        const T *__restrict__ dtdp, float *__restrict__ partial, size_t part_stride, long ntiles, const T *u_out,   \
+        const T *g_out, const T *target, T *u_in, T *g_in, int ldu, Ckpt ck, const float *__restrict__ scales
 template <class T, int W, int V, int NW, int C, bool SC>
-__global__ void __launch_bounds__(64 * NW) k_bwd_nn(int N, int l0, int nl, int B, const int* __restrict__ kind,
-                                                    const typename RowOf<T>::type* __restrict__ rows_inv,
-                                                    const T* __restrict__ dtdp, float* __restrict__ partial,
-                                                    size_t part_stride, long ntiles, const T* u_out, const T* g_out,
-                                                    const T* target, T* u_in, T* g_in, int ldu, Ckpt ck,
-                                                    const float* __restrict__ scales) {
+__device__ __forceinline__ void bwd_nn_body(MP_BWD_NN_PARAMS) {
     typedef typename RowOf<T>::type R;
     extern __shared__ unsigned char smem_raw[];
     R* stage = reinterpret_cast<R*>(smem_raw);  // [2][NW][C][NP/2]
@@ -1467,6 +1519,18 @@ __global__ void __launch_bounds__(64 * NW) k_bwd_nn(int N, int l0, int nl, int B
         first = false;
     }
 }
+template <class T, int W, int V, int NW, int C, bool SC>
+__global__ void __launch_bounds__(64 * NW) k_bwd_nn(MP_BWD_NN_PARAMS) {
+    bwd_nn_body<T, W, V, NW, C, SC>(N, l0, nl, B, kind, rows_inv, dtdp, partial, part_stride, ntiles, u_out, g_out,
+                                    target, u_in, g_in, ldu, ck, scales);
+}
+// The widest tile (W = 16), compiled for two waves per SIMD also in four-wave workgroups.
+template <class T, int W, int V, int NW, int C, bool SC>
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2))) k_bwd_nn_w16(MP_BWD_NN_PARAMS) {
+    bwd_nn_body<T, W, V, NW, C, SC>(N, l0, nl, B, kind, rows_inv, dtdp, partial, part_stride, ntiles, u_out, g_out,
+                                    target, u_in, g_in, ldu, ck, scales);
+}
+#undef MP_BWD_NN_PARAMS
 
 // PIPE: 0 = bwd_tile; 4 or 8 = bwd_tile_pipe for workgroups of up to PIPE waves.
 // SC: scaled-unitary crossings, adjoint rows = scales (per waveguide) times rows_inv.
@@ -1550,18 +1614,20 @@ struct Plan {
     int scaled;       // generic crossings asserted to be c U: adjoint rows = |det T| x inverse rows
     const float* scales = nullptr;  // scaled: per-waveguide |det T| table (set by the caller of a launch)
     int nn = 1;       // the NN-specialised adjoint may serve this plan (off: MESHPROP_GENERIC_ADJOINT)
+    int sorder = 0;   // > 0: K-form table and dp partials in the NN adjoint's stage order (W/2 slots per lane)
 };
 
 // Stage depth the NN-specialised adjoint is compiled for: two buffers of C layers of the K-form
-// stage fill the 64 KiB (four waves) or 128 KiB (eight waves) budget, C = 16 / W.
+// stage fill the 64 KiB (four waves) or 128 KiB (eight waves) budget, C = 16 / W (one layer at
+// W = 16).
 __host__ __device__ constexpr int nn_stage_depth(int W) { return 16 / W; }
 
 // Whether the NN-specialised adjoint (k_bwd_nn) serves a plan: pipelined unitary (or
 // scaled-unitary) adjoint of complex fields on an NN geometry, one lane group per wave, four- or
-// eight-wave workgroups and the stage depth it is compiled for.
+// eight-wave workgroups and the stage depth it is compiled for (W = 16: one vector per lane).
 inline bool nn_adjoint(const Plan& P) {
-    return P.nn && P.pipe && !P.fft && P.uni && !P.real && P.LPV == 64 && P.W >= 2 && P.W <= 8 &&
-           (P.waves == 4 || P.waves == 8) && P.C == nn_stage_depth(P.W);
+    return P.nn && P.pipe && !P.fft && P.uni && !P.real && P.LPV == 64 && P.W >= 2 && P.W <= 16 &&
+           (P.W < 16 || P.V == 1) && (P.waves == 4 || P.waves == 8) && P.C == nn_stage_depth(P.W);
 }
 
 // LDS of one wave's stage layer in the adjoint: one entry per waveguide, or one per crossing

@@ -67,11 +67,17 @@ static int go_nn(const Plan& P, const int* kind, const void* ri, const void* dtd
     constexpr bool SC = MP_SC_SET != 0;
     const size_t shmem = (size_t)2 * NW * C * (W * 32) * sizeof(R);
     static std::atomic<unsigned long long> lds_set{0};
-    const void* fn = (const void*)k_bwd_nn<T, W, V, NW, C, SC>;
-    if (int rc = lds_limit_once(fn, lds_set, shmem)) return rc;
-    hipLaunchKernelGGL((k_bwd_nn<T, W, V, NW, C, SC>), dim3(P.grid), dim3(64 * NW), shmem, st, P.N, l0, nl, P.B, kind,
-                       (const R*)ri, (const T*)dtdp, partial, ps, P.ntiles, (const T*)u_out, (const T*)g_out,
-                       (const T*)target, (T*)u_in, (T*)g_in, ldu, ck, P.scales);
+    // W = 16: the entry compiled for two waves per SIMD.
+    auto kern = [] {
+        if constexpr (W >= 16)
+            return k_bwd_nn_w16<T, W, V, NW, C, SC>;
+        else
+            return k_bwd_nn<T, W, V, NW, C, SC>;
+    }();
+    if (int rc = lds_limit_once((const void*)kern, lds_set, shmem)) return rc;
+    hipLaunchKernelGGL(kern, dim3(P.grid), dim3(64 * NW), shmem, st, P.N, l0, nl, P.B, kind, (const R*)ri,
+                       (const T*)dtdp, partial, ps, P.ntiles, (const T*)u_out, (const T*)g_out, (const T*)target,
+                       (T*)u_in, (T*)g_in, ldu, ck, P.scales);
     return MESHPROP_OK;
 }
 
@@ -95,20 +101,20 @@ template <class T, int W, int V>
 static int go(const Plan& P, const int* kind, const void* ri, const void* ra, const void* dtdp, float* partial,
               size_t ps, int l0, int nl, const void* u_out, const void* g_out, const void* target, void* u_in,
               void* g_in, int ldu, hipStream_t st, Ckpt ck) {
-    // The pipelined stage is compiled for the tiles it can serve (two buffers of two layers
-    // fit the LDS budget for W <= 4, and for W = 8 with the per-crossing K-form stage), once
-    // for four-wave and once for eight-wave groups.
-    if constexpr (W <= 8) {
-        if constexpr (std::is_same<T, float2>::value && W >= 2) {
-            // Unitary (or, in the scaled object, scaled-unitary) adjoints on NN geometries.
-            if (nn_adjoint(P) && (MP_SC_SET || !P.scaled)) {
-                if (P.waves == 8)
-                    return go_nn<T, W, V, 8>(P, kind, ri, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in, g_in,
-                                             ldu, st, ck);
-                return go_nn<T, W, V, 4>(P, kind, ri, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in, g_in, ldu,
+    // Unitary (or, in the scaled object, scaled-unitary) adjoints on NN geometries.
+    if constexpr (std::is_same<T, float2>::value && W >= 2 && (W < 16 || V == 1)) {
+        if (nn_adjoint(P) && (MP_SC_SET || !P.scaled)) {
+            if (P.waves == 8)
+                return go_nn<T, W, V, 8>(P, kind, ri, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in, g_in, ldu,
                                          st, ck);
-            }
+            return go_nn<T, W, V, 4>(P, kind, ri, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in, g_in, ldu,
+                                     st, ck);
         }
+    }
+    // The general pipelined stage is compiled for the tiles it can serve (two buffers of two
+    // layers fit the LDS budget for W <= 4, and for W = 8 with the per-crossing K-form stage),
+    // once for four-wave and once for eight-wave groups.
+    if constexpr (W <= 8) {
         if (P.pipe) {
             if (P.waves > 4) return go2<T, W, V, kPipeWaves>(MP_GO_ARGS);
             return go2<T, W, V, 4>(MP_GO_ARGS);

@@ -27,22 +27,29 @@ namespace mp {
 // entries; wave q sums workgroups [q*nwg/4, (q+1)*nwg/4) with four interleaved accumulators and
 // the four wave sums are added in wave order through LDS: one launch, a fixed summation order
 // (bitwise reproducible), and four waves' worth of loads in flight per entry.
+// kso > 0: the partials are in the NN adjoint's stage order (position p*64 + t holds slot
+// t*kso + p); the blocks then run over positions (coalesced partial reads) and scatter to dp.
 __global__ void __launch_bounds__(256) k_dp_reduce(int L, int Nh, int NPh, int np, int nwg, size_t E,
                                                    const float* __restrict__ partial, float* __restrict__ dp, int ldp,
-                                                   int assign) {
+                                                   int assign, int kso) {
     __shared__ float red[4][64];
     const int lane = threadIdx.x & 63, wq = threadIdx.x >> 6;
     const long idx = (long)blockIdx.x * 64 + lane;
-    const long total = (long)L * Nh * np;
-    const bool ok = idx < total;
-    int k = 0, mm = 0, l = 0;
-    float acc = 0.f;
+    const int Nr = kso > 0 ? NPh : Nh;  // entries per layer the blocks run over
+    const long total = (long)L * Nr * np;
+    bool ok = idx < total;
+    int k = 0, mm = 0, l = 0, pos = 0;
     if (ok) {
         k = (int)(idx % np);
         const long lm = idx / np;
-        mm = (int)(lm % Nh);
-        l = (int)(lm / Nh);
-        const float* src = partial + ((size_t)l * NPh + mm) * np + k;
+        pos = (int)(lm % Nr);
+        l = (int)(lm / Nr);
+        mm = kso > 0 ? (pos % 64) * kso + pos / 64 : pos;
+        ok = mm < Nh;
+    }
+    float acc = 0.f;
+    if (ok) {
+        const float* src = partial + ((size_t)l * NPh + pos) * np + k;
         const int w0 = (int)((long)nwg * wq / 4), w1 = (int)((long)nwg * (wq + 1) / 4);
         float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
         int w = w0;
@@ -160,6 +167,7 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
     // Generic crossings asserted to be scaled unitaries run the unitary adjoint.
     P.scaled = adj && X == X_GENERIC && (flags & MESHPROP_SCALED_UNITARY) != 0;
     P.scales = nullptr;
+    P.sorder = 0;
     P.nn = (flags & MESHPROP_GENERIC_ADJOINT) ? 0 : 1;
     // The fused forward + adjoint runs the unitary adjoint: unitary or scaled-unitary crossings.
     if (op == MESHPROP_OP_FWDGRAD && X == X_GENERIC && !P.scaled) return MESHPROP_ERR_UNSUPPORTED;
@@ -196,12 +204,17 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
     if (adj && P.uni && P.W == 8 && !(tile && tile->v >= 1)) P.V = 1;
     P.NP = P.LPV * P.W;
     P.rowbytes = P.real ? sizeof(float2) : sizeof(float4);
+    // Widest unitary NN adjoint (W = 16, one vector per lane): the NN-specialised kernel with a
+    // pipelined stage of one layer (two buffers of 8 KiB per wave).
+    const bool nn16 = adj && P.nn && P.W == 16 && P.V == 1 && P.uni && !P.real && !fft && P.LPV == 64 &&
+                      !(tile && tile->chunk > 1);
     {
         // Eight-wave adjoint workgroups exist only with the pipelined stage (two buffers of two
-        // layers of eight waves in the wide budget).
+        // layers of eight waves in the wide budget; one layer at W = 16).
         const bool pipe_w = P.W <= 4 || (P.W == 8 && P.uni);  // widths with a pipelined stage
-        const bool wide_ok = adj && pipe_w && (P.NP / 2) % 64 == 0 &&
-                             (size_t)4 * kPipeWaves * stage_layer_bytes(P) <= stage_budget(kPipeWaves);
+        const bool wide_ok = adj && ((pipe_w && (P.NP / 2) % 64 == 0 &&
+                                      (size_t)4 * kPipeWaves * stage_layer_bytes(P) <= stage_budget(kPipeWaves)) ||
+                                     nn16);
         const int cap = adj ? (wide_ok ? kPipeWaves : bwd_max_waves(P.W, P.V, P.uni)) : 4;
         // Automatic choice: eight-wave adjoint workgroups for unitary crossings when the batch
         // still fills every CU with them (half the dp partials of four-wave groups; measured at
@@ -253,9 +266,13 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
                 C = Cp;
             }
         }
+        if (nn16) {
+            P.pipe = 1;
+            C = 1;
+        }
         if (P.waves > 4 && !P.pipe) return MESHPROP_ERR_UNSUPPORTED;  // only the pipelined stage sums 8 waves
         P.C = C;
-
+        P.sorder = (nn_adjoint(P) && nn_stage_order(P.W)) ? P.W / 2 : 0;
     } else {
         P.C = 0;
         P.pipe = 0;
@@ -350,10 +367,10 @@ int run_impl(int op, int X, int N, int L, int B, int fft, const int* g0, const i
         dim3 grid((P.NP / 2 + 255) / 256, L);  // one thread per slot
         if (P.real)
             hipLaunchKernelGGL((k_rows<float2>), grid, dim3(256), 0, st, X, N, L, P.NP, P.LPV, fft, g0, g1, p, ldp, sp, lds,
-                               (const float*)dp, tabs, kind, (float2*)tab_a, (float2*)tab_b, (float*)dtdp, scl);
+                               (const float*)dp, tabs, kind, (float2*)tab_a, (float2*)tab_b, (float*)dtdp, scl, P.sorder);
         else
             hipLaunchKernelGGL((k_rows<float4>), grid, dim3(256), 0, st, X, N, L, P.NP, P.LPV, fft, g0, g1, p, ldp, sp, lds,
-                               (const float*)dp, tabs, kind, (float4*)tab_a, (float4*)tab_b, (float2*)dtdp, scl);
+                               (const float*)dp, tabs, kind, (float4*)tab_a, (float4*)tab_b, (float2*)dtdp, scl, P.sorder);
         if ((rc = check_launch())) return rc;
     }
 
@@ -416,10 +433,11 @@ int run_impl(int op, int X, int N, int L, int B, int fft, const int* g0, const i
         if ((rc = check_launch())) return rc;
     }
     if ((op == MESHPROP_OP_BACKDIFF || op == MESHPROP_OP_FWDGRAD) && L > 0 && do_red) {
-        const long total = (long)L * (N / 2) * P.np;
+        const long total = (long)L * (P.sorder > 0 ? P.NP / 2 : N / 2) * P.np;
         const size_t E = (size_t)L * (P.NP / 2) * P.np;
         hipLaunchKernelGGL(k_dp_reduce, dim3((unsigned)((total + 63) / 64)), dim3(256), 0, st, L, N / 2, P.NP / 2,
-                           P.np, P.grid, E, (const float*)partial, dp, ldp, (flags & MESHPROP_DP_ASSIGN) ? 1 : 0);
+                           P.np, P.grid, E, (const float*)partial, dp, ldp, (flags & MESHPROP_DP_ASSIGN) ? 1 : 0,
+                           P.sorder);
         if ((rc = check_launch())) return rc;
     }
     return MESHPROP_OK;
@@ -501,10 +519,14 @@ int meshprop_run_noalloc(int op, int X, int N, int L, int B, int fft, const int*
 size_t meshprop_workspace_size(int op, int X, int N, int L, int B, int fft, const meshprop_tile* tile) {
     Plan P;
     size_t total = 0;
-    if (make_plan(op, X, N, L, B, fft, tile, P) == MESHPROP_OK) total = P.total;
-    // Generic crossings: enough for the scaled-unitary plan as well (MESHPROP_SCALED_UNITARY).
-    if (X == X_GENERIC && make_plan(op, X, N, L, B, fft, tile, P, MESHPROP_SCALED_UNITARY) == MESHPROP_OK)
-        total = std::max(total, P.total);
+    // Enough for every plan the flags of a call can select: the scaled-unitary plan of generic
+    // crossings (MESHPROP_SCALED_UNITARY) and the general adjoint (MESHPROP_GENERIC_ADJOINT, whose
+    // workgroups can be smaller, hence more dp partials).
+    for (const int fl : {0, (int)MESHPROP_GENERIC_ADJOINT, (int)MESHPROP_SCALED_UNITARY,
+                         (int)(MESHPROP_SCALED_UNITARY | MESHPROP_GENERIC_ADJOINT)}) {
+        if ((fl & MESHPROP_SCALED_UNITARY) && X != X_GENERIC) continue;
+        if (make_plan(op, X, N, L, B, fft, tile, P, fl) == MESHPROP_OK) total = std::max(total, P.total);
+    }
     return total;
 }
 

@@ -99,6 +99,7 @@ class Case:
         self.out, self.dout = torch.empty_like(self.u), torch.empty_like(self.u)
         self.dp = torch.zeros((L, N // 2, npar), dtype=torch.float32, device=dev)
         self.flags = Lb.SCALED_UNITARY if (X == "mzi_generic" and op == Lb.OP_FWDGRAD) else 0
+        self.extra_flags = 0  # e.g. GENERIC_ADJOINT, to time the general adjoint kernel
         self.dt = dt
 
     def run(self, tile, ws, nb):
@@ -112,10 +113,11 @@ class Case:
                    self.dout, N, tile=tile, workspace=ws, ws_bytes=nb)
         elif op == Lb.OP_BACKDIFF:
             Lb.run(op, X, N, L, B, False, *g, self.p, self.dp, self.ldp, self.s, self.lds, self.u, None, self.out,
-                   None, N, target=self.y, tile=tile, workspace=ws, ws_bytes=nb)
+                   None, N, target=self.y, tile=tile, workspace=ws, ws_bytes=nb, flags=self.extra_flags)
         else:
             Lb.run(op, X, N, L, B, False, *g, self.p, self.dp, self.ldp, self.s, self.lds, self.u, None, self.out,
-                   None, N, target=self.y, tile=tile, workspace=ws, ws_bytes=nb, flags=self.flags | Lb.DP_ASSIGN)
+                   None, N, target=self.y, tile=tile, workspace=ws, ws_bytes=nb,
+                   flags=self.flags | self.extra_flags | Lb.DP_ASSIGN)
 
     def time(self, tile, reps: int = 5, span_ms: float = 0.5) -> float:
         """Median ms of one call with this tile (inf when the library rejects the tile): each of
