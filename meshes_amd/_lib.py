@@ -138,6 +138,19 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def _check_dtypes(X, params, fields):
+    """The kernels read float32 parameters and complex64 fields (float32 for orthogonal
+    crossings) through raw pointers: any other element type is refused here."""
+    xc = X_CODE[X] if isinstance(X, str) else X
+    want = torch.float32 if xc == X_CODE["orth"] else torch.complex64
+    for t in params:
+        if t is not None and t.dtype != torch.float32:
+            raise TypeError(f"parameters must be float32, got {t.dtype}")
+    for t in fields:
+        if t is not None and t.dtype != want:
+            raise TypeError(f"fields must be {want}, got {t.dtype}")
+
+
 def tile_struct(tile):
     if tile is None:
         return None
@@ -186,6 +199,7 @@ def run(op, X, N, L, B, fft, g0, g1, p, dp, ldp, s, lds, u_a, du_a, u_b, du_b, l
         ckpt_every=0, tile=None, workspace=None, ws_bytes=0, stream=None, flags=0):
     """Thin call of meshprop_run on torch tensors (device pointers) and the current stream."""
     lib = load()
+    _check_dtypes(X, (p, dp, s), (u_a, du_a, u_b, du_b, target, ckpt))
     if stream is None:
         stream = torch.cuda.current_stream().cuda_stream
     rc = lib.meshprop_run(op, X_CODE[X] if isinstance(X, str) else X, N, L, B, int(fft), _ptr(g0), _ptr(g1),
@@ -200,6 +214,7 @@ def run_noalloc(op, X, N, L, B, fft, g0, g1, p, dp, ldp, s, lds, u_a, du_a, u_b,
     """meshprop_run_noalloc: meshprop_run with the library's static device scratch in place of a
     workspace (nothing is allocated; calls that overlap on one device race)."""
     lib = load()
+    _check_dtypes(X, (p, dp, s), (u_a, du_a, u_b, du_b, target, ckpt))
     if stream is None:
         stream = torch.cuda.current_stream().cuda_stream
     rc = lib.meshprop_run_noalloc(op, X_CODE[X] if isinstance(X, str) else X, N, L, B, int(fft), _ptr(g0), _ptr(g1),

@@ -1276,7 +1276,10 @@ __device__ __forceinline__ void bwd_tile_pipe(const int N, const int l0, const i
 // 1 KiB of consecutive LDS.  The conversion, the K-form table (written by the prologue in the
 // same order) and the dp partials are then indexed by stage position, and the dp reduction
 // maps positions back to slots.
-__host__ __device__ constexpr bool nn_stage_order(int W) { return W >= 16; }
+// Measured against slot order (MI355X, fwd+grad, Clements MZI): N=256 L=256 B=4096 0.2890 ms
+// against 0.2916 ms; N=512 L=512 B=4096 1.483 against 1.515 ms; at W = 16 slot order made the
+// one-layer stage spill and conflict (W = 2 has one slot per lane: both orders agree).
+__host__ __device__ constexpr bool nn_stage_order(int W) { return W >= 4; }
 
 // K-form stage entries of one NN layer (lane t owns slots t*W/2 .. t*W/2 + W/2 - 1; an odd
 // layer pairs the lane's last waveguide with the next lane's first); entry p at wl[p * ES].

@@ -90,10 +90,11 @@ class Case:
         dt = torch.float32 if real else torch.complex64
 
         def field():
-            a = rng.standard_normal((B, N)).astype(np.float32)
+            a = rng.standard_normal((B, N))
             if not real:
-                a = (a + 1j * rng.standard_normal((B, N))).astype(np.complex64)
-            return torch.as_tensor(a / np.sqrt(N)).to(dev)
+                a = a + 1j * rng.standard_normal((B, N))
+            # the kernels read complex64 (float32 for real fields): cast after the scaling
+            return torch.as_tensor((a / np.sqrt(N)).astype(np.float32 if real else np.complex64)).to(dev)
 
         self.u, self.du, self.y = field(), field(), field()
         self.out, self.dout = torch.empty_like(self.u), torch.empty_like(self.u)

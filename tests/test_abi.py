@@ -200,3 +200,36 @@ def test_tile_table_consulted_without_gpu(lib):
     assert checked > 100
     # an explicit tile wins over the table
     assert L.resolve_tile(L.OP_FWDPROP, "orth", 512, 4096, (8, 2, 4, 0))[:3] == (8, 2, 4)
+
+
+def test_wrappers_refuse_other_element_types():
+    """The ctypes wrappers hand raw pointers to kernels that read complex64 fields (float32 for
+    orthogonal crossings) and float32 parameters: other element types are refused."""
+    import torch
+
+    import meshes_amd._lib as L
+    f32, c64 = torch.zeros(4, dtype=torch.float32), torch.zeros(4, dtype=torch.complex64)
+    L._check_dtypes("mzi", (f32, None), (c64, None, c64))
+    L._check_dtypes("orth", (f32,), (f32,))
+    with pytest.raises(TypeError):
+        L._check_dtypes("mzi", (f32,), (c64.to(torch.complex128),))
+    with pytest.raises(TypeError):
+        L._check_dtypes("orth", (f32,), (c64,))
+    with pytest.raises(TypeError):
+        L._check_dtypes("sym", (f32.double(),), (c64,))
+
+
+def test_w16_adjoint_plans_without_gpu(lib):
+    """Unitary NN meshes of N > 512 run the NN-specialised adjoint with one-layer stages; the
+    general adjoint flag, FFT geometries and real fields keep the general kernel; the workspace
+    covers the general plan's larger partial count."""
+    import meshes_amd._lib as L
+    for tile in (None, (16, 1, 4, 0), (16, 1, 8, 0)):
+        assert L.adjoint_kernel(L.OP_FWDGRAD, "mzi", 1024, 64, 2048, tile=tile) == "nn"
+        assert L.resolve_tile(L.OP_BACKDIFF, "mzi", 1024, 2048, tile)[3] == 1
+    assert L.adjoint_kernel(L.OP_BACKDIFF, "mzi", 1024, 64, 2048, flags=L.GENERIC_ADJOINT) == "general"
+    assert L.adjoint_kernel(L.OP_BACKDIFF, "mzi", 1024, 64, 2048, fft=True) == "general"
+    assert L.adjoint_kernel(L.OP_BACKDIFF, "orth", 1024, 64, 2048) == "general"
+    a = L.workspace_size(L.OP_BACKDIFF, "mzi", 1024, 64, 2048, False, (16, 1, 8, 0))
+    b = L.workspace_size(L.OP_BACKDIFF, "mzi", 1024, 64, 2048, False, (16, 1, 4, 0))
+    assert a == b  # eight-wave plan sized for the four-wave general plan the flag may select
