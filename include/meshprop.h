@@ -184,10 +184,13 @@ enum meshprop_flags {
      * T^H = |det T| T^-1, and fwd+grad accepts the generic crossing.  The prologue checks the
      * assertion in FP64 per crossing; where it fails, dp and dJ/du_in come out NaN. */
     MESHPROP_SCALED_UNITARY = 128,
-    /* backdiff / fwd+grad of unitary crossings on neighbour (NN) geometries: run the general
-     * adjoint kernel instead of the one specialised for such meshes (their results are
-     * bitwise equal; for comparison and timing). */
-    MESHPROP_GENERIC_ADJOINT = 256
+    /* Neighbour (NN) geometries: run the general propagation kernels instead of the ones
+     * specialised for such meshes -- the forward of fwdprop and fwd+grad, and the unitary
+     * adjoint of backdiff and fwd+grad (bitwise equal results when both run the same
+     * workgroup shape; for comparison and timing).  MESHPROP_GENERIC_ADJOINT is the earlier
+     * name of the same flag. */
+    MESHPROP_GENERIC_KERNELS = 256,
+    MESHPROP_GENERIC_ADJOINT = MESHPROP_GENERIC_KERNELS
 };
 
 /*
@@ -302,6 +305,12 @@ int meshprop_resolve_tile(int op, int X, int N, int B, meshprop_tile *out);
  * unitary crossings on NN geometries, 0 = the general one (also for ops without an adjoint),
  * -status for arguments meshprop_run would reject.  fft as in meshprop_run, flags its flags. */
 int meshprop_adjoint_kernel(int op, int X, int N, int L, int B, int fft, const meshprop_tile *tile, int flags);
+
+/* Which forward kernel a fwdprop / fwd+grad call with these arguments runs (no reference
+ * counterpart): 1 = the one specialised for NN geometries with rows staged through LDS, 0 = the
+ * general one (also for ops without this forward), -status for arguments meshprop_run would
+ * reject. */
+int meshprop_forward_kernel(int op, int X, int N, int L, int B, int fft, const meshprop_tile *tile, int flags);
 
 /* Last HIP error code seen by a failing call on this thread (hipError_t as int).  A HIP error
  * left pending on the thread by earlier, unrelated work is cleared when a call starts launching,

@@ -33,7 +33,7 @@ EXPORTS = (
     "meshprop_group_create", "meshprop_group_destroy", "meshprop_group_workspace_size",
     "meshprop_group_tables_size", "meshprop_group_upload", "meshprop_group_launch", "meshprop_group_signature",
     "meshprop_group_entry_info", "meshprop_phase_screen", "meshprop_phase_screen_workspace_size",
-    "meshprop_hvp", "meshprop_hvp_workspace_size", "meshprop_run_noalloc", "meshprop_adjoint_kernel",
+    "meshprop_hvp", "meshprop_hvp_workspace_size", "meshprop_run_noalloc", "meshprop_adjoint_kernel", "meshprop_forward_kernel",
 )
 
 # Bytes of the static device scratch behind meshprop_run_noalloc (MESHPROP_NOALLOC_SCRATCH_BYTES).
@@ -89,6 +89,8 @@ def load():
     lib.meshprop_resolve_tile.restype = c_int
     lib.meshprop_adjoint_kernel.argtypes = [c_int, c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(Tile), c_int]
     lib.meshprop_adjoint_kernel.restype = c_int
+    lib.meshprop_forward_kernel.argtypes = lib.meshprop_adjoint_kernel.argtypes
+    lib.meshprop_forward_kernel.restype = c_int
     lib.meshprop_last_hip_error.restype = c_int
     lib.meshprop_status_string.restype = ctypes.c_char_p
     lib.meshprop_status_string.argtypes = [c_int]
@@ -178,6 +180,17 @@ def resolve_tile(op, X, N, B, tile=None):
     return (t.w, t.v, t.waves, t.chunk)
 
 
+def forward_kernel(op, X, N, L, B, fft=False, tile=None, flags=0) -> str:
+    """Which forward kernel a fwdprop / fwd+grad call runs: "nn" (NN geometry, rows staged
+    through LDS) or "general"."""
+    lib = load()
+    rc = lib.meshprop_forward_kernel(op, X_CODE[X] if isinstance(X, str) else X, N, L, B, int(fft),
+                                     tile_struct(tile), int(flags))
+    if rc < 0:
+        check(-rc, "forward_kernel")
+    return "nn" if rc == 1 else "general"
+
+
 def adjoint_kernel(op, X, N, L, B, fft=False, tile=None, flags=0) -> str:
     """Which adjoint kernel a backdiff / fwd+grad call runs: "nn" (specialised for unitary
     crossings on NN geometries) or "general"."""
@@ -192,7 +205,7 @@ def adjoint_kernel(op, X, N, L, B, fft=False, tile=None, flags=0) -> str:
 ONLY_PROLOGUE, SKIP_PROLOGUE, SKIP_REDUCE, ONLY_REDUCE, DP_ASSIGN = 1, 2, 4, 8, 16
 ONLY_FORWARD, SKIP_FORWARD = 32, 64
 SCALED_UNITARY = 128
-GENERIC_ADJOINT = 256
+GENERIC_KERNELS = GENERIC_ADJOINT = 256  # general kernels instead of the NN-specialised ones
 
 
 def run(op, X, N, L, B, fft, g0, g1, p, dp, ldp, s, lds, u_a, du_a, u_b, du_b, ldu, target=None, ckpt=None,

@@ -481,6 +481,136 @@ __global__ void __launch_bounds__(256) k_fwd(int N, int l0, int nl, int B, int L
     fwd_tile<T, W, V, const int*>(N, l0, nl, B, LPV, NP, kind, rows, u_in, u_out, ldu, blockIdx.x, ck);
 }
 
+// ---- neighbour-mesh forward ------------------------------------------------------------
+// NN geometries (layer kinds 0 and 1) whose vectors fill one wave (LPV = 64): one layer per
+// loop iteration with the field tile updated in place (one body per shift parity).  Same
+// arithmetic per waveguide as fwd_tile (bitwise equal results).
+template <class T, int W, int V, bool ODD>
+__device__ __forceinline__ void fwd_layer_nn(T (&u)[V][W], const typename RowOf<T>::type (&r)[W]) {
+    if constexpr (!ODD) {
+#pragma unroll
+        for (int k = 0; k < W; k += 2) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                const T a = u[v][k], b = u[v][k + 1];
+                u[v][k] = rapply(r[k], a, b);
+                u[v][k + 1] = rapply(r[k + 1], b, a);
+            }
+        }
+    } else {
+        T hi[V], lo[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            hi[v] = lane_next(u[v][0]);
+            lo[v] = lane_prev(u[v][W - 1]);
+        }
+#pragma unroll
+        for (int k = 1; k + 1 < W; k += 2) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                const T a = u[v][k], b = u[v][k + 1];
+                u[v][k] = rapply(r[k], a, b);
+                u[v][k + 1] = rapply(r[k + 1], b, a);
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            u[v][W - 1] = rapply(r[W - 1], u[v][W - 1], hi[v]);
+            u[v][0] = rapply(r[0], u[v][0], lo[v]);
+        }
+    }
+}
+
+// Rows staged through LDS: the workgroup (four waves) copies each layer's row table into a
+// ring of three LDS slots once, and every wave reads its rows from there.  A row feeds 8 FMAs
+// per vector, so with rows read from L2 by every wave the texture address unit, at 64 B per
+// clock and CU, bounds a one-vector tile near half the FMA rate (counters at N=1024 L=1024
+// B=2048: TA busy 92 % of the kernel, VALU issue 44 %); through LDS each layer's rows pass
+// the TA once per workgroup.  Layer l: read its rows from slot l % 3, request the rows of
+// layer l + 2 into registers, apply the layer, write the rows of layer l + 1 (requested one
+// layer earlier) to their slot, and one barrier publishes them.  The slot written at layer l
+// was last read at layer l - 2, before the barrier of layer l - 1.
+template <class T, int W, int V>
+__device__ __forceinline__ void fwd_tile_nn(const int N, const int l0, const int nl, const int B, ConstInt* kind,
+                                            const typename RowOf<T>::type* __restrict__ rows, const T* u_in,
+                                            T* u_out, const int ldu, const long tile, const Ckpt ck,
+                                            typename RowOf<T>::type* __restrict__ lrows) {
+    typedef typename RowOf<T>::type R;
+    constexpr int LPV = 64, NP = 64 * W;
+    constexpr int CP = NP / 256;  // entries of a layer each thread copies (workgroups of 256 threads)
+    static_assert(NP % 256 == 0, "whole layers per workgroup copy");
+    const Lanes q = lanes_of<V>(LPV, tile);
+    const int tid = threadIdx.x;
+    T u[V][W];
+    load_tile(u, u_in, q, N, B, ldu);
+    if (nl > 0) {
+        const __amdgpu_buffer_rsrc_t rrs = table_rsrc(rows + (size_t)l0 * NP);
+        const unsigned coff = (unsigned)tid * (unsigned)sizeof(R);
+        R cp[2][CP];  // rows of the next two layers on their way to LDS
+        auto fetch = [&](R (&c)[CP], const int l) {
+            const int lc = l < nl ? l : nl - 1;  // past the end: the last layer again (in bounds, unused)
+#pragma unroll
+            for (int e = 0; e < CP; ++e) c[e] = buf_load<R>(rrs, coff, (lc * NP + e * 256) * (int)sizeof(R));
+        };
+        auto publish = [&](const R (&c)[CP], const int l) {
+            R* dst = lrows + (l % 3) * NP + tid;
+#pragma unroll
+            for (int e = 0; e < CP; ++e) dst[e * 256] = c[e];
+        };
+        // LDS writes of this thread done, then the workgroup barrier (row requests in flight
+        // stay in flight: no wait on the vector memory counter).
+        auto lds_barrier = [] { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); };
+        fetch(cp[0], 0);
+        fetch(cp[1], 1);
+        publish(cp[0], 0);
+        lds_barrier();
+        T* const ckb = reinterpret_cast<T*>(ck.base);
+        int ck_l = 0x7fffffff, ck_c = 0;
+        if (ckb) {
+            ck_c = (l0 + ck.every - 1) / ck.every;
+            ck_l = ck_c * ck.every - l0;
+        }
+        int kd = kind[l0];
+        // Two layers per iteration: the register sets of the requested rows alternate.
+        auto layer = [&](const int l, R (&cnext)[CP], R (&cafter)[CP]) {
+            const int kn = kind[l0 + (l + 1 < nl ? l + 1 : l)];
+            const R* cur = lrows + (l % 3) * NP + q.t;
+            R r[W];
+#pragma unroll
+            for (int k = 0; k < W; ++k) r[k] = cur[k * LPV];
+            fetch(cafter, l + 2);
+            if (l == ck_l) {
+                store_tile(u, ckb + (size_t)ck_c * ck.stride, q, N, B, ldu);
+                ++ck_c;
+                ck_l += ck.every;
+            }
+            if (kd & 1)
+                fwd_layer_nn<T, W, V, true>(u, r);
+            else
+                fwd_layer_nn<T, W, V, false>(u, r);
+            publish(cnext, l + 1);
+            lds_barrier();
+            kd = kn;
+        };
+        int l = 0;
+        for (; l + 1 < nl; l += 2) {
+            layer(l, cp[1], cp[0]);
+            layer(l + 1, cp[0], cp[1]);
+        }
+        if (l < nl) layer(l, cp[1], cp[0]);
+    }
+    store_tile(u, u_out, q, N, B, ldu);
+}
+
+template <class T, int W, int V>
+__global__ void __launch_bounds__(256) k_fwd_nn(int N, int l0, int nl, int B, const int* __restrict__ kind,
+                                                const typename RowOf<T>::type* __restrict__ rows, const T* u_in,
+                                                T* u_out, int ldu, Ckpt ck) {
+    typedef typename RowOf<T>::type R;
+    __shared__ R lrows[3 * 64 * W];
+    fwd_tile_nn<T, W, V>(N, l0, nl, B, (ConstInt*)kind, rows, u_in, u_out, ldu, blockIdx.x, ck, lrows);
+}
+
 // ---- fwddiff --------------------------------------------------------------------------
 template <class T, int W, int V>
 __global__ void __launch_bounds__(256) k_fwddiff(int N, int L, int B, int LPV, int NP, const int* __restrict__ kind,
@@ -1611,12 +1741,13 @@ struct Plan {
     int np, ns, real;
     long ntiles;
     int grid;         // workgroups of the propagation kernel
+    int ncu = 256;    // compute units of the device the plan was made for
     size_t rowbytes;  // sizeof(row) = 16 (complex) or 8 (real)
     size_t off_kind, off_a, off_b, off_dtdp, off_partial, total;  // workspace layout
     int uni;          // crossings are unitary (T^-1 = T^H) or, with scaled, scaled unitaries
     int scaled;       // generic crossings asserted to be c U: adjoint rows = |det T| x inverse rows
     const float* scales = nullptr;  // scaled: per-waveguide |det T| table (set by the caller of a launch)
-    int nn = 1;       // the NN-specialised adjoint may serve this plan (off: MESHPROP_GENERIC_ADJOINT)
+    int nn = 1;       // the NN-specialised kernels may serve this plan (off: MESHPROP_GENERIC_KERNELS)
     int sorder = 0;   // > 0: K-form table and dp partials in the NN adjoint's stage order (W/2 slots per lane)
 };
 
@@ -1624,6 +1755,18 @@ struct Plan {
 // stage fill the 64 KiB (four waves) or 128 KiB (eight waves) budget, C = 16 / W (one layer at
 // W = 16).
 __host__ __device__ constexpr int nn_stage_depth(int W) { return 16 / W; }
+
+// Whether the NN-specialised forward (k_fwd_nn, rows staged through LDS) serves a plan: NN
+// geometry, one lane group per wave, four-wave workgroups, the widest tile (W = 16) and at least
+// two waves per SIMD (ncu: the device's CUs).  Measured fwdprop, Clements MZI, L = N (general
+// forward -> staged): N=1024 B=2048 1.048 -> 0.700 ms, B=4096 (two vectors per lane) 1.979 ->
+// 0.897 ms, N=768 B=2048 0.754 -> 0.635 ms; with one wave per SIMD (N=1024 B=1024) 0.523 ->
+// 0.607 ms, and at W = 8 (N=512 B=4096) 0.254 -> 0.277 ms, W = 4 (N=256 B=4096) 0.071 -> 0.087
+// ms: there a row feeds two vectors, the address unit is not the limit and the per-layer
+// barrier costs more than it saves.
+inline bool nn_forward(const Plan& P, const int ncu) {
+    return P.nn && !P.fft && P.LPV == 64 && P.W == 16 && P.waves == 4 && P.ntiles * P.waves >= 8L * ncu;
+}
 
 // Whether the NN-specialised adjoint (k_bwd_nn) serves a plan: pipelined unitary (or
 // scaled-unitary) adjoint of complex fields on an NN geometry, one lane group per wave, four- or

@@ -9,8 +9,15 @@ template <class T, int W, int V>
 static void go(const Plan& P, const int* kind, const void* rows, int l0, int nl, const void* u_in, void* u_out,
                int ldu, hipStream_t st, Ckpt ck) {
     typedef typename RowOf<T>::type R;
-    hipLaunchKernelGGL((k_fwd<T, W, V>), dim3(P.grid), dim3(64 * P.waves), 0, st, P.N, l0, nl, P.B, P.LPV, P.NP,
-                       kind, (const R*)rows, (const T*)u_in, (T*)u_out, ldu, ck);
+    if constexpr (W == 16) {
+        if (nn_forward(P, P.ncu)) {
+            hipLaunchKernelGGL((k_fwd_nn<T, W, V>), dim3(P.grid), dim3(64 * P.waves), 0, st, P.N, l0, nl, P.B, kind,
+                               (const R*)rows, (const T*)u_in, (T*)u_out, ldu, ck);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_fwd<T, W, V>), dim3(P.grid), dim3(64 * P.waves), 0, st, P.N, l0, nl, P.B, P.LPV, P.NP, kind,
+                       (const R*)rows, (const T*)u_in, (T*)u_out, ldu, ck);
 }
 
 #if MP_ILP_SET

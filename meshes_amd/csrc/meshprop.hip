@@ -168,7 +168,7 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
     P.scaled = adj && X == X_GENERIC && (flags & MESHPROP_SCALED_UNITARY) != 0;
     P.scales = nullptr;
     P.sorder = 0;
-    P.nn = (flags & MESHPROP_GENERIC_ADJOINT) ? 0 : 1;
+    P.nn = (flags & MESHPROP_GENERIC_KERNELS) ? 0 : 1;
     // The fused forward + adjoint runs the unitary adjoint: unitary or scaled-unitary crossings.
     if (op == MESHPROP_OP_FWDGRAD && X == X_GENERIC && !P.scaled) return MESHPROP_ERR_UNSUPPORTED;
     if (N <= 0 || (N & 1) || L < 0 || B < 0) return MESHPROP_ERR_ARG;
@@ -184,6 +184,7 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
     P.real = (X == X_ORTH);
     P.uni = (X != X_GENERIC) || P.scaled;
     const int ncu = num_cus();
+    P.ncu = ncu;
     // Without an explicit tile, neighbour meshes take the measured table's (a requested stage
     // depth is kept); butterfly meshes and untabulated cases take the heuristic.
     meshprop_tile tabled;
@@ -393,7 +394,7 @@ int run_impl(int op, int X, int N, int L, int B, int fft, const int* g0, const i
             // launches: on MI355X they ran faster than one kernel doing both per tile.
             Plan PF;
             const meshprop_tile tf = {P.W, 0, 0, 0};
-            rc = make_plan(MESHPROP_OP_FWDPROP, X, N, L, B, fft, &tf, PF);
+            rc = make_plan(MESHPROP_OP_FWDPROP, X, N, L, B, fft, &tf, PF, flags & MESHPROP_GENERIC_KERNELS);
             if (!rc && (PF.W != P.W || PF.NP != P.NP)) rc = MESHPROP_ERR_UNSUPPORTED;
             if (!rc && !(flags & MESHPROP_SKIP_FORWARD)) {
                 rc = launch_fwd_any(PF, kind, tab_a, 0, L, u_a, u_b, ldu, st);
@@ -520,10 +521,10 @@ size_t meshprop_workspace_size(int op, int X, int N, int L, int B, int fft, cons
     Plan P;
     size_t total = 0;
     // Enough for every plan the flags of a call can select: the scaled-unitary plan of generic
-    // crossings (MESHPROP_SCALED_UNITARY) and the general adjoint (MESHPROP_GENERIC_ADJOINT, whose
+    // crossings (MESHPROP_SCALED_UNITARY) and the general kernels (MESHPROP_GENERIC_KERNELS, whose
     // workgroups can be smaller, hence more dp partials).
-    for (const int fl : {0, (int)MESHPROP_GENERIC_ADJOINT, (int)MESHPROP_SCALED_UNITARY,
-                         (int)(MESHPROP_SCALED_UNITARY | MESHPROP_GENERIC_ADJOINT)}) {
+    for (const int fl : {0, (int)MESHPROP_GENERIC_KERNELS, (int)MESHPROP_SCALED_UNITARY,
+                         (int)(MESHPROP_SCALED_UNITARY | MESHPROP_GENERIC_KERNELS)}) {
         if ((fl & MESHPROP_SCALED_UNITARY) && X != X_GENERIC) continue;
         if (make_plan(op, X, N, L, B, fft, tile, P, fl) == MESHPROP_OK) total = std::max(total, P.total);
     }
@@ -547,6 +548,22 @@ int meshprop_adjoint_kernel(int op, int X, int N, int L, int B, int fft, const m
     if (rc) return -rc;
     if (op != MESHPROP_OP_BACKDIFF && op != MESHPROP_OP_FWDGRAD) return 0;
     return nn_adjoint(P) ? 1 : 0;
+}
+
+int meshprop_forward_kernel(int op, int X, int N, int L, int B, int fft, const meshprop_tile* tile, int flags) {
+    Plan P;
+    int rc = make_plan(op, X, N, L, B, fft, tile, P, flags);
+    if (rc) return -rc;
+    if (op == MESHPROP_OP_FWDGRAD) {
+        // the forward of fwd+grad runs at the adjoint's width (as run_impl plans it)
+        const meshprop_tile tf = {P.W, 0, 0, 0};
+        Plan PF;
+        rc = make_plan(MESHPROP_OP_FWDPROP, X, N, L, B, fft, &tf, PF, flags & MESHPROP_GENERIC_KERNELS);
+        if (rc) return -rc;
+        return nn_forward(PF, PF.ncu) ? 1 : 0;
+    }
+    if (op != MESHPROP_OP_FWDPROP) return 0;
+    return nn_forward(P, P.ncu) ? 1 : 0;
 }
 
 int meshprop_run(int op, int X, int N, int L, int B, int fft, const int* geom0, const int* geom1, const float* p,

@@ -28,15 +28,15 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--tile", type=str, default="")
-    ap.add_argument("--generic-adjoint", action="store_true",
-                    help="run the general adjoint kernel instead of the NN-specialised one")
+    ap.add_argument("--generic-kernels", action="store_true",
+                    help="run the general forward and adjoint kernels instead of the NN-specialised ones")
     args = ap.parse_args(argv)
     L = args.L or args.N
     tile = tuple(int(t) for t in args.tile.split(",")) if args.tile else None
     op = OPS[args.op]
     case = Case(op, args.X, args.N, L, args.B)
-    if args.generic_adjoint:
-        case.extra_flags = Lb.GENERIC_ADJOINT
+    if args.generic_kernels:
+        case.extra_flags = Lb.GENERIC_KERNELS
     nb = Lb.workspace_size(op, args.X, args.N, L, args.B, False, tile)
     ws = torch.empty(max(nb, 256), dtype=torch.uint8, device="cuda")
     for _ in range(args.warmup):
@@ -52,7 +52,7 @@ def main(argv=None):
     out = {"op": args.op, "X": args.X, "N": args.N, "L": L, "B": args.B,
            "tile": list(Lb.resolve_tile(op, args.X, args.N, args.B, tile)) if not (args.X == "mzi_generic" and
                                                                                   op == Lb.OP_FWDGRAD) else tile,
-           "generic_adjoint": bool(args.generic_adjoint), "ms_per_call": ms, "updates_per_s": args.B * case.n_active / (ms * 1e-3),
+           "generic_kernels": bool(args.generic_kernels), "ms_per_call": ms, "updates_per_s": args.B * case.n_active / (ms * 1e-3),
            "device": torch.cuda.get_device_name(0)}
     print(json.dumps(out), flush=True)
 

@@ -100,7 +100,7 @@ class Case:
         self.out, self.dout = torch.empty_like(self.u), torch.empty_like(self.u)
         self.dp = torch.zeros((L, N // 2, npar), dtype=torch.float32, device=dev)
         self.flags = Lb.SCALED_UNITARY if (X == "mzi_generic" and op == Lb.OP_FWDGRAD) else 0
-        self.extra_flags = 0  # e.g. GENERIC_ADJOINT, to time the general adjoint kernel
+        self.extra_flags = 0  # e.g. GENERIC_KERNELS, to time the general kernels
         self.dt = dt
 
     def run(self, tile, ws, nb):
@@ -108,10 +108,10 @@ class Case:
         g = (self.lens, self.shifts)
         if op == Lb.OP_FWDPROP:
             Lb.run(op, X, N, L, B, False, *g, self.p, None, self.ldp, self.s, self.lds, self.u, None, self.out, None,
-                   N, tile=tile, workspace=ws, ws_bytes=nb)
+                   N, tile=tile, workspace=ws, ws_bytes=nb, flags=self.extra_flags)
         elif op == Lb.OP_FWDDIFF:
             Lb.run(op, X, N, L, B, False, *g, self.p, self.dp, self.ldp, self.s, self.lds, self.u, self.du, self.out,
-                   self.dout, N, tile=tile, workspace=ws, ws_bytes=nb)
+                   self.dout, N, tile=tile, workspace=ws, ws_bytes=nb, flags=self.extra_flags)
         elif op == Lb.OP_BACKDIFF:
             Lb.run(op, X, N, L, B, False, *g, self.p, self.dp, self.ldp, self.s, self.lds, self.u, None, self.out,
                    None, N, target=self.y, tile=tile, workspace=ws, ws_bytes=nb, flags=self.extra_flags)

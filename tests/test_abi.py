@@ -233,3 +233,16 @@ def test_w16_adjoint_plans_without_gpu(lib):
     a = L.workspace_size(L.OP_BACKDIFF, "mzi", 1024, 64, 2048, False, (16, 1, 8, 0))
     b = L.workspace_size(L.OP_BACKDIFF, "mzi", 1024, 64, 2048, False, (16, 1, 4, 0))
     assert a == b  # eight-wave plan sized for the four-wave general plan the flag may select
+
+
+def test_forward_kernel_choice_without_gpu(lib):
+    """The LDS-staged forward serves wide NN meshes with at least two waves per SIMD (this
+    host-side plan assumes the 256 CUs of an MI355X when no device is present)."""
+    import meshes_amd._lib as L
+    assert L.forward_kernel(L.OP_FWDPROP, "mzi", 1024, 64, 2048) == "nn"
+    assert L.forward_kernel(L.OP_FWDGRAD, "mzi", 1024, 64, 2048) == "nn"
+    assert L.forward_kernel(L.OP_FWDPROP, "mzi", 1024, 64, 1024) == "general"  # one wave per SIMD
+    assert L.forward_kernel(L.OP_FWDPROP, "mzi", 512, 64, 4096) == "general"
+    assert L.forward_kernel(L.OP_FWDPROP, "mzi", 1024, 64, 2048, fft=True) == "general"
+    assert L.forward_kernel(L.OP_FWDPROP, "mzi", 1024, 64, 2048, flags=L.GENERIC_KERNELS) == "general"
+    assert L.forward_kernel(L.OP_BACKDIFF, "mzi", 1024, 64, 2048) == "general"

@@ -37,7 +37,7 @@ CASES = [
 WIDE = [
     ("mzi", 1024, 37, 100, None), ("mzi", 1024, 20, 64, (16, 1, 4)), ("mzi", 1000, 13, 70, (16, 1, 8)),
     ("mzi", 600, 9, 33, (16, 1, 4)), ("sym", 1024, 11, 64, (16, 1, 8)), ("sym", 770, 1, 9, (16, 1, 4)),
-    ("mzi", 1024, 1024, 40, (16, 1, 8)),
+    ("mzi", 1024, 1024, 40, (16, 1, 8)), ("mzi", 1024, 16, 2048, None),
 ]
 
 
@@ -195,3 +195,45 @@ def test_nn_adjoint_not_used_elsewhere(gpu):
     assert Lb.adjoint_kernel(Lb.OP_BACKDIFF, "mzi", 1024, 8, 64, fft=True) == "general"
     assert Lb.adjoint_kernel(Lb.OP_BACKDIFF, "orth", 1024, 8, 64) == "general"
     assert Lb.adjoint_kernel(Lb.OP_FWDPROP, "mzi", 256, 8, 64) == "general"
+
+
+# Staged forward (W = 16, at least two waves per SIMD): odd and even depths, one and two
+# vectors per lane, padded widths, real fields, lossy crossings with checkpoints.
+FWD = [("mzi", 1024, 13, 2048, None), ("mzi", 1024, 8, 4096, (16, 2, 4)), ("mzi", 1000, 7, 2100, (16, 1, 4)),
+       ("sym", 600, 9, 2048, (16, 1, 4)), ("orth", 1024, 6, 2048, (16, 1, 4)),
+       ("mzi_generic", 800, 10, 2048, (16, 1, 4)),
+       ("mzi", 1024, 1, 2048, None), ("mzi", 1024, 2, 2048, None)]
+
+
+@pytest.mark.parametrize("case", FWD, ids=[_id(c) for c in FWD])
+def test_nn_forward(gpu, case):
+    """fwdprop of wide NN meshes runs k_fwd_nn (rows staged through LDS): bitwise equal to the
+    general forward and within 1e-5 of the oracle; with checkpoints the stored fields are
+    bitwise equal too."""
+    import meshes_amd._lib as Lb
+    X, N, L, B, tile = case
+    t4 = None if tile is None else tuple(tile) + (0,)
+    assert Lb.forward_kernel(Lb.OP_FWDPROP, X, N, L, B, tile=t4) == "nn"
+    assert Lb.forward_kernel(Lb.OP_FWDPROP, X, N, L, B, tile=t4, flags=Lb.GENERIC_KERNELS) == "general"
+    g, p, s, u, rng = _setup(X, "nn", N, L, B, seed=11 * N + L, lossy=X == "mzi_generic")
+    mesh = _mesh(g, X, p, s)
+    tile = None if tile is None else tuple(tile) + (0,)
+    ub = torch.as_tensor(u).cuda().contiguous()
+    K = 4
+    res = {}
+    for flags in (0, Lb.GENERIC_KERNELS):
+        out = torch.empty_like(ub)
+        ck = torch.empty(((L + K - 1) // K, B, N), dtype=ub.dtype, device=ub.device)
+        ldp, lds = mesh._ld()
+        g0, g1 = mesh._geom()
+        sp = mesh.p_splitter if mesh.p_splitter.numel() else None
+        nb = Lb.workspace_size(Lb.OP_FWDPROP, X, N, L, B, False, tile)
+        ws = torch.empty(max(nb, 1), dtype=torch.uint8, device=ub.device)
+        Lb.run(Lb.OP_FWDPROP, X, N, L, B, False, g0, g1, mesh.p_phase, None, ldp, sp, lds, ub, None, out, None, N,
+               ckpt=ck, ckpt_every=K, tile=tile, workspace=ws, ws_bytes=nb, flags=flags)
+        res[flags] = (out, ck)
+    torch.cuda.synchronize()
+    assert torch.equal(res[0][0], res[Lb.GENERIC_KERNELS][0])
+    assert torch.equal(res[0][1], res[Lb.GENERIC_KERNELS][1])
+    ref = O.mesh_fwd(g, X, p.astype(np.float64), _f64(s), u.astype(np.complex128 if X != "orth" else np.float64))
+    assert O.rel_l2(_np(res[0][0]), ref) < TOL
