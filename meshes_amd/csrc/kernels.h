@@ -1504,10 +1504,14 @@ __device__ __forceinline__ void bwd_tile_nn(const int N, const int l0, const int
     };
     // Conversion item of the previous stage (k-th of this thread): the NW waves' entries of one
     // slot, its K-form coefficients and the partial it lands in.
+    // Eight-wave workgroups of the widest tile read an item's stage entries after the layer
+    // instead of before it: eight entries held across the layer spill its registers.
+    constexpr bool LATE = LEAN;
     struct Item {
-        R a[NW];
+        R a[LATE ? 1 : NW];
         KT dd[np];
         float* out;
+        const R* s0;
         bool on;
     } it;
     int pc0 = 0;
@@ -1520,8 +1524,12 @@ __device__ __forceinline__ void bwd_tile_nn(const int N, const int l0, const int
         const int m = (wbase % NPh) + lane;
         const int la = l0 + last - (pc0 + lcc);
         const R* s0 = pbuf + lcc * NPh + m;
+        if constexpr (LATE) {
+            it.s0 = s0;
+        } else {
 #pragma unroll
-        for (int wv = 0; wv < NW; ++wv) it.a[wv] = s0[wv * WSTRIDE];
+            for (int wv = 0; wv < NW; ++wv) it.a[wv] = s0[wv * WSTRIDE];
+        }
         const KT* src = reinterpret_cast<const KT*>(dtdp) + (size_t)la * np * NPh + m;
 #pragma unroll
         for (int e = 0; e < np; ++e) it.dd[e] = src[e * NPh];
@@ -1529,9 +1537,17 @@ __device__ __forceinline__ void bwd_tile_nn(const int N, const int l0, const int
     };
     auto finish = [&]() {
         if (!it.on) return;
-        R e = it.a[0];
+        R a[NW];
 #pragma unroll
-        for (int wv = 1; wv < NW; ++wv) e = row_add(e, it.a[wv]);
+        for (int wv = 0; wv < NW; ++wv) {
+            if constexpr (LATE)
+                a[wv] = it.s0[wv * WSTRIDE];
+            else
+                a[wv] = it.a[wv];
+        }
+        R e = a[0];
+#pragma unroll
+        for (int wv = 1; wv < NW; ++wv) e = row_add(e, a[wv]);
         float val[np];
 #pragma unroll
         for (int k = 0; k < np; ++k) val[k] = kdot(e, it.dd[k]);

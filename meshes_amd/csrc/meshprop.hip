@@ -194,6 +194,11 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
         if (const TileEntry* e = table_tile(op, P.scaled ? X_MZI : X, N, B)) {
             if ((long)e->w * 64 >= N && tile_ok(op, e->w, e->v)) {
                 tabled = meshprop_tile{e->w, e->v, e->waves, tile ? tile->chunk : 0};
+                // The table's W = 16 unitary adjoint entries were measured on the general kernel
+                // (four-wave workgroups at most); the NN-specialised one takes eight-wave
+                // workgroups whenever the batch gives every CU one (N=1024 L=1024 B=2048:
+                // backdiff 3.21 ms against 3.41 ms with four waves, fwd+grad 3.91 against 4.14).
+                if (adj && P.nn && P.uni && e->w == 16) tabled.waves = 0;
                 tile = &tabled;
             }
         }

@@ -195,7 +195,12 @@ def test_tile_table_consulted_without_gpu(lib):
             B = 2 * N + 1 if fat else 2 * N
             got = L.resolve_tile(op, names[X], N, B)
             cap = 4 if op == L.OP_FWDPROP or op == L.OP_FWDDIFF else 8  # the plan clamps waves
-            assert got[:2] == (W, V) and got[2] == min(waves, cap), (op, X, fat, n0, N, got)
+            assert got[:2] == (W, V), (op, X, fat, n0, N, got)
+            if W == 16 and X in (0, 1) and op in (L.OP_BACKDIFF, L.OP_FWDGRAD):
+                # the NN-specialised W = 16 adjoint picks its workgroup size by batch
+                assert got[2] == (8 if B >= 8 * 256 else 4), (op, X, fat, n0, N, got)
+            else:
+                assert got[2] == min(waves, cap), (op, X, fat, n0, N, got)
             checked += 1
     assert checked > 100
     # an explicit tile wins over the table
