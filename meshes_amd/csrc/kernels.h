@@ -1471,6 +1471,7 @@ __device__ __forceinline__ void bwd_tile_nn(const int N, const int l0, const int
     static_assert(C > 1 || nn_stage_order(W), "one-layer stages are compiled for the stage order only");
     // Widest tile: the rows are loaded right before their layer (one row set live).
     constexpr bool LEAN = W >= 16;
+    static_assert(!(LEAN && NW == 8) || C == 1, "staged rows: one layer per stage");
     const int lane = threadIdx.x & 63;
     const int wave_u = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int last = nl - 1;
@@ -1502,6 +1503,33 @@ __device__ __forceinline__ void bwd_tile_nn(const int N, const int l0, const int
             for (int k = 0; k < W; ++k) f[k] = buf_load<float>(srs, soff, (lr * NP + k * LPV) * (int)sizeof(float));
         }
     };
+    // Eight-wave workgroups of the widest tile stage the rows through LDS as the forward does
+    // (k_fwd_nn): every wave read a layer's 16 KiB of rows through the address unit, right
+    // before the layer (no register room for a prefetch).  Two slots behind the stage buffers:
+    // the rows of local layer s + 1 are requested during layer s and written to slot (s+1) & 1,
+    // which was last read during layer s - 1, before the stage barrier of that layer; the stage
+    // barrier of layer s publishes them.
+    constexpr bool RS = LEAN && NW == 8 && !SC;
+    constexpr int CP = RS ? NP / NTHR : 1;  // row entries each thread copies per layer
+    R* const lrow = stage + 2 * BSTRIDE;    // [2][NP] (RS only)
+    R cp[CP];
+    auto fetch_rows = [&](const int s) {
+        const int lr = s < last ? last - s : 0;
+#pragma unroll
+        for (int e = 0; e < CP; ++e)
+            cp[e] = buf_load<R>(rrs, (unsigned)threadIdx.x * (unsigned)sizeof(R), (lr * NP + e * NTHR) * (int)sizeof(R));
+    };
+    auto publish_rows = [&](const int s) {
+#pragma unroll
+        for (int e = 0; e < CP; ++e) lrow[(s & 1) * NP + threadIdx.x + e * NTHR] = cp[e];
+    };
+    if constexpr (RS) {
+        if (nl > 0) {
+            fetch_rows(0);
+            publish_rows(0);
+            __syncthreads();
+        }
+    }
     // Conversion item of the previous stage (k-th of this thread): the NW waves' entries of one
     // slot, its K-form coefficients and the partial it lands in.
     // Eight-wave workgroups of the widest tile read an item's stage entries after the layer
@@ -1582,10 +1610,16 @@ __device__ __forceinline__ void bwd_tile_nn(const int N, const int l0, const int
                 const int cur = (lc + PAR) & 1;  // register sets of this layer (a constant)
                 const int rs = LEAN ? 0 : cur;
                 if (lc < NI && have_prev) issue(lc, ITEMS);
-                if constexpr (LEAN)
+                if constexpr (RS) {
+                    const R* lr = lrow + ((c0 + lc) & 1) * NP + lane;
+#pragma unroll
+                    for (int k = 0; k < W; ++k) ri[0][k] = lr[k * LPV];
+                    fetch_rows(c0 + lc + 1);
+                } else if constexpr (LEAN) {
                     rows_at(c0 + lc, ri[0], sc[0]);
-                else
+                } else {
                     rows_at(c0 + lc + 1, ri[cur ^ 1], sc[cur ^ 1]);
+                }
                 const bool odd = ODD >= 0 ? ODD != 0 : (kd[lc] & 1) != 0;
                 if constexpr (C == 1)  // one-layer stages update the state in place
                     bwd_layer_nn<T, W, V, SC>(odd, ua, ga, ua, ga, ri[rs], wl, lane, sc[rs]);
@@ -1594,6 +1628,7 @@ __device__ __forceinline__ void bwd_tile_nn(const int N, const int l0, const int
                 else
                     bwd_layer_nn<T, W, V, SC>(odd, ub, gb, ua, ga, ri[rs], wl + lc * NPh, lane, sc[rs]);
                 if (lc < NI && have_prev) finish();
+                if constexpr (RS) publish_rows(c0 + lc + 1);
             }
         }
         if (have_prev) {
@@ -1766,6 +1801,13 @@ struct Plan {
     int nn = 1;       // the NN-specialised kernels may serve this plan (off: MESHPROP_GENERIC_KERNELS)
     int sorder = 0;   // > 0: K-form table and dp partials in the NN adjoint's stage order (W/2 slots per lane)
 };
+
+// Dynamic LDS of the NN-specialised adjoint: two stage buffers of NW waves x C layers x NP/2
+// K-form entries, and for eight-wave workgroups of the widest tile (W = 16, not scaled) two row
+// slots of NP rows.
+__host__ __device__ constexpr size_t nn_lds_bytes(int W, int NW, int C, bool SC, size_t rowbytes) {
+    return (size_t)2 * NW * C * (W * 32) * rowbytes + ((W >= 16 && NW == 8 && !SC) ? (size_t)2 * 64 * W * rowbytes : 0);
+}
 
 // Stage depth the NN-specialised adjoint is compiled for: two buffers of C layers of the K-form
 // stage fill the 64 KiB (four waves) or 128 KiB (eight waves) budget, C = 16 / W (one layer at

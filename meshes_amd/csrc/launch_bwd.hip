@@ -65,7 +65,7 @@ static int go_nn(const Plan& P, const int* kind, const void* ri, const void* dtd
     typedef typename RowOf<T>::type R;
     constexpr int C = nn_stage_depth(W);
     constexpr bool SC = MP_SC_SET != 0;
-    const size_t shmem = (size_t)2 * NW * C * (W * 32) * sizeof(R);
+    const size_t shmem = nn_lds_bytes(W, NW, C, SC, sizeof(R));
     static std::atomic<unsigned long long> lds_set{0};
     // W = 16: the entry compiled for two waves per SIMD.
     auto kern = [] {
