@@ -1411,6 +1411,12 @@ __device__ __forceinline__ void bwd_tile_pipe(const int N, const int l0, const i
 // one-layer stage spill and conflict (W = 2 has one slot per lane: both orders agree).
 __host__ __device__ constexpr bool nn_stage_order(int W) { return W >= 4; }
 
+// Whether the NN-specialised adjoint stages its rows through LDS: eight-wave workgroups, W >= 8;
+// the scaled-unitary variant keeps its per-layer loads.  Measured against per-wave row loads
+// (fwd+grad, Clements MZI, L = N): N=1024 B=2048 3.91 -> 3.54 ms, N=512 B=4096 1.487 -> 1.404 ms;
+// at W = 4 (N=256 B=4096, two vectors per lane) 0.2891 -> 0.2942 ms, so it stays off there.
+__host__ __device__ constexpr bool nn_rows_staged(int W, int NW, bool SC) { return W >= 8 && NW == 8 && !SC; }
+
 // K-form stage entries of one NN layer (lane t owns slots t*W/2 .. t*W/2 + W/2 - 1; an odd
 // layer pairs the lane's last waveguide with the next lane's first); entry p at wl[p * ES].
 template <class T, int W, int ES>
@@ -1503,29 +1509,39 @@ __device__ __forceinline__ void bwd_tile_nn(const int N, const int l0, const int
             for (int k = 0; k < W; ++k) f[k] = buf_load<float>(srs, soff, (lr * NP + k * LPV) * (int)sizeof(float));
         }
     };
-    // Eight-wave workgroups of the widest tile stage the rows through LDS as the forward does
-    // (k_fwd_nn): every wave read a layer's 16 KiB of rows through the address unit, right
-    // before the layer (no register room for a prefetch).  Two slots behind the stage buffers:
-    // the rows of local layer s + 1 are requested during layer s and written to slot (s+1) & 1,
-    // which was last read during layer s - 1, before the stage barrier of that layer; the stage
-    // barrier of layer s publishes them.
-    constexpr bool RS = LEAN && NW == 8 && !SC;
-    constexpr int CP = RS ? NP / NTHR : 1;  // row entries each thread copies per layer
-    R* const lrow = stage + 2 * BSTRIDE;    // [2][NP] (RS only)
-    R cp[CP];
-    auto fetch_rows = [&](const int s) {
+    // Eight-wave workgroups of the wide tiles stage the rows through LDS as the forward does
+    // (k_fwd_nn): every wave reading a layer's rows through the address unit (16 KiB per wave
+    // and layer at W = 16) bounds the kernel.  Two slots of C layers behind the stage buffers:
+    // during stage k each thread requests its share of the rows of stage k + 1 and writes them
+    // to slot (k+1) & 1 at the end of the stage (that slot was last read during stage k - 1,
+    // before its barrier); the stage barrier publishes them.
+    constexpr bool RS = nn_rows_staged(W, NW, SC);
+    // Row entries each thread copies per layer; with fewer entries than threads (W = 4 in eight
+    // waves) the first NP threads copy one each.
+    constexpr int CP = (RS && NP >= NTHR) ? NP / NTHR : 1;
+    static_assert(!RS || NP % NTHR == 0 || NP < NTHR, "row copy: whole entries per thread");
+    const bool copier = NP >= NTHR || (int)threadIdx.x < NP;
+    R* const lrow = stage + 2 * BSTRIDE;    // [2][C][NP] (RS only)
+    R cp[C][CP];
+    auto fetch_rows = [&](R (&c)[CP], const int s) {
+        if (!copier) return;
         const int lr = s < last ? last - s : 0;
 #pragma unroll
         for (int e = 0; e < CP; ++e)
-            cp[e] = buf_load<R>(rrs, (unsigned)threadIdx.x * (unsigned)sizeof(R), (lr * NP + e * NTHR) * (int)sizeof(R));
+            c[e] = buf_load<R>(rrs, (unsigned)threadIdx.x * (unsigned)sizeof(R), (lr * NP + e * NTHR) * (int)sizeof(R));
     };
-    auto publish_rows = [&](const int s) {
+    auto publish_rows = [&](const int k) {  // the fetched rows of stage k into its slot
+        if (!copier) return;
+        R* dst = lrow + (k & 1) * (C * NP) + threadIdx.x;
 #pragma unroll
-        for (int e = 0; e < CP; ++e) lrow[(s & 1) * NP + threadIdx.x + e * NTHR] = cp[e];
+        for (int lc = 0; lc < C; ++lc)
+#pragma unroll
+            for (int e = 0; e < CP; ++e) dst[lc * NP + e * NTHR] = cp[lc][e];
     };
     if constexpr (RS) {
         if (nl > 0) {
-            fetch_rows(0);
+#pragma unroll
+            for (int lc = 0; lc < C; ++lc) fetch_rows(cp[lc], lc);
             publish_rows(0);
             __syncthreads();
         }
@@ -1581,7 +1597,7 @@ __device__ __forceinline__ void bwd_tile_nn(const int N, const int l0, const int
         for (int k = 0; k < np; ++k) val[k] = kdot(e, it.dd[k]);
         put_dp<np>(it.out, val, first);
     };
-    if (!LEAN && nl > 0) rows_at(0, ri[0], sc[0]);
+    if (!LEAN && !RS && nl > 0) rows_at(0, ri[0], sc[0]);
     bool have_prev = false;
     // One stage of cl layers starting at local layer c0 (GUARD: cl < C, the last stage); PAR is
     // the parity of c0, which picks the register sets of its layers; ODD is the shift parity of
@@ -1611,10 +1627,10 @@ __device__ __forceinline__ void bwd_tile_nn(const int N, const int l0, const int
                 const int rs = LEAN ? 0 : cur;
                 if (lc < NI && have_prev) issue(lc, ITEMS);
                 if constexpr (RS) {
-                    const R* lr = lrow + ((c0 + lc) & 1) * NP + lane;
+                    const R* lr = lrow + (((c0 / C) & 1) * C + lc) * NP + lane;
 #pragma unroll
-                    for (int k = 0; k < W; ++k) ri[0][k] = lr[k * LPV];
-                    fetch_rows(c0 + lc + 1);
+                    for (int k = 0; k < W; ++k) ri[rs][k] = lr[k * LPV];
+                    fetch_rows(cp[lc], c0 + C + lc);
                 } else if constexpr (LEAN) {
                     rows_at(c0 + lc, ri[0], sc[0]);
                 } else {
@@ -1628,9 +1644,9 @@ __device__ __forceinline__ void bwd_tile_nn(const int N, const int l0, const int
                 else
                     bwd_layer_nn<T, W, V, SC>(odd, ub, gb, ua, ga, ri[rs], wl + lc * NPh, lane, sc[rs]);
                 if (lc < NI && have_prev) finish();
-                if constexpr (RS) publish_rows(c0 + lc + 1);
             }
         }
+        if constexpr (RS) publish_rows(c0 / C + 1);
         if (have_prev) {
             // Items of the previous stage that found no layer to hide under.
             const int done = GUARD ? (cl < NI ? cl : NI) : (C < NI ? C : NI);
@@ -1803,10 +1819,9 @@ struct Plan {
 };
 
 // Dynamic LDS of the NN-specialised adjoint: two stage buffers of NW waves x C layers x NP/2
-// K-form entries, and for eight-wave workgroups of the widest tile (W = 16, not scaled) two row
-// slots of NP rows.
+// K-form entries, and with staged rows two slots of C layers of NP rows.
 __host__ __device__ constexpr size_t nn_lds_bytes(int W, int NW, int C, bool SC, size_t rowbytes) {
-    return (size_t)2 * NW * C * (W * 32) * rowbytes + ((W >= 16 && NW == 8 && !SC) ? (size_t)2 * 64 * W * rowbytes : 0);
+    return (size_t)2 * NW * C * (W * 32) * rowbytes + (nn_rows_staged(W, NW, SC) ? (size_t)2 * C * 64 * W * rowbytes : 0);
 }
 
 // Stage depth the NN-specialised adjoint is compiled for: two buffers of C layers of the K-form
