@@ -282,6 +282,14 @@ int meshprop_phase_screen(int op, int N, int B, const float *phi, float *dphi, v
                           void *workspace, size_t workspace_bytes, void *stream);
 size_t meshprop_phase_screen_workspace_size(int N, int B);
 
+/* Waveguide permutation of B vectors (rows of ldu elements): u_out[b][w] = u_in[b][perm[w]]
+ * for w < N, perm a device int32[N] permutation (the gather v <- v[p] of mesh.py:47-48, 71-72
+ * and c/dot.cpp:150-152, 289-291, applied between layers of a permuted StructuredMeshNetwork;
+ * the reference GPU path rejects permutations, mesh.py:573).  elem_bytes: 8 for complex64, 4
+ * for float32 fields.  u_in and u_out must not overlap.  Asynchronous on stream. */
+int meshprop_permute(int N, int B, const int *perm, const void *u_in, void *u_out, int ldu, int elem_bytes,
+                     void *stream);
+
 /* Exact Hessian-vector product along the parameter direction q, by forward-over-reverse
  * differentiation (no reference counterpart: gpu/mesh.py:236-288 takes central differences of
  * two backdiffs).  Inputs: the forward output u_out and its tangent du_out along q (from

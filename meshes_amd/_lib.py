@@ -33,6 +33,7 @@ EXPORTS = (
     "meshprop_group_create", "meshprop_group_destroy", "meshprop_group_workspace_size",
     "meshprop_group_tables_size", "meshprop_group_upload", "meshprop_group_launch", "meshprop_group_signature",
     "meshprop_group_entry_info", "meshprop_phase_screen", "meshprop_phase_screen_workspace_size",
+    "meshprop_permute",
     "meshprop_hvp", "meshprop_hvp_workspace_size", "meshprop_run_noalloc", "meshprop_adjoint_kernel", "meshprop_forward_kernel",
 )
 
@@ -116,6 +117,8 @@ def load():
     lib.meshprop_phase_screen.restype = c_int
     lib.meshprop_phase_screen_workspace_size.argtypes = [c_int, c_int]
     lib.meshprop_phase_screen_workspace_size.restype = c_size_t
+    lib.meshprop_permute.argtypes = [c_int, c_int, vp, vp, vp, c_int, c_int, vp]
+    lib.meshprop_permute.restype = c_int
     lib.meshprop_hvp_workspace_size.argtypes = [c_int] * 5
     lib.meshprop_hvp_workspace_size.restype = c_size_t
     lib.meshprop_hvp.argtypes = ([c_int] * 5 + [vp] * 4 + [c_int, vp, c_int] + [vp] * 7 + [c_int, vp, c_size_t, vp])
@@ -245,6 +248,20 @@ def phase_screen(op, N, B, phi, dphi, u, du, ldu, workspace=None, ws_bytes=0, st
     rc = lib.meshprop_phase_screen(op, N, B, _ptr(phi), _ptr(dphi), _ptr(u), _ptr(du), ldu, _ptr(workspace),
                                    ctypes.c_size_t(ws_bytes), ctypes.c_void_p(stream))
     check(rc, "phase_screen")
+
+
+def permute(N, B, perm, u_in, u_out, ldu, stream=None):
+    """meshprop_permute on torch tensors: u_out[b, w] = u_in[b, perm[w]] (perm int32 on the device)."""
+    lib = load()
+    if stream is None:
+        stream = torch.cuda.current_stream().cuda_stream
+    if perm.dtype != torch.int32 or u_in.dtype != u_out.dtype:
+        raise TypeError("perm must be int32 and both fields of one element type")
+    eb = 8 if u_in.dtype == torch.complex64 else 4 if u_in.dtype == torch.float32 else 0
+    if not eb:
+        raise TypeError(f"fields must be complex64 or float32, got {u_in.dtype}")
+    check(lib.meshprop_permute(N, B, _ptr(perm), _ptr(u_in), _ptr(u_out), ldu, eb, ctypes.c_void_p(stream)),
+          "permute")
 
 
 def phase_screen_workspace_size(N, B) -> int:

@@ -100,8 +100,12 @@ def strides_from_perms(N: int, lens, shifts, perms) -> Tuple[np.ndarray, List[np
     return np.asarray(strides, dtype=np.int64), slots
 
 
-def to_gpu(net, tile=None) -> MeshNetworkGPU:
-    """MeshNetworkGPU of a StructuredMeshNetwork-like object (the replacement of mesh.py:568-579)."""
+def to_gpu(net, tile=None):
+    """MeshNetworkGPU of a StructuredMeshNetwork-like object (the replacement of mesh.py:568-579).
+
+    Permutations that pair waveguides at power-of-two strides map onto the FFT geometry; any
+    other permutations give a PermutedMeshGPU (segments of layers between gathers, with the same
+    call interface), where the reference GPU path rejects them (mesh.py:573)."""
     name = type(net.X).__name__
     if name not in X_NAMES:
         raise NotImplementedError(f"no GPU model for crossing {name}")
@@ -114,12 +118,25 @@ def to_gpu(net, tile=None) -> MeshNetworkGPU:
     p_cr, s_cr, phi = _packed(net)
     perm = list(net.perm) if getattr(net, "perm", None) is not None else [None] * (L + 1)
     splitter = np.iterable(net.p_splitter) or np.any(np.asarray(net.p_splitter) != 0)
-    if all(q is None for q in perm):
+    butterfly = True
+    if any(q is not None for q in perm):
+        try:
+            strides, slots = strides_from_perms(N, lens, shifts, perm)
+        except ValueError:
+            butterfly = False  # any other permutations: segments between gathers
+    if all(q is None for q in perm) or not butterfly:
         p_slot = np.nan_to_num(_grid(p_cr, N, lens, shifts, n_p))
         s_slot = np.nan_to_num(_grid(s_cr, N, lens, shifts, n_s)) if splitter else np.zeros(0)
         kw = {}
+        if not butterfly:
+            from .permuted import PermutedMeshGPU
+            p_full = p_slot.astype(np.float32).reshape(-1)
+            if net.is_phase:
+                p_full = np.concatenate([p_full, np.asarray(phi, np.float32)])
+            return PermutedMeshGPU(N, L, np.asarray(lens), np.asarray(shifts), perm, p_full,
+                                   np.asarray(s_slot, np.float32).reshape(-1), X, phi_pos=net.phi_pos,
+                                   is_phase=bool(net.is_phase), tile=tile)
     else:
-        strides, slots = strides_from_perms(N, lens, shifts, perm)
         p_slot = np.zeros((L, N // 2, n_p))
         s_slot = np.zeros((L, N // 2, n_s))
         ind = 0
@@ -159,7 +176,9 @@ class MeshOptimizer:
         # packed crossing k -> flat slot index of the GPU mesh
         N, lens, shifts = int(mesh.N), list(mesh.lens), list(mesh.shifts)
         perm = list(mesh.perm) if getattr(mesh, "perm", None) is not None else [None] * (len(lens) + 1)
-        if all(q is None for q in perm):
+        from .permuted import PermutedMeshGPU
+        if all(q is None for q in perm) or isinstance(self.mesh, PermutedMeshGPU):
+            # NN slots (also for a PermutedMeshGPU: its segments keep the NN geometry)
             self._inds = np.concatenate([np.arange(n * (N // 2) + s // 2, n * (N // 2) + s // 2 + k)
                                          for n, (s, k) in enumerate(zip(shifts, lens))])
         else:

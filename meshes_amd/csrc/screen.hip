@@ -1,6 +1,7 @@
 // screen.hip -- the phase screen of is_phase meshes: u_w <- exp(i phi_w) u_w on every vector,
 // its forward-mode derivative and its adjoint with dJ/dphi (c/dot.cpp:97-143 is the CPU
-// semantics; the reference GPU path rejects is_phase, gpu/mesh.py:65).
+// semantics; the reference GPU path rejects is_phase, gpu/mesh.py:65); and the waveguide
+// permutation (gather) between the layers of a permuted mesh (c/dot.cpp:150-152, 289-291).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -62,6 +63,19 @@ __global__ void __launch_bounds__(256) k_screen_reduce(int N, int nblk, const fl
 
 inline int nblocks(int B) { return (B + kScreenVectors - 1) / kScreenVectors; }
 
+// out[b][w] = in[b][perm[w]] for w < N: one thread per output element (coalesced writes; the
+// reads of a vector stay within its N contiguous elements).  E: element type (complex64 as
+// float2, or float).
+template <class E>
+__global__ void __launch_bounds__(256) k_permute(int N, int B, const int* __restrict__ perm, const E* __restrict__ in,
+                                                 E* __restrict__ out, int ldu) {
+    const int w = blockIdx.x * 256 + threadIdx.x;
+    if (w >= N) return;
+    const int src = perm[w];
+    const long b1 = ((long)blockIdx.y + 1) * kScreenVectors < B ? ((long)blockIdx.y + 1) * kScreenVectors : B;
+    for (long b = (long)blockIdx.y * kScreenVectors; b < b1; ++b) out[(size_t)b * ldu + w] = in[(size_t)b * ldu + src];
+}
+
 }  // namespace
 
 extern "C" {
@@ -93,6 +107,24 @@ int meshprop_phase_screen(int op, int N, int B, const float* phi, float* dphi, v
                            (const float*)workspace, dphi);
         if (hipGetLastError() != hipSuccess) return MESHPROP_ERR_HIP;
     }
+    return MESHPROP_OK;
+}
+
+int meshprop_permute(int N, int B, const int* perm, const void* u_in, void* u_out, int ldu, int elem_bytes,
+                     void* stream) {
+    if (N <= 0 || B < 0 || ldu < N || (elem_bytes != 8 && elem_bytes != 4)) return MESHPROP_ERR_ARG;
+    if (B == 0) return MESHPROP_OK;
+    if (!perm || !u_in || !u_out || u_in == u_out) return MESHPROP_ERR_ARG;
+    (void)hipGetLastError();
+    hipStream_t st = (hipStream_t)stream;
+    const dim3 grid((N + 255) / 256, nblocks(B));
+    if (elem_bytes == 8)
+        hipLaunchKernelGGL(k_permute<float2>, grid, dim3(256), 0, st, N, B, perm, (const float2*)u_in, (float2*)u_out,
+                           ldu);
+    else
+        hipLaunchKernelGGL(k_permute<float>, grid, dim3(256), 0, st, N, B, perm, (const float*)u_in, (float*)u_out,
+                           ldu);
+    if (hipGetLastError() != hipSuccess) return MESHPROP_ERR_HIP;
     return MESHPROP_OK;
 }
 

@@ -296,8 +296,6 @@ class MeshNetworkGPU:
         """
         assert vpos in ["in", "out"]
         p, s = self._defaults(p_phase, p_splitter)
-        ldp, lds = self._ld()
-        g0, g1 = self._geom()
         if vpos == "in":
             vout = self.dot(v, p_phase, p_splitter)
         else:
@@ -313,6 +311,15 @@ class MeshNetworkGPU:
         assert out_dp.numel() == self.n_phase
         assert out_dp.device == self.device, f"out_dp is on {out_dp.device}, the mesh on {self.device}"
         gin = self._out_buffer(out_dJdv, B)
+        self._backdiff(vb, gb, p, s, out_dp, gin)
+        return out_dp, self._result(gin, out_dJdv)
+
+    def _backdiff(self, vb, gb, p, s, out_dp, gin):
+        """Adjoint walk of (B, N) output field vb and gradient gb (screens included): accumulates
+        dJ/dp into out_dp, writes dJ/dv_in into gin and returns the reconstructed input field."""
+        B = vb.shape[0]
+        ldp, lds = self._ld()
+        g0, g1 = self._geom()
         uin = torch.empty_like(vb)
         n_cr_p = self.n_cr * self.X_np
         if self.is_phase and self.phi_pos == "out":
@@ -325,7 +332,7 @@ class MeshNetworkGPU:
                  flags=self._adj_flags(s))
         if self.is_phase and self.phi_pos == "in":
             self._screen(_lib.OP_BACKDIFF, B, p[n_cr_p:], out_dp[n_cr_p:], uin, gin)
-        return out_dp, self._result(gin, out_dJdv)
+        return uin
 
     def grad_target_l2(self, v, y, p_phase=None, p_splitter=None, out_dp=None):
         """Fused fwd+grad of J = sum |U v - y|^2 (the loss of gpu/test.py:131): returns

@@ -67,3 +67,13 @@ def test_grid_phase_matches_the_slot_layout():
     mask = O.active_mask(g)
     assert np.all(np.isnan(grid[~mask]))
     assert np.array_equal(grid[mask].reshape(-1), np.arange(n_cr * 2, dtype=float))
+
+
+def test_permuted_mesh_segments():
+    """A permuted mesh splits at every layer with a gather before it (the gathers before layer 0
+    and after the last run outside the segments)."""
+    from meshes_amd.permuted import segment_bounds
+    P = np.arange(4)
+    assert segment_bounds(5, [None] * 6) == [0, 5]
+    assert segment_bounds(5, [P, None, P, None, None, P]) == [0, 2, 5]
+    assert segment_bounds(3, [None, P, P, None]) == [0, 1, 2, 3]
