@@ -33,6 +33,7 @@ enum { B_KIND = 1, B_FWD = 2, B_INV = 4, B_DTDP = 8 };
 
 struct GMesh {
     int what, var, X, N, L, B, fft, LPV, NP, C, np, ns, ldp, lds, ldu, S, build, assign;
+    int kso;  // > 0: K table and dp partials in the NN adjoint's stage order (W = 16 NN meshes)
     long ntiles, E;
     long pro_chunks;  // prologue blocks (of 256 slots) per layer
     const int *g0, *g1;
@@ -103,7 +104,7 @@ __global__ void __launch_bounds__(256) k_group_prologue(const unsigned char* __r
         }
         if (M.build & B_DTDP) tabs |= TAB_KUNI;  // grouped meshes are unitary: K-form
         slot_tables_at<float4>(M.X, M.N, M.NP, M.LPV, M.fft, M.g0, M.g1, M.p, M.ldp, M.s, M.lds, nullptr, tabs,
-                               M.kind, ta, tb, M.dtdp, nullptr, l, m);
+                               M.kind, ta, tb, M.dtdp, nullptr, l, m, M.kso);
     }
 }
 
@@ -123,15 +124,22 @@ __global__ void __launch_bounds__(256) k_group_reduce(const unsigned char* __res
         KMesh& M = meshes[e];
         const long idx = (u - begin[e]) * 64 + lane;
         const int Nh = M.N / 2;
-        const bool ok = idx < (long)M.L * Nh * M.np;
-        int k = 0, m = 0, l = 0;
-        float acc = 0.f;
+        // Stage-order partials (kso > 0): units run over positions (coalesced reads) and
+        // position p*64 + t lands in slot t*kso + p.
+        const int Nr = M.kso > 0 ? M.NP / 2 : Nh;
+        bool ok = idx < (long)M.L * Nr * M.np;
+        int k = 0, m = 0, l = 0, pos = 0;
         if (ok) {
             k = (int)(idx % M.np);
             const long lm = idx / M.np;
-            m = (int)(lm % Nh);
-            l = (int)(lm / Nh);
-            const float* src = M.partial + ((size_t)l * (M.NP / 2) + m) * M.np + k;
+            pos = (int)(lm % Nr);
+            l = (int)(lm / Nr);
+            m = M.kso > 0 ? (pos % 64) * M.kso + pos / 64 : pos;
+            ok = m < Nh;
+        }
+        float acc = 0.f;
+        if (ok) {
+            const float* src = M.partial + ((size_t)l * (M.NP / 2) + pos) * M.np + k;
             const size_t E = (size_t)M.E;
             const int s0 = (int)((long)M.S * wq / 4), s1 = (int)((long)M.S * (wq + 1) / 4);
             // Eight interleaved accumulators: eight loads in flight per lane (the sums are
@@ -176,10 +184,21 @@ __device__ __forceinline__ void group_item(KMesh& M, const int slot, float4* sta
     for (long tile = slot; tile < M.ntiles; tile += M.S) {
         if (M.what & G_FWD)
             group_fwd_tile<W, V>(&M, tile);
-        if (M.what & G_GRAD)
+        if (M.what & G_GRAD) {
+            if constexpr (W == 16 && V == 1) {
+                if (M.kso > 0) {
+                    // NN meshes: the NN-specialised adjoint (one-layer stages in stage order).
+                    bwd_tile_nn<float2, 16, 1, kFamWaves[1], 1, false>(
+                        M.N, 0, M.L, M.B, (KInt*)M.kind, M.rows_inv, M.dtdp, part, M.u_out, M.g_out, M.target,
+                        M.u_rec, M.g_in, M.ldu, tile, first, stage, Ckpt{nullptr, 0, 0}, nullptr);
+                    first = false;
+                    continue;
+                }
+            }
             bwd_tile<float2, W, V, true, KInt*>(M.N, 0, M.L, M.B, M.LPV, M.NP, M.C, (KInt*)M.kind, M.rows_inv,
                                          M.rows_inv, M.dtdp, part, M.u_out, M.g_out,
                                          M.target, M.u_rec, M.g_in, M.ldu, tile, first, stage);
+        }
         first = false;
     }
 }
@@ -237,7 +256,7 @@ using namespace mp;
 
 struct meshprop_group {
     struct Info {
-        int W, V, LPV, NP, C, S, var, fam, what, build, np, ns;
+        int W, V, LPV, NP, C, S, var, fam, what, build, np, ns, kso;
         long ntiles, E, pro_units, red_units;
         bool own_tables;
         size_t off_tab, off_partial;  // workspace offsets (tables when own_tables)
@@ -400,6 +419,10 @@ int plan_group(meshprop_group& g) {
         if (C < depth) C = depth;
         if ((size_t)C * per_layer > kGroupStage) return MESHPROP_ERR_UNSUPPORTED;
         f.C = C;
+        // W = 16 NN meshes of unitary crossings run the NN-specialised adjoint: one-layer stages
+        // (two buffers in the stage budget), K table and partials in its stage order.
+        f.kso = (f.W == 16 && f.V == 1 && !e.fft && f.LPV == 64 && (e.X == X_MZI || e.X == X_SYM)) ? 8 : 0;
+        if (f.kso) f.C = 1;
         const bool grad = f.what & G_GRAD;
         f.S = (e.B == 0) ? 0 : grad ? (int)std::min<long>(f.ntiles, 2L * g.ncu) : (int)f.ntiles;
         f.E = (long)e.L * (f.NP / 2) * f.np;
@@ -424,7 +447,8 @@ int plan_group(meshprop_group& g) {
         if (grad && e.L > 0 && f.S > 0) off = a256(off + (size_t)f.S * f.E * sizeof(float));
         f.pro_units = f.build ? (long)e.L * ((f.NP / 2 + 255) / 256) : 0;
         const bool assign = (e.flags & MESHPROP_GROUP_DP_ASSIGN) != 0;
-        f.red_units = (grad && e.L > 0 && (f.S > 0 || assign)) ? ((long)e.L * (e.N / 2) * f.np + 63) / 64 : 0;
+        const long red_slots = f.kso ? f.NP / 2 : e.N / 2;  // positions of a stage-order mesh
+        f.red_units = (grad && e.L > 0 && (f.S > 0 || assign)) ? ((long)e.L * red_slots * f.np + 63) / 64 : 0;
         g.pro_total += f.pro_units;
         g.red_total += f.red_units;
         // Work items, costed by layers x tile work x tiles per item.
@@ -497,6 +521,7 @@ void build_image(const meshprop_group& g, unsigned char* ws) {
         m.LPV = f.LPV;
         m.NP = f.NP;
         m.C = f.C;
+        m.kso = f.kso;
         m.np = f.np;
         m.ns = f.ns;
         m.ldp = e.ldp;

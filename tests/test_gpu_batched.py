@@ -92,6 +92,7 @@ MIX = [
     ("mzi", "butterfly", 16, 100, "grad"), ("mzi", "butterfly", 64, 33, "fwd"), ("mzi", "butterfly", 256, 20, "grad"),
     ("sym", "nn", 48, 12, "grad"), ("mzi", "nn", 200, 7, "backdiff"), ("mzi_generic", "nn", 96, 11, "fwd"),
     ("mzi", "clements", 1024, 2, "grad"), ("mzi", "butterfly", 1024, 6, "grad"),
+    ("mzi", "nn", 700, 9, "backdiff"), ("sym", "clements", 1024, 5, "grad"),
 ]
 
 
@@ -223,7 +224,8 @@ def test_group_edges_and_accumulate(gpu):
 
 def test_group_resident_tables(gpu):
     from meshes_amd.batched import MeshGroup, tables_for
-    specs = [Spec(31, "mzi", "clements", 128, 20, "grad"), Spec(32, "mzi", "butterfly", 512, 6, "fwd")]
+    specs = [Spec(31, "mzi", "clements", 128, 20, "grad"), Spec(32, "mzi", "butterfly", 512, 6, "fwd"),
+             Spec(33, "mzi", "clements", 1024, 4, "grad")]
     tabs = [tables_for(s.mesh) for s in specs]
     jobs = [s.job(tables=t, build_tables=True) for s, t in zip(specs, tabs)]
     MeshGroup(jobs).run()
