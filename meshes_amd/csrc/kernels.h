@@ -1454,7 +1454,7 @@ __device__ __forceinline__ void bwd_layer_nn(const bool odd, const T (&u)[V][W],
     kstage_write_nn<T, W, nn_stage_order(W) ? 64 : 1>(odd, accA, accB, wl);
 }
 
-template <class T, int W, int V, int NW, int C, bool SC>
+template <class T, int W, int V, int NW, int C, bool SC, bool RSX = nn_rows_staged(W, NW, SC)>
 __device__ __forceinline__ void bwd_tile_nn(const int N, const int l0, const int nl, const int B, ConstInt* kind,
                                             const typename RowOf<T>::type* __restrict__ rows_inv,
                                             const T* __restrict__ dtdp, float* __restrict__ my_partial,
@@ -1515,7 +1515,7 @@ __device__ __forceinline__ void bwd_tile_nn(const int N, const int l0, const int
     // during stage k each thread requests its share of the rows of stage k + 1 and writes them
     // to slot (k+1) & 1 at the end of the stage (that slot was last read during stage k - 1,
     // before its barrier); the stage barrier publishes them.
-    constexpr bool RS = nn_rows_staged(W, NW, SC);
+    constexpr bool RS = RSX;
     // Row entries each thread copies per layer; with fewer entries than threads (W = 4 in eight
     // waves) the first NP threads copy one each.
     constexpr int CP = (RS && NP >= NTHR) ? NP / NTHR : 1;
@@ -1706,7 +1706,7 @@ __device__ __forceinline__ void bwd_tile_nn(const int N, const int l0, const int
     int N, int l0, int nl, int B, const int *__restrict__ kind, const typename RowOf<T>::type *__restrict__ rows_inv, \
         const T *__restrict__ dtdp, float *__restrict__ partial, size_t part_stride, long ntiles, const T *u_out,   \
         const T *g_out, const T *target, T *u_in, T *g_in, int ldu, Ckpt ck, const float *__restrict__ scales
-template <class T, int W, int V, int NW, int C, bool SC>
+template <class T, int W, int V, int NW, int C, bool SC, bool RSX>
 __device__ __forceinline__ void bwd_nn_body(MP_BWD_NN_PARAMS) {
     typedef typename RowOf<T>::type R;
     extern __shared__ unsigned char smem_raw[];
@@ -1714,21 +1714,21 @@ __device__ __forceinline__ void bwd_nn_body(MP_BWD_NN_PARAMS) {
     float* my_partial = partial + (size_t)blockIdx.x * part_stride;
     bool first = true;
     for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        bwd_tile_nn<T, W, V, NW, C, SC>(N, l0, nl, B, (ConstInt*)kind, rows_inv, dtdp, my_partial, u_out, g_out,
-                                        target, u_in, g_in, ldu, tile, first, stage, ck, scales);
+        bwd_tile_nn<T, W, V, NW, C, SC, RSX>(N, l0, nl, B, (ConstInt*)kind, rows_inv, dtdp, my_partial, u_out,
+                                             g_out, target, u_in, g_in, ldu, tile, first, stage, ck, scales);
         first = false;
     }
 }
-template <class T, int W, int V, int NW, int C, bool SC>
+template <class T, int W, int V, int NW, int C, bool SC, bool RSX = nn_rows_staged(W, NW, SC)>
 __global__ void __launch_bounds__(64 * NW) k_bwd_nn(MP_BWD_NN_PARAMS) {
-    bwd_nn_body<T, W, V, NW, C, SC>(N, l0, nl, B, kind, rows_inv, dtdp, partial, part_stride, ntiles, u_out, g_out,
-                                    target, u_in, g_in, ldu, ck, scales);
+    bwd_nn_body<T, W, V, NW, C, SC, RSX>(N, l0, nl, B, kind, rows_inv, dtdp, partial, part_stride, ntiles, u_out,
+                                         g_out, target, u_in, g_in, ldu, ck, scales);
 }
 // The widest tile (W = 16), compiled for two waves per SIMD also in four-wave workgroups.
-template <class T, int W, int V, int NW, int C, bool SC>
+template <class T, int W, int V, int NW, int C, bool SC, bool RSX = nn_rows_staged(W, NW, SC)>
 __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2))) k_bwd_nn_w16(MP_BWD_NN_PARAMS) {
-    bwd_nn_body<T, W, V, NW, C, SC>(N, l0, nl, B, kind, rows_inv, dtdp, partial, part_stride, ntiles, u_out, g_out,
-                                    target, u_in, g_in, ldu, ck, scales);
+    bwd_nn_body<T, W, V, NW, C, SC, RSX>(N, l0, nl, B, kind, rows_inv, dtdp, partial, part_stride, ntiles, u_out,
+                                         g_out, target, u_in, g_in, ldu, ck, scales);
 }
 #undef MP_BWD_NN_PARAMS
 
@@ -1820,9 +1820,15 @@ struct Plan {
 
 // Dynamic LDS of the NN-specialised adjoint: two stage buffers of NW waves x C layers x NP/2
 // K-form entries, and with staged rows two slots of C layers of NP rows.
-__host__ __device__ constexpr size_t nn_lds_bytes(int W, int NW, int C, bool SC, size_t rowbytes) {
-    return (size_t)2 * NW * C * (W * 32) * rowbytes + (nn_rows_staged(W, NW, SC) ? (size_t)2 * C * 64 * W * rowbytes : 0);
+__host__ __device__ constexpr size_t nn_lds_bytes(int W, int NW, int C, bool rs, size_t rowbytes) {
+    return (size_t)2 * NW * C * (W * 32) * rowbytes + (rs ? (size_t)2 * C * 64 * W * rowbytes : 0);
 }
+
+// Four-wave workgroups of the widest tile also stage their rows when the grid leaves at most
+// one workgroup per CU (small batches): the rows then arrive from LDS instead of L2, which
+// shortens each layer's critical path; with more workgroups the 96 KiB of LDS would cost
+// occupancy.
+inline bool nn_rows_staged_small(const Plan& P) { return P.W == 16 && P.waves == 4 && P.grid <= P.ncu; }
 
 // Stage depth the NN-specialised adjoint is compiled for: two buffers of C layers of the K-form
 // stage fill the 64 KiB (four waves) or 128 KiB (eight waves) budget, C = 16 / W (one layer at

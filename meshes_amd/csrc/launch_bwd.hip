@@ -58,21 +58,26 @@ static int go1(const Plan& P, const int* kind, const void* ri, const void* ra, c
 
 // NN-specialised unitary adjoint (k_bwd_nn) for NW-wave workgroups; scaled-unitary crossings
 // in the scaled object.
-template <class T, int W, int V, int NW>
+template <class T, int W, int V, int NW, bool RS = nn_rows_staged(W, NW, MP_SC_SET != 0)>
 static int go_nn(const Plan& P, const int* kind, const void* ri, const void* dtdp, float* partial, size_t ps, int l0,
                  int nl, const void* u_out, const void* g_out, const void* target, void* u_in, void* g_in, int ldu,
                  hipStream_t st, Ckpt ck) {
     typedef typename RowOf<T>::type R;
     constexpr int C = nn_stage_depth(W);
     constexpr bool SC = MP_SC_SET != 0;
-    const size_t shmem = nn_lds_bytes(W, NW, C, SC, sizeof(R));
+    if constexpr (W >= 16 && NW == 4 && !SC && !RS) {
+        if (nn_rows_staged_small(P))
+            return go_nn<T, W, V, NW, true>(P, kind, ri, dtdp, partial, ps, l0, nl, u_out, g_out, target, u_in, g_in,
+                                            ldu, st, ck);
+    }
+    const size_t shmem = nn_lds_bytes(W, NW, C, RS, sizeof(R));
     static std::atomic<unsigned long long> lds_set{0};
     // W = 16: the entry compiled for two waves per SIMD.
     auto kern = [] {
         if constexpr (W >= 16)
-            return k_bwd_nn_w16<T, W, V, NW, C, SC>;
+            return k_bwd_nn_w16<T, W, V, NW, C, SC, RS>;
         else
-            return k_bwd_nn<T, W, V, NW, C, SC>;
+            return k_bwd_nn<T, W, V, NW, C, SC, RS>;
     }();
     if (int rc = lds_limit_once((const void*)kern, lds_set, shmem)) return rc;
     hipLaunchKernelGGL(kern, dim3(P.grid), dim3(64 * NW), shmem, st, P.N, l0, nl, P.B, kind, (const R*)ri,
