@@ -1411,11 +1411,13 @@ __device__ __forceinline__ void bwd_tile_pipe(const int N, const int l0, const i
 // one-layer stage spill and conflict (W = 2 has one slot per lane: both orders agree).
 __host__ __device__ constexpr bool nn_stage_order(int W) { return W >= 4; }
 
-// Whether the NN-specialised adjoint stages its rows through LDS: eight-wave workgroups, W >= 8;
-// the scaled-unitary variant keeps its per-layer loads.  Measured against per-wave row loads
-// (fwd+grad, Clements MZI, L = N): N=1024 B=2048 3.91 -> 3.54 ms, N=512 B=4096 1.487 -> 1.404 ms;
-// at W = 4 (N=256 B=4096, two vectors per lane) 0.2891 -> 0.2942 ms, so it stays off there.
-__host__ __device__ constexpr bool nn_rows_staged(int W, int NW, bool SC) { return W >= 8 && NW == 8 && !SC; }
+// Whether the NN-specialised adjoint stages its rows through LDS: eight-wave workgroups, W >= 8
+// (scaled-unitary adjoints stage the rows and load their per-waveguide factors per wave, one
+// layer ahead).  Measured against per-wave row loads (fwd+grad, Clements MZI, L = N): N=1024
+// B=2048 3.91 -> 3.54 ms, N=512 B=4096 1.487 -> 1.404 ms; the lossy N=512 L=512 B=8192 Adam
+// step with checkpoints every 16 layers 4.56 -> 4.28 ms; at W = 4 (N=256 B=4096, two vectors
+// per lane) 0.2891 -> 0.2942 ms, so it stays off there.
+__host__ __device__ constexpr bool nn_rows_staged(int W, int NW, bool) { return W >= 8 && NW == 8; }
 
 // K-form stage entries of one NN layer (lane t owns slots t*W/2 .. t*W/2 + W/2 - 1; an odd
 // layer pairs the lane's last waveguide with the next lane's first); entry p at wl[p * ES].
@@ -1597,7 +1599,17 @@ __device__ __forceinline__ void bwd_tile_nn(const int N, const int l0, const int
         for (int k = 0; k < np; ++k) val[k] = kdot(e, it.dd[k]);
         put_dp<np>(it.out, val, first);
     };
+    // Per-waveguide factors of a scaled-unitary adjoint with staged rows: one layer ahead,
+    // by buffer loads (the one-layer stages of the widest tile load them right before the layer).
+    auto scales_at = [&](const int s, float (&f)[SC ? W : 1]) {
+        if constexpr (SC) {
+            const int lr = s < last ? last - s : 0;
+#pragma unroll
+            for (int k = 0; k < W; ++k) f[k] = buf_load<float>(srs, soff, (lr * NP + k * LPV) * (int)sizeof(float));
+        }
+    };
     if (!LEAN && !RS && nl > 0) rows_at(0, ri[0], sc[0]);
+    if (RS && SC && !LEAN && nl > 0) scales_at(0, sc[0]);
     bool have_prev = false;
     // One stage of cl layers starting at local layer c0 (GUARD: cl < C, the last stage); PAR is
     // the parity of c0, which picks the register sets of its layers; ODD is the shift parity of
@@ -1630,6 +1642,12 @@ __device__ __forceinline__ void bwd_tile_nn(const int N, const int l0, const int
                     const R* lr = lrow + (((c0 / C) & 1) * C + lc) * NP + lane;
 #pragma unroll
                     for (int k = 0; k < W; ++k) ri[rs][k] = lr[k * LPV];
+                    if constexpr (SC) {
+                        if constexpr (LEAN)
+                            scales_at(c0 + lc, sc[0]);
+                        else
+                            scales_at(c0 + lc + 1, sc[cur ^ 1]);
+                    }
                     fetch_rows(cp[lc], c0 + C + lc);
                 } else if constexpr (LEAN) {
                     rows_at(c0 + lc, ri[0], sc[0]);
