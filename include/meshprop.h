@@ -296,8 +296,10 @@ int meshprop_permute(int N, int B, const int *perm, const void *u_in, void *u_ou
  * meshprop_fwddiff with dp = q), and either target (loss sum |u_out - target|^2: dJ/du_out =
  * 2 (u_out - target), tangent 2 du_out) or dJdu_out with its tangent d_dJdu_out along q.
  * Accumulates grad += dJ/dp (nullable) and hvp += (d2J/dp2) q in the slot layout of p.  Unitary
- * complex crossings (MESHPROP_MZI, MESHPROP_SYM), N <= 512; other crossings return
- * MESHPROP_ERR_UNSUPPORTED.  Workspace: meshprop_hvp_workspace_size bytes. */
+ * crossings: MESHPROP_MZI and MESHPROP_SYM (complex64 fields) with N <= 1024 on NN meshes and
+ * N <= 512 on FFT meshes; MESHPROP_ORTH (float32 fields) with N <= 1024 on NN meshes.  Other
+ * crossings and sizes return MESHPROP_ERR_UNSUPPORTED.  Workspace: meshprop_hvp_workspace_size
+ * bytes (it depends on fft). */
 size_t meshprop_hvp_workspace_size(int X, int N, int L, int B, int fft);
 int meshprop_hvp(int X, int N, int L, int B, int fft, const int *geom0, const int *geom1,
                  const float *p, const float *q, int ldp, const float *s, int lds,

@@ -276,6 +276,7 @@ def hvp(X, N, L, B, fft, g0, g1, p, q, ldp, s, lds, u_out, du_out, dJdu_out, d_d
         workspace, ws_bytes, stream=None):
     """meshprop_hvp on torch tensors."""
     lib = load()
+    _check_dtypes(X, (p, q, s, grad, hv), (u_out, du_out, dJdu_out, d_dJdu_out, target))
     if stream is None:
         stream = torch.cuda.current_stream().cuda_stream
     rc = lib.meshprop_hvp(X_CODE[X] if isinstance(X, str) else X, N, L, B, int(fft), _ptr(g0), _ptr(g1), _ptr(p),

@@ -11,8 +11,9 @@
 // With K_k = T^H dT/dp_k and its tangent dK_k along q (both anti-Hermitian), every LDS stage
 // end converts, per slot,
 //   grad_k += Re sum K_k Q,   hvp_k += Re sum (K_k dQ + dK_k Q)
-// into per-workgroup partials, reduced afterwards in fixed order.  Unitary complex crossings
-// (MZI, SYM), one vector per lane group, N up to 512.
+// into per-workgroup partials, reduced afterwards in fixed order.  Unitary crossings: MZI and
+// SYM (complex fields) and orth (real fields) on NN meshes up to N = 1024 (k_hvp_nn); MZI and
+// SYM on FFT meshes up to N = 512 (k_hvp, one vector per lane group).
 #include <algorithm>
 
 #include "kernels.h"
@@ -20,19 +21,22 @@
 namespace mp {
 
 // Tangent of the K-form coefficients along the parameter direction q, for the exact
-// Hessian-vector product of unitary complex crossings: K_k = T^H dT/dp_k is anti-Hermitian for
+// Hessian-vector product of unitary crossings: K_k = T^H dT/dp_k is anti-Hermitian for
 // every p, hence so is dK_k = (sum_m q_m dT/dp_m)^H dT/dp_k + T^H S_k with the contracted second
 // derivatives S_k = sum_m q_m d2T/dp_k dp_m; stored like K (float4 (a, d, c) per (layer,
-// parameter, slot), zero for inactive slots).
+// parameter, slot), or the float c of a real crossing; zero for inactive slots).  kso > 0:
+// stage order, slot m at position (m % kso) * 64 + m / kso, as the K table of the NN walk.
+template <class KT>
 __global__ void k_dkform(int X, int N, int L, int NP, int fft, const int* __restrict__ g0, const int* __restrict__ g1,
                          const float* __restrict__ p, const float* __restrict__ q, int ldp, const float* __restrict__ s,
-                         int lds, float4* __restrict__ dk) {
+                         int lds, KT* __restrict__ dk, int kso) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     const int l = blockIdx.y;
     const int NPh = NP / 2;
     if (l >= L || m >= NPh) return;
     const int np = n_phase_of(X), ns = n_split_of(X);
-    float4* out = dk + (size_t)l * np * NPh + m;
+    const int km = kso > 0 ? (m % kso) * 64 + m / kso : m;
+    KT* out = dk + (size_t)l * np * NPh + km;
     int i;
     if (fft) {
         const int st = g0[l];
@@ -48,7 +52,7 @@ __global__ void k_dkform(int X, int N, int L, int NP, int fft, const int* __rest
     bool upper = false;
     const bool act = (i >= 0) && wg_role(fft, g0, g1, l, i, N, mm, upper) && upper && mm == m;
     if (!act) {
-        for (int k = 0; k < np; ++k) out[(size_t)k * NPh] = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int k = 0; k < np; ++k) out[(size_t)k * NPh] = KT{};
         return;
     }
     const float* pp = p + (size_t)l * ldp + (size_t)m * np;
@@ -63,7 +67,10 @@ __global__ void k_dkform(int X, int N, int L, int NP, int fft, const int* __rest
     for (int k = 0; k < np; ++k) {
         const M2 dK = madd(mmul(mherm(Dq), dT[k]), mmul(mherm(Tm), S2[k]));
         const double cr = 0.5 * (dK.a01.r - dK.a10.r), ci = 0.5 * (dK.a01.i + dK.a10.i);
-        out[(size_t)k * NPh] = make_float4((float)dK.a00.i, (float)dK.a11.i, (float)cr, (float)ci);
+        if constexpr (sizeof(KT) == sizeof(float4))
+            out[(size_t)k * NPh] = KT{(float)dK.a00.i, (float)dK.a11.i, (float)cr, (float)ci};
+        else
+            out[(size_t)k * NPh] = KT((float)cr);
     }
 }
 
@@ -282,17 +289,250 @@ __global__ void __launch_bounds__(64 * kHvpWaves) __attribute__((amdgpu_waves_pe
     }
 }
 
-// out[l*ldp + m*np + k] += sum over workgroups (in order) of partial[wg][(l*NPh + m)*np + k], m < N/2.
+// ---- NN meshes --------------------------------------------------------------------------
+// One nearest-neighbour layer of the HVP walk for one vector per wave (lane t holds waveguides
+// t*W .. t*W + W - 1), the state updated in place; ODD: the layer pairs (2p+1, 2p+2) in the
+// lane and the lane's last waveguide with the next lane's first.  The Q and dQ entries of the
+// lane's slot p go to wq[p * 64] and wdq[p * 64] (stage order).
+template <class T, int W, bool ODD>
+__device__ __forceinline__ void hvp_layer_nn(T (&u)[W], T (&du)[W], T (&g)[W], T (&dg)[W],
+                                             const typename RowOf<T>::type (&ri)[W],
+                                             const typename RowOf<T>::type (&rd)[W], typename RowOf<T>::type* wq,
+                                             typename RowOf<T>::type* wdq) {
+    constexpr int H = W / 2, ES = 64;
+    const T z = fzero(T());
+    // In-lane crossing of waveguides k (upper) and j = k + 1, entry p.
+    auto pair = [&](const int k, const int j, const int p) __attribute__((always_inline)) {
+        const T xk = rapply(ri[k], u[k], u[j]), xj = rapply(ri[j], u[j], u[k]);
+        const T dxk = rapply_d(ri[k], rd[k], u[k], u[j], du[k], du[j]);
+        const T dxj = rapply_d(ri[j], rd[j], u[j], u[k], du[j], du[k]);
+        const T gk = rapply(ri[k], g[k], g[j]), gj = rapply(ri[j], g[j], g[k]);
+        const T dgk = rapply_d(ri[k], rd[k], g[k], g[j], dg[k], dg[j]);
+        const T dgj = rapply_d(ri[j], rd[j], g[j], g[k], dg[j], dg[k]);
+        wq[p * ES] = kquad(cjim(gk, xk, z), cjim(gj, xj, z), cjfma(gk, xj, z), cjfma(gj, xk, z));
+        wdq[p * ES] = kquad(cjim(gk, dxk, cjim(dgk, xk, z)), cjim(gj, dxj, cjim(dgj, xj, z)),
+                            cjfma(gk, dxj, cjfma(dgk, xj, z)), cjfma(gj, dxk, cjfma(dgj, xk, z)));
+        u[k] = xk;
+        u[j] = xj;
+        du[k] = dxk;
+        du[j] = dxj;
+        g[k] = gk;
+        g[j] = gj;
+        dg[k] = dgk;
+        dg[j] = dgj;
+    };
+    if constexpr (!ODD) {
+#pragma unroll
+        for (int p = 0; p < H; ++p) pair(2 * p, 2 * p + 1, p);
+    } else {
+        // Lane-boundary waveguides: their new values from the neighbours' layer inputs (lanes
+        // without a neighbour read 0; their rows are the identity), read before any update.
+        const T hu = lane_next(u[0]), hdu = lane_next(du[0]), hg = lane_next(g[0]), hdg = lane_next(dg[0]);
+        const T lu = lane_prev(u[W - 1]), ldu = lane_prev(du[W - 1]), lg = lane_prev(g[W - 1]),
+                ldg = lane_prev(dg[W - 1]);
+        const T xh = rapply(ri[W - 1], u[W - 1], hu), gh = rapply(ri[W - 1], g[W - 1], hg);
+        const T dxh = rapply_d(ri[W - 1], rd[W - 1], u[W - 1], hu, du[W - 1], hdu);
+        const T dgh = rapply_d(ri[W - 1], rd[W - 1], g[W - 1], hg, dg[W - 1], hdg);
+        const T xl = rapply(ri[0], u[0], lu), gl = rapply(ri[0], g[0], lg);
+        const T dxl = rapply_d(ri[0], rd[0], u[0], lu, du[0], ldu);
+        const T dgl = rapply_d(ri[0], rd[0], g[0], lg, dg[0], ldg);
+#pragma unroll
+        for (int p = 0; p + 1 < H; ++p) pair(2 * p + 1, 2 * p + 2, p);
+        u[W - 1] = xh;
+        du[W - 1] = dxh;
+        g[W - 1] = gh;
+        dg[W - 1] = dgh;
+        u[0] = xl;
+        du[0] = dxl;
+        g[0] = gl;
+        dg[0] = dgl;
+        // Crossing (W-1 of this lane, 0 of the next): this lane's rows of Q and dQ, and the
+        // next lane's (computed there against this lane's reconstructed x, dx).
+        const T nx = lane_next(xl), ndx = lane_next(dxl), px = lane_prev(xh), pdx = lane_prev(dxh);
+        const T A0 = cjim(gl, xl, z), B0 = cjfma(gl, px, z);
+        const T dA0 = cjim(gl, dxl, cjim(dgl, xl, z)), dB0 = cjfma(gl, pdx, cjfma(dgl, px, z));
+        wq[(H - 1) * ES] = kquad(cjim(gh, xh, z), a_next(A0), cjfma(gh, nx, z), lane_next(B0));
+        wdq[(H - 1) * ES] = kquad(cjim(gh, dxh, cjim(dgh, xh, z)), a_next(dA0), cjfma(gh, ndx, cjfma(dgh, nx, z)),
+                                  lane_next(dB0));
+    }
+}
+
+// HVP walk of NN meshes: NW waves per workgroup, one vector per wave, the layers walked
+// backwards with one-layer stages (buffer s & 1 of [2][NW][Q, dQ][NP/2] entries); while layer
+// s is applied, the workgroup converts layer s - 1 (its K and dK coefficients requested before
+// the layer, its NW entries summed in wave order after it).  PF: rows of the next layer are
+// requested one layer ahead (two row register sets); otherwise right before their layer.
+// part_g may be null (no gradient).
+template <class T, int W, int NW, bool PF>
+__device__ __forceinline__ void hvp_nn_body(const int N, const int L, const int B, const int* __restrict__ kind,
+                                            const typename RowOf<T>::type* __restrict__ rows_r,
+                                            const typename RowOf<T>::type* __restrict__ rows_d,
+                                            const typename KOf<T>::type* __restrict__ ktab,
+                                            const typename KOf<T>::type* __restrict__ dktab,
+                                            float* __restrict__ part_g, float* __restrict__ part_h,
+                                            const size_t part_stride, const long ntiles, const T* u_out,
+                                            const T* du_out, const T* g_out, const T* dg_out, const T* target,
+                                            const int ldu) {
+    typedef typename RowOf<T>::type R;
+    typedef typename KOf<T>::type KT;
+    constexpr int LPV = 64, NP = 64 * W, NPh = NP / 2;
+    constexpr int np = NPhase<T>::value;
+    constexpr int NTHR = 64 * NW;
+    constexpr int NI = (NPh + NTHR - 1) / NTHR;  // conversion items per thread and layer
+    constexpr int WSTRIDE = 2 * NPh;             // one wave's Q then dQ entries
+    constexpr int BSTRIDE = NW * WSTRIDE;        // one buffer
+    static_assert(NPh % 64 == 0, "whole waves of conversion items");
+    extern __shared__ unsigned char smem_raw[];
+    R* stage = reinterpret_cast<R*>(smem_raw);
+    const int lane = threadIdx.x & 63;
+    const int wave_u = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    float* pg = part_g ? part_g + (size_t)blockIdx.x * part_stride : nullptr;
+    float* ph = part_h + (size_t)blockIdx.x * part_stride;
+    const __amdgpu_buffer_rsrc_t rrs = table_rsrc(rows_r), drs = table_rsrc(rows_d);
+    const unsigned roff = (unsigned)lane * (unsigned)sizeof(R);
+    auto rows_at = [&](const int s, R (&r)[W], R (&d)[W]) __attribute__((always_inline)) {
+        const int la = s < L ? L - 1 - s : 0;
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            r[k] = buf_load<R>(rrs, roff, (la * NP + k * LPV) * (int)sizeof(R));
+            d[k] = buf_load<R>(drs, roff, (la * NP + k * LPV) * (int)sizeof(R));
+        }
+    };
+    bool first = true;
+    for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const Lanes q = lanes_of<1>(LPV, tile);
+        T u[1][W], du[1][W], g[1][W], dg[1][W];
+        load_tile(u, u_out, q, N, B, ldu);
+        load_tile(du, du_out, q, N, B, ldu);
+        if (target != nullptr) {
+            T y[1][W];
+            load_tile(y, target, q, N, B, ldu);
+#pragma unroll
+            for (int k = 0; k < W; ++k) {
+                g[0][k] = fsub2(u[0][k], y[0][k]);
+                dg[0][k] = fscl(du[0][k], 2.f);
+            }
+        } else {
+            load_tile(g, g_out, q, N, B, ldu);
+            load_tile(dg, dg_out, q, N, B, ldu);
+        }
+        // Conversion items of the previous layer pla (entries in pbuf): item k of a thread is
+        // position k*NTHR + threadIdx.x; whole waves are on or off.
+        struct Item {
+            KT kk[np], dk[np];
+        } it[NI];
+        int pla = 0;
+        const R* pbuf = stage;
+        auto issue = [&]() __attribute__((always_inline)) {
+#pragma unroll
+            for (int k = 0; k < NI; ++k) {
+                const int m = k * NTHR + wave_u * 64 + lane;
+                if (k * NTHR + wave_u * 64 >= NPh) continue;
+                const KT* kp = ktab + (size_t)pla * np * NPh + m;
+                const KT* dp = dktab + (size_t)pla * np * NPh + m;
+#pragma unroll
+                for (int e = 0; e < np; ++e) {
+                    it[k].kk[e] = kp[e * NPh];
+                    it[k].dk[e] = dp[e * NPh];
+                }
+            }
+        };
+        auto finish = [&]() __attribute__((always_inline)) {
+#pragma unroll
+            for (int k = 0; k < NI; ++k) {
+                const int m = k * NTHR + wave_u * 64 + lane;
+                if (k * NTHR + wave_u * 64 >= NPh) continue;
+                const R* s0 = pbuf + m;
+                R Qs = s0[0], dQs = s0[NPh];
+#pragma unroll
+                for (int wv = 1; wv < NW; ++wv) {
+                    Qs = row_add(Qs, s0[wv * WSTRIDE]);
+                    dQs = row_add(dQs, s0[wv * WSTRIDE + NPh]);
+                }
+                float gv[np], hv[np];
+#pragma unroll
+                for (int e = 0; e < np; ++e) {
+                    gv[e] = kdot(Qs, it[k].kk[e]);
+                    hv[e] = kdot(dQs, it[k].kk[e]) + kdot(Qs, it[k].dk[e]);
+                }
+                const size_t o = ((size_t)pla * NPh + m) * np;
+                if (pg) put_dp<np>(pg + o, gv, first);
+                put_dp<np>(ph + o, hv, first);
+            }
+        };
+        // W >= 8: the coefficients are requested at the conversion (held across the layer
+        // they push the tile past the registers of two waves per SIMD).
+        constexpr bool LATE = W >= 8;
+        auto layer = [&](const int s, const R (&ri)[W], const R (&rd)[W]) __attribute__((always_inline)) {
+            R* buf = stage + (s & 1) * BSTRIDE;
+            if (s > 0 && !LATE) issue();
+            R* wq = buf + wave_u * WSTRIDE + lane;
+            if (kind[L - 1 - s] & 1)
+                hvp_layer_nn<T, W, true>(u[0], du[0], g[0], dg[0], ri, rd, wq, wq + NPh);
+            else
+                hvp_layer_nn<T, W, false>(u[0], du[0], g[0], dg[0], ri, rd, wq, wq + NPh);
+            if (s > 0) {
+                if (LATE) issue();
+                finish();
+            }
+            __syncthreads();
+            pla = L - 1 - s;
+            pbuf = buf;
+        };
+        if constexpr (PF) {
+            R rA[W], dA[W], rB[W], dB[W];
+            rows_at(0, rA, dA);
+            for (int s = 0; s < L; s += 2) {
+                rows_at(s + 1, rB, dB);
+                layer(s, rA, dA);
+                if (s + 1 < L) {
+                    rows_at(s + 2, rA, dA);
+                    layer(s + 1, rB, dB);
+                }
+            }
+        } else {
+            for (int s = 0; s < L; ++s) {
+                R r[W], d[W];
+                rows_at(s, r, d);
+                layer(s, r, d);
+            }
+        }
+        // The last layer, converted after its barrier.
+        if (L > 0) {
+            issue();
+            finish();
+        }
+        __syncthreads();  // the next tile reuses the buffers
+        first = false;
+    }
+}
+
+template <class T, int W, int NW, bool PF>
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(W <= 8 ? 2 : 1))) k_hvp_nn(int N, int L, int B, const int* __restrict__ kind,
+                                                    const typename RowOf<T>::type* __restrict__ rows_r,
+                                                    const typename RowOf<T>::type* __restrict__ rows_d,
+                                                    const typename KOf<T>::type* __restrict__ ktab,
+                                                    const typename KOf<T>::type* __restrict__ dktab,
+                                                    float* __restrict__ part_g, float* __restrict__ part_h,
+                                                    size_t part_stride, long ntiles, const T* u_out, const T* du_out,
+                                                    const T* g_out, const T* dg_out, const T* target, int ldu) {
+    hvp_nn_body<T, W, NW, PF>(N, L, B, kind, rows_r, rows_d, ktab, dktab, part_g, part_h, part_stride, ntiles, u_out,
+                              du_out, g_out, dg_out, target, ldu);
+}
+
+// out[l*ldp + m*np + k] += sum over workgroups (in order) of partial[wg][(l*NPh + pos)*np + k],
+// m < N/2, pos = m (kso == 0) or the stage-order position (m % kso) * 64 + m / kso.
 __global__ void __launch_bounds__(256) k_hvp_reduce(int L, int Nh, int NPh, int np, int nwg, size_t stride,
                                                     const float* __restrict__ partial, float* __restrict__ out,
-                                                    int ldp) {
+                                                    int ldp, int kso) {
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long)L * Nh * np) return;
     const int k = idx % np;
     const long lm = idx / np;
     const int m = lm % Nh;
     const int l = lm / Nh;
-    const float* src = partial + ((size_t)l * NPh + m) * np + k;
+    const int pos = kso > 0 ? (m % kso) * 64 + m / kso : m;
+    const float* src = partial + ((size_t)l * NPh + pos) * np + k;
     float a0 = 0.f, a1 = 0.f;
     int w = 0;
     for (; w + 1 < nwg; w += 2) {
@@ -310,44 +550,70 @@ using namespace mp;
 namespace {
 
 struct HvpPlan {
-    int W, LPV, NP, C, grid;
+    int W, LPV, NP, C, grid, nn, nw, kso, np;
     long ntiles;
-    size_t E, off_kind, off_r, off_d, off_t1, off_t2, off_pg, off_ph, total;
+    size_t E, rowbytes, kbytes, shmem, off_kind, off_r, off_d, off_t1, off_t2, off_pg, off_ph, total;
 };
 
 inline size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-int hvp_plan(int X, int N, int L, int B, HvpPlan& P) {
-    if (X != X_MZI && X != X_SYM) return (X < 0 || X > 3) ? MESHPROP_ERR_ARG : MESHPROP_ERR_UNSUPPORTED;
+// Waves per workgroup of the NN walk: eight for the narrow tiles, four for W >= 8 (the stage of
+// Q and dQ entries then takes 64 KiB per workgroup at W = 8, 128 KiB at W = 16).
+constexpr int hvp_nn_waves(int W) { return W <= 4 ? 8 : 4; }
+
+int hvp_plan(int X, int N, int L, int B, int fft, HvpPlan& P) {
+    if (X < 0 || X > 3) return MESHPROP_ERR_ARG;
     if (N <= 0 || (N & 1) || L < 0 || B < 0) return MESHPROP_ERR_ARG;
-    if (N > 512) return MESHPROP_ERR_UNSUPPORTED;
-    P.W = N <= 128 ? 2 : N <= 256 ? 4 : 8;
-    int lpv = 1;
-    while (lpv < (N + P.W - 1) / P.W) lpv <<= 1;
-    P.LPV = lpv;
-    P.NP = lpv * P.W;
-    const int G = 64 / lpv;
-    P.ntiles = std::max<long>(1, ((long)B + kHvpWaves * G - 1) / (kHvpWaves * G));
+    if (X == X_GENERIC) return MESHPROP_ERR_UNSUPPORTED;
+    P.nn = !fft;
+    P.np = n_phase_of(X);
+    const bool real = X == X_ORTH;
+    P.rowbytes = real ? sizeof(float2) : sizeof(float4);
+    P.kbytes = real ? sizeof(float) : sizeof(float4);
     int dev = 0, ncu = 256;
     if (hipGetDevice(&dev) == hipSuccess)
         if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0) ncu = 256;
-    P.grid = (int)std::min<long>(P.ntiles, 2L * ncu);
-    const size_t per_layer = (size_t)kHvpWaves * 2 * (P.NP / 2) * sizeof(float4);  // Q and dQ per slot
-    P.C = (int)std::max<size_t>(1, std::min<size_t>(8, kHvpStage / per_layer));
-    if ((size_t)P.C * per_layer > kHvpStage) return MESHPROP_ERR_UNSUPPORTED;
+    if (P.nn) {
+        if (N > 1024) return MESHPROP_ERR_UNSUPPORTED;
+        P.W = N <= 128 ? 2 : N <= 256 ? 4 : N <= 512 ? 8 : 16;
+        P.LPV = 64;
+        P.NP = 64 * P.W;
+        P.nw = hvp_nn_waves(P.W);
+        P.kso = P.W / 2;
+        P.C = 1;
+        P.ntiles = std::max<long>(1, ((long)B + P.nw - 1) / P.nw);
+        P.grid = (int)std::min<long>(P.ntiles, (P.W >= 16 ? 1L : 2L) * ncu);
+        P.shmem = (size_t)2 * P.nw * 2 * (P.NP / 2) * P.rowbytes;
+    } else {
+        if (real || N > 512) return MESHPROP_ERR_UNSUPPORTED;
+        P.W = N <= 128 ? 2 : N <= 256 ? 4 : 8;
+        int lpv = 1;
+        while (lpv < (N + P.W - 1) / P.W) lpv <<= 1;
+        P.LPV = lpv;
+        P.NP = lpv * P.W;
+        P.nw = kHvpWaves;
+        P.kso = 0;
+        const int G = 64 / lpv;
+        P.ntiles = std::max<long>(1, ((long)B + kHvpWaves * G - 1) / (kHvpWaves * G));
+        P.grid = (int)std::min<long>(P.ntiles, 2L * ncu);
+        const size_t per_layer = (size_t)kHvpWaves * 2 * (P.NP / 2) * sizeof(float4);  // Q and dQ per slot
+        P.C = (int)std::max<size_t>(1, std::min<size_t>(8, kHvpStage / per_layer));
+        if ((size_t)P.C * per_layer > kHvpStage) return MESHPROP_ERR_UNSUPPORTED;
+        P.shmem = (size_t)P.C * per_layer;
+    }
     const size_t NPh = P.NP / 2;
-    P.E = (size_t)L * NPh * 2;
+    P.E = (size_t)L * NPh * P.np;
     size_t off = 0;
     P.off_kind = off;
     off = al256(off + (size_t)L * sizeof(int));
     P.off_r = off;
-    off = al256(off + (size_t)L * P.NP * sizeof(float4));
+    off = al256(off + (size_t)L * P.NP * P.rowbytes);
     P.off_d = off;
-    off = al256(off + (size_t)L * P.NP * sizeof(float4));
+    off = al256(off + (size_t)L * P.NP * P.rowbytes);
     P.off_t1 = off;
-    off = al256(off + (size_t)L * NPh * 8 * sizeof(float2));
+    off = al256(off + (size_t)L * NPh * P.np * P.kbytes);
     P.off_t2 = off;
-    off = al256(off + (size_t)L * NPh * 8 * sizeof(float2));
+    off = al256(off + (size_t)L * NPh * P.np * P.kbytes);
     P.off_pg = off;
     off = al256(off + (size_t)P.grid * P.E * sizeof(float));
     P.off_ph = off;
@@ -359,8 +625,7 @@ int hvp_plan(int X, int N, int L, int B, HvpPlan& P) {
 template <int W>
 void launch_hvp(const HvpPlan& P, int N, int L, int B, unsigned char* base, const void* u_out, const void* du_out,
                 const void* g, const void* dg, const void* target, int ldu, hipStream_t st) {
-    const size_t shmem = (size_t)kHvpWaves * P.C * 2 * (P.NP / 2) * sizeof(float4);
-    hipLaunchKernelGGL((k_hvp<W>), dim3(P.grid), dim3(64 * kHvpWaves), shmem, st, N, L, B, P.LPV, P.NP, P.C,
+    hipLaunchKernelGGL((k_hvp<W>), dim3(P.grid), dim3(64 * kHvpWaves), P.shmem, st, N, L, B, P.LPV, P.NP, P.C,
                        (const int*)(base + P.off_kind), (const float4*)(base + P.off_r),
                        (const float4*)(base + P.off_d), (const float4*)(base + P.off_t1),
                        (const float4*)(base + P.off_t2), (float*)(base + P.off_pg), (float*)(base + P.off_ph), P.E,
@@ -368,14 +633,42 @@ void launch_hvp(const HvpPlan& P, int N, int L, int B, unsigned char* base, cons
                        (const float2*)target, ldu);
 }
 
+template <class T, int W>
+int launch_hvp_nn(const HvpPlan& P, int N, int L, int B, unsigned char* base, bool with_grad, const void* u_out,
+                  const void* du_out, const void* g, const void* dg, const void* target, int ldu, hipStream_t st) {
+    typedef typename RowOf<T>::type R;
+    typedef typename KOf<T>::type KT;
+    constexpr int NW = hvp_nn_waves(W);
+    constexpr bool PF = W <= 4;
+    static std::atomic<unsigned long long> lds_set{0};
+    if (int rc = lds_limit_once((const void*)k_hvp_nn<T, W, NW, PF>, lds_set, P.shmem)) return rc;
+    hipLaunchKernelGGL((k_hvp_nn<T, W, NW, PF>), dim3(P.grid), dim3(64 * NW), P.shmem, st, N, L, B,
+                       (const int*)(base + P.off_kind), (const R*)(base + P.off_r), (const R*)(base + P.off_d),
+                       (const KT*)(base + P.off_t1), (const KT*)(base + P.off_t2),
+                       with_grad ? (float*)(base + P.off_pg) : nullptr, (float*)(base + P.off_ph), P.E, P.ntiles,
+                       (const T*)u_out, (const T*)du_out, (const T*)g, (const T*)dg, (const T*)target, ldu);
+    return MESHPROP_OK;
+}
+
+template <class T>
+int launch_hvp_nn_any(const HvpPlan& P, int N, int L, int B, unsigned char* base, bool with_grad, const void* u_out,
+                      const void* du_out, const void* g, const void* dg, const void* target, int ldu,
+                      hipStream_t st) {
+    switch (P.W) {
+        case 2: return launch_hvp_nn<T, 2>(P, N, L, B, base, with_grad, u_out, du_out, g, dg, target, ldu, st);
+        case 4: return launch_hvp_nn<T, 4>(P, N, L, B, base, with_grad, u_out, du_out, g, dg, target, ldu, st);
+        case 8: return launch_hvp_nn<T, 8>(P, N, L, B, base, with_grad, u_out, du_out, g, dg, target, ldu, st);
+        default: return launch_hvp_nn<T, 16>(P, N, L, B, base, with_grad, u_out, du_out, g, dg, target, ldu, st);
+    }
+}
+
 }  // namespace
 
 extern "C" {
 
 size_t meshprop_hvp_workspace_size(int X, int N, int L, int B, int fft) {
-    (void)fft;
     HvpPlan P;
-    return hvp_plan(X, N, L, B, P) == MESHPROP_OK ? P.total : 0;
+    return hvp_plan(X, N, L, B, fft, P) == MESHPROP_OK ? P.total : 0;
 }
 
 int meshprop_hvp(int X, int N, int L, int B, int fft, const int* geom0, const int* geom1, const float* p,
@@ -383,12 +676,12 @@ int meshprop_hvp(int X, int N, int L, int B, int fft, const int* geom0, const in
                  const void* dJdu_out, const void* d_dJdu_out, const void* target, float* grad, float* hvp, int ldu,
                  void* workspace, size_t workspace_bytes, void* stream) {
     HvpPlan P;
-    int rc = hvp_plan(X, N, L, B, P);
+    int rc = hvp_plan(X, N, L, B, fft, P);
     if (rc) return rc;
     if (B == 0 || L == 0) return MESHPROP_OK;
     if (ldu < N || !u_out || !du_out || !hvp || !p || !q || !geom0 || (!fft && !geom1)) return MESHPROP_ERR_ARG;
     if (!target && (!dJdu_out || !d_dJdu_out)) return MESHPROP_ERR_ARG;
-    if (ldp < (N / 2) * 2) return MESHPROP_ERR_ARG;
+    if (ldp < (N / 2) * P.np) return MESHPROP_ERR_ARG;
     const int ns = n_split_of(X);
     if (s && ns && lds < (N / 2) * ns) return MESHPROP_ERR_ARG;
     if (!workspace || workspace_bytes < P.total || (((uintptr_t)workspace) & 255)) return MESHPROP_ERR_WORKSPACE;
@@ -396,29 +689,47 @@ int meshprop_hvp(int X, int N, int L, int B, int fft, const int* geom0, const in
     hipStream_t st = (hipStream_t)stream;
     unsigned char* base = (unsigned char*)workspace;
     const float* sp = ns ? s : nullptr;
-    hipLaunchKernelGGL((k_rows<float4>), dim3((P.NP / 2 + 255) / 256, L), dim3(256), 0, st, X, N, L, P.NP, P.LPV, fft,
-                       geom0, geom1, p, ldp, sp, lds, q, (int)(TAB_INV | TAB_DIFF_ADJ | TAB_KUNI),
-                       (int*)(base + P.off_kind), (float4*)(base + P.off_r), (float4*)(base + P.off_d),
-                       (float2*)(base + P.off_t1), nullptr, 0);
-    if (hipGetLastError() != hipSuccess) return MESHPROP_ERR_HIP;
+    const int tabs = TAB_INV | TAB_DIFF_ADJ | TAB_KUNI;
     const dim3 g2((P.NP / 2 + 255) / 256, L);
-    hipLaunchKernelGGL(k_dkform, g2, dim3(256), 0, st, X, N, L, P.NP, fft, geom0, geom1, p, q, ldp, sp, lds,
-                       (float4*)(base + P.off_t2));
-    if (hipGetLastError() != hipSuccess) return MESHPROP_ERR_HIP;
-    switch (P.W) {
-        case 2: launch_hvp<2>(P, N, L, B, base, u_out, du_out, dJdu_out, d_dJdu_out, target, ldu, st); break;
-        case 4: launch_hvp<4>(P, N, L, B, base, u_out, du_out, dJdu_out, d_dJdu_out, target, ldu, st); break;
-        default: launch_hvp<8>(P, N, L, B, base, u_out, du_out, dJdu_out, d_dJdu_out, target, ldu, st); break;
+    if (X == X_ORTH) {
+        hipLaunchKernelGGL((k_rows<float2>), g2, dim3(256), 0, st, X, N, L, P.NP, P.LPV, fft, geom0, geom1, p, ldp,
+                           sp, lds, q, tabs, (int*)(base + P.off_kind), (float2*)(base + P.off_r),
+                           (float2*)(base + P.off_d), (float*)(base + P.off_t1), nullptr, P.kso);
+        if (hipGetLastError() != hipSuccess) return MESHPROP_ERR_HIP;
+        hipLaunchKernelGGL(k_dkform<float>, g2, dim3(256), 0, st, X, N, L, P.NP, fft, geom0, geom1, p, q, ldp, sp,
+                           lds, (float*)(base + P.off_t2), P.kso);
+    } else {
+        hipLaunchKernelGGL((k_rows<float4>), g2, dim3(256), 0, st, X, N, L, P.NP, P.LPV, fft, geom0, geom1, p, ldp,
+                           sp, lds, q, tabs, (int*)(base + P.off_kind), (float4*)(base + P.off_r),
+                           (float4*)(base + P.off_d), (float2*)(base + P.off_t1), nullptr, P.kso);
+        if (hipGetLastError() != hipSuccess) return MESHPROP_ERR_HIP;
+        hipLaunchKernelGGL(k_dkform<float4>, g2, dim3(256), 0, st, X, N, L, P.NP, fft, geom0, geom1, p, q, ldp, sp,
+                           lds, (float4*)(base + P.off_t2), P.kso);
     }
     if (hipGetLastError() != hipSuccess) return MESHPROP_ERR_HIP;
-    const long total = (long)L * (N / 2) * 2;
+    if (P.nn) {
+        rc = (X == X_ORTH)
+                 ? launch_hvp_nn_any<float>(P, N, L, B, base, grad != nullptr, u_out, du_out, dJdu_out, d_dJdu_out,
+                                            target, ldu, st)
+                 : launch_hvp_nn_any<float2>(P, N, L, B, base, grad != nullptr, u_out, du_out, dJdu_out, d_dJdu_out,
+                                             target, ldu, st);
+        if (rc) return rc;
+    } else {
+        switch (P.W) {
+            case 2: launch_hvp<2>(P, N, L, B, base, u_out, du_out, dJdu_out, d_dJdu_out, target, ldu, st); break;
+            case 4: launch_hvp<4>(P, N, L, B, base, u_out, du_out, dJdu_out, d_dJdu_out, target, ldu, st); break;
+            default: launch_hvp<8>(P, N, L, B, base, u_out, du_out, dJdu_out, d_dJdu_out, target, ldu, st); break;
+        }
+    }
+    if (hipGetLastError() != hipSuccess) return MESHPROP_ERR_HIP;
+    const long total = (long)L * (N / 2) * P.np;
     const dim3 gr((unsigned)((total + 255) / 256));
     if (grad) {
-        hipLaunchKernelGGL(k_hvp_reduce, gr, dim3(256), 0, st, L, N / 2, P.NP / 2, 2, P.grid, P.E,
-                           (const float*)(base + P.off_pg), grad, ldp);
+        hipLaunchKernelGGL(k_hvp_reduce, gr, dim3(256), 0, st, L, N / 2, P.NP / 2, P.np, P.grid, P.E,
+                           (const float*)(base + P.off_pg), grad, ldp, P.kso);
     }
-    hipLaunchKernelGGL(k_hvp_reduce, gr, dim3(256), 0, st, L, N / 2, P.NP / 2, 2, P.grid, P.E,
-                       (const float*)(base + P.off_ph), hvp, ldp);
+    hipLaunchKernelGGL(k_hvp_reduce, gr, dim3(256), 0, st, L, N / 2, P.NP / 2, P.np, P.grid, P.E,
+                       (const float*)(base + P.off_ph), hvp, ldp, P.kso);
     if (hipGetLastError() != hipSuccess) return MESHPROP_ERR_HIP;
     return MESHPROP_OK;
 }

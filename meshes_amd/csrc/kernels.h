@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <type_traits>
 
 #include "crossing.h"
@@ -1878,6 +1879,20 @@ inline bool nn_adjoint(const Plan& P) {
 // A HIP error left pending on the calling thread by earlier, unrelated work is cleared before
 // an entry point's first launch, so the launch checks that follow report this call's own.
 inline void clear_pending_hip_error() { (void)hipGetLastError(); }
+
+// Raises a kernel's dynamic LDS limit above the default 64 KiB once per device (the attribute
+// persists), and reports a failure instead of launching into it.
+inline int lds_limit_once(const void* fn, std::atomic<unsigned long long>& done, size_t shmem) {
+    if (shmem <= 64 * 1024) return MESHPROP_OK;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return MESHPROP_ERR_HIP;
+    const unsigned long long bit = 1ull << dev;
+    if (done.load(std::memory_order_acquire) & bit) return MESHPROP_OK;
+    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem) != hipSuccess)
+        return MESHPROP_ERR_HIP;
+    done.fetch_or(bit, std::memory_order_acq_rel);
+    return MESHPROP_OK;
+}
 
 inline size_t stage_layer_bytes(const Plan& P) { return (size_t)(P.uni ? P.NP / 2 : P.NP) * P.rowbytes; }
 

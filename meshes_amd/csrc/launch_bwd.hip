@@ -3,8 +3,6 @@
 // except ilp_tile() and forwards those to launch_bwd_ilp.hip (MP_ILP_SET = 1, iterative-ilp
 // scheduler); launch_bwd_sc.hip (MP_SC_SET = 1) holds the scaled-unitary adjoints of every
 // complex tile.
-#include <atomic>
-
 #include "kernels.h"
 #include "tiles.h"
 
@@ -13,20 +11,6 @@
 #endif
 
 namespace mp {
-
-// Raises a kernel's dynamic LDS limit above the default 64 KiB once per device (the attribute
-// persists), and reports a failure instead of launching into it.
-static int lds_limit_once(const void* fn, std::atomic<unsigned long long>& done, size_t shmem) {
-    if (shmem <= 64 * 1024) return MESHPROP_OK;
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return MESHPROP_ERR_HIP;
-    const unsigned long long bit = 1ull << dev;
-    if (done.load(std::memory_order_acquire) & bit) return MESHPROP_OK;
-    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem) != hipSuccess)
-        return MESHPROP_ERR_HIP;
-    done.fetch_or(bit, std::memory_order_acq_rel);
-    return MESHPROP_OK;
-}
 
 template <class T, int W, int V, bool UNI, int PIPE, bool SC>
 static int go1(const Plan& P, const int* kind, const void* ri, const void* ra, const void* dtdp, float* partial,

@@ -394,8 +394,13 @@ class MeshNetworkGPU:
 
     # ---- Hessian-vector products (gpu/mesh.py:236-288) --------------------------------
     def exact_hvp_supported(self) -> bool:
-        """Forward-over-reverse HVP kernel: unitary complex crossings, N <= 512, no phase screen."""
-        return self.X in ("mzi", "sym") and self.N <= 512 and not self.is_phase
+        """Forward-over-reverse HVP kernel: unitary crossings (MZI, SYM; orth on NN meshes) up to
+        N = 1024 on NN meshes and N = 512 on FFT meshes, no phase screen."""
+        if self.is_phase or self.X not in ("mzi", "sym", "orth"):
+            return False
+        if self.fft:
+            return self.X != "orth" and self.N <= 512
+        return self.N <= 1024
 
     def _hvp_exact(self, q, out, dout, g, dg, p, s):
         B = out.shape[0]

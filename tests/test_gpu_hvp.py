@@ -1,9 +1,10 @@
 """Exact Hessian-vector products (forward-over-reverse kernel) against the FP64 oracle.
 
 The oracle's hvp_inverse is pinned by central differences of its gradient; here the GPU result
-(meshprop_hvp after fwddiff along q) must match it within 1e-5 relative L2, for MZI and SYM
-crossings on NN and FFT geometries and N up to 512, and MeshNetworkGPU.hvp_target / hess_target
-must agree with the oracle and be symmetric.
+(meshprop_hvp after fwddiff along q) must match it within 1e-5 relative L2: MZI and SYM on NN
+meshes up to N = 1024 (every NN tile width, W = 2 .. 16) and on FFT meshes up to N = 512, orth
+(real fields) on NN meshes, several vector tiles per workgroup; MeshNetworkGPU.hvp_target /
+hess_target must agree with the oracle and be symmetric.
 """
 import zlib
 
@@ -17,7 +18,10 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-5
 CASES = [("mzi", "nn", 16, 9, 7), ("mzi", "clements", 64, 0, 20), ("sym", "nn", 40, 11, 13),
          ("mzi", "fft", 128, 7, 10), ("mzi", "clements", 256, 0, 6), ("mzi", "nn", 512, 12, 5),
-         ("sym", "fft", 256, 8, 9)]
+         ("sym", "fft", 256, 8, 9), ("mzi", "nn", 130, 5, 4200), ("sym", "nn", 300, 10, 70),
+         ("mzi", "nn", 700, 9, 6), ("sym", "nn", 1000, 7, 5), ("mzi", "clements", 1024, 0, 3),
+         ("orth", "nn", 96, 14, 9), ("orth", "clements", 128, 0, 8), ("orth", "nn", 250, 11, 33),
+         ("orth", "nn", 600, 8, 4), ("orth", "nn", 1024, 6, 5)]
 
 
 def _geom(rng, kind, N, L):
@@ -39,10 +43,13 @@ def test_hvp_kernel_matches_oracle(gpu, case):
     p = rng.uniform(0, 2 * np.pi, (g.L, N // 2, n_p)).astype(np.float32)
     s = (0.05 * rng.standard_normal((g.L, N // 2, n_s))).astype(np.float32)
     q = rng.standard_normal(p.shape).astype(np.float32)
-    u = O.unit_cn(rng, B, N).astype(np.complex64)
-    t = O.unit_cn(rng, B, N).astype(np.complex64)
+    real = O.REAL_FIELD[X]
+    fdt = np.float32 if real else np.complex64
+    u = O.unit_cn(rng, B, N, real=real).astype(fdt)
+    t = O.unit_cn(rng, B, N, real=real).astype(fdt)
     kw = dict(strides=g.strides) if g.kind == "fft" else {}
     mesh = MeshNetworkGPU(N, g.L, g.lens, g.shifts, p, s, X, is_phase=False, **kw)
+    assert mesh.exact_hvp_supported()
     out, dout = mesh.dot(torch.as_tensor(u.T), dp=q)
     qt = torch.as_tensor(q.reshape(-1)).cuda()
     grad = torch.zeros(mesh.n_phase, dtype=torch.float32, device="cuda")

@@ -307,9 +307,10 @@ def test_d2T_finite_difference(X):
     assert np.allclose(d2, np.swapaxes(d2, 1, 2))
 
 
-@pytest.mark.parametrize("X,kind", [("mzi", "nn"), ("sym", "nn"), ("mzi", "fft")])
+@pytest.mark.parametrize("X,kind", [("mzi", "nn"), ("sym", "nn"), ("mzi", "fft"), ("orth", "nn")])
 def test_hvp_finite_difference_of_gradient(X, kind):
-    """hvp_inverse against central differences (h = 1e-5) of the FP64 gradient along q."""
+    """hvp_inverse against central differences (h = 1e-5) of the FP64 gradient along q (real
+    fields for orth)."""
     rng = np.random.RandomState(43)
     N, L, B = 12, 9, 5
     g = O.random_geometry_nn(rng, N, L) if kind == "nn" else O.Geometry(16, 6, strides=rng.randint(0, 4, size=6))
@@ -317,8 +318,8 @@ def test_hvp_finite_difference_of_gradient(X, kind):
     n_p = O.N_PHASE[X]
     p = rng.uniform(0, 2 * np.pi, (g.L, N // 2, n_p))
     s = 0.1 * rng.standard_normal((g.L, N // 2, O.N_SPLIT[X]))
-    u = O.unit_cn(rng, B, N)
-    t = O.unit_cn(rng, B, N)
+    u = O.unit_cn(rng, B, N, real=O.REAL_FIELD[X])
+    t = O.unit_cn(rng, B, N, real=O.REAL_FIELD[X])
     q = rng.standard_normal(p.shape)
 
     def grad(pp):
