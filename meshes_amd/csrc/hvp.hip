@@ -294,21 +294,39 @@ __global__ void __launch_bounds__(64 * kHvpWaves) __attribute__((amdgpu_waves_pe
 // t*W .. t*W + W - 1), the state updated in place; ODD: the layer pairs (2p+1, 2p+2) in the
 // lane and the lane's last waveguide with the next lane's first.  The Q and dQ entries of the
 // lane's slot p go to wq[p * 64] and wdq[p * 64] (stage order).
-template <class T, int W, bool ODD>
-__device__ __forceinline__ void hvp_layer_nn(T (&u)[W], T (&du)[W], T (&g)[W], T (&dg)[W],
-                                             const typename RowOf<T>::type (&ri)[W],
-                                             const typename RowOf<T>::type (&rd)[W], typename RowOf<T>::type* wq,
-                                             typename RowOf<T>::type* wdq) {
+// Row sources of one layer: registers loaded beforehand (RowRegs), or buffer loads issued where
+// a crossing needs them (RowLoads; the compiler schedules them against the register budget).
+template <class R, int W>
+struct RowRegs {
+    const R (&ri)[W];
+    const R (&rd)[W];
+    __device__ __forceinline__ R r(const int k) const { return ri[k]; }
+    __device__ __forceinline__ R d(const int k) const { return rd[k]; }
+};
+template <class R>
+struct RowLoads {
+    __amdgpu_buffer_rsrc_t rr, dr;
+    unsigned off;
+    int base;  // byte offset of the layer's k = 0 entries
+    __device__ __forceinline__ R r(const int k) const { return buf_load<R>(rr, off, base + k * 64 * (int)sizeof(R)); }
+    __device__ __forceinline__ R d(const int k) const { return buf_load<R>(dr, off, base + k * 64 * (int)sizeof(R)); }
+};
+
+template <class T, int W, bool ODD, class RS>
+__device__ __forceinline__ void hvp_layer_nn(T (&u)[W], T (&du)[W], T (&g)[W], T (&dg)[W], const RS& rs,
+                                             typename RowOf<T>::type* wq, typename RowOf<T>::type* wdq) {
+    typedef typename RowOf<T>::type R;
     constexpr int H = W / 2, ES = 64;
     const T z = fzero(T());
     // In-lane crossing of waveguides k (upper) and j = k + 1, entry p.
     auto pair = [&](const int k, const int j, const int p) __attribute__((always_inline)) {
-        const T xk = rapply(ri[k], u[k], u[j]), xj = rapply(ri[j], u[j], u[k]);
-        const T dxk = rapply_d(ri[k], rd[k], u[k], u[j], du[k], du[j]);
-        const T dxj = rapply_d(ri[j], rd[j], u[j], u[k], du[j], du[k]);
-        const T gk = rapply(ri[k], g[k], g[j]), gj = rapply(ri[j], g[j], g[k]);
-        const T dgk = rapply_d(ri[k], rd[k], g[k], g[j], dg[k], dg[j]);
-        const T dgj = rapply_d(ri[j], rd[j], g[j], g[k], dg[j], dg[k]);
+        const R rk = rs.r(k), rj = rs.r(j), ek = rs.d(k), ej = rs.d(j);
+        const T xk = rapply(rk, u[k], u[j]), xj = rapply(rj, u[j], u[k]);
+        const T dxk = rapply_d(rk, ek, u[k], u[j], du[k], du[j]);
+        const T dxj = rapply_d(rj, ej, u[j], u[k], du[j], du[k]);
+        const T gk = rapply(rk, g[k], g[j]), gj = rapply(rj, g[j], g[k]);
+        const T dgk = rapply_d(rk, ek, g[k], g[j], dg[k], dg[j]);
+        const T dgj = rapply_d(rj, ej, g[j], g[k], dg[j], dg[k]);
         wq[p * ES] = kquad(cjim(gk, xk, z), cjim(gj, xj, z), cjfma(gk, xj, z), cjfma(gj, xk, z));
         wdq[p * ES] = kquad(cjim(gk, dxk, cjim(dgk, xk, z)), cjim(gj, dxj, cjim(dgj, xj, z)),
                             cjfma(gk, dxj, cjfma(dgk, xj, z)), cjfma(gj, dxk, cjfma(dgj, xk, z)));
@@ -330,12 +348,13 @@ __device__ __forceinline__ void hvp_layer_nn(T (&u)[W], T (&du)[W], T (&g)[W], T
         const T hu = lane_next(u[0]), hdu = lane_next(du[0]), hg = lane_next(g[0]), hdg = lane_next(dg[0]);
         const T lu = lane_prev(u[W - 1]), ldu = lane_prev(du[W - 1]), lg = lane_prev(g[W - 1]),
                 ldg = lane_prev(dg[W - 1]);
-        const T xh = rapply(ri[W - 1], u[W - 1], hu), gh = rapply(ri[W - 1], g[W - 1], hg);
-        const T dxh = rapply_d(ri[W - 1], rd[W - 1], u[W - 1], hu, du[W - 1], hdu);
-        const T dgh = rapply_d(ri[W - 1], rd[W - 1], g[W - 1], hg, dg[W - 1], hdg);
-        const T xl = rapply(ri[0], u[0], lu), gl = rapply(ri[0], g[0], lg);
-        const T dxl = rapply_d(ri[0], rd[0], u[0], lu, du[0], ldu);
-        const T dgl = rapply_d(ri[0], rd[0], g[0], lg, dg[0], ldg);
+        const R rh = rs.r(W - 1), eh = rs.d(W - 1), rl = rs.r(0), el = rs.d(0);
+        const T xh = rapply(rh, u[W - 1], hu), gh = rapply(rh, g[W - 1], hg);
+        const T dxh = rapply_d(rh, eh, u[W - 1], hu, du[W - 1], hdu);
+        const T dgh = rapply_d(rh, eh, g[W - 1], hg, dg[W - 1], hdg);
+        const T xl = rapply(rl, u[0], lu), gl = rapply(rl, g[0], lg);
+        const T dxl = rapply_d(rl, el, u[0], lu, du[0], ldu);
+        const T dgl = rapply_d(rl, el, g[0], lg, dg[0], ldg);
 #pragma unroll
         for (int p = 0; p + 1 < H; ++p) pair(2 * p + 1, 2 * p + 2, p);
         u[W - 1] = xh;
@@ -360,10 +379,11 @@ __device__ __forceinline__ void hvp_layer_nn(T (&u)[W], T (&du)[W], T (&g)[W], T
 // HVP walk of NN meshes: NW waves per workgroup, one vector per wave, the layers walked
 // backwards with one-layer stages (buffer s & 1 of [2][NW][Q, dQ][NP/2] entries); while layer
 // s is applied, the workgroup converts layer s - 1 (its K and dK coefficients requested before
-// the layer, its NW entries summed in wave order after it).  PF: rows of the next layer are
-// requested one layer ahead (two row register sets); otherwise right before their layer.
-// part_g may be null (no gradient).
-template <class T, int W, int NW, bool PF>
+// the layer, its NW entries summed in wave order after it).  Rows: PF, those of the next
+// layer requested one layer ahead (two row register sets); JIT, each crossing's rows loaded
+// where it is applied; otherwise a whole layer's rows right before it.  part_g may be null (no
+// gradient).
+template <class T, int W, int NW, bool PF, bool JIT>
 __device__ __forceinline__ void hvp_nn_body(const int N, const int L, const int B, const int* __restrict__ kind,
                                             const typename RowOf<T>::type* __restrict__ rows_r,
                                             const typename RowOf<T>::type* __restrict__ rows_d,
@@ -463,14 +483,14 @@ __device__ __forceinline__ void hvp_nn_body(const int N, const int L, const int 
         // W >= 8: the coefficients are requested at the conversion (held across the layer
         // they push the tile past the registers of two waves per SIMD).
         constexpr bool LATE = W >= 8;
-        auto layer = [&](const int s, const R (&ri)[W], const R (&rd)[W]) __attribute__((always_inline)) {
+        auto layer = [&](const int s, const auto& rsrc) __attribute__((always_inline)) {
             R* buf = stage + (s & 1) * BSTRIDE;
             if (s > 0 && !LATE) issue();
             R* wq = buf + wave_u * WSTRIDE + lane;
             if (kind[L - 1 - s] & 1)
-                hvp_layer_nn<T, W, true>(u[0], du[0], g[0], dg[0], ri, rd, wq, wq + NPh);
+                hvp_layer_nn<T, W, true>(u[0], du[0], g[0], dg[0], rsrc, wq, wq + NPh);
             else
-                hvp_layer_nn<T, W, false>(u[0], du[0], g[0], dg[0], ri, rd, wq, wq + NPh);
+                hvp_layer_nn<T, W, false>(u[0], du[0], g[0], dg[0], rsrc, wq, wq + NPh);
             if (s > 0) {
                 if (LATE) issue();
                 finish();
@@ -484,17 +504,19 @@ __device__ __forceinline__ void hvp_nn_body(const int N, const int L, const int 
             rows_at(0, rA, dA);
             for (int s = 0; s < L; s += 2) {
                 rows_at(s + 1, rB, dB);
-                layer(s, rA, dA);
+                layer(s, RowRegs<R, W>{rA, dA});
                 if (s + 1 < L) {
                     rows_at(s + 2, rA, dA);
-                    layer(s + 1, rB, dB);
+                    layer(s + 1, RowRegs<R, W>{rB, dB});
                 }
             }
+        } else if constexpr (JIT) {
+            for (int s = 0; s < L; ++s) layer(s, RowLoads<R>{rrs, drs, roff, (L - 1 - s) * NP * (int)sizeof(R)});
         } else {
             for (int s = 0; s < L; ++s) {
                 R r[W], d[W];
                 rows_at(s, r, d);
-                layer(s, r, d);
+                layer(s, RowRegs<R, W>{r, d});
             }
         }
         // The last layer, converted after its barrier.
@@ -507,7 +529,7 @@ __device__ __forceinline__ void hvp_nn_body(const int N, const int L, const int 
     }
 }
 
-template <class T, int W, int NW, bool PF>
+template <class T, int W, int NW, bool PF, bool JIT>
 __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(W <= 8 ? 2 : 1))) k_hvp_nn(int N, int L, int B, const int* __restrict__ kind,
                                                     const typename RowOf<T>::type* __restrict__ rows_r,
                                                     const typename RowOf<T>::type* __restrict__ rows_d,
@@ -516,7 +538,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(W 
                                                     float* __restrict__ part_g, float* __restrict__ part_h,
                                                     size_t part_stride, long ntiles, const T* u_out, const T* du_out,
                                                     const T* g_out, const T* dg_out, const T* target, int ldu) {
-    hvp_nn_body<T, W, NW, PF>(N, L, B, kind, rows_r, rows_d, ktab, dktab, part_g, part_h, part_stride, ntiles, u_out,
+    hvp_nn_body<T, W, NW, PF, JIT>(N, L, B, kind, rows_r, rows_d, ktab, dktab, part_g, part_h, part_stride, ntiles, u_out,
                               du_out, g_out, dg_out, target, ldu);
 }
 
@@ -639,10 +661,17 @@ int launch_hvp_nn(const HvpPlan& P, int N, int L, int B, unsigned char* base, bo
     typedef typename RowOf<T>::type R;
     typedef typename KOf<T>::type KT;
     constexpr int NW = hvp_nn_waves(W);
+    // Row sources, as measured (MI355X, Clements L = N): rows of the next layer prefetched at
+    // W <= 4 (MZI N=256 B=4096 HVP 1.04 ms against 1.13 with per-crossing loads); per-crossing
+    // loads at W = 8 (no spills at two waves per SIMD: N=512 5.77 -> 4.59 ms) and for real
+    // W = 16 fields (N=1024 orth 8.73 -> 7.65 ms); the complex W = 16 tile (one wave per SIMD,
+    // bound by its 128 KiB stage) keeps a whole layer of rows in registers (15.5 ms against 18.2
+    // with per-crossing loads and 20.8 with only the tangent rows loaded per crossing).
     constexpr bool PF = W <= 4;
+    constexpr bool JIT = W == 8 || (W >= 16 && sizeof(T) == sizeof(float));
     static std::atomic<unsigned long long> lds_set{0};
-    if (int rc = lds_limit_once((const void*)k_hvp_nn<T, W, NW, PF>, lds_set, P.shmem)) return rc;
-    hipLaunchKernelGGL((k_hvp_nn<T, W, NW, PF>), dim3(P.grid), dim3(64 * NW), P.shmem, st, N, L, B,
+    if (int rc = lds_limit_once((const void*)k_hvp_nn<T, W, NW, PF, JIT>, lds_set, P.shmem)) return rc;
+    hipLaunchKernelGGL((k_hvp_nn<T, W, NW, PF, JIT>), dim3(P.grid), dim3(64 * NW), P.shmem, st, N, L, B,
                        (const int*)(base + P.off_kind), (const R*)(base + P.off_r), (const R*)(base + P.off_d),
                        (const KT*)(base + P.off_t1), (const KT*)(base + P.off_t2),
                        with_grad ? (float*)(base + P.off_pg) : nullptr, (float*)(base + P.off_ph), P.E, P.ntiles,
