@@ -16,14 +16,15 @@
 namespace mp {
 
 // Kernel families: family 0 holds the tiles with W <= 8 in four-wave workgroups (two per CU),
-// family 1 the W = 16 tile in two-wave workgroups, whose registers allow one wave per SIMD.
+// family 1 the W = 16 tile in four-wave workgroups (one per CU: its registers allow one wave
+// per SIMD); NN meshes there run the LDS-staged forward and the NN-specialised adjoint.
 // Variant v of the persistent kernels: (W, V) = kGroupTiles[v].
 struct GTile {
     int w, v, fam;
 };
 constexpr GTile kGroupTiles[] = {{2, 1, 0}, {2, 2, 0}, {2, 4, 0}, {4, 1, 0}, {4, 2, 0}, {8, 1, 0}, {16, 1, 1}};
 constexpr int kGroupVariants = sizeof(kGroupTiles) / sizeof(kGroupTiles[0]);
-constexpr int kFamWaves[2] = {4, 2};
+constexpr int kFamWaves[2] = {4, 4};
 static_assert(kFamWaves[0] <= kStageWaves && kFamWaves[1] <= kStageWaves, "stage conversion sums at most kStageWaves waves");
 constexpr size_t kGroupStage = 64 * 1024;  // LDS of one workgroup (backdiff stage)
 constexpr size_t kGroupLds = kGroupStage + 16;  // stage + the queue slot
@@ -169,10 +170,23 @@ __global__ void __launch_bounds__(256) k_group_reduce(const unsigned char* __res
 // The forward tile runs as a called function: inlined beside the adjoint it raises the
 // kernel's register allocation above 256 (one wave per SIMD instead of two).  Its arguments
 // arrive in vector registers and are made scalar again with readfirstlane.
+// W = 16 NN meshes (kso > 0: one 64-lane vector per wave) run the forward with their rows
+// staged through LDS (fwd_tile_nn, its three-slot ring in the stage buffer, which no adjoint
+// uses until the forward's last barrier): each wave reading 16 KiB of rows per layer through
+// the address unit bounds the general tile.
 template <int W, int V>
-__device__ __noinline__ void group_fwd_tile(KMesh* Mp, const long tile_v) {
+__device__ __noinline__ void group_fwd_tile(KMesh* Mp, const long tile_v, float4* stage) {
     KMesh& M = *uniform_ptr(Mp);
     const long tile = (long)__builtin_amdgcn_readfirstlane((unsigned)tile_v);
+    if constexpr (W == 16 && V == 1) {
+        static_assert(3 * 64 * W * sizeof(float4) <= kGroupStage, "forward row ring inside the stage buffer");
+        static_assert(kFamWaves[1] == 4, "fwd_tile_nn copies rows with 256 threads");
+        if (M.kso > 0) {
+            fwd_tile_nn<float2, 16, 1>(M.N, 0, M.L, M.B, (KInt*)M.kind, M.rows_fwd, M.u_in, M.u_out, M.ldu, tile,
+                                       Ckpt{nullptr, 0, 0}, stage);
+            return;
+        }
+    }
     fwd_tile<float2, W, V, KInt*>(M.N, 0, M.L, M.B, M.LPV, M.NP, (KInt*)M.kind, M.rows_fwd, M.u_in, M.u_out, M.ldu,
                                   tile);
 }
@@ -183,7 +197,7 @@ __device__ __forceinline__ void group_item(KMesh& M, const int slot, float4* sta
     bool first = true;
     for (long tile = slot; tile < M.ntiles; tile += M.S) {
         if (M.what & G_FWD)
-            group_fwd_tile<W, V>(&M, tile);
+            group_fwd_tile<W, V>(&M, tile, stage);
         if (M.what & G_GRAD) {
             if constexpr (W == 16 && V == 1) {
                 if (M.kso > 0) {
