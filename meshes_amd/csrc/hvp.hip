@@ -542,29 +542,6 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(W 
                               du_out, g_out, dg_out, target, ldu);
 }
 
-// out[l*ldp + m*np + k] += sum over workgroups (in order) of partial[wg][(l*NPh + pos)*np + k],
-// m < N/2, pos = m (kso == 0) or the stage-order position (m % kso) * 64 + m / kso.
-__global__ void __launch_bounds__(256) k_hvp_reduce(int L, int Nh, int NPh, int np, int nwg, size_t stride,
-                                                    const float* __restrict__ partial, float* __restrict__ out,
-                                                    int ldp, int kso) {
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long)L * Nh * np) return;
-    const int k = idx % np;
-    const long lm = idx / np;
-    const int m = lm % Nh;
-    const int l = lm / Nh;
-    const int pos = kso > 0 ? (m % kso) * 64 + m / kso : m;
-    const float* src = partial + ((size_t)l * NPh + pos) * np + k;
-    float a0 = 0.f, a1 = 0.f;
-    int w = 0;
-    for (; w + 1 < nwg; w += 2) {
-        a0 += src[(size_t)w * stride];
-        a1 += src[(size_t)(w + 1) * stride];
-    }
-    if (w < nwg) a0 += src[(size_t)w * stride];
-    out[(size_t)l * ldp + (size_t)m * np + k] += a0 + a1;
-}
-
 }  // namespace mp
 
 using namespace mp;
@@ -751,16 +728,12 @@ int meshprop_hvp(int X, int N, int L, int B, int fft, const int* geom0, const in
         }
     }
     if (hipGetLastError() != hipSuccess) return MESHPROP_ERR_HIP;
-    const long total = (long)L * (N / 2) * P.np;
-    const dim3 gr((unsigned)((total + 255) / 256));
-    if (grad) {
-        hipLaunchKernelGGL(k_hvp_reduce, gr, dim3(256), 0, st, L, N / 2, P.NP / 2, P.np, P.grid, P.E,
-                           (const float*)(base + P.off_pg), grad, ldp, P.kso);
-    }
-    hipLaunchKernelGGL(k_hvp_reduce, gr, dim3(256), 0, st, L, N / 2, P.NP / 2, P.np, P.grid, P.E,
-                       (const float*)(base + P.off_ph), hvp, ldp, P.kso);
-    if (hipGetLastError() != hipSuccess) return MESHPROP_ERR_HIP;
-    return MESHPROP_OK;
+    // The partials are summed in workgroup order (the adjoint's reduction, accumulating).
+    if (grad && (rc = launch_dp_reduce(L, N / 2, P.NP / 2, P.np, P.grid, P.E, (const float*)(base + P.off_pg), grad,
+                                       ldp, 0, P.kso, st)))
+        return rc;
+    return launch_dp_reduce(L, N / 2, P.NP / 2, P.np, P.grid, P.E, (const float*)(base + P.off_ph), hvp, ldp, 0,
+                            P.kso, st);
 }
 
 }  // extern "C"

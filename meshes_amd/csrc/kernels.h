@@ -1908,6 +1908,10 @@ int launch_fwddiff_any(const Plan& P, const int* kind, const void* rows, const v
 int launch_bwd_any(const Plan& P, const int* kind, const void* ri, const void* ra, const void* dtdp, float* partial,
                    size_t part_stride, int l0, int nl, const void* u_out, const void* g_out, const void* target,
                    void* u_in, void* g_in, int ldu, hipStream_t st, Ckpt ck = Ckpt{nullptr, 0, 0});
+// dp (+)= the per-workgroup partials (nwg of them, stride E floats) in fixed order; kso > 0:
+// partials in the NN adjoint's stage order.  Defined with k_dp_reduce in meshprop.hip.
+int launch_dp_reduce(int L, int Nh, int NPh, int np, int nwg, size_t E, const float* partial, float* dp, int ldp,
+                     int assign, int kso, hipStream_t st);
 // The tiles compiled with an ILP-oriented scheduler (one object per direction).
 int launch_fwd_ilp(const Plan& P, const int* kind, const void* rows, int l0, int nl, const void* u_in, void* u_out,
                    int ldu, hipStream_t st, Ckpt ck);

@@ -71,6 +71,17 @@ __global__ void __launch_bounds__(256) k_dp_reduce(int L, int Nh, int NPh, int n
     }
 }
 
+// Host launcher of k_dp_reduce for the other translation units (the HVP reductions):
+// dp[l*ldp + m*np + k] (+)= the nwg partials (stride E) of slot m, in workgroup order.
+int launch_dp_reduce(int L, int Nh, int NPh, int np, int nwg, size_t E, const float* partial, float* dp, int ldp,
+                     int assign, int kso, hipStream_t st) {
+    const long total = (long)L * (kso > 0 ? NPh : Nh) * np;
+    if (total <= 0) return MESHPROP_OK;
+    hipLaunchKernelGGL(k_dp_reduce, dim3((unsigned)((total + 63) / 64)), dim3(256), 0, st, L, Nh, NPh, np, nwg, E,
+                       partial, dp, ldp, assign, kso);
+    return hipGetLastError() == hipSuccess ? MESHPROP_OK : MESHPROP_ERR_HIP;
+}
+
 }  // namespace mp
 
 
