@@ -257,15 +257,16 @@ int meshprop_group_upload(const meshprop_group *g, void *workspace, size_t works
  * uploaded into the workspace.  The launch shape depends on the device and on
  * meshprop_group_signature() only. */
 int meshprop_group_launch(const meshprop_group *g, void *workspace, size_t workspace_bytes, void *stream);
-/* Launch-shape signature (MESHPROP_GROUP_SIG_* bits: which of the prologue, the two
+/* Launch-shape signature (MESHPROP_GROUP_SIG_* bits: which of the prologue, the three
  * persistent kernel families and the reduction the plan launches).  Plans with equal
  * signatures uploaded into the same workspace replay the same captured graph. */
 int meshprop_group_signature(const meshprop_group *g);
 enum meshprop_group_signature_bits {
-    MESHPROP_GROUP_SIG_NARROW = 1,    /* items of meshes with N <= 512 */
-    MESHPROP_GROUP_SIG_WIDE = 2,      /* items of meshes with N > 512 */
-    MESHPROP_GROUP_SIG_PROLOGUE = 4,  /* row tables to build */
-    MESHPROP_GROUP_SIG_REDUCE = 8     /* dp to reduce */
+    MESHPROP_GROUP_SIG_NARROW = 1,        /* items of meshes with N <= 512 */
+    MESHPROP_GROUP_SIG_WIDE = 2,          /* items of NN meshes with N > 512 */
+    MESHPROP_GROUP_SIG_PROLOGUE = 4,      /* row tables to build */
+    MESHPROP_GROUP_SIG_REDUCE = 8,        /* dp to reduce */
+    MESHPROP_GROUP_SIG_WIDE_GENERAL = 16  /* items of other meshes with N > 512 (FFT geometry) */
 };
 /* Resolved tile and number of work items of entry i. */
 int meshprop_group_entry_info(const meshprop_group *g, int i, meshprop_tile *tile, int *items);

@@ -15,17 +15,20 @@
 
 namespace mp {
 
-// Kernel families: family 0 holds the tiles with W <= 8 in four-wave workgroups (two per CU),
-// family 1 the W = 16 tile in four-wave workgroups (one per CU: its registers allow one wave
-// per SIMD); NN meshes there run the LDS-staged forward and the NN-specialised adjoint.
-// Variant v of the persistent kernels: (W, V) = kGroupTiles[v].
+// Kernel families, each a persistent kernel of four-wave workgroups: family 0 holds the tiles
+// with W <= 8 (two workgroups per CU); family 1 the W = 16 tile of NN meshes (LDS-staged
+// forward, NN-specialised adjoint; compiled for two waves per SIMD, two workgroups per CU);
+// family 2 the W = 16 tile of the other meshes (general forward and adjoint, whose registers
+// allow one wave per SIMD).  Kept apart, the general W = 16 code does not hold family 1 to one
+// wave per SIMD.  Variant v of the persistent kernels: (W, V) = kGroupTiles[v].
 struct GTile {
     int w, v, fam;
 };
 constexpr GTile kGroupTiles[] = {{2, 1, 0}, {2, 2, 0}, {2, 4, 0}, {4, 1, 0}, {4, 2, 0}, {8, 1, 0}, {16, 1, 1}};
 constexpr int kGroupVariants = sizeof(kGroupTiles) / sizeof(kGroupTiles[0]);
-constexpr int kFamWaves[2] = {4, 4};
-static_assert(kFamWaves[0] <= kStageWaves && kFamWaves[1] <= kStageWaves, "stage conversion sums at most kStageWaves waves");
+constexpr int kGroupFams = 3;
+constexpr int kFamWaves[kGroupFams] = {4, 4, 4};
+static_assert(kFamWaves[0] <= kStageWaves && kFamWaves[2] <= kStageWaves, "stage conversion sums at most kStageWaves waves");
 constexpr size_t kGroupStage = 64 * 1024;  // LDS of one workgroup (backdiff stage)
 constexpr size_t kGroupLds = kGroupStage + 16;  // stage + the queue slot
 
@@ -50,10 +53,10 @@ struct GMesh {
 
 struct GHeader {
     int nmesh, pad0;
-    int nitems[2];
-    int grab[2], done[2];  // queue counters; the last workgroup out of a launch resets them
+    int nitems[kGroupFams];
+    int grab[kGroupFams], done[kGroupFams];  // queue counters; the last workgroup out of a launch resets them
     long pro_total, red_total;
-    long off_pro, off_red, off_mesh, off_items[2];
+    long off_pro, off_red, off_mesh, off_items[kGroupFams];
 };
 
 // Descriptors are read through the constant address space: they do not change while a launch
@@ -174,51 +177,47 @@ __global__ void __launch_bounds__(256) k_group_reduce(const unsigned char* __res
 // staged through LDS (fwd_tile_nn, its three-slot ring in the stage buffer, which no adjoint
 // uses until the forward's last barrier): each wave reading 16 KiB of rows per layer through
 // the address unit bounds the general tile.
-template <int W, int V>
+template <int W, int V, bool NN>
 __device__ __noinline__ void group_fwd_tile(KMesh* Mp, const long tile_v, float4* stage) {
     KMesh& M = *uniform_ptr(Mp);
     const long tile = (long)__builtin_amdgcn_readfirstlane((unsigned)tile_v);
-    if constexpr (W == 16 && V == 1) {
+    if constexpr (NN) {
+        static_assert(W == 16 && V == 1, "the NN family holds the W = 16 tile");
         static_assert(3 * 64 * W * sizeof(float4) <= kGroupStage, "forward row ring inside the stage buffer");
         static_assert(kFamWaves[1] == 4, "fwd_tile_nn copies rows with 256 threads");
-        if (M.kso > 0) {
-            fwd_tile_nn<float2, 16, 1>(M.N, 0, M.L, M.B, (KInt*)M.kind, M.rows_fwd, M.u_in, M.u_out, M.ldu, tile,
-                                       Ckpt{nullptr, 0, 0}, stage);
-            return;
-        }
+        fwd_tile_nn<float2, 16, 1>(M.N, 0, M.L, M.B, (KInt*)M.kind, M.rows_fwd, M.u_in, M.u_out, M.ldu, tile,
+                                   Ckpt{nullptr, 0, 0}, stage);
+    } else {
+        fwd_tile<float2, W, V, KInt*>(M.N, 0, M.L, M.B, M.LPV, M.NP, (KInt*)M.kind, M.rows_fwd, M.u_in, M.u_out,
+                                      M.ldu, tile);
     }
-    fwd_tile<float2, W, V, KInt*>(M.N, 0, M.L, M.B, M.LPV, M.NP, (KInt*)M.kind, M.rows_fwd, M.u_in, M.u_out, M.ldu,
-                                  tile);
 }
 
-template <int W, int V>
+// NN: the W = 16 NN family (kso > 0: one-layer stages in stage order).
+template <int W, int V, bool NN = false>
 __device__ __forceinline__ void group_item(KMesh& M, const int slot, float4* stage) {
     float* part = (M.what & G_GRAD) ? M.partial + (size_t)slot * M.E : nullptr;
     bool first = true;
     for (long tile = slot; tile < M.ntiles; tile += M.S) {
         if (M.what & G_FWD)
-            group_fwd_tile<W, V>(&M, tile, stage);
+            group_fwd_tile<W, V, NN>(&M, tile, stage);
         if (M.what & G_GRAD) {
-            if constexpr (W == 16 && V == 1) {
-                if (M.kso > 0) {
-                    // NN meshes: the NN-specialised adjoint (one-layer stages in stage order).
-                    bwd_tile_nn<float2, 16, 1, kFamWaves[1], 1, false>(
-                        M.N, 0, M.L, M.B, (KInt*)M.kind, M.rows_inv, M.dtdp, part, M.u_out, M.g_out, M.target,
-                        M.u_rec, M.g_in, M.ldu, tile, first, stage, Ckpt{nullptr, 0, 0}, nullptr);
-                    first = false;
-                    continue;
-                }
-            }
-            bwd_tile<float2, W, V, true, KInt*>(M.N, 0, M.L, M.B, M.LPV, M.NP, M.C, (KInt*)M.kind, M.rows_inv,
-                                         M.rows_inv, M.dtdp, part, M.u_out, M.g_out,
-                                         M.target, M.u_rec, M.g_in, M.ldu, tile, first, stage);
+            if constexpr (NN)
+                bwd_tile_nn<float2, 16, 1, kFamWaves[1], 1, false>(
+                    M.N, 0, M.L, M.B, (KInt*)M.kind, M.rows_inv, M.dtdp, part, M.u_out, M.g_out, M.target, M.u_rec,
+                    M.g_in, M.ldu, tile, first, stage, Ckpt{nullptr, 0, 0}, nullptr);
+            else
+                bwd_tile<float2, W, V, true, KInt*>(M.N, 0, M.L, M.B, M.LPV, M.NP, M.C, (KInt*)M.kind, M.rows_inv,
+                                                    M.rows_inv, M.dtdp, part, M.u_out, M.g_out, M.target, M.u_rec,
+                                                    M.g_in, M.ldu, tile, first, stage);
         }
         first = false;
     }
 }
 
 template <int FAM>
-__global__ void __launch_bounds__(64 * kFamWaves[FAM]) k_group(unsigned char* __restrict__ ws) {
+__global__ void __launch_bounds__(64 * kFamWaves[FAM]) __attribute__((amdgpu_waves_per_eu(FAM == 1 ? 2 : 1)))
+k_group(unsigned char* __restrict__ ws) {
     GHeader* H = reinterpret_cast<GHeader*>(ws);
     KMesh* meshes = (KMesh*)(ws + H->off_mesh);
     KInt* items = (KInt*)(ws + H->off_items[FAM]);  // (mesh, slot) pairs
@@ -246,6 +245,8 @@ __global__ void __launch_bounds__(64 * kFamWaves[FAM]) k_group(unsigned char* __
                 case 5: group_item<8, 1>(M, ws_slot, stage); break;
                 default: break;
             }
+        } else if constexpr (FAM == 1) {
+            group_item<16, 1, true>(M, ws_slot, stage);
         } else {
             group_item<16, 1>(M, ws_slot, stage);
         }
@@ -278,10 +279,10 @@ struct meshprop_group {
     };
     std::vector<meshprop_group_entry> ent;
     std::vector<Info> info;
-    std::vector<int2> items[2];
+    std::vector<int2> items[kGroupFams];
     size_t header_bytes = 0, total = 0;
     long pro_total = 0, red_total = 0;
-    int ncu = 256, grid[2] = {0, 0}, dev = 0;
+    int ncu = 256, grid[kGroupFams] = {0, 0, 0}, dev = 0;
     int sig = 0;  // MESHPROP_GROUP_SIG_* bits: which launches the plan needs
     mutable std::vector<unsigned char> image;
 };
@@ -328,10 +329,10 @@ int cu_count() {
 }
 
 // Per-device launch shape of the grouped kernels (resident workgroups per family) and the
-// second stream the W = 16 family runs on, set up once per device.
+// second stream the W = 16 families run on, set up once per device.
 struct DeviceGroupConfig {
     std::once_flag once;
-    int ncu = 256, grid[2] = {256, 256};
+    int ncu = 256, grid[kGroupFams] = {256, 256, 256};
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::mutex lock;  // orders fork / join sequences of concurrent host threads
@@ -345,18 +346,23 @@ DeviceGroupConfig* device_config(int dev) {
     DeviceGroupConfig& c = g_dev_cfg[dev];
     std::call_once(c.once, [&c]() {
         c.ncu = cu_count();
-        (void)hipFuncSetAttribute((const void*)k_group<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGroupLds);
-        (void)hipFuncSetAttribute((const void*)k_group<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGroupLds);
-        for (int fam = 0; fam < 2; ++fam) {
+        const void* fn[kGroupFams] = {(const void*)k_group<0>, (const void*)k_group<1>, (const void*)k_group<2>};
+        bool attr = true;
+        for (int fam = 0; fam < kGroupFams; ++fam) {
+            // The stage and queue slot exceed the default dynamic LDS limit of 64 KiB.
+            attr = attr && hipFuncSetAttribute(fn[fam], hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGroupLds) ==
+                               hipSuccess;
             int nb = 0;
-            hipError_t rc = fam == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_group<0>, 64 * kFamWaves[0],
-                                                                                       kGroupLds)
-                                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_group<1>, 64 * kFamWaves[1],
-                                                                                       kGroupLds);
+            hipError_t rc = fam == 0   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_group<0>, 64 * kFamWaves[0],
+                                                                                      kGroupLds)
+                            : fam == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_group<1>, 64 * kFamWaves[1],
+                                                                                      kGroupLds)
+                                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_group<2>, 64 * kFamWaves[2],
+                                                                                      kGroupLds);
             if (rc != hipSuccess || nb < 1) nb = 1;
             c.grid[fam] = nb * c.ncu;
         }
-        c.ok = hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking) == hipSuccess &&
+        c.ok = attr && hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking) == hipSuccess &&
                hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming) == hipSuccess &&
                hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming) == hipSuccess;
     });
@@ -382,6 +388,21 @@ int check_entry(const meshprop_group_entry& e) {
     }
     if (((uintptr_t)e.tables) & 255) return MESHPROP_ERR_ARG;
     return MESHPROP_OK;
+}
+
+// Offsets of the families' item lists in the plan image (after header, prologue and reduction
+// unit tables and the mesh descriptors); returns the image size.
+size_t item_offsets(const meshprop_group& g, size_t (&it)[kGroupFams]) {
+    const int n = (int)g.ent.size();
+    const size_t hdr = a256(sizeof(GHeader));
+    const size_t pro = hdr, red = a256(pro + (size_t)(n + 1) * sizeof(long));
+    const size_t mesh = a256(red + (size_t)(n + 1) * sizeof(long));
+    size_t off = a256(mesh + (size_t)n * sizeof(GMesh));
+    for (int fam = 0; fam < kGroupFams; ++fam) {
+        it[fam] = off;
+        off = a256(off + g.items[fam].size() * sizeof(int2));
+    }
+    return off;
 }
 
 int plan_group(meshprop_group& g) {
@@ -420,7 +441,10 @@ int plan_group(meshprop_group& g) {
             }
         }
         f.var = group_var(f.W, f.V);
-        f.fam = kGroupTiles[f.var].fam;
+        // W = 16 NN meshes of unitary crossings run the NN family: one-layer stages (two buffers
+        // in the stage budget), K table and partials in the NN adjoint's stage order.
+        f.kso = (f.W == 16 && f.V == 1 && !e.fft && f.LPV == 64 && (e.X == X_MZI || e.X == X_SYM)) ? 8 : 0;
+        f.fam = kGroupTiles[f.var].fam == 0 ? 0 : f.kso ? 1 : 2;
         const int nw = kFamWaves[f.fam];
         f.ntiles = ((long)e.B + (long)nw * G * f.V - 1) / ((long)nw * G * f.V);
         // Layers per LDS stage: at least one unrolled group of the adjoint loop.
@@ -433,9 +457,6 @@ int plan_group(meshprop_group& g) {
         if (C < depth) C = depth;
         if ((size_t)C * per_layer > kGroupStage) return MESHPROP_ERR_UNSUPPORTED;
         f.C = C;
-        // W = 16 NN meshes of unitary crossings run the NN-specialised adjoint: one-layer stages
-        // (two buffers in the stage budget), K table and partials in its stage order.
-        f.kso = (f.W == 16 && f.V == 1 && !e.fft && f.LPV == 64 && (e.X == X_MZI || e.X == X_SYM)) ? 8 : 0;
         if (f.kso) f.C = 1;
         const bool grad = f.what & G_GRAD;
         f.S = (e.B == 0) ? 0 : grad ? (int)std::min<long>(f.ntiles, 2L * g.ncu) : (int)f.ntiles;
@@ -474,20 +495,15 @@ int plan_group(meshprop_group& g) {
         }
     }
     std::stable_sort(cand.begin(), cand.end(), [](const Cand& a, const Cand& b) { return a.cost > b.cost; });
-    g.items[0].clear();
-    g.items[1].clear();
+    for (int fam = 0; fam < kGroupFams; ++fam) g.items[fam].clear();
     for (const Cand& c : cand) g.items[c.fam].push_back(make_int2(c.e, c.s));
     // Device plan at the start of the workspace, data after it.
-    const size_t hdr = a256(sizeof(GHeader));
-    const size_t pro = hdr, red = a256(pro + (size_t)(n + 1) * sizeof(long));
-    const size_t mesh = a256(red + (size_t)(n + 1) * sizeof(long));
-    const size_t it0 = a256(mesh + (size_t)n * sizeof(GMesh));
-    const size_t it1 = a256(it0 + g.items[0].size() * sizeof(int2));
-    g.header_bytes = a256(it1 + g.items[1].size() * sizeof(int2));
+    size_t it[kGroupFams];
+    g.header_bytes = item_offsets(g, it);
     g.total = g.header_bytes + off;
-    g.grid[0] = dc ? dc->grid[0] : g.ncu;
-    g.grid[1] = dc ? dc->grid[1] : g.ncu;
+    for (int fam = 0; fam < kGroupFams; ++fam) g.grid[fam] = dc ? dc->grid[fam] : g.ncu;
     g.sig = (g.items[0].empty() ? 0 : MESHPROP_GROUP_SIG_NARROW) | (g.items[1].empty() ? 0 : MESHPROP_GROUP_SIG_WIDE) |
+            (g.items[2].empty() ? 0 : MESHPROP_GROUP_SIG_WIDE_GENERAL) |
             (g.pro_total > 0 ? MESHPROP_GROUP_SIG_PROLOGUE : 0) | (g.red_total > 0 ? MESHPROP_GROUP_SIG_REDUCE : 0);
     return MESHPROP_OK;
 }
@@ -497,21 +513,21 @@ void build_image(const meshprop_group& g, unsigned char* ws) {
     const size_t hdr = a256(sizeof(GHeader));
     const size_t pro = hdr, red = a256(pro + (size_t)(n + 1) * sizeof(long));
     const size_t mesh = a256(red + (size_t)(n + 1) * sizeof(long));
-    const size_t it0 = a256(mesh + (size_t)n * sizeof(GMesh));
-    const size_t it1 = a256(it0 + g.items[0].size() * sizeof(int2));
+    size_t it[kGroupFams];
+    (void)item_offsets(g, it);
     g.image.assign(g.header_bytes, 0);
     unsigned char* im = g.image.data();
     GHeader* H = reinterpret_cast<GHeader*>(im);
     H->nmesh = n;
-    H->nitems[0] = (int)g.items[0].size();
-    H->nitems[1] = (int)g.items[1].size();
     H->pro_total = g.pro_total;
     H->red_total = g.red_total;
     H->off_pro = (long)pro;
     H->off_red = (long)red;
     H->off_mesh = (long)mesh;
-    H->off_items[0] = (long)it0;
-    H->off_items[1] = (long)it1;
+    for (int fam = 0; fam < kGroupFams; ++fam) {
+        H->nitems[fam] = (int)g.items[fam].size();
+        H->off_items[fam] = (long)it[fam];
+    }
     long* pb = reinterpret_cast<long*>(im + pro);
     long* rb = reinterpret_cast<long*>(im + red);
     GMesh* M = reinterpret_cast<GMesh*>(im + mesh);
@@ -567,8 +583,8 @@ void build_image(const meshprop_group& g, unsigned char* ws) {
     }
     pb[n] = pacc;
     rb[n] = racc;
-    if (!g.items[0].empty()) std::memcpy(im + it0, g.items[0].data(), g.items[0].size() * sizeof(int2));
-    if (!g.items[1].empty()) std::memcpy(im + it1, g.items[1].data(), g.items[1].size() * sizeof(int2));
+    for (int fam = 0; fam < kGroupFams; ++fam)
+        if (!g.items[fam].empty()) std::memcpy(im + it[fam], g.items[fam].data(), g.items[fam].size() * sizeof(int2));
 }
 
 int hip_status(hipError_t e) {
@@ -635,27 +651,34 @@ int meshprop_group_launch(const meshprop_group* g, void* ws, size_t bytes, void*
         hipLaunchKernelGGL(k_group_prologue, dim3(4 * g->ncu), dim3(256), 0, st, (const unsigned char*)w);
         if ((e = hipGetLastError()) != hipSuccess) return hip_status(e);
     }
-    const bool narrow = g->sig & MESHPROP_GROUP_SIG_NARROW, wide = g->sig & MESHPROP_GROUP_SIG_WIDE;
-    if (wide && narrow) {
-        // The W = 16 family runs beside family 0 on a second stream (fork / join by events,
-        // which a graph capture records as two parallel branches).
-        if (!dc->ok) return MESHPROP_ERR_HIP;
-        std::lock_guard<std::mutex> hold(dc->lock);
+    if (!dc->ok) return MESHPROP_ERR_HIP;
+    const bool narrow = g->sig & MESHPROP_GROUP_SIG_NARROW;
+    const bool wide = g->sig & MESHPROP_GROUP_SIG_WIDE, wide_gen = g->sig & MESHPROP_GROUP_SIG_WIDE_GENERAL;
+    // The W = 16 families (one after the other) run beside family 0 on a second stream (fork /
+    // join by events, which a graph capture records as two parallel branches).
+    const bool fork = narrow && (wide || wide_gen);
+    std::unique_lock<std::mutex> hold(dc->lock, std::defer_lock);
+    hipStream_t sw = st;
+    if (fork) {
+        hold.lock();
         if ((e = hipEventRecord(dc->ev_fork, st)) != hipSuccess) return hip_status(e);
         if ((e = hipStreamWaitEvent(dc->side, dc->ev_fork, 0)) != hipSuccess) return hip_status(e);
-        hipLaunchKernelGGL((k_group<1>), dim3(g->grid[1]), dim3(64 * kFamWaves[1]), kGroupLds, dc->side, w);
+        sw = dc->side;
+    }
+    if (wide) {
+        hipLaunchKernelGGL((k_group<1>), dim3(g->grid[1]), dim3(64 * kFamWaves[1]), kGroupLds, sw, w);
         if ((e = hipGetLastError()) != hipSuccess) return hip_status(e);
-        if ((e = hipEventRecord(dc->ev_join, dc->side)) != hipSuccess) return hip_status(e);
-        hipLaunchKernelGGL((k_group<0>), dim3(g->grid[0]), dim3(64 * kFamWaves[0]), kGroupLds, st, w);
+    }
+    if (wide_gen) {
+        hipLaunchKernelGGL((k_group<2>), dim3(g->grid[2]), dim3(64 * kFamWaves[2]), kGroupLds, sw, w);
         if ((e = hipGetLastError()) != hipSuccess) return hip_status(e);
-        if ((e = hipStreamWaitEvent(st, dc->ev_join, 0)) != hipSuccess) return hip_status(e);
-    } else if (wide) {
-        hipLaunchKernelGGL((k_group<1>), dim3(g->grid[1]), dim3(64 * kFamWaves[1]), kGroupLds, st, w);
-        if ((e = hipGetLastError()) != hipSuccess) return hip_status(e);
-    } else if (narrow) {
+    }
+    if (fork && (e = hipEventRecord(dc->ev_join, dc->side)) != hipSuccess) return hip_status(e);
+    if (narrow) {
         hipLaunchKernelGGL((k_group<0>), dim3(g->grid[0]), dim3(64 * kFamWaves[0]), kGroupLds, st, w);
         if ((e = hipGetLastError()) != hipSuccess) return hip_status(e);
     }
+    if (fork && (e = hipStreamWaitEvent(st, dc->ev_join, 0)) != hipSuccess) return hip_status(e);
     if (g->sig & MESHPROP_GROUP_SIG_REDUCE) {
         hipLaunchKernelGGL(k_group_reduce, dim3(8 * g->ncu), dim3(256), 0, st, (const unsigned char*)w);
         if ((e = hipGetLastError()) != hipSuccess) return hip_status(e);
