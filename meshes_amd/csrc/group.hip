@@ -37,7 +37,8 @@ enum { B_KIND = 1, B_FWD = 2, B_INV = 4, B_DTDP = 8 };
 
 struct GMesh {
     int what, var, X, N, L, B, fft, LPV, NP, C, np, ns, ldp, lds, ldu, S, build, assign;
-    int kso;  // > 0: K table and dp partials in the NN adjoint's stage order (W = 16 NN meshes)
+    int nnadj;  // gradient items run the NN-specialised adjoint (NN meshes, one 64-lane vector group)
+    int kso;    // > 0: K table and dp partials in the NN adjoint's stage order (W >= 4 NN meshes)
     long ntiles, E;
     long pro_chunks;  // prologue blocks (of 256 slots) per layer
     const int *g0, *g1;
@@ -193,7 +194,9 @@ __device__ __noinline__ void group_fwd_tile(KMesh* Mp, const long tile_v, float4
     }
 }
 
-// NN: the W = 16 NN family (kso > 0: one-layer stages in stage order).
+// NN: the W = 16 NN family (one-layer stages in stage order).  In family 0, NN meshes of one
+// 64-lane vector group (nnadj) run the NN-specialised adjoint as the plain path does: stages
+// of 16 / W layers, four waves (64 KiB of LDS), entries in stage order at W >= 4.
 template <int W, int V, bool NN = false>
 __device__ __forceinline__ void group_item(KMesh& M, const int slot, float4* stage) {
     float* part = (M.what & G_GRAD) ? M.partial + (size_t)slot * M.E : nullptr;
@@ -202,14 +205,27 @@ __device__ __forceinline__ void group_item(KMesh& M, const int slot, float4* sta
         if (M.what & G_FWD)
             group_fwd_tile<W, V, NN>(&M, tile, stage);
         if (M.what & G_GRAD) {
-            if constexpr (NN)
+            if constexpr (NN) {
                 bwd_tile_nn<float2, 16, 1, kFamWaves[1], 1, false>(
                     M.N, 0, M.L, M.B, (KInt*)M.kind, M.rows_inv, M.dtdp, part, M.u_out, M.g_out, M.target, M.u_rec,
                     M.g_in, M.ldu, tile, first, stage, Ckpt{nullptr, 0, 0}, nullptr);
-            else
+            } else {
+                if constexpr (W <= 8) {
+                    static_assert(nn_lds_bytes(W, kFamWaves[0], nn_stage_depth(W), false, sizeof(float4)) <=
+                                      kGroupStage,
+                                  "NN adjoint stage inside the group stage");
+                    if (M.nnadj) {
+                        bwd_tile_nn<float2, W, V, kFamWaves[0], nn_stage_depth(W), false, false>(
+                            M.N, 0, M.L, M.B, (KInt*)M.kind, M.rows_inv, M.dtdp, part, M.u_out, M.g_out, M.target,
+                            M.u_rec, M.g_in, M.ldu, tile, first, stage, Ckpt{nullptr, 0, 0}, nullptr);
+                        first = false;
+                        continue;
+                    }
+                }
                 bwd_tile<float2, W, V, true, KInt*>(M.N, 0, M.L, M.B, M.LPV, M.NP, M.C, (KInt*)M.kind, M.rows_inv,
                                                     M.rows_inv, M.dtdp, part, M.u_out, M.g_out, M.target, M.u_rec,
                                                     M.g_in, M.ldu, tile, first, stage);
+            }
         }
         first = false;
     }
@@ -271,7 +287,7 @@ using namespace mp;
 
 struct meshprop_group {
     struct Info {
-        int W, V, LPV, NP, C, S, var, fam, what, build, np, ns, kso;
+        int W, V, LPV, NP, C, S, var, fam, what, build, np, ns, nnadj, kso;
         long ntiles, E, pro_units, red_units;
         bool own_tables;
         size_t off_tab, off_partial;  // workspace offsets (tables when own_tables)
@@ -441,10 +457,12 @@ int plan_group(meshprop_group& g) {
             }
         }
         f.var = group_var(f.W, f.V);
-        // W = 16 NN meshes of unitary crossings run the NN family: one-layer stages (two buffers
-        // in the stage budget), K table and partials in the NN adjoint's stage order.
-        f.kso = (f.W == 16 && f.V == 1 && !e.fft && f.LPV == 64 && (e.X == X_MZI || e.X == X_SYM)) ? 8 : 0;
-        f.fam = kGroupTiles[f.var].fam == 0 ? 0 : f.kso ? 1 : 2;
+        // NN meshes of unitary crossings with one 64-lane vector group run the NN-specialised
+        // adjoint (the W = 16 ones as family 1, with one-layer stages): K table and partials in
+        // its stage order at W >= 4.
+        f.nnadj = (!e.fft && f.LPV == 64 && (e.X == X_MZI || e.X == X_SYM) && (f.W < 16 || f.V == 1)) ? 1 : 0;
+        f.kso = (f.nnadj && nn_stage_order(f.W)) ? f.W / 2 : 0;
+        f.fam = kGroupTiles[f.var].fam == 0 ? 0 : f.nnadj ? 1 : 2;
         const int nw = kFamWaves[f.fam];
         f.ntiles = ((long)e.B + (long)nw * G * f.V - 1) / ((long)nw * G * f.V);
         // Layers per LDS stage: at least one unrolled group of the adjoint loop.
@@ -457,7 +475,7 @@ int plan_group(meshprop_group& g) {
         if (C < depth) C = depth;
         if ((size_t)C * per_layer > kGroupStage) return MESHPROP_ERR_UNSUPPORTED;
         f.C = C;
-        if (f.kso) f.C = 1;
+        if (f.nnadj) f.C = nn_stage_depth(f.W);
         const bool grad = f.what & G_GRAD;
         f.S = (e.B == 0) ? 0 : grad ? (int)std::min<long>(f.ntiles, 2L * g.ncu) : (int)f.ntiles;
         f.E = (long)e.L * (f.NP / 2) * f.np;
@@ -551,6 +569,7 @@ void build_image(const meshprop_group& g, unsigned char* ws) {
         m.LPV = f.LPV;
         m.NP = f.NP;
         m.C = f.C;
+        m.nnadj = f.nnadj;
         m.kso = f.kso;
         m.np = f.np;
         m.ns = f.ns;
