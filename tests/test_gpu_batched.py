@@ -115,6 +115,29 @@ def test_group_matches_oracle(gpu):
         sp.check(j)
 
 
+NN_TILES = [(100, (2, 1)), (100, (2, 2)), (120, (2, 4)), (200, (4, 1)), (250, (4, 2)), (400, (8, 1)),
+            (64, (2, 1))]
+
+
+@pytest.mark.parametrize("N,wv", NN_TILES, ids=[f"N{n}-W{w}V{v}" for n, (w, v) in NN_TILES])
+def test_group_nn_adjoint_tiles(gpu, N, wv):
+    """Every family-0 tile of the NN-specialised adjoint in a grouped launch (NN meshes with one
+    64-lane vector group; N = 64 at W = 2 has LPV = 32 and runs the general adjoint) against the
+    FP64 oracle, with splitter errors, both losses and dJ/dx."""
+    from meshes_amd.batched import MeshGroup
+    w, v = wv
+    specs = [Spec(7 * N + v, "mzi", "clements", N, 150, "grad", loss="target", gx=True, errors=True),
+             Spec(7 * N + v + 1, "sym", "nn", N, 97, "grad", loss="dy", gx=True, errors=True)]
+    jobs = [sp.job(tile=(w, v, 0, 0)) for sp in specs]
+    grp = MeshGroup(jobs)
+    grp.run()
+    torch.cuda.synchronize()
+    for i, (sp, j) in enumerate(zip(specs, jobs)):
+        tile, _ = grp.info(i)
+        assert tuple(tile[:2]) == (w, v), tile
+        sp.check(j)
+
+
 def test_group_alone_equals_mix(gpu):
     from meshes_amd.batched import MeshGroup
     specs = _specs()
