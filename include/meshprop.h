@@ -317,10 +317,10 @@ int meshprop_resolve_tile(int op, int X, int N, int B, meshprop_tile *out);
  * -status for arguments meshprop_run would reject.  fft as in meshprop_run, flags its flags. */
 int meshprop_adjoint_kernel(int op, int X, int N, int L, int B, int fft, const meshprop_tile *tile, int flags);
 
-/* Which forward kernel a fwdprop / fwd+grad call with these arguments runs (no reference
- * counterpart): 1 = the one specialised for NN geometries with rows staged through LDS, 0 = the
- * general one (also for ops without this forward), -status for arguments meshprop_run would
- * reject. */
+/* Which forward kernel a fwdprop / fwd+grad (or forward-mode, fwddiff) call with these
+ * arguments runs (no reference counterpart): 1 = the one specialised for NN geometries with
+ * rows staged through LDS, 0 = the general one (also for backdiff), -status for arguments
+ * meshprop_run would reject. */
 int meshprop_forward_kernel(int op, int X, int N, int L, int B, int fft, const meshprop_tile *tile, int flags);
 
 /* Last HIP error code seen by a failing call on this thread (hipError_t as int).  A HIP error

@@ -184,8 +184,8 @@ def resolve_tile(op, X, N, B, tile=None):
 
 
 def forward_kernel(op, X, N, L, B, fft=False, tile=None, flags=0) -> str:
-    """Which forward kernel a fwdprop / fwd+grad call runs: "nn" (NN geometry, rows staged
-    through LDS) or "general"."""
+    """Which forward kernel a fwdprop / fwd+grad / fwddiff call runs: "nn" (NN geometry, rows
+    staged through LDS) or "general"."""
     lib = load()
     rc = lib.meshprop_forward_kernel(op, X_CODE[X] if isinstance(X, str) else X, N, L, B, int(fft),
                                      tile_struct(tile), int(flags))

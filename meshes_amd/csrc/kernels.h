@@ -613,6 +613,124 @@ __global__ void __launch_bounds__(256) k_fwd_nn(int N, int l0, int nl, int B, co
 }
 
 // ---- fwddiff --------------------------------------------------------------------------
+// NN-specialised forward-mode walk of the widest tile (W = 16, one vector per wave): u and du
+// updated in place per layer, the layer parity a compile-time body, both row tables staged
+// through LDS as the forward's (k_fwd_nn): a three-slot ring of (rows, tangent rows) per
+// layer, requested two layers ahead by the NW waves of the workgroup, each crossing's rows read
+// from LDS where it is applied.  Each wave reading 32 KiB of rows per layer through the address
+// unit bounds the general fwddiff tile.
+template <class T, int W, bool ODD>
+__device__ __forceinline__ void fwddiff_layer_nn(T (&u)[W], T (&du)[W], const typename RowOf<T>::type* lr,
+                                                 const typename RowOf<T>::type* ld) {
+    typedef typename RowOf<T>::type R;
+    constexpr int H = W / 2;
+    auto pair = [&](const int k, const int j) __attribute__((always_inline)) {
+        const R rk = lr[k * 64], rj = lr[j * 64], ek = ld[k * 64], ej = ld[j * 64];
+        const T a = u[k], b = u[j], da = du[k], db = du[j];
+        u[k] = rapply(rk, a, b);
+        u[j] = rapply(rj, b, a);
+        du[k] = rapply_d(rk, ek, a, b, da, db);
+        du[j] = rapply_d(rj, ej, b, a, db, da);
+    };
+    if constexpr (!ODD) {
+#pragma unroll
+        for (int p = 0; p < H; ++p) pair(2 * p, 2 * p + 1);
+    } else {
+        const T hu = lane_next(u[0]), hd = lane_next(du[0]), lu = lane_prev(u[W - 1]), lo = lane_prev(du[W - 1]);
+#pragma unroll
+        for (int p = 0; p + 1 < H; ++p) pair(2 * p + 1, 2 * p + 2);
+        const R rh = lr[(W - 1) * 64], eh = ld[(W - 1) * 64], rl = lr[0], el = ld[0];
+        const T a = u[W - 1], b = u[0], da = du[W - 1], db = du[0];
+        u[W - 1] = rapply(rh, a, hu);
+        du[W - 1] = rapply_d(rh, eh, a, hu, da, hd);
+        u[0] = rapply(rl, b, lu);
+        du[0] = rapply_d(rl, el, b, lu, db, lo);
+    }
+}
+
+template <class T, int W, int NW>
+__device__ __forceinline__ void fwddiff_tile_nn(const int N, const int nl, const int B, ConstInt* kind,
+                                                const typename RowOf<T>::type* __restrict__ rows,
+                                                const typename RowOf<T>::type* __restrict__ drows, const T* u_in,
+                                                const T* du_in, T* u_out, T* du_out, const int ldu, const long tile,
+                                                typename RowOf<T>::type* __restrict__ lrows) {
+    typedef typename RowOf<T>::type R;
+    constexpr int LPV = 64, NP = 64 * W, NTHR = 64 * NW;
+    constexpr int CP = NP / NTHR;  // entries of a layer's table each thread copies
+    static_assert(NP % NTHR == 0, "whole layers per workgroup copy");
+    const Lanes q = lanes_of<1>(LPV, tile);
+    const int tid = threadIdx.x;
+    T u[1][W], du[1][W];
+    load_tile(u, u_in, q, N, B, ldu);
+    if (du_in != nullptr) {
+        load_tile(du, du_in, q, N, B, ldu);
+    } else {
+#pragma unroll
+        for (int k = 0; k < W; ++k) du[0][k] = fzero(T());
+    }
+    if (nl > 0) {
+        const __amdgpu_buffer_rsrc_t rrs = table_rsrc(rows), drs = table_rsrc(drows);
+        const unsigned coff = (unsigned)tid * (unsigned)sizeof(R);
+        R cp[2][2][CP];  // [register set][table][entry]: rows of the next two layers on their way to LDS
+        auto fetch = [&](R (&c)[2][CP], const int l) {
+            const int lc = l < nl ? l : nl - 1;  // past the end: the last layer again (in bounds, unused)
+#pragma unroll
+            for (int e = 0; e < CP; ++e) {
+                c[0][e] = buf_load<R>(rrs, coff, (lc * NP + e * NTHR) * (int)sizeof(R));
+                c[1][e] = buf_load<R>(drs, coff, (lc * NP + e * NTHR) * (int)sizeof(R));
+            }
+        };
+        auto publish = [&](const R (&c)[2][CP], const int l) {
+            R* dst = lrows + (l % 3) * 2 * NP + tid;
+#pragma unroll
+            for (int e = 0; e < CP; ++e) {
+                dst[e * NTHR] = c[0][e];
+                dst[NP + e * NTHR] = c[1][e];
+            }
+        };
+        auto lds_barrier = [] { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); };
+        fetch(cp[0], 0);
+        fetch(cp[1], 1);
+        publish(cp[0], 0);
+        lds_barrier();
+        int kd = kind[0];
+        auto layer = [&](const int l, R (&cnext)[2][CP], R (&cafter)[2][CP]) {
+            const int kn = kind[l + 1 < nl ? l + 1 : l];
+            const R* cur = lrows + (l % 3) * 2 * NP + q.t;
+            fetch(cafter, l + 2);
+            if (kd & 1)
+                fwddiff_layer_nn<T, W, true>(u[0], du[0], cur, cur + NP);
+            else
+                fwddiff_layer_nn<T, W, false>(u[0], du[0], cur, cur + NP);
+            publish(cnext, l + 1);
+            lds_barrier();
+            kd = kn;
+        };
+        int l = 0;
+        for (; l + 1 < nl; l += 2) {
+            layer(l, cp[1], cp[0]);
+            layer(l + 1, cp[0], cp[1]);
+        }
+        if (l < nl) layer(l, cp[1], cp[0]);
+    }
+    store_tile(u, u_out, q, N, B, ldu);
+    if (du_out != nullptr) store_tile(du, du_out, q, N, B, ldu);
+}
+
+constexpr int kFwdDiffNNWaves = 8;
+
+template <class T, int W>
+__global__ void __launch_bounds__(64 * kFwdDiffNNWaves) k_fwddiff_nn(int N, int L, int B, const int* __restrict__ kind,
+                                                                   const typename RowOf<T>::type* __restrict__ rows,
+                                                                   const typename RowOf<T>::type* __restrict__ drows,
+                                                                   const T* u_in, const T* du_in, T* u_out, T* du_out,
+                                                                   int ldu) {
+    typedef typename RowOf<T>::type R;
+    extern __shared__ unsigned char smem_raw[];
+    fwddiff_tile_nn<T, W, kFwdDiffNNWaves>(N, L, B, (ConstInt*)kind, rows, drows, u_in, du_in, u_out, du_out, ldu,
+                                           blockIdx.x, reinterpret_cast<R*>(smem_raw));
+}
+
 template <class T, int W, int V>
 __global__ void __launch_bounds__(256) k_fwddiff(int N, int L, int B, int LPV, int NP, const int* __restrict__ kind,
                                                  const typename RowOf<T>::type* __restrict__ rows,
@@ -1862,6 +1980,16 @@ __host__ __device__ constexpr int nn_stage_depth(int W) { return 16 / W; }
 // 0.607 ms, and at W = 8 (N=512 B=4096) 0.254 -> 0.277 ms, W = 4 (N=256 B=4096) 0.071 -> 0.087
 // ms: there a row feeds two vectors, the address unit is not the limit and the per-layer
 // barrier costs more than it saves.
+// The NN-specialised forward-mode walk (k_fwddiff_nn) serves W = 16 NN meshes of one vector
+// per wave when the batch gives every CU an eight-wave workgroup (its LDS ring holds three
+// layers of both row tables).  Measured against the general tile (MI355X, L = N): MZI N=1024
+// B=2048 1.654 ms against 2.345, N=768 1.233 against 1.808, orth N=1024 0.829 against 1.83; at
+// B=256 (32 workgroups) 1.377 against 1.109.
+inline bool nn_fwddiff(const Plan& P) {
+    return P.nn && !P.fft && P.LPV == 64 && P.W == 16 && P.V == 1 && (long)P.B >= 8L * P.ncu;
+}
+inline size_t nn_fwddiff_lds(const Plan& P) { return (size_t)3 * 2 * P.NP * P.rowbytes; }
+
 inline bool nn_forward(const Plan& P, const int ncu) {
     return P.nn && !P.fft && P.LPV == 64 && P.W == 16 && P.waves == 4 && P.ntiles * P.waves >= 8L * ncu;
 }

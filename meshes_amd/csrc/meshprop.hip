@@ -578,6 +578,7 @@ int meshprop_forward_kernel(int op, int X, int N, int L, int B, int fft, const m
         if (rc) return -rc;
         return nn_forward(PF, PF.ncu) ? 1 : 0;
     }
+    if (op == MESHPROP_OP_FWDDIFF) return nn_fwddiff(P) ? 1 : 0;
     if (op != MESHPROP_OP_FWDPROP) return 0;
     return nn_forward(P, P.ncu) ? 1 : 0;
 }

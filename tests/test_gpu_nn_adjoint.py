@@ -237,3 +237,45 @@ def test_nn_forward(gpu, case):
     assert torch.equal(res[0][1], res[Lb.GENERIC_KERNELS][1])
     ref = O.mesh_fwd(g, X, p.astype(np.float64), _f64(s), u.astype(np.complex128 if X != "orth" else np.float64))
     assert O.rel_l2(_np(res[0][0]), ref) < TOL
+
+
+# Staged forward-mode walk (W = 16, one vector per wave, a batch that gives every CU a
+# workgroup): odd and even depths, one and two layers, complex and real fields.
+FWDDIFF = [("mzi", 1024, 9, 2100), ("sym", 700, 12, 2048), ("orth", 1024, 7, 2048), ("mzi", 1024, 1, 2048),
+           ("mzi", 1000, 2, 2064), ("mzi_generic", 800, 6, 2048)]
+
+
+@pytest.mark.parametrize("case", FWDDIFF, ids=["-".join(map(str, c)) for c in FWDDIFF])
+def test_nn_fwddiff(gpu, case):
+    """fwddiff of wide NN meshes runs k_fwddiff_nn (both row tables staged through LDS):
+    bitwise equal to the general fwddiff and within 1e-5 of the oracle's JVP."""
+    import meshes_amd._lib as Lb
+    X, N, L, B = case
+    assert Lb.forward_kernel(Lb.OP_FWDDIFF, X, N, L, B) == "nn"
+    assert Lb.forward_kernel(Lb.OP_FWDDIFF, X, N, L, B, flags=Lb.GENERIC_KERNELS) == "general"
+    assert Lb.forward_kernel(Lb.OP_FWDDIFF, X, N, L, 256) == "general"
+    g, p, s, u, rng = _setup(X, "nn", N, L, B, seed=13 * N + L, lossy=X == "mzi_generic")
+    mesh = _mesh(g, X, p, s)
+    dp = rng.standard_normal(p.shape).astype(np.float32)
+    du = (rng.standard_normal(u.shape) + (0 if O.REAL_FIELD[X] else 1j * rng.standard_normal(u.shape)))
+    du = du.astype(u.dtype)
+    ub, dub = torch.as_tensor(u).cuda().contiguous(), torch.as_tensor(du).cuda().contiguous()
+    dpt = torch.as_tensor(dp.reshape(-1)).cuda()
+    g0, g1 = mesh._geom()
+    ldp, lds = mesh._ld()
+    sp = mesh.p_splitter if mesh.p_splitter.numel() else None
+    res = {}
+    for flags in (0, Lb.GENERIC_KERNELS):
+        out, dout = torch.empty_like(ub), torch.empty_like(ub)
+        nb = Lb.workspace_size(Lb.OP_FWDDIFF, X, N, L, B, False)
+        ws = torch.empty(max(nb, 1), dtype=torch.uint8, device=ub.device)
+        Lb.run(Lb.OP_FWDDIFF, X, N, L, B, False, g0, g1, mesh.p_phase, dpt, ldp, sp, lds, ub, dub, out, dout, N,
+               workspace=ws, ws_bytes=nb, flags=flags)
+        res[flags] = (out, dout)
+    torch.cuda.synchronize()
+    assert torch.equal(res[0][0], res[Lb.GENERIC_KERNELS][0])
+    assert torch.equal(res[0][1], res[Lb.GENERIC_KERNELS][1])
+    ref, dref = O.mesh_jvp(g, X, p.astype(np.float64), _f64(s), u.astype(np.complex128), dp.astype(np.float64),
+                           du.astype(np.complex128))
+    assert O.rel_l2(_np(res[0][0]), ref) < TOL
+    assert O.rel_l2(_np(res[0][1]), dref) < TOL

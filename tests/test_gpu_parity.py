@@ -63,6 +63,7 @@ CASES = [
     ("mzi", "nn", 1000, 4, 9), ("mzi", "nn", 1024, 3, 5), ("mzi", "fft", 256, 10, 21), ("mzi", "fft", 32, 8, 40),
     ("mzi", "fft", 1024, 10, 6), ("sym", "nn", 200, 8, 17), ("sym", "fft", 128, 7, 19), ("orth", "nn", 256, 10, 41),
     ("orth", "nn", 38, 9, 13), ("orth", "fft", 512, 9, 8), ("mzi_generic", "nn", 96, 8, 12),
+    ("orth", "nn", 700, 11, 19), ("sym", "nn", 600, 21, 70),
 ]
 
 
