@@ -1980,13 +1980,16 @@ __host__ __device__ constexpr int nn_stage_depth(int W) { return 16 / W; }
 // 0.607 ms, and at W = 8 (N=512 B=4096) 0.254 -> 0.277 ms, W = 4 (N=256 B=4096) 0.071 -> 0.087
 // ms: there a row feeds two vectors, the address unit is not the limit and the per-layer
 // barrier costs more than it saves.
-// The NN-specialised forward-mode walk (k_fwddiff_nn) serves W = 16 NN meshes of one vector
+// The NN-specialised forward-mode walk (k_fwddiff_nn) serves W >= 8 NN meshes of one vector
 // per wave when the batch gives every CU an eight-wave workgroup (its LDS ring holds three
 // layers of both row tables).  Measured against the general tile (MI355X, L = N): MZI N=1024
-// B=2048 1.654 ms against 2.345, N=768 1.233 against 1.808, orth N=1024 0.829 against 1.83; at
-// B=256 (32 workgroups) 1.377 against 1.109.
+// B=2048 1.654 ms against 2.345, N=768 1.233 against 1.808, orth N=1024 0.829 against 1.83;
+// N=512 B=4096 0.785 against 0.978, B=8192 1.40 against 1.81, N=300 0.454 against 0.579, orth
+// N=512 0.373 against 0.505.  Not at N=1024 B=256 (32 workgroups: 1.377 against 1.109) nor at
+// W = 4 (one vector per wave in four-wave workgroups: N=256 B=4096 0.246 against 0.214, B=16384
+// 0.780 against 0.747, orth 0.119 against 0.101).
 inline bool nn_fwddiff(const Plan& P) {
-    return P.nn && !P.fft && P.LPV == 64 && P.W == 16 && P.V == 1 && (long)P.B >= 8L * P.ncu;
+    return P.nn && !P.fft && P.LPV == 64 && P.W >= 8 && (long)P.B >= 8L * P.ncu;
 }
 inline size_t nn_fwddiff_lds(const Plan& P) { return (size_t)3 * 2 * P.NP * P.rowbytes; }
 

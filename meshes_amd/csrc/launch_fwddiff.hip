@@ -8,14 +8,15 @@ template <class T, int W, int V>
 static int go(const Plan& P, const int* kind, const void* rows, const void* drows, const void* u_in,
               const void* du_in, void* u_out, void* du_out, int ldu, hipStream_t st) {
     typedef typename RowOf<T>::type R;
-    if constexpr (W == 16 && V == 1) {
+    if constexpr (W >= 8) {
         if (nn_fwddiff(P)) {
-            // One vector per wave: B / 8 eight-wave workgroups, each through the whole depth.
+            // One vector per wave: one workgroup per NW vectors, each through the whole depth.
+            constexpr int NW = kFwdDiffNNWaves;
             static std::atomic<unsigned long long> lds_set{0};
             const size_t shmem = nn_fwddiff_lds(P);
             if (int rc = lds_limit_once((const void*)k_fwddiff_nn<T, W>, lds_set, shmem)) return rc;
-            const long grid = ((long)P.B + kFwdDiffNNWaves - 1) / kFwdDiffNNWaves;
-            hipLaunchKernelGGL((k_fwddiff_nn<T, W>), dim3((unsigned)grid), dim3(64 * kFwdDiffNNWaves), shmem, st,
+            const long grid = ((long)P.B + NW - 1) / NW;
+            hipLaunchKernelGGL((k_fwddiff_nn<T, W>), dim3((unsigned)grid), dim3(64 * NW), shmem, st,
                                P.N, P.L, P.B, kind, (const R*)rows, (const R*)drows, (const T*)u_in, (const T*)du_in,
                                (T*)u_out, (T*)du_out, ldu);
             return MESHPROP_OK;

@@ -239,10 +239,11 @@ def test_nn_forward(gpu, case):
     assert O.rel_l2(_np(res[0][0]), ref) < TOL
 
 
-# Staged forward-mode walk (W = 16, one vector per wave, a batch that gives every CU a
+# Staged forward-mode walk (W >= 8, one vector per wave, a batch that gives every CU a
 # workgroup): odd and even depths, one and two layers, complex and real fields.
 FWDDIFF = [("mzi", 1024, 9, 2100), ("sym", 700, 12, 2048), ("orth", 1024, 7, 2048), ("mzi", 1024, 1, 2048),
-           ("mzi", 1000, 2, 2064), ("mzi_generic", 800, 6, 2048)]
+           ("mzi", 1000, 2, 2064), ("mzi_generic", 800, 6, 2048), ("mzi", 512, 11, 4100), ("orth", 300, 8, 2048),
+           ("sym", 400, 3, 2048)]
 
 
 @pytest.mark.parametrize("case", FWDDIFF, ids=["-".join(map(str, c)) for c in FWDDIFF])
