@@ -207,8 +207,9 @@ enum meshprop_flags {
  * Entry fields.  op: MESHPROP_OP_FWDPROP (u_in -> u_out), MESHPROP_OP_BACKDIFF (u_out,
  * dJdu_out or target -> dp, dJdu_in and, if u_in is set, the reconstructed input written
  * to u_in) or MESHPROP_OP_FWDGRAD (u_in -> u_out, then the adjoint of that output).
- * Gradient entries need a unitary crossing (MESHPROP_MZI, MESHPROP_SYM); forward entries
- * take any complex crossing.  MESHPROP_ORTH (real fields) is not grouped.  target, when
+ * Gradient entries need a unitary crossing (MESHPROP_MZI, MESHPROP_SYM, MESHPROP_ORTH);
+ * forward entries take any crossing.  MESHPROP_ORTH entries carry float32 fields, the others
+ * complex64; one plan may mix them.  target, when
  * set, replaces dJdu_out by 2 (u_out - target).  dp is accumulated into (or, with
  * MESHPROP_GROUP_DP_ASSIGN in flags, overwritten on its N/2 slots per layer).  tile.w, tile.v
  * and tile.chunk pick the tile (0 = default); tile.waves is ignored.  tables, when set,

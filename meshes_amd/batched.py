@@ -56,9 +56,8 @@ class GroupJob:
     def __post_init__(self):
         assert self.op in OPS, self.op
         m = self.mesh
-        assert m.X != "orth", "real (orth) meshes are not grouped"
         if self.op != "fwd":
-            assert m.X in ("mzi", "sym"), "gradient jobs need unitary crossings"
+            assert m.X in ("mzi", "sym", "orth"), "gradient jobs need unitary crossings"
         ref = self.x if self.op != "backdiff" else self.y
         assert ref is not None and ref.ndim == 2 and ref.shape[1] == m.N
         self.B = int(ref.shape[0])
@@ -66,11 +65,11 @@ class GroupJob:
         for name in ("x", "y", "target", "dy", "gx"):
             t = getattr(self, name)
             if t is not None:
-                assert t.shape == (self.B, m.N) and t.dtype == torch.complex64 and t.is_contiguous(), name
+                assert t.shape == (self.B, m.N) and t.dtype == m.dtype and t.is_contiguous(), name
                 assert not (t.is_conj() or t.is_neg()), f"{name}: resolve lazy conjugate/negative views first"
                 assert t.device == dev, name
         if self.y is None:
-            self.y = torch.empty((self.B, m.N), dtype=torch.complex64, device=dev)
+            self.y = torch.empty((self.B, m.N), dtype=m.dtype, device=dev)
         if self.op != "fwd":
             assert self.target is not None or self.dy is not None, "gradient jobs need target or dy"
             if self.dp is None:

@@ -35,8 +35,12 @@ constexpr size_t kGroupLds = kGroupStage + 16;  // stage + the queue slot
 enum { G_FWD = 1, G_GRAD = 2 };  // what an item runs
 enum { B_KIND = 1, B_FWD = 2, B_INV = 4, B_DTDP = 8 };
 
+// Row tables, K tables and fields are typed for complex meshes (float4 rows, float2 fields);
+// the descriptor of a real (orth) mesh holds the same buffers as float2 rows, float K
+// coefficients and float fields, read through casts where real is set.
 struct GMesh {
     int what, var, X, N, L, B, fft, LPV, NP, C, np, ns, ldp, lds, ldu, S, build, assign;
+    int real;
     int nnadj;  // gradient items run the NN-specialised adjoint (NN meshes, one 64-lane vector group)
     int kso;    // > 0: K table and dp partials in the NN adjoint's stage order (W >= 4 NN meshes)
     long ntiles, E;
@@ -108,8 +112,13 @@ __global__ void __launch_bounds__(256) k_group_prologue(const unsigned char* __r
             ta = M.rows_inv;
         }
         if (M.build & B_DTDP) tabs |= TAB_KUNI;  // grouped meshes are unitary: K-form
-        slot_tables_at<float4>(M.X, M.N, M.NP, M.LPV, M.fft, M.g0, M.g1, M.p, M.ldp, M.s, M.lds, nullptr, tabs,
-                               M.kind, ta, tb, M.dtdp, nullptr, l, m, M.kso);
+        if (M.real)
+            slot_tables_at<float2>(M.X, M.N, M.NP, M.LPV, M.fft, M.g0, M.g1, M.p, M.ldp, M.s, M.lds, nullptr, tabs,
+                                   M.kind, reinterpret_cast<float2*>(ta), reinterpret_cast<float2*>(tb),
+                                   reinterpret_cast<float*>(M.dtdp), nullptr, l, m, M.kso);
+        else
+            slot_tables_at<float4>(M.X, M.N, M.NP, M.LPV, M.fft, M.g0, M.g1, M.p, M.ldp, M.s, M.lds, nullptr, tabs,
+                                   M.kind, ta, tb, M.dtdp, nullptr, l, m, M.kso);
     }
 }
 
@@ -178,53 +187,66 @@ __global__ void __launch_bounds__(256) k_group_reduce(const unsigned char* __res
 // staged through LDS (fwd_tile_nn, its three-slot ring in the stage buffer, which no adjoint
 // uses until the forward's last barrier): each wave reading 16 KiB of rows per layer through
 // the address unit bounds the general tile.
-template <int W, int V, bool NN>
+// T: float2 (complex fields) or float (real fields, orth).
+template <class T, int W, int V, bool NN>
 __device__ __noinline__ void group_fwd_tile(KMesh* Mp, const long tile_v, float4* stage) {
+    typedef typename RowOf<T>::type R;
     KMesh& M = *uniform_ptr(Mp);
     const long tile = (long)__builtin_amdgcn_readfirstlane((unsigned)tile_v);
+    const R* rows = reinterpret_cast<const R*>(M.rows_fwd);
+    const T* u_in = reinterpret_cast<const T*>(M.u_in);
+    T* u_out = reinterpret_cast<T*>(M.u_out);
     if constexpr (NN) {
         static_assert(W == 16 && V == 1, "the NN family holds the W = 16 tile");
         static_assert(3 * 64 * W * sizeof(float4) <= kGroupStage, "forward row ring inside the stage buffer");
         static_assert(kFamWaves[1] == 4, "fwd_tile_nn copies rows with 256 threads");
-        fwd_tile_nn<float2, 16, 1>(M.N, 0, M.L, M.B, (KInt*)M.kind, M.rows_fwd, M.u_in, M.u_out, M.ldu, tile,
-                                   Ckpt{nullptr, 0, 0}, stage);
+        fwd_tile_nn<T, 16, 1>(M.N, 0, M.L, M.B, (KInt*)M.kind, rows, u_in, u_out, M.ldu, tile, Ckpt{nullptr, 0, 0},
+                              reinterpret_cast<R*>(stage));
     } else {
-        fwd_tile<float2, W, V, KInt*>(M.N, 0, M.L, M.B, M.LPV, M.NP, (KInt*)M.kind, M.rows_fwd, M.u_in, M.u_out,
-                                      M.ldu, tile);
+        fwd_tile<T, W, V, KInt*>(M.N, 0, M.L, M.B, M.LPV, M.NP, (KInt*)M.kind, rows, u_in, u_out, M.ldu, tile);
     }
 }
 
 // NN: the W = 16 NN family (one-layer stages in stage order).  In family 0, NN meshes of one
 // 64-lane vector group (nnadj) run the NN-specialised adjoint as the plain path does: stages
 // of 16 / W layers, four waves (64 KiB of LDS), entries in stage order at W >= 4.
-template <int W, int V, bool NN = false>
-__device__ __forceinline__ void group_item(KMesh& M, const int slot, float4* stage) {
+template <class T, int W, int V, bool NN = false>
+__device__ __forceinline__ void group_item(KMesh& M, const int slot, float4* stage_raw) {
+    typedef typename RowOf<T>::type R;
     float* part = (M.what & G_GRAD) ? M.partial + (size_t)slot * M.E : nullptr;
+    const R* rows_inv = reinterpret_cast<const R*>(M.rows_inv);
+    const T* ktab = reinterpret_cast<const T*>(M.dtdp);
+    const T* u_out = reinterpret_cast<const T*>(M.u_out);
+    const T* g_out = reinterpret_cast<const T*>(M.g_out);
+    const T* target = reinterpret_cast<const T*>(M.target);
+    T* u_rec = reinterpret_cast<T*>(M.u_rec);
+    T* g_in = reinterpret_cast<T*>(M.g_in);
+    R* stage = reinterpret_cast<R*>(stage_raw);
     bool first = true;
     for (long tile = slot; tile < M.ntiles; tile += M.S) {
         if (M.what & G_FWD)
-            group_fwd_tile<W, V, NN>(&M, tile, stage);
+            group_fwd_tile<T, W, V, NN>(&M, tile, stage_raw);
         if (M.what & G_GRAD) {
             if constexpr (NN) {
-                bwd_tile_nn<float2, 16, 1, kFamWaves[1], 1, false>(
-                    M.N, 0, M.L, M.B, (KInt*)M.kind, M.rows_inv, M.dtdp, part, M.u_out, M.g_out, M.target, M.u_rec,
-                    M.g_in, M.ldu, tile, first, stage, Ckpt{nullptr, 0, 0}, nullptr);
+                bwd_tile_nn<T, 16, 1, kFamWaves[1], 1, false>(M.N, 0, M.L, M.B, (KInt*)M.kind, rows_inv, ktab, part,
+                                                              u_out, g_out, target, u_rec, g_in, M.ldu, tile, first,
+                                                              stage, Ckpt{nullptr, 0, 0}, nullptr);
             } else {
                 if constexpr (W <= 8) {
                     static_assert(nn_lds_bytes(W, kFamWaves[0], nn_stage_depth(W), false, sizeof(float4)) <=
                                       kGroupStage,
                                   "NN adjoint stage inside the group stage");
                     if (M.nnadj) {
-                        bwd_tile_nn<float2, W, V, kFamWaves[0], nn_stage_depth(W), false, false>(
-                            M.N, 0, M.L, M.B, (KInt*)M.kind, M.rows_inv, M.dtdp, part, M.u_out, M.g_out, M.target,
-                            M.u_rec, M.g_in, M.ldu, tile, first, stage, Ckpt{nullptr, 0, 0}, nullptr);
+                        bwd_tile_nn<T, W, V, kFamWaves[0], nn_stage_depth(W), false, false>(
+                            M.N, 0, M.L, M.B, (KInt*)M.kind, rows_inv, ktab, part, u_out, g_out, target, u_rec, g_in,
+                            M.ldu, tile, first, stage, Ckpt{nullptr, 0, 0}, nullptr);
                         first = false;
                         continue;
                     }
                 }
-                bwd_tile<float2, W, V, true, KInt*>(M.N, 0, M.L, M.B, M.LPV, M.NP, M.C, (KInt*)M.kind, M.rows_inv,
-                                                    M.rows_inv, M.dtdp, part, M.u_out, M.g_out, M.target, M.u_rec,
-                                                    M.g_in, M.ldu, tile, first, stage);
+                bwd_tile<T, W, V, true, KInt*>(M.N, 0, M.L, M.B, M.LPV, M.NP, M.C, (KInt*)M.kind, rows_inv, rows_inv,
+                                               ktab, part, u_out, g_out, target, u_rec, g_in, M.ldu, tile, first,
+                                               stage);
             }
         }
         first = false;
@@ -232,7 +254,7 @@ __device__ __forceinline__ void group_item(KMesh& M, const int slot, float4* sta
 }
 
 template <int FAM>
-__global__ void __launch_bounds__(64 * kFamWaves[FAM]) __attribute__((amdgpu_waves_per_eu(FAM == 1 ? 2 : 1)))
+__global__ void __launch_bounds__(64 * kFamWaves[FAM]) __attribute__((amdgpu_waves_per_eu(FAM <= 1 ? 2 : 1)))
 k_group(unsigned char* __restrict__ ws) {
     GHeader* H = reinterpret_cast<GHeader*>(ws);
     KMesh* meshes = (KMesh*)(ws + H->off_mesh);
@@ -252,19 +274,34 @@ k_group(unsigned char* __restrict__ ws) {
         const int wm = items[2 * it], ws_slot = items[2 * it + 1];
         KMesh& M = meshes[wm];
         if constexpr (FAM == 0) {
-            switch (M.var) {
-                case 0: group_item<2, 1>(M, ws_slot, stage); break;
-                case 1: group_item<2, 2>(M, ws_slot, stage); break;
-                case 2: group_item<2, 4>(M, ws_slot, stage); break;
-                case 3: group_item<4, 1>(M, ws_slot, stage); break;
-                case 4: group_item<4, 2>(M, ws_slot, stage); break;
-                case 5: group_item<8, 1>(M, ws_slot, stage); break;
-                default: break;
+            if (M.real) {
+                switch (M.var) {
+                    case 0: group_item<float, 2, 1>(M, ws_slot, stage); break;
+                    case 1: group_item<float, 2, 2>(M, ws_slot, stage); break;
+                    case 2: group_item<float, 2, 4>(M, ws_slot, stage); break;
+                    case 3: group_item<float, 4, 1>(M, ws_slot, stage); break;
+                    case 4: group_item<float, 4, 2>(M, ws_slot, stage); break;
+                    case 5: group_item<float, 8, 1>(M, ws_slot, stage); break;
+                    default: break;
+                }
+            } else {
+                switch (M.var) {
+                    case 0: group_item<float2, 2, 1>(M, ws_slot, stage); break;
+                    case 1: group_item<float2, 2, 2>(M, ws_slot, stage); break;
+                    case 2: group_item<float2, 2, 4>(M, ws_slot, stage); break;
+                    case 3: group_item<float2, 4, 1>(M, ws_slot, stage); break;
+                    case 4: group_item<float2, 4, 2>(M, ws_slot, stage); break;
+                    case 5: group_item<float2, 8, 1>(M, ws_slot, stage); break;
+                    default: break;
+                }
             }
         } else if constexpr (FAM == 1) {
-            group_item<16, 1, true>(M, ws_slot, stage);
+            group_item<float2, 16, 1, true>(M, ws_slot, stage);
         } else {
-            group_item<16, 1>(M, ws_slot, stage);
+            if (M.real)
+                group_item<float, 16, 1>(M, ws_slot, stage);
+            else
+                group_item<float2, 16, 1>(M, ws_slot, stage);
         }
         if (threadIdx.x == 0) s_next = (int)gridDim.x + atomicAdd(&H->grab[FAM], 1);
         __syncthreads();
@@ -287,7 +324,7 @@ using namespace mp;
 
 struct meshprop_group {
     struct Info {
-        int W, V, LPV, NP, C, S, var, fam, what, build, np, ns, nnadj, kso;
+        int W, V, LPV, NP, C, S, var, fam, what, build, np, ns, nnadj, kso, real;
         long ntiles, E, pro_units, red_units;
         bool own_tables;
         size_t off_tab, off_partial;  // workspace offsets (tables when own_tables)
@@ -389,7 +426,6 @@ int check_entry(const meshprop_group_entry& e) {
     if (e.op != MESHPROP_OP_FWDPROP && e.op != MESHPROP_OP_BACKDIFF && e.op != MESHPROP_OP_FWDGRAD)
         return MESHPROP_ERR_ARG;
     if (e.X < 0 || e.X > 3) return MESHPROP_ERR_ARG;
-    if (e.X == X_ORTH) return MESHPROP_ERR_UNSUPPORTED;
     if (e.op != MESHPROP_OP_FWDPROP && e.X == X_GENERIC) return MESHPROP_ERR_UNSUPPORTED;
     if (e.N <= 0 || (e.N & 1) || e.L < 0 || e.B < 0) return MESHPROP_ERR_ARG;
     if (e.N > 1024) return MESHPROP_ERR_UNSUPPORTED;
@@ -461,6 +497,7 @@ int plan_group(meshprop_group& g) {
         // adjoint (the W = 16 ones as family 1, with one-layer stages): K table and partials in
         // its stage order at W >= 4.
         f.nnadj = (!e.fft && f.LPV == 64 && (e.X == X_MZI || e.X == X_SYM) && (f.W < 16 || f.V == 1)) ? 1 : 0;
+        f.real = (e.X == X_ORTH) ? 1 : 0;  // real meshes run the general adjoint (families 0 and 2)
         f.kso = (f.nnadj && nn_stage_order(f.W)) ? f.W / 2 : 0;
         f.fam = kGroupTiles[f.var].fam == 0 ? 0 : f.nnadj ? 1 : 2;
         const int nw = kFamWaves[f.fam];
@@ -570,6 +607,7 @@ void build_image(const meshprop_group& g, unsigned char* ws) {
         m.NP = f.NP;
         m.C = f.C;
         m.nnadj = f.nnadj;
+        m.real = f.real;
         m.kso = f.kso;
         m.np = f.np;
         m.ns = f.ns;
@@ -640,7 +678,7 @@ size_t meshprop_group_workspace_size(const meshprop_group* g) { return g ? g->to
 
 size_t meshprop_group_tables_size(int X, int N, int L, int fft, const meshprop_tile* tile) {
     (void)fft;
-    if (X < 0 || X > 3 || X == X_ORTH || N <= 0 || (N & 1) || N > 1024 || L < 0) return 0;
+    if (X < 0 || X > 3 || N <= 0 || (N & 1) || N > 1024 || L < 0) return 0;
     meshprop_tile t = tile ? *tile : meshprop_tile{0, 0, 0, 0};
     size_t k, f, i, d, total;
     table_layout(N, L, group_w(N, t), k, f, i, d, total);

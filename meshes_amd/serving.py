@@ -37,7 +37,7 @@ class Request:
     rid: int
     mesh_id: int
     op: str                       # 'fwd' or 'grad'
-    x: torch.Tensor               # (B, N) complex64 on the device
+    x: torch.Tensor               # (B, N) complex64 on the device (float32 for orth meshes)
     target: Optional[torch.Tensor] = None
     t_arrive: float = 0.0         # seconds after the stream starts
     y: Optional[torch.Tensor] = None
@@ -106,7 +106,7 @@ class MeshServer:
         self.direct_launches = 0
         # Row tables of every mesh, built once and kept resident.
         self.tables = [tables_for(m) for m in self.meshes]
-        build = [GroupJob(m, "fwd", x=torch.empty((0, m.N), dtype=torch.complex64, device=self.device),
+        build = [GroupJob(m, "fwd", x=torch.empty((0, m.N), dtype=m.dtype, device=self.device),
                           tables=t, build_tables=True) for m, t in zip(self.meshes, self.tables)]
         g = MeshGroup(build)
         g.run()
@@ -115,7 +115,7 @@ class MeshServer:
         # Per-mesh entry templates (everything but the request's op, batch and buffers).
         self._tmpl = []
         for m, t in zip(self.meshes, self.tables):
-            e = _entry(GroupJob(m, "fwd", x=torch.empty((0, m.N), dtype=torch.complex64, device=self.device),
+            e = _entry(GroupJob(m, "fwd", x=torch.empty((0, m.N), dtype=m.dtype, device=self.device),
                                 tables=t, build_tables=False))
             e.flags = _lib.GROUP_DP_ASSIGN
             self._tmpl.append(e)
@@ -128,26 +128,32 @@ class MeshServer:
         return request_footprint(self.meshes[mesh_id], op, B, self.ncu)
 
     # -- graphs -----------------------------------------------------------------------------
+    @staticmethod
+    def _family(m) -> int:
+        """Persistent kernel family of a mesh's grouped items: 0 for N <= 512, 1 for wider NN
+        meshes of MZI / SYM crossings, 2 for the other wide meshes."""
+        if m.N <= 512:
+            return 0
+        return 1 if (not m.fft and m.X in ("mzi", "sym")) else 2
+
     def _capture_slots(self):
         """Capture each slot's launch once per signature a batch can have: forward-only or
-        with gradients, with meshes up to N = 512, above it, or both."""
-        narrow = [i for i, m in enumerate(self.meshes) if m.N <= 512][:1]
-        wide = [i for i, m in enumerate(self.meshes) if m.N > 512][:1]
+        with gradients, with items of any non-empty set of the kernel families present."""
+        rep = {}
+        for i, m in enumerate(self.meshes):
+            rep.setdefault(self._family(m), i)
+        fams = sorted(rep)
         combos = []
         for op in ("fwd", "grad"):
-            if narrow:
-                combos.append([(narrow[0], op)])
-            if wide:
-                combos.append([(wide[0], op)])
-            if narrow and wide:
-                combos.append([(narrow[0], op), (wide[0], op)])
+            for mask in range(1, 1 << len(fams)):
+                combos.append([(rep[f], op) for k, f in enumerate(fams) if mask >> k & 1])
         for s in self.slots:
             with torch.cuda.stream(s.stream):
                 for combo in combos:
                     reqs = []
                     for mid, op in combo:
                         N = self.meshes[mid].N
-                        x = torch.zeros((1, N), dtype=torch.complex64, device=self.device)
+                        x = torch.zeros((1, N), dtype=self.meshes[mid].dtype, device=self.device)
                         reqs.append(Request(-1, mid, op, x, torch.zeros_like(x) if op == "grad" else None))
                     self._plan(s, reqs)
                     s.group.capture()
@@ -168,7 +174,7 @@ class MeshServer:
                 e = ents[k]
                 e.B = r.B
                 e.u_in = r.x.data_ptr()
-                r.y = torch.empty((r.B, m.N), dtype=torch.complex64, device=self.device)
+                r.y = torch.empty((r.B, m.N), dtype=m.dtype, device=self.device)
                 e.u_out = r.y.data_ptr()
                 if r.op == "grad":
                     r.dp = torch.empty(m.n_phase, dtype=torch.float32, device=self.device)
@@ -202,7 +208,7 @@ class MeshServer:
         ldp, lds = m.N // 2 * m.X_np, m.N // 2 * m.X_ns
         sp = m.p_splitter if m.p_splitter.numel() else None
         with torch.cuda.stream(slot.stream):
-            r.y = torch.empty((r.B, m.N), dtype=torch.complex64, device=self.device)
+            r.y = torch.empty((r.B, m.N), dtype=m.dtype, device=self.device)
             r.screen = None
             if r.op == "grad":
                 r.dp = torch.empty(m.n_phase, dtype=torch.float32, device=self.device)
@@ -310,7 +316,7 @@ class MeshServer:
             need = max(_lib.workspace_size(op, m.X, m.N, m.L, r.B, m.fft) for op in ops)
             if ws.numel() < need:
                 ws = torch.empty(need, dtype=torch.uint8, device=self.device)
-            r.y = torch.empty((r.B, m.N), dtype=torch.complex64, device=self.device)
+            r.y = torch.empty((r.B, m.N), dtype=m.dtype, device=self.device)
             _lib.run(_lib.OP_FWDPROP, m.X, m.N, m.L, r.B, m.fft, g0, g1, m.p_phase, None, ldp, s, lds, r.x, None,
                      r.y, None, m.N, workspace=ws, ws_bytes=need)
             if r.op == "grad":
@@ -328,7 +334,7 @@ def request_footprint(m: MeshNetworkGPU, op: str, B: int, ncu: int) -> int:
     and, for gradients, its dp and dp partials (one slab per partial slot of the grouped
     plan: min(vector tiles, 2 x CUs) slabs of L x N/2 x 2 floats), plus its share of the
     plan (descriptor, queue entries, alignment)."""
-    field_bytes = B * m.N * 8
+    field_bytes = B * m.N * m.dtype.itemsize
     W = _width(m.N)
     lpv = _lpv(m.N, W)
     waves = 2 if W == 16 else 4
@@ -379,7 +385,7 @@ def poisson_requests(meshes: List[MeshNetworkGPU], rate: float, n: int, seed: in
         op = "grad" if rng.uniform() < grad_frac else "fwd"
 
         def unit():
-            z = torch.randn((B, m.N), dtype=torch.complex64, device=dev, generator=gen)
+            z = torch.randn((B, m.N), dtype=m.dtype, device=dev, generator=gen)
             return z / torch.linalg.norm(z, dim=1, keepdim=True)
         out.append(Request(k, mid, op, unit(), unit() if op == "grad" else None, float(t[k])))
     return out

@@ -159,18 +159,23 @@ def test_group_plan_without_gpu(lib):
         assert lib.meshprop_group_launch(plan, ctypes.c_void_p(4096 + 8), 1 << 40, None) == 2
     finally:
         lib.meshprop_group_destroy(plan)
-    bad = [dict(N=63), dict(op=L.OP_FWDDIFF), dict(X=L.ORTH), dict(X=L.MZI_GENERIC), dict(N=2048), dict(ldu=32),
+    bad = [dict(N=63), dict(op=L.OP_FWDDIFF), dict(X=L.MZI_GENERIC), dict(N=2048), dict(ldu=32),
            dict(target=None), dict(dp=None), dict(u_in=None), dict(ldp=10)]
-    want = [1, 1, 3, 3, 3, 1, 1, 1, 1, 1]
+    want = [1, 1, 3, 3, 1, 1, 1, 1, 1]
     for kw, rc in zip(bad, want):
         one = (L.GroupEntry * 1)(_entry(L, **kw))
         assert lib.meshprop_group_create(one, 1, ctypes.byref(plan)) == rc, kw
+    # real-field (orth) gradient entries are grouped
+    one = (L.GroupEntry * 1)(_entry(L, X=L.ORTH))
+    assert lib.meshprop_group_create(one, 1, ctypes.byref(plan)) == 0
+    lib.meshprop_group_destroy(plan)
     # forward entries take the lossy crossing; tables size covers kind + two row tables + dT/dp
     one = (L.GroupEntry * 1)(_entry(L, op=L.OP_FWDPROP, X=L.MZI_GENERIC, ldp=64))
     assert lib.meshprop_group_create(one, 1, ctypes.byref(plan)) == 0
     lib.meshprop_group_destroy(plan)
     assert lib.meshprop_group_tables_size(L.MZI, 256, 256, 0, None) >= 256 * 256 * 16 * 2 + 256 * 128 * 64
-    assert lib.meshprop_group_tables_size(L.ORTH, 256, 256, 0, None) == 0
+    assert lib.meshprop_group_tables_size(L.ORTH, 256, 256, 0, None) >= 256 * 256 * 8 * 2 + 256 * 128 * 4
+    assert lib.meshprop_group_tables_size(L.ORTH, 2048, 16, 0, None) == 0
 
 
 def test_tile_table_consulted_without_gpu(lib):

@@ -34,6 +34,11 @@ def _c64(a):
     return torch.as_tensor(np.ascontiguousarray(a.astype(np.complex64))).cuda()
 
 
+def _field(a, real):
+    """Device field: complex64, or float32 for real (orth) meshes."""
+    return torch.as_tensor(np.ascontiguousarray(a.astype(np.float32 if real else np.complex64))).cuda()
+
+
 class Spec:
     """One job with its FP64 reference."""
 
@@ -44,24 +49,27 @@ class Spec:
         n_p, n_s = O.N_PHASE[X], O.N_SPLIT[X]
         self.p = rng.uniform(0, 2 * np.pi, (g.L, N // 2, n_p)).astype(np.float32)
         self.s = (0.05 * rng.standard_normal((g.L, N // 2, n_s))).astype(np.float32) if (errors and n_s) else None
-        self.x = O.unit_cn(rng, B, N).astype(np.complex64)
-        self.t = O.unit_cn(rng, B, N).astype(np.complex64)
-        self.dy = (rng.standard_normal((B, N)) + 1j * rng.standard_normal((B, N))).astype(np.complex64)
+        self.real = real = O.REAL_FIELD[X]
+        fdt = np.float32 if real else np.complex64
+        self.x = O.unit_cn(rng, B, N, real=real).astype(fdt)
+        self.t = O.unit_cn(rng, B, N, real=real).astype(fdt)
+        dy = rng.standard_normal((B, N))
+        self.dy = (dy if real else dy + 1j * rng.standard_normal((B, N))).astype(fdt)
         self.want_gx = gx
         self.mesh = _mesh(g, X, self.p, self.s)
 
     def job(self, **kw):
         from meshes_amd.batched import GroupJob
-        x = _c64(self.x)
+        x = _field(self.x, self.real)
         if self.op == "fwd":
             return GroupJob(self.mesh, "fwd", x=x, **kw)
         gx = torch.empty_like(x) if self.want_gx else None
-        tgt = _c64(self.t) if self.loss == "target" else None
-        dy = _c64(self.dy) if self.loss == "dy" else None
+        tgt = _field(self.t, self.real) if self.loss == "target" else None
+        dy = _field(self.dy, self.real) if self.loss == "dy" else None
         if self.op == "grad":
             return GroupJob(self.mesh, "grad", x=x, target=tgt, dy=dy, gx=gx, **kw)
         # backdiff: y given (the forward output), x receives the reconstruction
-        y = _c64(self.ref_y())
+        y = _field(self.ref_y(), self.real)
         return GroupJob(self.mesh, "backdiff", x=torch.empty_like(x), y=y, target=tgt, dy=dy, gx=gx, **kw)
 
     def ref_y(self):
@@ -93,6 +101,8 @@ MIX = [
     ("sym", "nn", 48, 12, "grad"), ("mzi", "nn", 200, 7, "backdiff"), ("mzi_generic", "nn", 96, 11, "fwd"),
     ("mzi", "clements", 1024, 2, "grad"), ("mzi", "butterfly", 1024, 6, "grad"),
     ("mzi", "nn", 700, 9, "backdiff"), ("sym", "clements", 1024, 5, "grad"),
+    ("orth", "clements", 128, 9, "grad"), ("orth", "nn", 700, 5, "grad"), ("orth", "butterfly", 64, 8, "fwd"),
+    ("orth", "butterfly", 1024, 3, "grad"), ("orth", "nn", 300, 6, "backdiff"),
 ]
 
 

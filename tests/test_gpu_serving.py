@@ -72,6 +72,62 @@ def test_served_equals_alone_plain_and_oracle(gpu):
     assert checked >= 10
 
 
+def test_served_orth_and_fft_meshes(gpu):
+    """Real-field (orth) meshes and FFT meshes beside NN ones: every kernel family of the grouped
+    launch (N <= 512; wider NN meshes of MZI crossings; the other wide meshes) in the captured
+    graphs of the slots.  Served results equal the same request alone (bitwise), the sequential
+    plain launches and, on the small orth meshes, the FP64 oracle (1e-5)."""
+    from meshes_amd import MeshNetworkGPU, geometry
+    from meshes_amd.batched import GroupJob, MeshGroup
+    from meshes_amd.serving import MeshServer, poisson_requests
+    rng = np.random.RandomState(12)
+    meshes, geoms = [], []
+    for X, kind, N in [("orth", "clements", 40), ("orth", "clements", 200), ("orth", "clements", 700),
+                       ("mzi", "butterfly", 128), ("mzi", "butterfly", 1024), ("mzi", "clements", 600)]:
+        if kind == "clements":
+            g = O.clements(N)
+            kw = {}
+        else:
+            g = O.Geometry(N, int(np.log2(N)), strides=np.arange(int(np.log2(N))))
+            kw = dict(strides=g.strides)
+        n_p = O.N_PHASE[X]
+        p = rng.uniform(0, 2 * np.pi, (g.L, N // 2, n_p)).astype(np.float32)
+        meshes.append(MeshNetworkGPU(N, g.L, g.lens, g.shifts, p, np.zeros(0), X, is_phase=False, **kw))
+        geoms.append((g, p))
+    server = MeshServer(meshes, hbm_budget=3 << 30, slots=3, direct=False)
+    reqs = poisson_requests(meshes, rate=50000.0, n=60, seed=9, b_max=200)
+    done, st = server.serve(reqs)
+    assert st["requests"] == 60
+    seq, _ = server.serve_sequential(_clone(reqs))
+    checked = 0
+    for r, q in zip(done, seq):
+        m = meshes[r.mesh_id]
+        assert r.y.dtype == m.dtype
+        if r.op == "grad":
+            job = GroupJob(m, "grad", x=r.x, target=r.target, tables=server.tables[r.mesh_id], build_tables=False)
+        else:
+            job = GroupJob(m, "fwd", x=r.x, tables=server.tables[r.mesh_id], build_tables=False)
+        MeshGroup([job]).run()
+        torch.cuda.synchronize()
+        assert torch.equal(job.y, r.y), (r.rid, m.X, m.N, r.op)
+        assert O.rel_l2(q.y.cpu().numpy(), r.y.cpu().numpy()) < TOL
+        if r.op == "grad":
+            assert torch.equal(job.dp, r.dp), (r.rid, m.X, m.N)
+            assert O.rel_l2(q.dp.cpu().numpy(), r.dp.cpu().numpy()) < TOL
+        if m.X == "orth" and m.N <= 200:
+            g, p = geoms[r.mesh_id]
+            p64 = p.astype(np.float64)
+            x = r.x.cpu().numpy().astype(np.float64)
+            y = O.mesh_fwd(g, "orth", p64, None, x)
+            assert O.rel_l2(r.y.cpu().numpy(), y) < TOL
+            if r.op == "grad":
+                yg = r.y.cpu().numpy().astype(np.float64)
+                rdp, _, _ = O.mesh_vjp(g, "orth", p64, None, yg, 2 * (yg - r.target.cpu().numpy().astype(np.float64)))
+                assert O.rel_l2(r.dp.cpu().numpy().reshape(rdp.shape), rdp) < TOL
+            checked += 1
+    assert checked >= 5
+
+
 def test_admission_respects_budget(gpu):
     from meshes_amd.serving import MeshServer, poisson_requests, request_footprint
     meshes = _meshes(seed=4)
