@@ -21,7 +21,9 @@ CASES = [("mzi", "nn", 16, 9, 7), ("mzi", "clements", 64, 0, 20), ("sym", "nn", 
          ("sym", "fft", 256, 8, 9), ("mzi", "nn", 130, 5, 4200), ("sym", "nn", 300, 10, 70),
          ("mzi", "nn", 700, 9, 6), ("sym", "nn", 1000, 7, 5), ("mzi", "clements", 1024, 0, 3),
          ("orth", "nn", 96, 14, 9), ("orth", "clements", 128, 0, 8), ("orth", "nn", 250, 11, 33),
-         ("orth", "nn", 600, 8, 4), ("orth", "nn", 1024, 6, 5)]
+         ("orth", "nn", 600, 8, 4), ("orth", "nn", 1024, 6, 5), ("mzi", "clements", 2, 0, 3),
+         ("sym", "clements", 6, 0, 1), ("mzi", "nn", 514, 1, 9), ("orth", "clements", 4, 0, 5),
+         ("mzi", "clements", 130, 0, 2049)]
 
 
 def _geom(rng, kind, N, L):
