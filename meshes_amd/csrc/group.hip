@@ -148,7 +148,7 @@ __global__ void __launch_bounds__(256) k_group_reduce(const unsigned char* __res
             const long lm = idx / M.np;
             pos = (int)(lm % Nr);
             l = (int)(lm / Nr);
-            m = M.kso > 0 ? (pos % 64) * M.kso + pos / 64 : pos;
+            m = stage_slot(pos, M.kso);
             ok = m < Nh;
         }
         float acc = 0.f;

@@ -25,7 +25,7 @@ namespace mp {
 // every p, hence so is dK_k = (sum_m q_m dT/dp_m)^H dT/dp_k + T^H S_k with the contracted second
 // derivatives S_k = sum_m q_m d2T/dp_k dp_m; stored like K (float4 (a, d, c) per (layer,
 // parameter, slot), or the float c of a real crossing; zero for inactive slots).  kso > 0:
-// stage order, slot m at position (m % kso) * 64 + m / kso, as the K table of the NN walk.
+// the NN walk's stage order (stage_pos), as its K table.
 template <class KT>
 __global__ void k_dkform(int X, int N, int L, int NP, int fft, const int* __restrict__ g0, const int* __restrict__ g1,
                          const float* __restrict__ p, const float* __restrict__ q, int ldp, const float* __restrict__ s,
@@ -35,7 +35,7 @@ __global__ void k_dkform(int X, int N, int L, int NP, int fft, const int* __rest
     const int NPh = NP / 2;
     if (l >= L || m >= NPh) return;
     const int np = n_phase_of(X), ns = n_split_of(X);
-    const int km = kso > 0 ? (m % kso) * 64 + m / kso : m;
+    const int km = stage_pos(m, kso);
     KT* out = dk + (size_t)l * np * NPh + km;
     int i;
     if (fft) {
@@ -308,13 +308,32 @@ struct RowLoads {
     __amdgpu_buffer_rsrc_t rr, dr;
     unsigned off;
     int base;  // byte offset of the layer's k = 0 entries
-    __device__ __forceinline__ R r(const int k) const { return buf_load<R>(rr, off, base + k * 64 * (int)sizeof(R)); }
-    __device__ __forceinline__ R d(const int k) const { return buf_load<R>(dr, off, base + k * 64 * (int)sizeof(R)); }
+    int kst;   // bytes between the entries of consecutive k
+    __device__ __forceinline__ R r(const int k) const { return buf_load<R>(rr, off, base + k * kst); }
+    __device__ __forceinline__ R d(const int k) const { return buf_load<R>(dr, off, base + k * kst); }
 };
 
-template <class T, int W, bool ODD, class RS>
+// A vector split over two waves (halves of 64*W waveguides): the lanes at the split see the
+// other half's boundary waveguide (its layer inputs, exchanged through LDS at the layer barrier)
+// in place of the neighbour a lane shift would give, and half 0 computes the first waveguide of
+// half 1 itself where the crossing between the halves needs its new values (no second barrier).
+template <class T>
+struct HvpEdge {
+    typedef typename RowOf<T>::type R;
+    bool hi, lo;     // this lane is half 0's lane 63 / half 1's lane 0
+    bool half0;      // wave-uniform: this wave holds half 0
+    T u, du, g, dg;  // the other half's boundary waveguide (layer inputs)
+    R r, d;          // (half 0) the rows of waveguide 64*W, the first of half 1
+};
+__device__ __forceinline__ float2 tsel(const bool c, const float2 a, const float2 b) {
+    return make_float2(c ? a.x : b.x, c ? a.y : b.y);
+}
+__device__ __forceinline__ float tsel(const bool c, const float a, const float b) { return c ? a : b; }
+
+template <class T, int W, bool ODD, bool SPLIT, class RS>
 __device__ __forceinline__ void hvp_layer_nn(T (&u)[W], T (&du)[W], T (&g)[W], T (&dg)[W], const RS& rs,
-                                             typename RowOf<T>::type* wq, typename RowOf<T>::type* wdq) {
+                                             typename RowOf<T>::type* wq, typename RowOf<T>::type* wdq,
+                                             const HvpEdge<T>& e) {
     typedef typename RowOf<T>::type R;
     constexpr int H = W / 2, ES = 64;
     const T z = fzero(T());
@@ -345,9 +364,19 @@ __device__ __forceinline__ void hvp_layer_nn(T (&u)[W], T (&du)[W], T (&g)[W], T
     } else {
         // Lane-boundary waveguides: their new values from the neighbours' layer inputs (lanes
         // without a neighbour read 0; their rows are the identity), read before any update.
-        const T hu = lane_next(u[0]), hdu = lane_next(du[0]), hg = lane_next(g[0]), hdg = lane_next(dg[0]);
-        const T lu = lane_prev(u[W - 1]), ldu = lane_prev(du[W - 1]), lg = lane_prev(g[W - 1]),
-                ldg = lane_prev(dg[W - 1]);
+        T hu = lane_next(u[0]), hdu = lane_next(du[0]), hg = lane_next(g[0]), hdg = lane_next(dg[0]);
+        T lu = lane_prev(u[W - 1]), ldu = lane_prev(du[W - 1]), lg = lane_prev(g[W - 1]), ldg = lane_prev(dg[W - 1]);
+        if constexpr (SPLIT) {
+            hu = tsel(e.hi, e.u, hu);
+            hdu = tsel(e.hi, e.du, hdu);
+            hg = tsel(e.hi, e.g, hg);
+            hdg = tsel(e.hi, e.dg, hdg);
+            lu = tsel(e.lo, e.u, lu);
+            ldu = tsel(e.lo, e.du, ldu);
+            lg = tsel(e.lo, e.g, lg);
+            ldg = tsel(e.lo, e.dg, ldg);
+        }
+        const T uW = u[W - 1], duW = du[W - 1], gW = g[W - 1], dgW = dg[W - 1];
         const R rh = rs.r(W - 1), eh = rs.d(W - 1), rl = rs.r(0), el = rs.d(0);
         const T xh = rapply(rh, u[W - 1], hu), gh = rapply(rh, g[W - 1], hg);
         const T dxh = rapply_d(rh, eh, u[W - 1], hu, du[W - 1], hdu);
@@ -367,12 +396,27 @@ __device__ __forceinline__ void hvp_layer_nn(T (&u)[W], T (&du)[W], T (&g)[W], T
         dg[0] = dgl;
         // Crossing (W-1 of this lane, 0 of the next): this lane's rows of Q and dQ, and the
         // next lane's (computed there against this lane's reconstructed x, dx).
-        const T nx = lane_next(xl), ndx = lane_next(dxl), px = lane_prev(xh), pdx = lane_prev(dxh);
+        T nx = lane_next(xl), ndx = lane_next(dxl);
+        const T px = lane_prev(xh), pdx = lane_prev(dxh);
         const T A0 = cjim(gl, xl, z), B0 = cjfma(gl, px, z);
         const T dA0 = cjim(gl, dxl, cjim(dgl, xl, z)), dB0 = cjfma(gl, pdx, cjfma(dgl, px, z));
-        wq[(H - 1) * ES] = kquad(cjim(gh, xh, z), a_next(A0), cjfma(gh, nx, z), lane_next(B0));
-        wdq[(H - 1) * ES] = kquad(cjim(gh, dxh, cjim(dgh, xh, z)), a_next(dA0), cjfma(gh, ndx, cjfma(dgh, nx, z)),
-                                  lane_next(dB0));
+        T na = a_next(A0), nb = lane_next(B0), nda = a_next(dA0), ndb = lane_next(dB0);
+        if constexpr (SPLIT) {
+            if (e.half0) {
+                // Half 0's lane 63: the first waveguide of half 1, from its layer inputs (hu ..)
+                // and this lane's (uW ..), as half 1 computes it.
+                const T x5 = rapply(e.r, hu, uW), g5 = rapply(e.r, hg, gW);
+                const T dx5 = rapply_d(e.r, e.d, hu, uW, hdu, duW), dg5 = rapply_d(e.r, e.d, hg, gW, hdg, dgW);
+                nx = tsel(e.hi, x5, nx);
+                ndx = tsel(e.hi, dx5, ndx);
+                na = tsel(e.hi, cjim(g5, x5, z), na);
+                nb = tsel(e.hi, cjfma(g5, xh, z), nb);
+                nda = tsel(e.hi, cjim(g5, dx5, cjim(dg5, x5, z)), nda);
+                ndb = tsel(e.hi, cjfma(g5, dxh, cjfma(dg5, xh, z)), ndb);
+            }
+        }
+        wq[(H - 1) * ES] = kquad(cjim(gh, xh, z), na, cjfma(gh, nx, z), nb);
+        wdq[(H - 1) * ES] = kquad(cjim(gh, dxh, cjim(dgh, xh, z)), nda, cjfma(gh, ndx, cjfma(dgh, nx, z)), ndb);
     }
 }
 
@@ -382,8 +426,11 @@ __device__ __forceinline__ void hvp_layer_nn(T (&u)[W], T (&du)[W], T (&g)[W], T
 // the layer, its NW entries summed in wave order after it).  Rows: PF, those of the next
 // layer requested one layer ahead (two row register sets); JIT, each crossing's rows loaded
 // where it is applied; otherwise a whole layer's rows right before it.  part_g may be null (no
-// gradient).
-template <class T, int W, int NW, bool PF, bool JIT>
+// gradient).  SPL = 2: each vector split over two adjacent waves (halves of 64*W waveguides,
+// the row tables laid out for 128 lanes per vector, the stage-order positions of half h after
+// those of half 0), the boundary waveguides of the halves exchanged through LDS behind the
+// stage buffers ([2 layer parities][NW/2 vectors][2 halves][u, du, g, dg]).
+template <class T, int W, int NW, bool PF, bool JIT, int SPL = 1>
 __device__ __forceinline__ void hvp_nn_body(const int N, const int L, const int B, const int* __restrict__ kind,
                                             const typename RowOf<T>::type* __restrict__ rows_r,
                                             const typename RowOf<T>::type* __restrict__ rows_d,
@@ -395,32 +442,49 @@ __device__ __forceinline__ void hvp_nn_body(const int N, const int L, const int 
                                             const int ldu) {
     typedef typename RowOf<T>::type R;
     typedef typename KOf<T>::type KT;
-    constexpr int LPV = 64, NP = 64 * W, NPh = NP / 2;
+    static_assert(SPL == 1 || SPL == 2, "one or two waves per vector");
+    static_assert(SPL == 1 || (NW % 2 == 0 && !PF), "split vectors: whole vector pairs, rows loaded per layer");
+    constexpr bool SPLIT = SPL > 1;
+    constexpr int NPW = 64 * W, NPhW = NPW / 2;  // waveguides and slots of one wave
+    constexpr int LPVE = 64 * SPL;               // lanes per vector in the row tables
+    constexpr int NP = NPW * SPL, NPh = NP / 2;  // of one vector
+    constexpr int VPW = NW / SPL;                // vectors per workgroup
     constexpr int np = NPhase<T>::value;
     constexpr int NTHR = 64 * NW;
     constexpr int NI = (NPh + NTHR - 1) / NTHR;  // conversion items per thread and layer
-    constexpr int WSTRIDE = 2 * NPh;             // one wave's Q then dQ entries
+    constexpr int WSTRIDE = 2 * NPhW;            // one wave's Q then dQ entries
     constexpr int BSTRIDE = NW * WSTRIDE;        // one buffer
-    static_assert(NPh % 64 == 0, "whole waves of conversion items");
+    static_assert(NPhW % 64 == 0, "whole waves of conversion items");
     extern __shared__ unsigned char smem_raw[];
     R* stage = reinterpret_cast<R*>(smem_raw);
+    T* edge = reinterpret_cast<T*>(stage + 2 * BSTRIDE);  // SPLIT: [2][VPW][2][4]
     const int lane = threadIdx.x & 63;
     const int wave_u = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int half = wave_u % SPL, vec = wave_u / SPL;
     float* pg = part_g ? part_g + (size_t)blockIdx.x * part_stride : nullptr;
     float* ph = part_h + (size_t)blockIdx.x * part_stride;
     const __amdgpu_buffer_rsrc_t rrs = table_rsrc(rows_r), drs = table_rsrc(rows_d);
-    const unsigned roff = (unsigned)lane * (unsigned)sizeof(R);
+    const unsigned roff = (unsigned)(half * 64 + lane) * (unsigned)sizeof(R);
     auto rows_at = [&](const int s, R (&r)[W], R (&d)[W]) __attribute__((always_inline)) {
         const int la = s < L ? L - 1 - s : 0;
 #pragma unroll
         for (int k = 0; k < W; ++k) {
-            r[k] = buf_load<R>(rrs, roff, (la * NP + k * LPV) * (int)sizeof(R));
-            d[k] = buf_load<R>(drs, roff, (la * NP + k * LPV) * (int)sizeof(R));
+            r[k] = buf_load<R>(rrs, roff, (la * NP + k * LPVE) * (int)sizeof(R));
+            d[k] = buf_load<R>(drs, roff, (la * NP + k * LPVE) * (int)sizeof(R));
         }
     };
+    HvpEdge<T> eg;
+    eg.hi = SPLIT && half == 0 && lane == 63;
+    eg.lo = SPLIT && half == 1 && lane == 0;
+    eg.half0 = half == 0;
     bool first = true;
     for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const Lanes q = lanes_of<1>(LPV, tile);
+        Lanes q;
+        q.lane = lane;
+        q.wave = wave_u;
+        q.grp = 0;
+        q.t = half * 64 + lane;
+        q.vbase = tile * VPW + vec;
         T u[1][W], du[1][W], g[1][W], dg[1][W];
         load_tile(u, u_out, q, N, B, ldu);
         load_tile(du, du_out, q, N, B, ldu);
@@ -435,6 +499,29 @@ __device__ __forceinline__ void hvp_nn_body(const int N, const int L, const int 
         } else {
             load_tile(g, g_out, q, N, B, ldu);
             load_tile(dg, dg_out, q, N, B, ldu);
+        }
+        // SPLIT: the boundary waveguides of the halves after layer s (s = -1: the loaded state)
+        // go to edge slot s & 1; layer s reads slot (s - 1) & 1 after the barrier between them.
+        auto put_edge = [&](const int s) __attribute__((always_inline)) {
+            if constexpr (SPLIT) {
+                T* ed = edge + (((s & 1) * VPW + vec) * 2 + half) * 4;
+                if (eg.hi) {
+                    ed[0] = u[0][W - 1];
+                    ed[1] = du[0][W - 1];
+                    ed[2] = g[0][W - 1];
+                    ed[3] = dg[0][W - 1];
+                }
+                if (eg.lo) {
+                    ed[0] = u[0][0];
+                    ed[1] = du[0][0];
+                    ed[2] = g[0][0];
+                    ed[3] = dg[0][0];
+                }
+            }
+        };
+        if constexpr (SPLIT) {
+            put_edge(-1);
+            __syncthreads();
         }
         // Conversion items of the previous layer pla (entries in pbuf): item k of a thread is
         // position k*NTHR + threadIdx.x; whole waves are on or off.
@@ -462,12 +549,15 @@ __device__ __forceinline__ void hvp_nn_body(const int N, const int L, const int 
             for (int k = 0; k < NI; ++k) {
                 const int m = k * NTHR + wave_u * 64 + lane;
                 if (k * NTHR + wave_u * 64 >= NPh) continue;
-                const R* s0 = pbuf + m;
-                R Qs = s0[0], dQs = s0[NPh];
+                // Position m belongs to half m / NPhW: the entries of that half's waves, summed
+                // in vector order.
+                const int hf = m / NPhW;
+                const R* s0 = pbuf + hf * WSTRIDE + (m - hf * NPhW);
+                R Qs = s0[0], dQs = s0[NPhW];
 #pragma unroll
-                for (int wv = 1; wv < NW; ++wv) {
-                    Qs = row_add(Qs, s0[wv * WSTRIDE]);
-                    dQs = row_add(dQs, s0[wv * WSTRIDE + NPh]);
+                for (int v = 1; v < VPW; ++v) {
+                    Qs = row_add(Qs, s0[v * SPL * WSTRIDE]);
+                    dQs = row_add(dQs, s0[v * SPL * WSTRIDE + NPhW]);
                 }
                 float gv[np], hv[np];
 #pragma unroll
@@ -487,14 +577,30 @@ __device__ __forceinline__ void hvp_nn_body(const int N, const int L, const int 
             R* buf = stage + (s & 1) * BSTRIDE;
             if (s > 0 && !LATE) issue();
             R* wq = buf + wave_u * WSTRIDE + lane;
-            if (kind[L - 1 - s] & 1)
-                hvp_layer_nn<T, W, true>(u[0], du[0], g[0], dg[0], rsrc, wq, wq + NPh);
+            const bool odd = (kind[L - 1 - s] & 1) != 0;
+            if constexpr (SPLIT) {
+                if (odd) {
+                    const T* ed = edge + ((((s - 1) & 1) * VPW + vec) * 2 + (1 - half)) * 4;
+                    eg.u = ed[0];
+                    eg.du = ed[1];
+                    eg.g = ed[2];
+                    eg.dg = ed[3];
+                    if (eg.half0) {
+                        const int off = ((L - 1 - s) * NP + 64) * (int)sizeof(R);  // waveguide NPW: k = 0, lane 64
+                        eg.r = buf_load<R>(rrs, 0u, off);
+                        eg.d = buf_load<R>(drs, 0u, off);
+                    }
+                }
+            }
+            if (odd)
+                hvp_layer_nn<T, W, true, SPLIT>(u[0], du[0], g[0], dg[0], rsrc, wq, wq + NPhW, eg);
             else
-                hvp_layer_nn<T, W, false>(u[0], du[0], g[0], dg[0], rsrc, wq, wq + NPh);
+                hvp_layer_nn<T, W, false, SPLIT>(u[0], du[0], g[0], dg[0], rsrc, wq, wq + NPhW, eg);
             if (s > 0) {
                 if (LATE) issue();
                 finish();
             }
+            put_edge(s);
             __syncthreads();
             pla = L - 1 - s;
             pbuf = buf;
@@ -511,7 +617,8 @@ __device__ __forceinline__ void hvp_nn_body(const int N, const int L, const int 
                 }
             }
         } else if constexpr (JIT) {
-            for (int s = 0; s < L; ++s) layer(s, RowLoads<R>{rrs, drs, roff, (L - 1 - s) * NP * (int)sizeof(R)});
+            for (int s = 0; s < L; ++s)
+                layer(s, RowLoads<R>{rrs, drs, roff, (L - 1 - s) * NP * (int)sizeof(R), LPVE * (int)sizeof(R)});
         } else {
             for (int s = 0; s < L; ++s) {
                 R r[W], d[W];
@@ -529,7 +636,7 @@ __device__ __forceinline__ void hvp_nn_body(const int N, const int L, const int 
     }
 }
 
-template <class T, int W, int NW, bool PF, bool JIT>
+template <class T, int W, int NW, bool PF, bool JIT, int SPL = 1>
 __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(W <= 8 ? 2 : 1))) k_hvp_nn(int N, int L, int B, const int* __restrict__ kind,
                                                     const typename RowOf<T>::type* __restrict__ rows_r,
                                                     const typename RowOf<T>::type* __restrict__ rows_d,
@@ -538,7 +645,7 @@ __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(W 
                                                     float* __restrict__ part_g, float* __restrict__ part_h,
                                                     size_t part_stride, long ntiles, const T* u_out, const T* du_out,
                                                     const T* g_out, const T* dg_out, const T* target, int ldu) {
-    hvp_nn_body<T, W, NW, PF, JIT>(N, L, B, kind, rows_r, rows_d, ktab, dktab, part_g, part_h, part_stride, ntiles, u_out,
+    hvp_nn_body<T, W, NW, PF, JIT, SPL>(N, L, B, kind, rows_r, rows_d, ktab, dktab, part_g, part_h, part_stride, ntiles, u_out,
                               du_out, g_out, dg_out, target, ldu);
 }
 
@@ -549,16 +656,17 @@ using namespace mp;
 namespace {
 
 struct HvpPlan {
-    int W, LPV, NP, C, grid, nn, nw, kso, np;
+    int W, LPV, NP, C, grid, nn, nw, kso, np, spl;
     long ntiles;
     size_t E, rowbytes, kbytes, shmem, off_kind, off_r, off_d, off_t1, off_t2, off_pg, off_ph, total;
 };
 
 inline size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-// Waves per workgroup of the NN walk: eight for the narrow tiles, four for W >= 8 (the stage of
-// Q and dQ entries then takes 64 KiB per workgroup at W = 8, 128 KiB at W = 16).
-constexpr int hvp_nn_waves(int W) { return W <= 4 ? 8 : 4; }
+// Waves per workgroup of the NN walk: eight for the narrow tiles and for split vectors (four
+// vectors sum their entries per conversion; 128 KiB of stage, one workgroup per CU at two waves
+// per SIMD), four for W = 8 with one wave per vector (64 KiB, two workgroups per CU).
+constexpr int hvp_nn_waves(int W, int spl) { return (W <= 4 || spl > 1) ? 8 : 4; }
 
 int hvp_plan(int X, int N, int L, int B, int fft, HvpPlan& P) {
     if (X < 0 || X > 3) return MESHPROP_ERR_ARG;
@@ -574,15 +682,20 @@ int hvp_plan(int X, int N, int L, int B, int fft, HvpPlan& P) {
         if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0) ncu = 256;
     if (P.nn) {
         if (N > 1024) return MESHPROP_ERR_UNSUPPORTED;
-        P.W = N <= 128 ? 2 : N <= 256 ? 4 : N <= 512 ? 8 : 16;
-        P.LPV = 64;
-        P.NP = 64 * P.W;
-        P.nw = hvp_nn_waves(P.W);
+        // N > 512: each vector split over two waves of W = 8 (128 lanes per vector).
+        P.W = N <= 128 ? 2 : N <= 256 ? 4 : 8;
+        P.spl = N > 512 ? 2 : 1;
+        P.LPV = 64 * P.spl;
+        P.NP = 64 * P.W * P.spl;
+        P.nw = hvp_nn_waves(P.W, P.spl);
         P.kso = P.W / 2;
         P.C = 1;
-        P.ntiles = std::max<long>(1, ((long)B + P.nw - 1) / P.nw);
-        P.grid = (int)std::min<long>(P.ntiles, (P.W >= 16 ? 1L : 2L) * ncu);
-        P.shmem = (size_t)2 * P.nw * 2 * (P.NP / 2) * P.rowbytes;
+        const int vpw = P.nw / P.spl;  // vectors per workgroup
+        P.ntiles = std::max<long>(1, ((long)B + vpw - 1) / vpw);
+        P.grid = (int)std::min<long>(P.ntiles, (P.spl > 1 ? 1L : 2L) * ncu);
+        // two stage buffers of [waves][Q, dQ][slots of one wave], and the split vectors' edges
+        P.shmem = (size_t)2 * P.nw * 2 * (64 * P.W / 2) * P.rowbytes +
+                  (P.spl > 1 ? (size_t)2 * vpw * 2 * 4 * (P.rowbytes / 2) : 0);
     } else {
         if (real || N > 512) return MESHPROP_ERR_UNSUPPORTED;
         P.W = N <= 128 ? 2 : N <= 256 ? 4 : 8;
@@ -592,6 +705,7 @@ int hvp_plan(int X, int N, int L, int B, int fft, HvpPlan& P) {
         P.NP = lpv * P.W;
         P.nw = kHvpWaves;
         P.kso = 0;
+        P.spl = 1;
         const int G = 64 / lpv;
         P.ntiles = std::max<long>(1, ((long)B + kHvpWaves * G - 1) / (kHvpWaves * G));
         P.grid = (int)std::min<long>(P.ntiles, 2L * ncu);
@@ -637,16 +751,27 @@ int launch_hvp_nn(const HvpPlan& P, int N, int L, int B, unsigned char* base, bo
                   const void* du_out, const void* g, const void* dg, const void* target, int ldu, hipStream_t st) {
     typedef typename RowOf<T>::type R;
     typedef typename KOf<T>::type KT;
-    constexpr int NW = hvp_nn_waves(W);
+    constexpr int NW = hvp_nn_waves(W, 1);
+    constexpr int NW2 = hvp_nn_waves(W, 2);
     // Row sources, as measured (MI355X, Clements L = N): rows of the next layer prefetched at
     // W <= 4 (MZI N=256 B=4096 HVP 1.04 ms against 1.13 with per-crossing loads); per-crossing
-    // loads at W = 8 (no spills at two waves per SIMD: N=512 5.77 -> 4.59 ms) and for real
-    // W = 16 fields (N=1024 orth 8.73 -> 7.65 ms); the complex W = 16 tile (one wave per SIMD,
-    // bound by its 128 KiB stage) keeps a whole layer of rows in registers (15.5 ms against 18.2
-    // with per-crossing loads and 20.8 with only the tangent rows loaded per crossing).
+    // loads at W = 8 (no spills at two waves per SIMD: N=512 5.77 -> 4.59 ms).
     constexpr bool PF = W <= 4;
-    constexpr bool JIT = W == 8 || (W >= 16 && sizeof(T) == sizeof(float));
+    constexpr bool JIT = W == 8;
     static std::atomic<unsigned long long> lds_set{0};
+    static std::atomic<unsigned long long> lds_set2{0};
+    if constexpr (W == 8) {
+        if (P.spl > 1) {
+            if (int rc = lds_limit_once((const void*)k_hvp_nn<T, W, NW2, PF, JIT, 2>, lds_set2, P.shmem)) return rc;
+            hipLaunchKernelGGL((k_hvp_nn<T, W, NW2, PF, JIT, 2>), dim3(P.grid), dim3(64 * NW2), P.shmem, st, N, L, B,
+                               (const int*)(base + P.off_kind), (const R*)(base + P.off_r),
+                               (const R*)(base + P.off_d), (const KT*)(base + P.off_t1),
+                               (const KT*)(base + P.off_t2), with_grad ? (float*)(base + P.off_pg) : nullptr,
+                               (float*)(base + P.off_ph), P.E, P.ntiles, (const T*)u_out, (const T*)du_out,
+                               (const T*)g, (const T*)dg, (const T*)target, ldu);
+            return MESHPROP_OK;
+        }
+    }
     if (int rc = lds_limit_once((const void*)k_hvp_nn<T, W, NW, PF, JIT>, lds_set, P.shmem)) return rc;
     hipLaunchKernelGGL((k_hvp_nn<T, W, NW, PF, JIT>), dim3(P.grid), dim3(64 * NW), P.shmem, st, N, L, B,
                        (const int*)(base + P.off_kind), (const R*)(base + P.off_r), (const R*)(base + P.off_d),
@@ -663,8 +788,7 @@ int launch_hvp_nn_any(const HvpPlan& P, int N, int L, int B, unsigned char* base
     switch (P.W) {
         case 2: return launch_hvp_nn<T, 2>(P, N, L, B, base, with_grad, u_out, du_out, g, dg, target, ldu, st);
         case 4: return launch_hvp_nn<T, 4>(P, N, L, B, base, with_grad, u_out, du_out, g, dg, target, ldu, st);
-        case 8: return launch_hvp_nn<T, 8>(P, N, L, B, base, with_grad, u_out, du_out, g, dg, target, ldu, st);
-        default: return launch_hvp_nn<T, 16>(P, N, L, B, base, with_grad, u_out, du_out, g, dg, target, ldu, st);
+        default: return launch_hvp_nn<T, 8>(P, N, L, B, base, with_grad, u_out, du_out, g, dg, target, ldu, st);
     }
 }
 

@@ -143,6 +143,21 @@ struct FieldOfRow<float2> {
     typedef float type;
 };
 
+// Stage order of the NN-specialised walks: lane t of a 64-lane wave owns kso consecutive slots
+// t*kso .. t*kso + kso - 1, and its slot p sits at position p*64 + t of the wave's block of
+// 64*kso positions; a vector split over several waves (lane index over the whole vector
+// >= 64) takes one block per wave, in order.  kso == 0: slot order.
+__host__ __device__ inline int stage_pos(const int m, const int kso) {
+    if (kso <= 0) return m;
+    const int ln = m / kso;
+    return (ln / 64) * (64 * kso) + (m % kso) * 64 + ln % 64;
+}
+__host__ __device__ inline int stage_slot(const int pos, const int kso) {
+    if (kso <= 0) return pos;
+    const int r = pos % (64 * kso);
+    return (pos - r) + (r % 64) * kso + r / 64;
+}
+
 // One thread per (layer, slot m < NP/2): the crossing of the slot is evaluated once (FP64) and
 // both of its waveguides' rows are written to every requested table, with its dT/dp for
 // TAB_DTDP; waveguides that no active crossing touches get identity rows (and inactive slots
@@ -162,10 +177,8 @@ __device__ void slot_tables_at(const int X, const int N, const int NP, const int
     const int kd = layer_kind(fft, g0, g1, l, NP);
     if (m == 0 && kind) kind[l] = kd;
     const int NPh = NP / 2;
-    // Index of slot m in the K-form table: slot order, or (kso > 0, one 64-lane vector per wave)
-    // the adjoint's stage order, in which lane t's kso slots t*kso .. t*kso + kso - 1 sit 64
-    // entries apart (position p*64 + t for slot t*kso + p).
-    const int km = kso > 0 ? (m % kso) * 64 + m / kso : m;
+    // Index of slot m in the K-form table: slot order, or (kso > 0) the NN walks' stage order.
+    const int km = stage_pos(m, kso);
     int i, j;
     bool act;
     if (fft) {

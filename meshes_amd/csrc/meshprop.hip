@@ -44,7 +44,7 @@ __global__ void __launch_bounds__(256) k_dp_reduce(int L, int Nh, int NPh, int n
         const long lm = idx / np;
         pos = (int)(lm % Nr);
         l = (int)(lm / Nr);
-        mm = kso > 0 ? (pos % 64) * kso + pos / 64 : pos;
+        mm = stage_slot(pos, kso);
         ok = mm < Nh;
     }
     float acc = 0.f;
