@@ -636,17 +636,16 @@ __device__ __forceinline__ void hvp_nn_body(const int N, const int L, const int 
     }
 }
 
+// The W <= 8 tiles (also the halves of split vectors) compiled for two waves per SIMD.
 template <class T, int W, int NW, bool PF, bool JIT, int SPL = 1>
-__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(W <= 8 ? 2 : 1))) k_hvp_nn(int N, int L, int B, const int* __restrict__ kind,
-                                                    const typename RowOf<T>::type* __restrict__ rows_r,
-                                                    const typename RowOf<T>::type* __restrict__ rows_d,
-                                                    const typename KOf<T>::type* __restrict__ ktab,
-                                                    const typename KOf<T>::type* __restrict__ dktab,
-                                                    float* __restrict__ part_g, float* __restrict__ part_h,
-                                                    size_t part_stride, long ntiles, const T* u_out, const T* du_out,
-                                                    const T* g_out, const T* dg_out, const T* target, int ldu) {
-    hvp_nn_body<T, W, NW, PF, JIT, SPL>(N, L, B, kind, rows_r, rows_d, ktab, dktab, part_g, part_h, part_stride, ntiles, u_out,
-                              du_out, g_out, dg_out, target, ldu);
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2)))
+k_hvp_nn(int N, int L, int B, const int* __restrict__ kind, const typename RowOf<T>::type* __restrict__ rows_r,
+         const typename RowOf<T>::type* __restrict__ rows_d, const typename KOf<T>::type* __restrict__ ktab,
+         const typename KOf<T>::type* __restrict__ dktab, float* __restrict__ part_g, float* __restrict__ part_h,
+         size_t part_stride, long ntiles, const T* u_out, const T* du_out, const T* g_out, const T* dg_out,
+         const T* target, int ldu) {
+    hvp_nn_body<T, W, NW, PF, JIT, SPL>(N, L, B, kind, rows_r, rows_d, ktab, dktab, part_g, part_h, part_stride,
+                                        ntiles, u_out, du_out, g_out, dg_out, target, ldu);
 }
 
 }  // namespace mp
