@@ -205,11 +205,8 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
         if (const TileEntry* e = table_tile(op, P.scaled ? X_MZI : X, N, B)) {
             if ((long)e->w * 64 >= N && tile_ok(op, e->w, e->v)) {
                 tabled = meshprop_tile{e->w, e->v, e->waves, tile ? tile->chunk : 0};
-                // The table's W = 16 unitary adjoint entries were measured on the general kernel
-                // (four-wave workgroups at most); the NN-specialised one takes eight-wave
-                // workgroups whenever the batch gives every CU one (N=1024 L=1024 B=2048:
-                // backdiff 3.21 ms against 3.41 ms with four waves, fwd+grad 3.91 against 4.14).
-                if (adj && P.nn && P.uni && e->w == 16) tabled.waves = 0;
+                // W = 16 unitary adjoints pick their workgroup size by batch (below).
+                if (adj && P.uni && e->w == 16) tabled.waves = 0;
                 tile = &tabled;
             }
         }
@@ -237,9 +234,12 @@ int make_plan(int op, int X, int N, int L, int B, int fft, const meshprop_tile* 
         // still fills every CU with them (half the dp partials of four-wave groups; measured at
         // N=256, L=256, B=4096: 0.406-0.413 ms per fwd+grad step against 0.419 ms).  The
         // general (lossy) adjoint keeps four waves: its eight-wave kernel spills registers.
+        // The NN-specialised W = 16 adjoint takes eight waves already when four-wave groups would
+        // fill every CU (MI355X, N=1024 L=1024 fwd+grad: B=1024 3.145 ms with eight waves against
+        // 3.486 with four; B=16..256 2.94-2.95 against 2.83-2.86; B=2048 backdiff 3.21 against 3.41).
         const bool auto_waves = !(tile && tile->waves >= 1 && tile->waves <= 8);
         if (auto_waves && wide_ok && P.uni) {
-            const long per_wg = (long)kPipeWaves * (64 / P.LPV) * P.V;
+            const long per_wg = (long)(nn16 ? 4 : kPipeWaves) * (64 / P.LPV) * P.V;
             if (((long)B + per_wg - 1) / per_wg >= ncu) P.waves = kPipeWaves;
         }
         if (P.waves > cap) P.waves = cap;

@@ -203,7 +203,7 @@ def test_tile_table_consulted_without_gpu(lib):
             assert got[:2] == (W, V), (op, X, fat, n0, N, got)
             if W == 16 and X in (0, 1) and op in (L.OP_BACKDIFF, L.OP_FWDGRAD):
                 # the NN-specialised W = 16 adjoint picks its workgroup size by batch
-                assert got[2] == (8 if B >= 8 * 256 else 4), (op, X, fat, n0, N, got)
+                assert got[2] == (8 if B >= 4 * 256 else 4), (op, X, fat, n0, N, got)
             else:
                 assert got[2] == min(waves, cap), (op, X, fat, n0, N, got)
             checked += 1
