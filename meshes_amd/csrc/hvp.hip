@@ -662,10 +662,12 @@ struct HvpPlan {
 
 inline size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-// Waves per workgroup of the NN walk: eight for the narrow tiles and for split vectors (four
-// vectors sum their entries per conversion; 128 KiB of stage, one workgroup per CU at two waves
-// per SIMD), four for W = 8 with one wave per vector (64 KiB, two workgroups per CU).
-constexpr int hvp_nn_waves(int W, int spl) { return (W <= 4 || spl > 1) ? 8 : 4; }
+// Waves per workgroup of the NN walk: eight (eight vectors, or four split ones, sum their
+// entries per conversion; at W = 8 the 128 KiB stage leaves one workgroup per CU, two waves per
+// SIMD).  Measured against four at W = 8 (MI355X, Clements L = N): MZI N=512 B=4096 3.77 ms
+// against 4.38, N=300 2.10 against 2.45, orth N=512 1.99 against 2.13; B=256 1.52 against 1.47;
+// split vectors at N=1024 B=2048 10.4 against 11.4.
+constexpr int hvp_nn_waves(int, int) { return 8; }
 
 int hvp_plan(int X, int N, int L, int B, int fft, HvpPlan& P) {
     if (X < 0 || X > 3) return MESHPROP_ERR_ARG;
@@ -691,7 +693,7 @@ int hvp_plan(int X, int N, int L, int B, int fft, HvpPlan& P) {
         P.C = 1;
         const int vpw = P.nw / P.spl;  // vectors per workgroup
         P.ntiles = std::max<long>(1, ((long)B + vpw - 1) / vpw);
-        P.grid = (int)std::min<long>(P.ntiles, (P.spl > 1 ? 1L : 2L) * ncu);
+        P.grid = (int)std::min<long>(P.ntiles, (P.W >= 8 ? 1L : 2L) * ncu);  // W = 8: one per CU (LDS)
         // two stage buffers of [waves][Q, dQ][slots of one wave], and the split vectors' edges
         P.shmem = (size_t)2 * P.nw * 2 * (64 * P.W / 2) * P.rowbytes +
                   (P.spl > 1 ? (size_t)2 * vpw * 2 * 4 * (P.rowbytes / 2) : 0);
