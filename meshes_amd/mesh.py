@@ -395,22 +395,38 @@ class MeshNetworkGPU:
     # ---- Hessian-vector products (gpu/mesh.py:236-288) --------------------------------
     def exact_hvp_supported(self) -> bool:
         """Forward-over-reverse HVP kernel: unitary crossings (MZI, SYM; orth on NN meshes) up to
-        N = 1024 on NN meshes and N = 512 on FFT meshes, no phase screen."""
-        if self.is_phase or self.X not in ("mzi", "sym", "orth"):
+        N = 1024 on NN meshes and N = 512 on FFT meshes, with no phase screen or one at the output."""
+        if (self.is_phase and self.phi_pos == "in") or self.X not in ("mzi", "sym", "orth"):
             return False
         if self.fft:
             return self.X != "orth" and self.N <= 512
         return self.N <= 1024
 
     def _hvp_exact(self, q, out, dout, g, dg, p, s):
+        """H q from the (B, N) output field, its tangent along q, the output gradient and its tangent.
+
+        With an output screen y = S z, S = diag(exp(i phi)): the screen's gradient entries
+        sum_b Im(g conj(y)) have the derivative sum_b Im(dg conj(y) + g conj(dy)); the mesh part runs
+        on z = S^H y with tangent S^H (dy - i dphi y), gradient S^H g and its tangent
+        S^H (dg - i dphi g), as the screen's adjoint hands them to the mesh."""
         B = out.shape[0]
         hv = torch.zeros(self.n_phase, dtype=torch.float32, device=self.device)
+        hv_screen = None
+        if self.is_phase:
+            n_m = self.n_cr * self.X_np
+            hv_screen = torch.sum(torch.imag(dg * out.conj() + g * dout.conj()), dim=0)
+            sh = torch.exp(-1j * p[n_m:]).to(self.dtype)
+            idq = (1j * q[n_m:]).to(self.dtype)
+            out, dout = sh * out, sh * (dout - idq * out)
+            g, dg = sh * g, sh * (dg - idq * g)
         nb = _lib.hvp_workspace_size(self.X, self.N, self.L, B, self.fft)
         ws = torch.empty(max(nb, 1), dtype=torch.uint8, device=self.device)
         ldp, lds = self._ld()
         g0, g1 = self._geom()
         _lib.hvp(self.X, self.N, self.L, B, self.fft, g0, g1, p, q, ldp, s if s.numel() else None, lds, out, dout,
                  g, dg, None, None, hv, self.N, ws, nb)
+        if hv_screen is not None:
+            hv[self.n_cr * self.X_np:] = hv_screen
         return hv
 
     def _hvp_difference(self, vec, v, w, p, p_splitter, eps):
@@ -442,7 +458,8 @@ class MeshNetworkGPU:
 
     def hvp_target(self, vec, U_target, p_phase=None, p_splitter=None, eps=1e-3):
         """H vec of L2norm(U_target) (gpu/mesh.py:253-266); exact where supported: the output
-        gradient 2 offdiag(U - 1) is affine, its tangent is 2 offdiag(dU)."""
+        gradient 2 offdiag(U - 1) (2 (U - 1) with a phase screen) is affine, its tangent is
+        2 offdiag(dU) (2 dU)."""
         J = [0]
         w = self._L2norm_fn(J)
         Ut = torch.as_tensor(np.asarray(U_target) if not isinstance(U_target, torch.Tensor) else U_target)
@@ -455,7 +472,8 @@ class MeshNetworkGPU:
         out, dout = self.dot(v, p, p_splitter, dp=q)
         g = w(out)
         dg = 2 * dout
-        dg = dg - torch.diag_embed(torch.diagonal(dg))
+        if not self.is_phase:  # the loss's diagonal is dropped without a screen
+            dg = dg - torch.diag_embed(torch.diagonal(dg))
         return self._hvp_exact(q, out.T, dout.T, as_field(g, self.N, self.dtype, self.device),
                                as_field(dg, self.N, self.dtype, self.device), p, s)
 

@@ -4,7 +4,8 @@ The oracle's hvp_inverse is pinned by central differences of its gradient; here 
 (meshprop_hvp after fwddiff along q) must match it within 1e-5 relative L2: MZI and SYM on NN
 meshes up to N = 1024 (every NN tile width, W = 2 .. 16) and on FFT meshes up to N = 512, orth
 (real fields) on NN meshes, several vector tiles per workgroup; MeshNetworkGPU.hvp_target /
-hess_target must agree with the oracle and be symmetric.
+hess_target must agree with the oracle and be symmetric; meshes with a phase screen (exact at the
+output, central differences at the input) against FP64 differences of the oracle's gradient.
 """
 import zlib
 
@@ -139,3 +140,63 @@ def test_hvp_untraceable_w_falls_back_to_gradient_difference(gpu):
         return torch.as_tensor(4 * np.abs(a) ** 2 * a, device=y.device)
     hv = mesh.hvp(q, torch.as_tensor(v.T).cuda(), w).cpu().numpy()
     assert O.rel_l2(hv, fd) < 1e-2
+
+
+def _screen_grad(g, X, p, ph, v, gfun, pos):
+    """FP64 gradient (mesh phases, then screen phases) of a loss with dJ/dout = gfun(out), the
+    screen exp(i ph) at the mesh input or output; the oracle's mesh adjoint composed with the
+    screen's, whose own gradient is pinned against loss differences with the screen tests."""
+    e = np.exp(1j * ph)
+    if pos == "in":
+        xs = v * e[None, :]
+        out = O.mesh_fwd(g, X, p, None, xs)
+        rdp, gxs, _ = O.mesh_vjp(g, X, p, None, out, gfun(out), u_in=xs)
+        dph = np.sum(np.imag(gxs * np.conj(xs)), axis=0)
+    else:
+        z = O.mesh_fwd(g, X, p, None, v)
+        out = z * e[None, :]
+        gout = gfun(out)
+        dph = np.sum(np.imag(gout * np.conj(out)), axis=0)
+        rdp = O.mesh_vjp(g, X, p, None, z, gout * np.conj(e)[None, :], u_in=v)[0]
+    return np.concatenate([rdp.reshape(-1), dph])
+
+
+SCREEN_CASES = [("mzi", "clements", 16, 0, 12, "out"), ("sym", "nn", 64, 9, 11, "out"),
+                ("mzi", "fft", 32, 5, 7, "out"), ("mzi", "nn", 40, 6, 5, "in")]
+
+
+@pytest.mark.parametrize("case", SCREEN_CASES, ids=["-".join(map(str, c)) for c in SCREEN_CASES])
+def test_hvp_with_phase_screen(gpu, case):
+    """hvp and hvp_target on meshes with a phase screen, against FP64 central differences of the
+    oracle's gradient (screen phases included).  An output screen takes the exact path (1e-5); an
+    input screen falls back to the reference's difference of FP32 gradients (1e-2)."""
+    from meshes_amd import MeshNetworkGPU
+    X, kind, N, L, B, pos = case
+    rng = np.random.RandomState(zlib.crc32(repr(case).encode()))
+    g = _geom(rng, kind, N, L)
+    n_p = O.N_PHASE[X]
+    p = rng.uniform(0, 2 * np.pi, (g.L, N // 2, n_p)).astype(np.float32)
+    ph = rng.uniform(0, 2 * np.pi, N).astype(np.float32)
+    mesh = MeshNetworkGPU(N, g.L, g.lens, g.shifts, np.concatenate([p.ravel(), ph]), np.zeros(0), X, phi_pos=pos,
+                          is_phase=True, strides=g.strides if kind == "fft" else None)
+    exact = pos == "out"
+    assert mesh.exact_hvp_supported() == exact
+    tol = TOL if exact else 1e-2
+    v = O.unit_cn(rng, B, N).astype(np.complex64)
+    q = rng.standard_normal(mesh.n_phase).astype(np.float32)
+    h = 1e-5
+    p64, ph64 = p.astype(np.float64), ph.astype(np.float64)
+    qm, qs = q[:p.size].astype(np.float64).reshape(p.shape), q[p.size:].astype(np.float64)
+
+    def fd(gfun, vin):
+        return (_screen_grad(g, X, p64 + h * qm, ph64 + h * qs, vin, gfun, pos)
+                - _screen_grad(g, X, p64 - h * qm, ph64 - h * qs, vin, gfun, pos)) / (2 * h)
+
+    def w(y):  # J = sum |y|^4
+        return 4 * (y * y.conj()).real * y
+    hv = mesh.hvp(q, torch.as_tensor(v.T).cuda(), w).cpu().numpy()
+    assert O.rel_l2(hv, fd(lambda o: 4 * np.abs(o) ** 2 * o, v.astype(np.complex128))) < tol
+    # L2norm keeps the diagonal with a screen: dJ/dU = 2 (U - 1)
+    U = np.linalg.qr(rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N)))[0].astype(np.complex64)
+    hv = mesh.hvp_target(q, U).cpu().numpy()
+    assert O.rel_l2(hv, fd(lambda o: 2 * (o - np.eye(N)), U.astype(np.complex128).conj())) < tol
