@@ -308,6 +308,15 @@ int meshprop_hvp(int X, int N, int L, int B, int fft, const int *geom0, const in
                  const void *u_out, const void *du_out, const void *dJdu_out, const void *d_dJdu_out,
                  const void *target, float *grad, float *hvp, int ldu,
                  void *workspace, size_t workspace_bytes, void *stream);
+/* meshprop_hvp that also writes the input gradient dJ/du_in and its tangent along q into dJdu_in
+ * and d_dJdu_in (both or neither; same layout and ldu as the fields): the walk's final state,
+ * which an input phase screen's Hessian entries need.  No reference counterpart, as above. */
+int meshprop_hvp_grad_in(int X, int N, int L, int B, int fft, const int *geom0, const int *geom1,
+                         const float *p, const float *q, int ldp, const float *s, int lds,
+                         const void *u_out, const void *du_out, const void *dJdu_out,
+                         const void *d_dJdu_out, const void *target, float *grad, float *hvp,
+                         void *dJdu_in, void *d_dJdu_in, int ldu,
+                         void *workspace, size_t workspace_bytes, void *stream);
 
 /* The tile the library would use for a call (after defaults and clamping). */
 int meshprop_resolve_tile(int op, int X, int N, int B, meshprop_tile *out);

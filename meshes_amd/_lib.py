@@ -34,7 +34,7 @@ EXPORTS = (
     "meshprop_group_tables_size", "meshprop_group_upload", "meshprop_group_launch", "meshprop_group_signature",
     "meshprop_group_entry_info", "meshprop_phase_screen", "meshprop_phase_screen_workspace_size",
     "meshprop_permute",
-    "meshprop_hvp", "meshprop_hvp_workspace_size", "meshprop_run_noalloc", "meshprop_adjoint_kernel", "meshprop_forward_kernel",
+    "meshprop_hvp", "meshprop_hvp_grad_in", "meshprop_hvp_workspace_size", "meshprop_run_noalloc", "meshprop_adjoint_kernel", "meshprop_forward_kernel",
 )
 
 # Bytes of the static device scratch behind meshprop_run_noalloc (MESHPROP_NOALLOC_SCRATCH_BYTES).
@@ -123,6 +123,9 @@ def load():
     lib.meshprop_hvp_workspace_size.restype = c_size_t
     lib.meshprop_hvp.argtypes = ([c_int] * 5 + [vp] * 4 + [c_int, vp, c_int] + [vp] * 7 + [c_int, vp, c_size_t, vp])
     lib.meshprop_hvp.restype = c_int
+    lib.meshprop_hvp_grad_in.argtypes = ([c_int] * 5 + [vp] * 4 + [c_int, vp, c_int] + [vp] * 9
+                                         + [c_int, vp, c_size_t, vp])
+    lib.meshprop_hvp_grad_in.restype = c_int
     _lib = lib
     return lib
 
@@ -273,12 +276,21 @@ def hvp_workspace_size(X, N, L, B, fft=False) -> int:
 
 
 def hvp(X, N, L, B, fft, g0, g1, p, q, ldp, s, lds, u_out, du_out, dJdu_out, d_dJdu_out, target, grad, hv, ldu,
-        workspace, ws_bytes, stream=None):
-    """meshprop_hvp on torch tensors."""
+        workspace, ws_bytes, stream=None, dJdu_in=None, d_dJdu_in=None):
+    """meshprop_hvp on torch tensors; with dJdu_in and d_dJdu_in, meshprop_hvp_grad_in (which also
+    writes the input gradient and its tangent)."""
     lib = load()
-    _check_dtypes(X, (p, q, s, grad, hv), (u_out, du_out, dJdu_out, d_dJdu_out, target))
+    _check_dtypes(X, (p, q, s, grad, hv), (u_out, du_out, dJdu_out, d_dJdu_out, target, dJdu_in, d_dJdu_in))
     if stream is None:
         stream = torch.cuda.current_stream().cuda_stream
+    if dJdu_in is not None or d_dJdu_in is not None:
+        rc = lib.meshprop_hvp_grad_in(X_CODE[X] if isinstance(X, str) else X, N, L, B, int(fft), _ptr(g0), _ptr(g1),
+                                      _ptr(p), _ptr(q), ldp, _ptr(s), lds, _ptr(u_out), _ptr(du_out), _ptr(dJdu_out),
+                                      _ptr(d_dJdu_out), _ptr(target), _ptr(grad), _ptr(hv), _ptr(dJdu_in),
+                                      _ptr(d_dJdu_in), ldu, _ptr(workspace), ctypes.c_size_t(ws_bytes),
+                                      ctypes.c_void_p(stream))
+        check(rc, "hvp_grad_in")
+        return
     rc = lib.meshprop_hvp(X_CODE[X] if isinstance(X, str) else X, N, L, B, int(fft), _ptr(g0), _ptr(g1), _ptr(p),
                           _ptr(q), ldp, _ptr(s), lds, _ptr(u_out), _ptr(du_out), _ptr(dJdu_out), _ptr(d_dJdu_out),
                           _ptr(target), _ptr(grad), _ptr(hv), ldu, _ptr(workspace), ctypes.c_size_t(ws_bytes),

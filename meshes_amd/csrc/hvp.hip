@@ -158,7 +158,8 @@ __global__ void __launch_bounds__(64 * kHvpWaves) __attribute__((amdgpu_waves_pe
     int N, int L, int B, int LPV, int NP, int C, const int* __restrict__ kind, const float4* __restrict__ rows_r,
     const float4* __restrict__ rows_d, const float4* __restrict__ ktab, const float4* __restrict__ dktab,
     float* __restrict__ part_g, float* __restrict__ part_h, size_t part_stride, long ntiles, const float2* u_out,
-    const float2* du_out, const float2* g_out, const float2* dg_out, const float2* target, int ldu) {
+    const float2* du_out, const float2* g_out, const float2* dg_out, const float2* target, int ldu, float2* g_in,
+    float2* dg_in) {
     extern __shared__ unsigned char smem_raw[];
     float4* stage = reinterpret_cast<float4*>(smem_raw);  // [nw][C][2][NP/2]: Q entries, then dQ
     const int nw = blockDim.x >> 6;
@@ -284,6 +285,10 @@ __global__ void __launch_bounds__(64 * kHvpWaves) __attribute__((amdgpu_waves_pe
                 step(s, rA, dA_);
                 if (s % C == C - 1 || s == L - 1) convert(s);
             }
+        }
+        if (g_in != nullptr) {  // the walk's final gradient and its tangent: those of the input
+            store_tile(g, g_in, q, N, B, ldu);
+            store_tile(dg, dg_in, q, N, B, ldu);
         }
         first = false;
     }
@@ -439,7 +444,7 @@ __device__ __forceinline__ void hvp_nn_body(const int N, const int L, const int 
                                             float* __restrict__ part_g, float* __restrict__ part_h,
                                             const size_t part_stride, const long ntiles, const T* u_out,
                                             const T* du_out, const T* g_out, const T* dg_out, const T* target,
-                                            const int ldu) {
+                                            const int ldu, T* g_in, T* dg_in) {
     typedef typename RowOf<T>::type R;
     typedef typename KOf<T>::type KT;
     static_assert(SPL == 1 || SPL == 2, "one or two waves per vector");
@@ -631,6 +636,10 @@ __device__ __forceinline__ void hvp_nn_body(const int N, const int L, const int 
             issue();
             finish();
         }
+        if (g_in != nullptr) {  // the walk's final gradient and its tangent: those of the input
+            store_tile(g, g_in, q, N, B, ldu);
+            store_tile(dg, dg_in, q, N, B, ldu);
+        }
         __syncthreads();  // the next tile reuses the buffers
         first = false;
     }
@@ -643,9 +652,9 @@ k_hvp_nn(int N, int L, int B, const int* __restrict__ kind, const typename RowOf
          const typename RowOf<T>::type* __restrict__ rows_d, const typename KOf<T>::type* __restrict__ ktab,
          const typename KOf<T>::type* __restrict__ dktab, float* __restrict__ part_g, float* __restrict__ part_h,
          size_t part_stride, long ntiles, const T* u_out, const T* du_out, const T* g_out, const T* dg_out,
-         const T* target, int ldu) {
+         const T* target, int ldu, T* g_in, T* dg_in) {
     hvp_nn_body<T, W, NW, PF, JIT, SPL>(N, L, B, kind, rows_r, rows_d, ktab, dktab, part_g, part_h, part_stride,
-                                        ntiles, u_out, du_out, g_out, dg_out, target, ldu);
+                                        ntiles, u_out, du_out, g_out, dg_out, target, ldu, g_in, dg_in);
 }
 
 }  // namespace mp
@@ -738,18 +747,19 @@ int hvp_plan(int X, int N, int L, int B, int fft, HvpPlan& P) {
 
 template <int W>
 void launch_hvp(const HvpPlan& P, int N, int L, int B, unsigned char* base, const void* u_out, const void* du_out,
-                const void* g, const void* dg, const void* target, int ldu, hipStream_t st) {
+                const void* g, const void* dg, const void* target, int ldu, void* g_in, void* dg_in, hipStream_t st) {
     hipLaunchKernelGGL((k_hvp<W>), dim3(P.grid), dim3(64 * kHvpWaves), P.shmem, st, N, L, B, P.LPV, P.NP, P.C,
                        (const int*)(base + P.off_kind), (const float4*)(base + P.off_r),
                        (const float4*)(base + P.off_d), (const float4*)(base + P.off_t1),
                        (const float4*)(base + P.off_t2), (float*)(base + P.off_pg), (float*)(base + P.off_ph), P.E,
                        P.ntiles, (const float2*)u_out, (const float2*)du_out, (const float2*)g, (const float2*)dg,
-                       (const float2*)target, ldu);
+                       (const float2*)target, ldu, (float2*)g_in, (float2*)dg_in);
 }
 
 template <class T, int W>
 int launch_hvp_nn(const HvpPlan& P, int N, int L, int B, unsigned char* base, bool with_grad, const void* u_out,
-                  const void* du_out, const void* g, const void* dg, const void* target, int ldu, hipStream_t st) {
+                  const void* du_out, const void* g, const void* dg, const void* target, int ldu, void* g_in,
+                  void* dg_in, hipStream_t st) {
     typedef typename RowOf<T>::type R;
     typedef typename KOf<T>::type KT;
     constexpr int NW = hvp_nn_waves(W, 1);
@@ -769,7 +779,7 @@ int launch_hvp_nn(const HvpPlan& P, int N, int L, int B, unsigned char* base, bo
                                (const R*)(base + P.off_d), (const KT*)(base + P.off_t1),
                                (const KT*)(base + P.off_t2), with_grad ? (float*)(base + P.off_pg) : nullptr,
                                (float*)(base + P.off_ph), P.E, P.ntiles, (const T*)u_out, (const T*)du_out,
-                               (const T*)g, (const T*)dg, (const T*)target, ldu);
+                               (const T*)g, (const T*)dg, (const T*)target, ldu, (T*)g_in, (T*)dg_in);
             return MESHPROP_OK;
         }
     }
@@ -778,18 +788,22 @@ int launch_hvp_nn(const HvpPlan& P, int N, int L, int B, unsigned char* base, bo
                        (const int*)(base + P.off_kind), (const R*)(base + P.off_r), (const R*)(base + P.off_d),
                        (const KT*)(base + P.off_t1), (const KT*)(base + P.off_t2),
                        with_grad ? (float*)(base + P.off_pg) : nullptr, (float*)(base + P.off_ph), P.E, P.ntiles,
-                       (const T*)u_out, (const T*)du_out, (const T*)g, (const T*)dg, (const T*)target, ldu);
+                       (const T*)u_out, (const T*)du_out, (const T*)g, (const T*)dg, (const T*)target, ldu,
+                       (T*)g_in, (T*)dg_in);
     return MESHPROP_OK;
 }
 
 template <class T>
 int launch_hvp_nn_any(const HvpPlan& P, int N, int L, int B, unsigned char* base, bool with_grad, const void* u_out,
                       const void* du_out, const void* g, const void* dg, const void* target, int ldu,
-                      hipStream_t st) {
+                      void* g_in, void* dg_in, hipStream_t st) {
     switch (P.W) {
-        case 2: return launch_hvp_nn<T, 2>(P, N, L, B, base, with_grad, u_out, du_out, g, dg, target, ldu, st);
-        case 4: return launch_hvp_nn<T, 4>(P, N, L, B, base, with_grad, u_out, du_out, g, dg, target, ldu, st);
-        default: return launch_hvp_nn<T, 8>(P, N, L, B, base, with_grad, u_out, du_out, g, dg, target, ldu, st);
+        case 2: return launch_hvp_nn<T, 2>(P, N, L, B, base, with_grad, u_out, du_out, g, dg, target, ldu, g_in, dg_in,
+                                                 st);
+        case 4: return launch_hvp_nn<T, 4>(P, N, L, B, base, with_grad, u_out, du_out, g, dg, target, ldu, g_in, dg_in,
+                                                 st);
+        default: return launch_hvp_nn<T, 8>(P, N, L, B, base, with_grad, u_out, du_out, g, dg, target, ldu, g_in, dg_in,
+                                                 st);
     }
 }
 
@@ -802,16 +816,18 @@ size_t meshprop_hvp_workspace_size(int X, int N, int L, int B, int fft) {
     return hvp_plan(X, N, L, B, fft, P) == MESHPROP_OK ? P.total : 0;
 }
 
-int meshprop_hvp(int X, int N, int L, int B, int fft, const int* geom0, const int* geom1, const float* p,
-                 const float* q, int ldp, const float* s, int lds, const void* u_out, const void* du_out,
-                 const void* dJdu_out, const void* d_dJdu_out, const void* target, float* grad, float* hvp, int ldu,
-                 void* workspace, size_t workspace_bytes, void* stream) {
+int meshprop_hvp_grad_in(int X, int N, int L, int B, int fft, const int* geom0, const int* geom1, const float* p,
+                         const float* q, int ldp, const float* s, int lds, const void* u_out, const void* du_out,
+                         const void* dJdu_out, const void* d_dJdu_out, const void* target, float* grad, float* hvp,
+                         void* dJdu_in, void* d_dJdu_in, int ldu, void* workspace, size_t workspace_bytes,
+                         void* stream) {
     HvpPlan P;
     int rc = hvp_plan(X, N, L, B, fft, P);
     if (rc) return rc;
     if (B == 0 || L == 0) return MESHPROP_OK;
     if (ldu < N || !u_out || !du_out || !hvp || !p || !q || !geom0 || (!fft && !geom1)) return MESHPROP_ERR_ARG;
     if (!target && (!dJdu_out || !d_dJdu_out)) return MESHPROP_ERR_ARG;
+    if (!dJdu_in != !d_dJdu_in) return MESHPROP_ERR_ARG;
     if (ldp < (N / 2) * P.np) return MESHPROP_ERR_ARG;
     const int ns = n_split_of(X);
     if (s && ns && lds < (N / 2) * ns) return MESHPROP_ERR_ARG;
@@ -841,15 +857,21 @@ int meshprop_hvp(int X, int N, int L, int B, int fft, const int* geom0, const in
     if (P.nn) {
         rc = (X == X_ORTH)
                  ? launch_hvp_nn_any<float>(P, N, L, B, base, grad != nullptr, u_out, du_out, dJdu_out, d_dJdu_out,
-                                            target, ldu, st)
+                                            target, ldu, dJdu_in, d_dJdu_in, st)
                  : launch_hvp_nn_any<float2>(P, N, L, B, base, grad != nullptr, u_out, du_out, dJdu_out, d_dJdu_out,
-                                             target, ldu, st);
+                                             target, ldu, dJdu_in, d_dJdu_in, st);
         if (rc) return rc;
     } else {
         switch (P.W) {
-            case 2: launch_hvp<2>(P, N, L, B, base, u_out, du_out, dJdu_out, d_dJdu_out, target, ldu, st); break;
-            case 4: launch_hvp<4>(P, N, L, B, base, u_out, du_out, dJdu_out, d_dJdu_out, target, ldu, st); break;
-            default: launch_hvp<8>(P, N, L, B, base, u_out, du_out, dJdu_out, d_dJdu_out, target, ldu, st); break;
+            case 2: launch_hvp<2>(P, N, L, B, base, u_out, du_out, dJdu_out, d_dJdu_out, target, ldu, dJdu_in, d_dJdu_in,
+                               st);
+                break;
+            case 4: launch_hvp<4>(P, N, L, B, base, u_out, du_out, dJdu_out, d_dJdu_out, target, ldu, dJdu_in, d_dJdu_in,
+                               st);
+                break;
+            default: launch_hvp<8>(P, N, L, B, base, u_out, du_out, dJdu_out, d_dJdu_out, target, ldu, dJdu_in, d_dJdu_in,
+                               st);
+                break;
         }
     }
     if (hipGetLastError() != hipSuccess) return MESHPROP_ERR_HIP;
@@ -859,6 +881,15 @@ int meshprop_hvp(int X, int N, int L, int B, int fft, const int* geom0, const in
         return rc;
     return launch_dp_reduce(L, N / 2, P.NP / 2, P.np, P.grid, P.E, (const float*)(base + P.off_ph), hvp, ldp, 0,
                             P.kso, st);
+}
+
+int meshprop_hvp(int X, int N, int L, int B, int fft, const int* geom0, const int* geom1, const float* p,
+                 const float* q, int ldp, const float* s, int lds, const void* u_out, const void* du_out,
+                 const void* dJdu_out, const void* d_dJdu_out, const void* target, float* grad, float* hvp, int ldu,
+                 void* workspace, size_t workspace_bytes, void* stream) {
+    return meshprop_hvp_grad_in(X, N, L, B, fft, geom0, geom1, p, q, ldp, s, lds, u_out, du_out, dJdu_out,
+                                d_dJdu_out, target, grad, hvp, nullptr, nullptr, ldu, workspace, workspace_bytes,
+                                stream);
 }
 
 }  // extern "C"

@@ -395,25 +395,30 @@ class MeshNetworkGPU:
     # ---- Hessian-vector products (gpu/mesh.py:236-288) --------------------------------
     def exact_hvp_supported(self) -> bool:
         """Forward-over-reverse HVP kernel: unitary crossings (MZI, SYM; orth on NN meshes) up to
-        N = 1024 on NN meshes and N = 512 on FFT meshes, with no phase screen or one at the output."""
-        if (self.is_phase and self.phi_pos == "in") or self.X not in ("mzi", "sym", "orth"):
+        N = 1024 on NN meshes and N = 512 on FFT meshes, with or without a phase screen."""
+        if self.X not in ("mzi", "sym", "orth"):
             return False
         if self.fft:
             return self.X != "orth" and self.N <= 512
         return self.N <= 1024
 
-    def _hvp_exact(self, q, out, dout, g, dg, p, s):
+    def _hvp_exact(self, q, out, dout, g, dg, p, s, v=None):
         """H q from the (B, N) output field, its tangent along q, the output gradient and its tangent.
 
         With an output screen y = S z, S = diag(exp(i phi)): the screen's gradient entries
         sum_b Im(g conj(y)) have the derivative sum_b Im(dg conj(y) + g conj(dy)); the mesh part runs
         on z = S^H y with tangent S^H (dy - i dphi y), gradient S^H g and its tangent
-        S^H (dg - i dphi g), as the screen's adjoint hands them to the mesh."""
+        S^H (dg - i dphi g), as the screen's adjoint hands them to the mesh.  With an input screen
+        x = S v: the kernel also returns the input gradient g_x and its tangent dg_x, and the screen's
+        entries are sum_b Im(dg_x conj(x) + g_x conj(dx)), dx = i dphi x."""
         B = out.shape[0]
         hv = torch.zeros(self.n_phase, dtype=torch.float32, device=self.device)
-        hv_screen = None
-        if self.is_phase:
-            n_m = self.n_cr * self.X_np
+        hv_screen = gx = dgx = None
+        n_m = self.n_cr * self.X_np
+        if self.is_phase and self.phi_pos == "in":
+            assert v is not None, "an input screen needs the input field"
+            gx, dgx = torch.empty_like(g), torch.empty_like(g)
+        elif self.is_phase:
             hv_screen = torch.sum(torch.imag(dg * out.conj() + g * dout.conj()), dim=0)
             sh = torch.exp(-1j * p[n_m:]).to(self.dtype)
             idq = (1j * q[n_m:]).to(self.dtype)
@@ -424,9 +429,13 @@ class MeshNetworkGPU:
         ldp, lds = self._ld()
         g0, g1 = self._geom()
         _lib.hvp(self.X, self.N, self.L, B, self.fft, g0, g1, p, q, ldp, s if s.numel() else None, lds, out, dout,
-                 g, dg, None, None, hv, self.N, ws, nb)
+                 g, dg, None, None, hv, self.N, ws, nb, dJdu_in=gx, d_dJdu_in=dgx)
+        if gx is not None:
+            x = v * torch.exp(1j * p[n_m:]).to(self.dtype)
+            dx = (1j * q[n_m:]).to(self.dtype) * x
+            hv_screen = torch.sum(torch.imag(dgx * x.conj() + gx * dx.conj()), dim=0)
         if hv_screen is not None:
-            hv[self.n_cr * self.X_np:] = hv_screen
+            hv[n_m:] = hv_screen
         return hv
 
     def _hvp_difference(self, vec, v, w, p, p_splitter, eps):
@@ -454,7 +463,7 @@ class MeshNetworkGPU:
             return self._hvp_difference(vec, v, w, p, p_splitter, eps)
         gb = as_field(g, self.N, self.dtype, self.device)
         dgb = as_field(dg, self.N, self.dtype, self.device)
-        return self._hvp_exact(vec, out.T, dout.T, gb, dgb, p, s)
+        return self._hvp_exact(vec, out.T, dout.T, gb, dgb, p, s, as_field(v, self.N, self.dtype, self.device))
 
     def hvp_target(self, vec, U_target, p_phase=None, p_splitter=None, eps=1e-3):
         """H vec of L2norm(U_target) (gpu/mesh.py:253-266); exact where supported: the output
@@ -475,7 +484,8 @@ class MeshNetworkGPU:
         if not self.is_phase:  # the loss's diagonal is dropped without a screen
             dg = dg - torch.diag_embed(torch.diagonal(dg))
         return self._hvp_exact(q, out.T, dout.T, as_field(g, self.N, self.dtype, self.device),
-                               as_field(dg, self.N, self.dtype, self.device), p, s)
+                               as_field(dg, self.N, self.dtype, self.device), p, s,
+                               as_field(v, self.N, self.dtype, self.device))
 
     def hess(self, v, w: Callable, p_phase=None, p_splitter=None, eps=1e-3):
         # (the reference passes these arguments to hvp_target by mistake, gpu/mesh.py:276)

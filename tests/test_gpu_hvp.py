@@ -4,8 +4,8 @@ The oracle's hvp_inverse is pinned by central differences of its gradient; here 
 (meshprop_hvp after fwddiff along q) must match it within 1e-5 relative L2: MZI and SYM on NN
 meshes up to N = 1024 (every NN tile width, W = 2 .. 16) and on FFT meshes up to N = 512, orth
 (real fields) on NN meshes, several vector tiles per workgroup; MeshNetworkGPU.hvp_target /
-hess_target must agree with the oracle and be symmetric; meshes with a phase screen (exact at the
-output, central differences at the input) against FP64 differences of the oracle's gradient.
+hess_target must agree with the oracle and be symmetric; meshes with a phase screen (input or output)
+against FP64 differences of the oracle's gradient.
 """
 import zlib
 
@@ -57,15 +57,28 @@ def test_hvp_kernel_matches_oracle(gpu, case):
     qt = torch.as_tensor(q.reshape(-1)).cuda()
     grad = torch.zeros(mesh.n_phase, dtype=torch.float32, device="cuda")
     hv = torch.zeros_like(grad)
+    gin = torch.full((B, N), float("nan"), dtype=mesh.dtype, device="cuda")
+    dgin = torch.full_like(gin, float("nan"))
     nb = Lb.hvp_workspace_size(X, N, g.L, B, mesh.fft)
     ws = torch.empty(nb, dtype=torch.uint8, device="cuda")
     g0, g1 = mesh._geom()
     Lb.hvp(X, N, g.L, B, mesh.fft, g0, g1, mesh.p_phase, qt, N // 2 * n_p, mesh.p_splitter, N // 2 * n_s, out.T,
-           dout.T, None, None, torch.as_tensor(t).cuda(), grad, hv, N, ws, nb)
+           dout.T, None, None, torch.as_tensor(t).cuda(), grad, hv, N, ws, nb, dJdu_in=gin, d_dJdu_in=dgin)
     rg, rh = O.hvp_inverse(g, X, p.astype(np.float64), s.astype(np.float64), u.astype(np.complex128),
                            q.astype(np.float64), t.astype(np.complex128))
     assert O.rel_l2(grad.cpu().numpy().reshape(rg.shape), rg) < TOL
     assert O.rel_l2(hv.cpu().numpy().reshape(rh.shape), rh) < TOL
+    # the input gradient (oracle adjoint) and its tangent along q (FP64 central difference)
+    s64, u64, t64 = s.astype(np.float64), u.astype(np.complex128), t.astype(np.complex128)
+
+    def g_input(pp):
+        o = O.mesh_fwd(g, X, pp, s64, u64)
+        return O.mesh_vjp(g, X, pp, s64, o, 2 * (o - t64), u_in=u64)[1]
+    h = 1e-5
+    p64, q64 = p.astype(np.float64), q.astype(np.float64)
+    assert O.rel_l2(gin.cpu().numpy(), g_input(p64)) < TOL
+    fd = (g_input(p64 + h * q64) - g_input(p64 - h * q64)) / (2 * h)
+    assert O.rel_l2(dgin.cpu().numpy(), fd) < TOL
 
 
 def test_hvp_target_and_hessian(gpu):
@@ -162,14 +175,14 @@ def _screen_grad(g, X, p, ph, v, gfun, pos):
 
 
 SCREEN_CASES = [("mzi", "clements", 16, 0, 12, "out"), ("sym", "nn", 64, 9, 11, "out"),
-                ("mzi", "fft", 32, 5, 7, "out"), ("mzi", "nn", 40, 6, 5, "in")]
+                ("mzi", "fft", 32, 5, 7, "out"), ("mzi", "nn", 40, 6, 5, "in"), ("sym", "clements", 16, 0, 9, "in"),
+                ("mzi", "fft", 64, 6, 4, "in")]
 
 
 @pytest.mark.parametrize("case", SCREEN_CASES, ids=["-".join(map(str, c)) for c in SCREEN_CASES])
 def test_hvp_with_phase_screen(gpu, case):
     """hvp and hvp_target on meshes with a phase screen, against FP64 central differences of the
-    oracle's gradient (screen phases included).  An output screen takes the exact path (1e-5); an
-    input screen falls back to the reference's difference of FP32 gradients (1e-2)."""
+    oracle's gradient (screen phases included), both screen positions on the exact path (1e-5)."""
     from meshes_amd import MeshNetworkGPU
     X, kind, N, L, B, pos = case
     rng = np.random.RandomState(zlib.crc32(repr(case).encode()))
@@ -179,9 +192,8 @@ def test_hvp_with_phase_screen(gpu, case):
     ph = rng.uniform(0, 2 * np.pi, N).astype(np.float32)
     mesh = MeshNetworkGPU(N, g.L, g.lens, g.shifts, np.concatenate([p.ravel(), ph]), np.zeros(0), X, phi_pos=pos,
                           is_phase=True, strides=g.strides if kind == "fft" else None)
-    exact = pos == "out"
-    assert mesh.exact_hvp_supported() == exact
-    tol = TOL if exact else 1e-2
+    assert mesh.exact_hvp_supported()
+    tol = TOL
     v = O.unit_cn(rng, B, N).astype(np.complex64)
     q = rng.standard_normal(mesh.n_phase).astype(np.float32)
     h = 1e-5
